@@ -1,0 +1,162 @@
+#!/usr/bin/env python3
+"""Throughput of the HZO phase-field step on one MI355X.
+
+Workload (the reference's two-precision comparison film): 256x256x32 voxels,
+200 seeded Voronoi grains with FE/AFE/DE fractions 0.6/0.2/0.2, per-phase
+Landau coefficients within +-20 % of the base values, triangular applied
+field +-3 MV/cm over 2000 steps, Jacobi-PCG depolarisation solve after every
+step to ||r|| <= 1e-8 ||b||.  A step = TDGL update + full PCG solve.
+
+Prints one JSON line (metric, value, roofline, cpu_baseline, ...).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+NX, NY, NZ, NG = 256, 256, 32, 200
+FRACS = (0.2, 0.6, 0.2)  # AFE, FE, DE
+PERIOD, AMP = 2000, 3.0
+
+
+def build_inputs():
+    import cmsl_amd as H
+    return H.make_film_inputs(NX, NY, NZ, NG, fracs=FRACS, seed=1, jitter=0.2)
+
+
+def algorithmic_bytes_per_cell(kclass: str, precision: str) -> float:
+    """Compulsory HBM bytes per voxel of one launch (each array touched once).
+
+    fp64: pcg_dir_spmv reads r, p_old, code (8+8+2) and writes p_new (8) = 26;
+          pcg_update reads x, p, r, code (8+8+8+2) and writes x, r (16) = 42;
+          pcg_setup reads Pmx,Pmy,Pmz, x, code (32+2) and writes r (8) = 42;
+          tdgl reads P (48), phi (8), cell word (4) and writes P (48) = 108.
+    """
+    table = {
+        "fp64": {"tdgl": 108.0, "pcg_setup": 42.0, "pcg_dir_spmv": 26.0, "pcg_update": 42.0},
+    }
+    return table[precision][kclass]
+
+
+def run_gpu(args, inp):
+    import cmsl_amd as H
+    from cmsl_amd import _lib as L
+
+    film = H.Film(inp, precision=L.PREC_FP64, pcg_rule=L.PCG_RULE_RELRES, tol_rel=1e-8, tol_abs=1e-30)
+    Ez = inp.field_nd(H.triangle_wave(PERIOD, AMP, args.warmup + args.steps))
+    film.run(Ez[: args.warmup])
+    film.synchronize()
+    t0 = time.perf_counter()
+    its = film.run(Ez[args.warmup:])
+    film.synchronize()
+    t1 = time.perf_counter()
+    # kernel timing pass: the same steps again, each launch bracketed by HIP
+    # events on the film's stream
+    film.set_timing(True)
+    Ez2 = inp.field_nd(H.triangle_wave(PERIOD, AMP, args.timing_steps, start=args.warmup + args.steps))
+    film.run(Ez2)
+    kt = film.kernel_times()
+    film.set_timing(False)
+    film.close()
+    return t1 - t0, its, kt
+
+
+def run_cpu_baseline(inp, nsteps):
+    """The FP64 oracle (a restatement of the reference kernels; the reference
+    itself is CUDA-only) on the same film, first `nsteps` steps."""
+    import cmsl_amd as H
+    from oracle import pyoracle as O
+
+    c = O.cfg(inp.nx, inp.ny, inp.nz, inp.del_h, inp.del_t, dep_flag=1, zbc=0,
+              eps0_nd=inp.scales["eps0_nd"], pcg_rule=1, tol_abs=1e-30, tol_rel=1e-8)
+    f = O.Film(c, inp.omega, inp.grain_phase, inp.TR, inp.phases, inp.iface)
+    P = inp.P0.copy()
+    phi = np.zeros(inp.N)
+    E3 = np.zeros(3 * inp.N)
+    Ez = inp.field_nd(H.triangle_wave(PERIOD, AMP, nsteps))
+    t0 = time.perf_counter()
+    its, _ = f.run(P, phi, E3, Ez)
+    dt = time.perf_counter() - t0
+    return dt, its
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--timing-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (the film fits one MI355X); --gpus {args.gpus} "
+              "is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    inp = build_inputs()
+    N = inp.N
+    dt, its, kt = run_gpu(args, inp)
+    value = N * args.steps / dt / 1e6
+
+    # dominant kernel: largest total time in the timing pass
+    tot = {k: c * ms for k, (c, ms) in kt.items() if c > 0}
+    dom = max(tot, key=tot.get)
+    cnt, ms = kt[dom]
+    bpc = algorithmic_bytes_per_cell(dom, "fp64")
+    achieved = bpc * N / (ms * 1e-3) / 1e9
+
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(prof):
+        with open(prof) as fh:
+            pm = json.load(fh)
+        traffic = pm.get("fp64", {}).get(dom)
+
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+        cdt, cits = run_cpu_baseline(inp, args.cpu_steps)
+        cpu = {"value": N * args.cpu_steps / cdt / 1e6, "unit": "Mcell-steps/s", "cores": threads,
+               "kind": "port",
+               "sample": f"FP64 oracle, first {args.cpu_steps} steps of the same film and field, "
+                         f"{int(np.sum(cits))} PCG iterations, {cdt:.1f} s"}
+
+    line = {
+        "metric": "Mcell-steps/s incl. PCG at 256x256x32 (FP64 plain)",
+        "value": value,
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Voronoi film, seed 1)",
+        "config": {"workload": "256x256x32 film, 200 grains FE/AFE/DE 0.6/0.2/0.2, +-20% Landau, "
+                               "triangle +-3 MV/cm / 2000 steps, PCG ||r||<=1e-8||b||",
+                   "nx": NX, "ny": NY, "nz": NZ, "grains": NG, "precision": "fp64"},
+        "pcg_iters_mean": float(np.mean(its)),
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "bytes_per_cell": bpc, "launch_ms": ms, "launches_timed": cnt},
+        "kernel_ms": {k: v[1] for k, v in kt.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,643 @@
+// hzo_api.hip -- C-ABI context layer: owns the device state of one film and
+// drives the step (replaces AllocateData/Cleanup, the device globals, the
+// step loop body of evolve.cu:220-237 and the host side of Depol_field).
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "host/errors.h"
+#include "hzo_internal.h"
+
+using hzo::set_error;
+
+#define HZO_HIP(call)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return set_error(HZO_ERR_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #call,          \
+                             hipGetErrorString(e_));                                           \
+    } while (0)
+
+struct hzo_ctx {
+    hzo_config cfg;
+    hzo::Grid g;
+    hzo::StepConsts c;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int nb = 0;
+
+    double* P[2] = {nullptr, nullptr};
+    int cur = 0;
+    double* phi = nullptr;
+    double* r = nullptr;
+    double* p[2] = {nullptr, nullptr};
+    uint32_t* cw = nullptr;
+    uint16_t* sc = nullptr;
+    double* TR = nullptr;
+    hzo::PhaseC* ptab = nullptr;
+    double* lut = nullptr;
+    double* partials = nullptr;
+    unsigned* counters = nullptr;
+    hzo::PcgState* st = nullptr;
+    double* diag = nullptr;
+    hzo::PcgState* st_host = nullptr;  // pinned
+    double* diag_host = nullptr;       // pinned [16]
+
+    int ng = 0;
+    std::vector<int32_t> omega, grain_phase, iface;
+    hzo_phase_nd ph[3];
+    bool have_micro = false, have_phase = false, have_state = false;
+    int last_iters = 8;
+
+    bool timing = false;
+    std::vector<hipEvent_t> ev[HZO_KCLASSES][2];
+    int nt[HZO_KCLASSES] = {0, 0, 0, 0, 0};
+};
+
+namespace {
+
+bool is_pow2(double v) {
+    if (!(v > 0.0) || !std::isfinite(v)) return false;
+    int e;
+    const double m = frexp(v, &e);
+    return m == 0.5;
+}
+
+// depol_varEPS.cu:10-15
+double harm(double a, double b) {
+    const double eps = 1e-20;
+    double denom = a + b;
+    if (denom < eps) denom = eps;
+    return (2.0 * a * b) / denom;
+}
+
+hzo::Fp64Bufs bufs(hzo_ctx* x) {
+    hzo::Fp64Bufs b;
+    b.Pold = x->P[x->cur];
+    b.Pnew = x->P[x->cur ^ 1];
+    b.phi = x->phi;
+    b.r = x->r;
+    b.p[0] = x->p[0];
+    b.p[1] = x->p[1];
+    b.cw = x->cw;
+    b.sc = x->sc;
+    b.TR = x->TR;
+    b.ptab = x->ptab;
+    b.lut = x->lut;
+    b.partials = x->partials;
+    b.counters = x->counters;
+    b.st = x->st;
+    b.diag = x->diag;
+    return b;
+}
+
+template <class F>
+int timed(hzo_ctx* x, int cls, F launch) {
+    const bool t = x->timing && x->nt[cls] < HZO_TIMING_MAX;
+    if (t) {
+        auto& s = x->ev[cls][0];
+        auto& e = x->ev[cls][1];
+        const int i = x->nt[cls];
+        if ((int)s.size() <= i) {
+            hipEvent_t a, b;
+            HZO_HIP(hipEventCreate(&a));
+            HZO_HIP(hipEventCreate(&b));
+            s.push_back(a);
+            e.push_back(b);
+        }
+        HZO_HIP(hipEventRecord(s[i], x->stream));
+        launch();
+        HZO_HIP(hipGetLastError());
+        HZO_HIP(hipEventRecord(e[i], x->stream));
+        x->nt[cls]++;
+    } else {
+        launch();
+        HZO_HIP(hipGetLastError());
+    }
+    return HZO_OK;
+}
+
+int check_ready(hzo_ctx* x) {
+    if (!x) return set_error(HZO_ERR_ARG, "null context");
+    if (!x->have_micro || !x->have_phase || !x->have_state)
+        return set_error(HZO_ERR_STATE, "context needs set_microstructure, set_phase_table and set_state first");
+    return HZO_OK;
+}
+
+// Stencil code per voxel (see hzo_internal.h): permittivity class of the
+// voxel and of its operator neighbours, identity rows flagged.
+void build_codes(hzo_ctx* x, std::vector<uint16_t>& sc) {
+    const int nx = x->g.nx, ny = x->g.ny, nz = x->g.nz;
+    const long long N = x->g.N;
+    sc.assign(N, 0);
+    auto cls = [&](long long id) -> unsigned {
+        const int gid = x->omega[id];
+        if (gid < 0 || gid >= x->ng) return 3u;
+        const int m = x->grain_phase[gid];
+        return m == 2 ? 2u : (m == 1 ? 1u : 0u);
+    };
+    const long long ref = x->c.ref_id;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const long long id = (long long)i + (long long)nx * ((long long)j + (long long)ny * k);
+                if ((x->c.zbc == 0 && (k == 0 || k == nz - 1)) || (x->c.zbc == 1 && id == ref)) {
+                    sc[id] = hzo::SC_IDENTITY;
+                    continue;
+                }
+                const int e = (i + 1) % nx, w = (i - 1 + nx) % nx;
+                const int n = (j + 1) % ny, s = (j - 1 + ny) % ny;
+                int up, dn;
+                if (x->c.zbc == 0) {
+                    up = k + 1;
+                    dn = k - 1;
+                } else if (k == 0) {
+                    up = dn = 1;
+                } else if (k == nz - 1) {
+                    up = dn = nz - 2;
+                } else {
+                    up = k + 1;
+                    dn = k - 1;
+                }
+                auto at = [&](int a, int b2, int c2) {
+                    return (long long)a + (long long)nx * ((long long)b2 + (long long)ny * c2);
+                };
+                const unsigned code = cls(id) | (cls(at(e, j, k)) << 2) | (cls(at(w, j, k)) << 4) |
+                                      (cls(at(i, n, k)) << 6) | (cls(at(i, s, k)) << 8) |
+                                      (cls(at(i, j, up)) << 10) | (cls(at(i, j, dn)) << 12);
+                sc[id] = (uint16_t)code;
+            }
+}
+
+int upload_micro(hzo_ctx* x) {
+    const long long N = x->g.N;
+    std::vector<uint32_t> cw(N);
+    for (long long id = 0; id < N; ++id) {
+        const int gid = x->omega[id];
+        if (gid < 0 || gid >= x->ng) {
+            cw[id] = hzo::CW_VOID;
+            continue;
+        }
+        const int m = x->grain_phase[gid];
+        const uint32_t ph = m == 2 ? 2u : (m == 1 ? 1u : 0u);
+        cw[id] = ((uint32_t)gid & hzo::CW_GID_MASK) | (ph << hzo::CW_PHASE_SHIFT) |
+                 (x->iface[id] ? hzo::CW_IFACE : 0u);
+    }
+    std::vector<uint16_t> sc;
+    build_codes(x, sc);
+    HZO_HIP(hipMemcpy(x->cw, cw.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice));
+    HZO_HIP(hipMemcpy(x->sc, sc.data(), sizeof(uint16_t) * N, hipMemcpyHostToDevice));
+    return HZO_OK;
+}
+
+int upload_phase(hzo_ctx* x) {
+    const double del_h = x->cfg.del_h, del_t = x->cfg.del_t;
+    hzo::PhaseC pt[hzo::PTAB_ENTRIES];
+    for (int q = 0; q < 3; ++q)
+        for (int f = 0; f < 2; ++f) {
+            const hzo_phase_nd& P = x->ph[q];
+            double g_loc = P.g, Lfac = P.L;
+            if (f) {  // kernels_evol.cu:142-146
+                g_loc *= 0.4;
+                Lfac *= (1.0 + 10.0);
+            }
+            hzo::PhaseC& c = pt[q * 2 + f];
+            c.A = P.kappa * del_t / (del_h * del_h);
+            c.one_m6A = 1.0 - 6.0 * c.A;
+            c.a0_2 = 2.0 * P.a0;
+            c.beta_4 = 4.0 * P.beta;
+            c.beta_12 = 12.0 * P.beta;
+            c.gamma_6 = 6.0 * P.gamma;
+            c.ag = 2.0 * (P.alpha + g_loc);
+            c.ags = 2.0 * (P.alpha - g_loc);
+            c.cL = Lfac * del_t;
+        }
+    std::vector<double> lut(hzo::LUT_DOUBLES, 0.0);
+    double eps[4] = {x->ph[0].eps_r, x->ph[1].eps_r, x->ph[2].eps_r, 0.0};
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) lut[hzo::LUT_HARM + a * 4 + b] = harm(eps[a], eps[b]);
+    const double h = del_h;
+    for (unsigned code = 0; code < (1u << 14); ++code) {
+        const double* Hr = &lut[hzo::LUT_HARM + (code & 3u) * 4];
+        const double ex_p = Hr[(code >> 2) & 3], ex_m = Hr[(code >> 4) & 3];
+        const double ey_p = Hr[(code >> 6) & 3], ey_m = Hr[(code >> 8) & 3];
+        const double ez_p = Hr[(code >> 10) & 3], ez_m = Hr[(code >> 12) & 3];
+        const double D = (ex_p + ex_m + ey_p + ey_m + ez_p + ez_m) / (h * h);  // depol_varEPS.cu:177-179
+        lut[hzo::LUT_DINV + code] = (D > 1e-30 ? 1.0 / D : 1.0);
+    }
+    lut[hzo::LUT_DINV + hzo::SC_IDENTITY] = 1.0;
+    HZO_HIP(hipMemcpy(x->ptab, pt, sizeof(pt), hipMemcpyHostToDevice));
+    HZO_HIP(hipMemcpy(x->lut, lut.data(), sizeof(double) * hzo::LUT_DOUBLES, hipMemcpyHostToDevice));
+    return HZO_OK;
+}
+
+int init_pcg_state(hzo_ctx* x) {
+    hzo::PcgState s;
+    memset(&s, 0, sizeof(s));
+    s.max_iter = x->cfg.max_iter;
+    s.rule = x->cfg.pcg_rule;
+    s.tol_abs2 = x->cfg.tol_abs * x->cfg.tol_abs;
+    s.tol_rel2 = x->cfg.tol_rel * x->cfg.tol_rel;
+    s.done = 1;
+    HZO_HIP(hipMemcpy(x->st, &s, sizeof(s), hipMemcpyHostToDevice));
+    return HZO_OK;
+}
+
+int tdgl(hzo_ctx* x, double Ez_ext) {
+    const hzo::Fp64Bufs b = bufs(x);
+    int rc = timed(x, 0, [&] { hzo::launch_tdgl_f64(x->stream, x->g, x->c, Ez_ext, b); });
+    if (rc) return rc;
+    x->cur ^= 1;  // kernelUpdate_P becomes a buffer swap
+    return HZO_OK;
+}
+
+int depol(hzo_ctx* x, hzo_pcg_stats* stats) {
+    const hzo::Fp64Bufs b = bufs(x);
+    int rc = timed(x, 1, [&] { hzo::launch_pcg_setup_f64(x->stream, x->g, x->c, b); });
+    if (rc) return rc;
+    const int maxit = x->cfg.max_iter;
+    int it = 0;
+    int batch = x->last_iters > 0 ? x->last_iters : 1;
+    while (true) {
+        for (int m = 0; m < batch && it < maxit; ++m) {
+            ++it;
+            rc = timed(x, 2, [&] { hzo::launch_pcg_dir_f64(x->stream, x->g, x->c, b, it); });
+            if (rc) return rc;
+            rc = timed(x, 3, [&] { hzo::launch_pcg_update_f64(x->stream, x->g, x->c, b, it); });
+            if (rc) return rc;
+        }
+        HZO_HIP(hipMemcpyAsync(x->st_host, x->st, sizeof(hzo::PcgState), hipMemcpyDeviceToHost, x->stream));
+        HZO_HIP(hipStreamSynchronize(x->stream));
+        if (x->st_host->done || it >= maxit) break;
+        batch = x->last_iters / 4 > 2 ? x->last_iters / 4 : 2;
+    }
+    const hzo::PcgState& s = *x->st_host;
+    x->last_iters = s.iters > 0 ? s.iters : 1;
+    if (stats) {
+        stats->iters = s.iters;
+        stats->status = s.status;
+        stats->refinements = s.refinements;
+        stats->reserved = 0;
+        stats->rz0 = s.rz0;
+        stats->rz = s.rz_old;
+        stats->rr0 = s.rr0;
+        stats->rr = s.rr;
+        stats->bb = s.bb;
+    }
+    return HZO_OK;
+}
+
+void free_all(hzo_ctx* x) {
+    (void)hipFree(x->P[0]);
+    (void)hipFree(x->P[1]);
+    (void)hipFree(x->phi);
+    (void)hipFree(x->r);
+    (void)hipFree(x->p[0]);
+    (void)hipFree(x->p[1]);
+    (void)hipFree(x->cw);
+    (void)hipFree(x->sc);
+    (void)hipFree(x->TR);
+    (void)hipFree(x->ptab);
+    (void)hipFree(x->lut);
+    (void)hipFree(x->partials);
+    (void)hipFree(x->counters);
+    (void)hipFree(x->st);
+    (void)hipFree(x->diag);
+    if (x->st_host) (void)hipHostFree(x->st_host);
+    if (x->diag_host) (void)hipHostFree(x->diag_host);
+    for (int c = 0; c < HZO_KCLASSES; ++c)
+        for (int q = 0; q < 2; ++q)
+            for (auto e : x->ev[c][q]) (void)hipEventDestroy(e);
+    if (x->stream) (void)hipStreamDestroy(x->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int hzo_device_count(int* n) {
+    if (!n) return set_error(HZO_ERR_ARG, "null argument");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *n = c;
+    return HZO_OK;
+}
+
+void hzo_config_default(hzo_config* cfg) {
+    if (!cfg) return;
+    memset(cfg, 0, sizeof(*cfg));
+    cfg->nx = cfg->ny = cfg->nz = 0;
+    cfg->del_h = 1.0;
+    cfg->del_t = 0.01;
+    cfg->dep_flag = 1;
+    cfg->zbc = HZO_ZBC_DIRICHLET;
+    cfg->eps0_nd = 1.0;
+    cfg->g_bot = 0.0;  // global.cu:140-144
+    cfg->g_top = 0.0;
+    cfg->precision = HZO_PREC_FP64;
+    cfg->pcg_rule = HZO_PCG_RULE_REFERENCE;
+    cfg->tol_abs = 1.0e-3;
+    cfg->tol_rel = 1.0e-2;
+    cfg->max_iter = 4000;
+    cfg->refine_every = 20;
+}
+
+int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
+    if (!cfg || !out) return set_error(HZO_ERR_ARG, "hzo_create: null argument");
+    *out = nullptr;
+    if (cfg->nx <= 0 || cfg->ny <= 0 || cfg->nz <= 0)
+        return set_error(HZO_ERR_ARG, "hzo_create: grid %dx%dx%d", cfg->nx, cfg->ny, cfg->nz);
+    if (!(cfg->del_h > 0.0) || !(cfg->del_t > 0.0)) return set_error(HZO_ERR_ARG, "hzo_create: del_h, del_t must be > 0");
+    if (cfg->zbc != HZO_ZBC_DIRICHLET && cfg->zbc != HZO_ZBC_NEUMANN)
+        return set_error(HZO_ERR_ARG, "hzo_create: zbc %d", cfg->zbc);
+    if (cfg->zbc == HZO_ZBC_NEUMANN && cfg->nz < 3)
+        return set_error(HZO_ERR_ARG, "hzo_create: Neumann mirror needs nz >= 3");
+    if (cfg->zbc == HZO_ZBC_DIRICHLET && cfg->nz < 2)
+        return set_error(HZO_ERR_ARG, "hzo_create: Dirichlet electrodes need nz >= 2");
+    if (cfg->precision != HZO_PREC_FP64)
+        return set_error(HZO_ERR_ARG, "hzo_create: precision %d not available in this build", cfg->precision);
+    if (cfg->pcg_rule != 0 && cfg->pcg_rule != 1) return set_error(HZO_ERR_ARG, "hzo_create: pcg_rule %d", cfg->pcg_rule);
+    if (cfg->max_iter < 0) return set_error(HZO_ERR_ARG, "hzo_create: max_iter < 0");
+    const long long N = (long long)cfg->nx * cfg->ny * cfg->nz;
+    if (N > (1LL << 31)) return set_error(HZO_ERR_ARG, "hzo_create: grid too large");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return set_error(HZO_ERR_HIP, "hzo_create: no HIP device available");
+
+    hzo_ctx* x = new hzo_ctx();
+    x->cfg = *cfg;
+    x->g.nx = cfg->nx;
+    x->g.ny = cfg->ny;
+    x->g.nz = cfg->nz;
+    x->g.N = N;
+    hzo::StepConsts& c = x->c;
+    c.nx = cfg->nx;
+    c.ny = cfg->ny;
+    c.nz = cfg->nz;
+    c.dep_flag = cfg->dep_flag;
+    c.zbc = cfg->zbc;
+    c.del_h = cfg->del_h;
+    c.del_t = cfg->del_t;
+    c.g_bot = cfg->g_bot;
+    c.g_top = cfg->g_top;
+    c.eps0 = cfg->eps0_nd;
+    c.two_h = 2.0 * cfg->del_h;
+    c.inv_two_h = 1.0 / c.two_h;
+    c.two_h_pow2 = is_pow2(c.two_h);
+    c.h2 = cfg->del_h * cfg->del_h;
+    c.inv_h2 = 1.0 / c.h2;
+    c.h2_pow2 = is_pow2(c.h2);
+    c.ref_id = (long long)(cfg->nx / 2) + (long long)cfg->nx * ((long long)(cfg->ny / 2) + (long long)cfg->ny * (cfg->nz / 2));
+
+    (void)hipGetDevice(&x->device);
+    x->nb = hzo::blocks_for(x->g);
+    auto fail = [&](int rc) {
+        free_all(x);
+        delete x;
+        return rc;
+    };
+#define HZO_ALLOC(ptr, bytes)                                                                        \
+    do {                                                                                             \
+        if (hipMalloc((void**)&(ptr), (bytes)) != hipSuccess)                                       \
+            return fail(set_error(HZO_ERR_NOMEM, "hzo_create: hipMalloc of %zu bytes failed", (size_t)(bytes))); \
+    } while (0)
+    if (hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(set_error(HZO_ERR_HIP, "hzo_create: stream creation failed"));
+    HZO_ALLOC(x->P[0], sizeof(double) * 6 * N);
+    HZO_ALLOC(x->P[1], sizeof(double) * 6 * N);
+    HZO_ALLOC(x->phi, sizeof(double) * N);
+    HZO_ALLOC(x->r, sizeof(double) * N);
+    HZO_ALLOC(x->p[0], sizeof(double) * N);
+    HZO_ALLOC(x->p[1], sizeof(double) * N);
+    HZO_ALLOC(x->cw, sizeof(uint32_t) * N);
+    HZO_ALLOC(x->sc, sizeof(uint16_t) * N);
+    HZO_ALLOC(x->ptab, sizeof(hzo::PhaseC) * hzo::PTAB_ENTRIES);
+    HZO_ALLOC(x->lut, sizeof(double) * hzo::LUT_DOUBLES);
+    HZO_ALLOC(x->partials, sizeof(double) * hzo::RED_MAXQ * (size_t)x->nb);
+    HZO_ALLOC(x->counters, sizeof(unsigned) * hzo::RED_COUNTERS);
+    HZO_ALLOC(x->st, sizeof(hzo::PcgState));
+    HZO_ALLOC(x->diag, sizeof(double) * 16);
+#undef HZO_ALLOC
+    if (hipHostMalloc((void**)&x->st_host, sizeof(hzo::PcgState), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&x->diag_host, sizeof(double) * 16, hipHostMallocDefault) != hipSuccess)
+        return fail(set_error(HZO_ERR_NOMEM, "hzo_create: pinned host allocation failed"));
+    if (hipMemset(x->counters, 0, sizeof(unsigned) * hzo::RED_COUNTERS) != hipSuccess ||
+        hipMemset(x->diag, 0, sizeof(double) * 16) != hipSuccess ||
+        hipMemset(x->p[0], 0, sizeof(double) * N) != hipSuccess ||
+        hipMemset(x->p[1], 0, sizeof(double) * N) != hipSuccess)
+        return fail(set_error(HZO_ERR_HIP, "hzo_create: memset failed"));
+    int rc = init_pcg_state(x);
+    if (rc) return fail(rc);
+    *out = x;
+    return HZO_OK;
+}
+
+void hzo_destroy(hzo_ctx* x) {
+    if (!x) return;
+    (void)hipSetDevice(x->device);
+    (void)hipStreamSynchronize(x->stream);
+    free_all(x);
+    delete x;
+}
+
+int hzo_set_microstructure(hzo_ctx* x, const int32_t* omega, int ng, const int32_t* grain_phase,
+                           const double* TR, const int32_t* iface) {
+    if (!x || !omega || !grain_phase || !TR || ng <= 0)
+        return set_error(HZO_ERR_ARG, "hzo_set_microstructure: bad argument");
+    if (ng > (int)hzo::CW_GID_MASK) return set_error(HZO_ERR_ARG, "hzo_set_microstructure: too many grains");
+    HZO_HIP(hipSetDevice(x->device));
+    const long long N = x->g.N;
+    x->ng = ng;
+    x->omega.assign(omega, omega + N);
+    x->grain_phase.assign(grain_phase, grain_phase + ng);
+    x->iface.assign(N, 0);
+    if (iface) {
+        for (long long i = 0; i < N; ++i) x->iface[i] = iface[i] ? 1 : 0;
+    } else {
+        int rc = hzo_interface_flag(x->g.nx, x->g.ny, x->g.nz, omega, ng, grain_phase, x->iface.data());
+        if (rc) return rc;
+    }
+    (void)hipFree(x->TR);
+    x->TR = nullptr;
+    if (hipMalloc((void**)&x->TR, sizeof(double) * 9 * (size_t)ng) != hipSuccess)
+        return set_error(HZO_ERR_NOMEM, "hzo_set_microstructure: TR allocation failed");
+    HZO_HIP(hipMemcpy(x->TR, TR, sizeof(double) * 9 * (size_t)ng, hipMemcpyHostToDevice));
+    int rc = upload_micro(x);
+    if (rc) return rc;
+    x->have_micro = true;
+    return HZO_OK;
+}
+
+int hzo_set_phase_table(hzo_ctx* x, const hzo_phase_nd ph[3]) {
+    if (!x || !ph) return set_error(HZO_ERR_ARG, "hzo_set_phase_table: null argument");
+    HZO_HIP(hipSetDevice(x->device));
+    for (int q = 0; q < 3; ++q) x->ph[q] = ph[q];
+    int rc = upload_phase(x);
+    if (rc) return rc;
+    x->have_phase = true;
+    return HZO_OK;
+}
+
+int hzo_set_state(hzo_ctx* x, const double* P, const double* phi) {
+    if (!x || !P) return set_error(HZO_ERR_ARG, "hzo_set_state: null argument");
+    HZO_HIP(hipSetDevice(x->device));
+    const long long N = x->g.N;
+    x->cur = 0;
+    HZO_HIP(hipMemcpy(x->P[0], P, sizeof(double) * 6 * N, hipMemcpyHostToDevice));
+    if (phi) {
+        HZO_HIP(hipMemcpy(x->phi, phi, sizeof(double) * N, hipMemcpyHostToDevice));
+    } else {  // init_phi_linear (depol_varEPS.cu:203-221)
+        std::vector<double> h(N);
+        const int nx = x->g.nx, ny = x->g.ny, nz = x->g.nz;
+        for (int k = 0; k < nz; ++k) {
+            double val = 0.0;
+            if (x->c.zbc == 0) {
+                const double t = (nz <= 1) ? 0.0 : (double)k / (double)(nz - 1);
+                val = (1.0 - t) * x->c.g_bot + t * x->c.g_top;
+            }
+            for (long long q = 0; q < (long long)nx * ny; ++q) h[(long long)nx * ny * k + q] = val;
+        }
+        HZO_HIP(hipMemcpy(x->phi, h.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    }
+    x->have_state = true;
+    x->last_iters = 8;
+    return HZO_OK;
+}
+
+int hzo_get_state(hzo_ctx* x, double* P, double* phi, double* E) {
+    if (!x) return set_error(HZO_ERR_ARG, "hzo_get_state: null context");
+    HZO_HIP(hipSetDevice(x->device));
+    HZO_HIP(hipStreamSynchronize(x->stream));
+    const long long N = x->g.N;
+    if (P) HZO_HIP(hipMemcpy(P, x->P[x->cur], sizeof(double) * 6 * N, hipMemcpyDeviceToHost));
+    std::vector<double> ph;
+    if (phi || E) {
+        ph.resize(N);
+        HZO_HIP(hipMemcpy(ph.data(), x->phi, sizeof(double) * N, hipMemcpyDeviceToHost));
+        if (phi) memcpy(phi, ph.data(), sizeof(double) * N);
+    }
+    if (E) {  // E = -grad phi, depol_varEPS.cu:223-276 (host evaluation of the same expressions)
+        const int nx = x->g.nx, ny = x->g.ny, nz = x->g.nz;
+        const double del_h = x->cfg.del_h;
+        auto at = [&](int i, int j, int k) { return (long long)i + (long long)nx * ((long long)j + (long long)ny * k); };
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    const long long id = at(i, j, k);
+                    const int e = (i + 1) % nx, w = (i - 1 + nx) % nx, n = (j + 1) % ny, s = (j - 1 + ny) % ny;
+                    E[id] = -(ph[at(e, j, k)] - ph[at(w, j, k)]) / (2.0 * del_h);
+                    E[N + id] = -(ph[at(i, n, k)] - ph[at(i, s, k)]) / (2.0 * del_h);
+                    double up, dn;
+                    if (x->c.zbc == 0) {
+                        up = (k == nz - 1) ? x->c.g_top : ph[at(i, j, k + 1)];
+                        dn = (k == 0) ? x->c.g_bot : ph[at(i, j, k - 1)];
+                    } else {
+                        const int uu = (k == 0) ? 1 : ((k == nz - 1) ? nz - 2 : k + 1);
+                        const int dd = (k == 0) ? 1 : ((k == nz - 1) ? nz - 2 : k - 1);
+                        up = ph[at(i, j, uu)];
+                        dn = ph[at(i, j, dd)];
+                    }
+                    E[2 * N + id] = -(up - dn) / (2 * del_h);
+                }
+    }
+    return HZO_OK;
+}
+
+int hzo_tdgl_step(hzo_ctx* x, double Ez_ext) {
+    int rc = check_ready(x);
+    if (rc) return rc;
+    HZO_HIP(hipSetDevice(x->device));
+    return tdgl(x, Ez_ext);
+}
+
+int hzo_depol_solve(hzo_ctx* x, hzo_pcg_stats* stats) {
+    int rc = check_ready(x);
+    if (rc) return rc;
+    HZO_HIP(hipSetDevice(x->device));
+    return depol(x, stats);
+}
+
+int hzo_step(hzo_ctx* x, double Ez_ext, hzo_pcg_stats* stats) {
+    int rc = check_ready(x);
+    if (rc) return rc;
+    HZO_HIP(hipSetDevice(x->device));
+    rc = tdgl(x, Ez_ext);
+    if (rc) return rc;
+    if (x->cfg.dep_flag) return depol(x, stats);
+    if (stats) memset(stats, 0, sizeof(*stats));
+    return HZO_OK;
+}
+
+int hzo_run(hzo_ctx* x, const double* Ez, int nsteps, int* its) {
+    int rc = check_ready(x);
+    if (rc) return rc;
+    if (nsteps < 0 || (nsteps > 0 && !Ez)) return set_error(HZO_ERR_ARG, "hzo_run: bad argument");
+    HZO_HIP(hipSetDevice(x->device));
+    for (int s = 0; s < nsteps; ++s) {
+        hzo_pcg_stats st;
+        memset(&st, 0, sizeof(st));
+        rc = tdgl(x, Ez[s]);
+        if (rc) return rc;
+        if (x->cfg.dep_flag) {
+            rc = depol(x, &st);
+            if (rc) return rc;
+        }
+        if (its) its[s] = st.iters;
+    }
+    return HZO_OK;
+}
+
+int hzo_diagnostics(hzo_ctx* x, double out[10]) {
+    int rc = check_ready(x);
+    if (rc) return rc;
+    if (!out) return set_error(HZO_ERR_ARG, "hzo_diagnostics: null output");
+    HZO_HIP(hipSetDevice(x->device));
+    hzo::launch_diag_f64(x->stream, x->g, x->P[x->cur], x->cw, x->partials, x->counters + 4, x->diag + 8);
+    HZO_HIP(hipGetLastError());
+    HZO_HIP(hipMemcpyAsync(x->diag_host, x->diag, sizeof(double) * 16, hipMemcpyDeviceToHost, x->stream));
+    HZO_HIP(hipStreamSynchronize(x->stream));
+    const double* d = x->diag_host;
+    out[0] = d[8];
+    out[1] = d[9];
+    out[2] = d[2];
+    out[3] = d[3];
+    for (int q = 0; q < 6; ++q) out[4 + q] = d[10 + q];
+    return HZO_OK;
+}
+
+int hzo_synchronize(hzo_ctx* x) {
+    if (!x) return set_error(HZO_ERR_ARG, "null context");
+    HZO_HIP(hipSetDevice(x->device));
+    HZO_HIP(hipStreamSynchronize(x->stream));
+    return HZO_OK;
+}
+
+int hzo_set_timing(hzo_ctx* x, int enable) {
+    if (!x) return set_error(HZO_ERR_ARG, "null context");
+    HZO_HIP(hipSetDevice(x->device));
+    HZO_HIP(hipStreamSynchronize(x->stream));
+    x->timing = enable != 0;
+    for (int c = 0; c < HZO_KCLASSES; ++c) x->nt[c] = 0;
+    return HZO_OK;
+}
+
+int hzo_kernel_times(hzo_ctx* x, int counts[HZO_KCLASSES], double mean_ms[HZO_KCLASSES]) {
+    if (!x || !counts || !mean_ms) return set_error(HZO_ERR_ARG, "hzo_kernel_times: null argument");
+    HZO_HIP(hipSetDevice(x->device));
+    HZO_HIP(hipStreamSynchronize(x->stream));
+    for (int c = 0; c < HZO_KCLASSES; ++c) {
+        double tot = 0.0;
+        for (int i = 0; i < x->nt[c]; ++i) {
+            float ms = 0.f;
+            HZO_HIP(hipEventElapsedTime(&ms, x->ev[c][0][i], x->ev[c][1][i]));
+            tot += ms;
+        }
+        counts[c] = x->nt[c];
+        mean_ms[c] = x->nt[c] ? tot / x->nt[c] : 0.0;
+    }
+    return HZO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,118 @@
+// hzo_internal.h -- device data layout and launch interface shared by the
+// HIP kernels and the C-ABI layer.  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/hzo.h"
+
+namespace hzo {
+
+// ---------------------------------------------------------------------------
+// Per-voxel words.
+//
+// cell word (uint32, TDGL): grain id in bits 0-27, phase in bits 28-29,
+// interface flag in bit 30, void (omega < 0 or >= ng) in bit 31.  Replaces
+// the reference's omega_d + is_interface_d + eight per-voxel parameter maps
+// (kernels_evol.cu:51-58, init_conf.cu:98-105): the maps hold per-phase
+// constants, so the phase index is all the TDGL needs per voxel.
+constexpr uint32_t CW_GID_MASK = 0x0FFFFFFFu;
+constexpr int CW_PHASE_SHIFT = 28;
+constexpr uint32_t CW_IFACE = 1u << 30;
+constexpr uint32_t CW_VOID = 1u << 31;
+
+// stencil code (uint16, Poisson operator): permittivity class (0 AFE, 1 FE,
+// 2 DE, 3 void with eps = 0) of the voxel and of its e,w,n,s,u,d neighbours
+// under the operator's index rules, 2 bits each (bits 0-13); bit 14 marks an
+// identity row (Dirichlet electrode plane or Neumann gauge node,
+// depol_varEPS.cu:53-71).  The harmonic face means and the Jacobi inverse
+// diagonal of every row are functions of this code alone and come from a
+// lookup table built on the host with the reference formulas
+// (depol_varEPS.cu:10-15, 126-180), so they are bit-identical to
+// per-voxel evaluation.
+constexpr uint16_t SC_IDENTITY = 1u << 14;
+constexpr int SC_LUT_SIZE = (1 << 14) + 1;  // index SC_IDENTITY maps to the last entry
+// lut layout (doubles): [0,16) harm table H[a*4+b]; [16, 16+SC_LUT_SIZE) dinv.
+constexpr int LUT_HARM = 0;
+constexpr int LUT_DINV = 16;
+constexpr int LUT_DOUBLES = 16 + SC_LUT_SIZE;
+
+// TDGL coefficients of one (phase, interface-flag) pair, precomputed on the
+// host with the reference's own expressions (kernels_evol.cu:140-153, 212):
+// identical doubles to the per-voxel evaluation.  Table index phase*2+iface.
+struct PhaseC {
+    double A;        // kappa*del_t/(del_h*del_h)
+    double one_m6A;  // 1.0 - 6.0*A
+    double a0_2;     // 2.0*a0
+    double beta_4;   // 4.0*beta
+    double beta_12;  // 12.0*beta
+    double gamma_6;  // 6.0*gamma
+    double ag;       // 2.0*(alpha + g_loc)
+    double ags;      // 2.0*(alpha - g_loc)
+    double cL;       // Lfac*del_t
+};
+constexpr int PTAB_ENTRIES = 6;
+
+// Device-resident PCG control block.  Scalars never travel to the host
+// inside the iteration; the host polls `done` between launch batches.
+struct PcgState {
+    double rz_old, rz0, rr, rr0, bb, alpha, beta, denom;
+    int iters, status, done, refinements;
+    int max_iter, rule, pad0, pad1;
+    double tol_abs2, tol_rel2;
+};
+
+// Reduction scratch: one completion counter per kernel class and the
+// per-block partial sums.
+constexpr int RED_COUNTERS = 8;
+constexpr int RED_MAXQ = 8;
+
+struct Grid {
+    int nx, ny, nz;
+    long long N;
+};
+
+struct StepConsts {
+    int nx, ny, nz;
+    int dep_flag, zbc;
+    double del_h, del_t;
+    double g_bot, g_top;
+    double eps0;
+    // divisions by 2h and h^2 in the reference become multiplications by the
+    // exact reciprocal only when the divisor is a power of two (then both
+    // give the same correctly rounded value).
+    double two_h, inv_two_h, h2, inv_h2;
+    int two_h_pow2, h2_pow2;
+    long long ref_id;  // Neumann gauge node
+};
+
+// Launch helpers (kernels_fp64.hip).
+struct Fp64Bufs {
+    const double* Pold;
+    double* Pnew;
+    double* phi;    // PCG x
+    double* r;
+    double* p[2];   // ping-pong search direction
+    const uint32_t* cw;
+    const uint16_t* sc;
+    const double* TR;
+    const PhaseC* ptab;
+    const double* lut;
+    double* partials;   // RED_MAXQ * nblocks_max
+    unsigned* counters; // RED_COUNTERS
+    PcgState* st;
+    double* diag;       // [16] results of reductions (local sums etc.)
+};
+
+int blocks_for(const Grid& g);
+
+void launch_tdgl_f64(hipStream_t s, const Grid& g, const StepConsts& c, double Ez_ext,
+                     const Fp64Bufs& b);
+void launch_pcg_setup_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b);
+void launch_pcg_dir_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int it);
+void launch_pcg_update_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int it);
+void launch_diag_f64(hipStream_t s, const Grid& g, const double* P, const uint32_t* cw,
+                     double* partials, unsigned* counters, double* out8);
+
+}  // namespace hzo

@@ -1,0 +1,500 @@
+// kernels_fp64.hip -- plain FP64 form of the HZO step on CDNA4 (gfx950).
+//
+//   k_tdgl_f64        kernelCalc_antiferro + kernelUpdate_P (kernels_evol.cu:47-242)
+//                     with E = -grad(phi) (depol_varEPS.cu:223-276) evaluated
+//                     in-kernel from the previous solve's potential.
+//   k_pcg_setup_f64   divergence + RHS + r = b - A x + z = D^-1 r and the
+//                     inner products r.z, r.r, b.b (depol_varEPS.cu:279-329,
+//                     :400-460)
+//   k_pcg_dir_f64     p = z + beta p fused with q = A p and p.q (:479-485, :515-519)
+//   k_pcg_update_f64  x += alpha p, r -= alpha A p, z = D^-1 r, r.z, r.r (:492-507)
+//   k_diag_f64        the eight lab-frame diagnostics sums (evolve.cu:259-357)
+//
+// Every floating-point expression keeps the reference's operand order, and
+// the file is compiled with -ffp-contract=off, so each voxel value is the
+// double the reference expression denotes.  Inner products are
+// deterministic (fixed-order grid reduction, reduce.hip.h).
+//
+// Launch shape: 64x4 workgroups over x-y tiles (x fastest: each wave reads a
+// contiguous 512-byte x-row segment), each thread walking KZ z-planes.
+#include "../hzo_internal.h"
+#include "reduce.hip.h"
+
+namespace hzo {
+
+constexpr int BX = 64, BY = 4, KZ = 4;
+
+int blocks_for(const Grid& g) {
+    return ((g.nx + BX - 1) / BX) * ((g.ny + BY - 1) / BY) * ((g.nz + KZ - 1) / KZ);
+}
+
+static dim3 grid3(const Grid& g) {
+    return dim3((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, (g.nz + KZ - 1) / KZ);
+}
+
+__device__ __forceinline__ long long vid(int i, int j, int k, int nx, int ny) {
+    return (long long)i + (long long)nx * ((long long)j + (long long)ny * k);
+}
+
+// ---------------------------------------------------------------- TDGL
+__global__ __launch_bounds__(256) void k_tdgl_f64(Fp64Bufs b, StepConsts c, double Ez_ext, int nb) {
+    const int nx = c.nx, ny = c.ny, nz = c.nz;
+    const long long N = (long long)nx * ny * nz;
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k0 = blockIdx.z * KZ;
+    double loc[2] = {0.0, 0.0};  // sum Pmz_local, sum Psz_local (evolve.cu:269-276)
+    if (i < nx && j < ny) {
+        const int e = (i + 1 == nx) ? 0 : i + 1;
+        const int w = (i == 0) ? nx - 1 : i - 1;
+        const int n = (j + 1 == ny) ? 0 : j + 1;
+        const int s = (j == 0) ? ny - 1 : j - 1;
+        const int k1 = min(k0 + KZ, nz);
+        for (int k = k0; k < k1; ++k) {
+            const long long id = vid(i, j, k, nx, ny);
+            const uint32_t cw = b.cw[id];
+            if (cw & CW_VOID) {  // never written by the reference: zero buffer
+#pragma unroll
+                for (int q = 0; q < 6; ++q) b.Pnew[q * N + id] = 0.0;
+                continue;
+            }
+            const int u = (k == nz - 1) ? k : k + 1;
+            const int d = (k == 0) ? k : k - 1;
+            const long long ie = vid(e, j, k, nx, ny), iw = vid(w, j, k, nx, ny);
+            const long long in = vid(i, n, k, nx, ny), is = vid(i, s, k, nx, ny);
+            const long long iu = vid(i, j, u, nx, ny), idn = vid(i, j, d, nx, ny);
+
+            double sum[6], ctr[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                const double* P = b.Pold + q * N;
+                sum[q] = P[ie] + P[iw] + P[in] + P[is] + P[iu] + P[idn];
+                ctr[q] = P[id];
+            }
+            const PhaseC pc = b.ptab[((cw >> CW_PHASE_SHIFT) & 3u) * 2u + ((cw & CW_IFACE) ? 1u : 0u)];
+            const double* T = b.TR + (long long)(cw & CW_GID_MASK) * 9;
+            const double tr11 = T[0], tr12 = T[1], tr13 = T[2];
+            const double tr21 = T[3], tr22 = T[4], tr23 = T[5];
+            const double tr31 = T[6], tr32 = T[7], tr33 = T[8];
+            const double pMx0 = ctr[0], pMy0 = ctr[1], pMz0 = ctr[2];
+            const double pSx0 = ctr[3], pSy0 = ctr[4], pSz0 = ctr[5];
+
+            const double pMXl = tr11 * pMx0 + tr12 * pMy0 + tr13 * pMz0;
+            const double pMYl = tr21 * pMx0 + tr22 * pMy0 + tr23 * pMz0;
+            const double pMZl = tr31 * pMx0 + tr32 * pMy0 + tr33 * pMz0;
+            const double pSXl = tr11 * pSx0 + tr12 * pSy0 + tr13 * pSz0;
+            const double pSYl = tr21 * pSx0 + tr22 * pSy0 + tr23 * pSz0;
+            const double pSZl = tr31 * pSx0 + tr32 * pSy0 + tr33 * pSz0;
+            loc[0] += pMZl;
+            loc[1] += pSZl;
+
+            const double dfdPmxl = pc.a0_2 * pMXl;
+            const double dfdPmyl = pc.a0_2 * pMYl;
+            const double dfdPsxl = pc.a0_2 * pSXl;
+            const double dfdPsyl = pc.a0_2 * pSYl;
+            const double phiMz2 = pMZl * pMZl;
+            const double phiMz3 = phiMz2 * pMZl;
+            const double phiMz4 = phiMz3 * pMZl;
+            const double phiMz5 = phiMz4 * pMZl;
+            const double phiSz2 = pSZl * pSZl;
+            const double phiSz3 = phiSz2 * pSZl;
+            const double phiSz4 = phiSz3 * pSZl;
+            const double phiSz5 = phiSz4 * pSZl;
+            const double dfdPmzl = pc.ag * pMZl + pc.beta_4 * phiMz3 + pc.beta_12 * pMZl * phiSz2 +
+                                   pc.gamma_6 * (phiMz5 + 10.0 * phiMz3 * phiSz2 + 5.0 * pMZl * phiSz4);
+            const double dfdPszl = pc.ags * pSZl + pc.beta_4 * phiSz3 + pc.beta_12 * pSZl * phiMz2 +
+                                   pc.gamma_6 * (phiSz5 + 10.0 * phiMz2 * phiSz3 + 5.0 * pSZl * phiMz4);
+            double dfdPmx_b = tr11 * dfdPmxl + tr21 * dfdPmyl + tr31 * dfdPmzl;
+            double dfdPmy_b = tr12 * dfdPmxl + tr22 * dfdPmyl + tr32 * dfdPmzl;
+            double dfdPmz_b = tr13 * dfdPmxl + tr23 * dfdPmyl + tr33 * dfdPmzl;
+            const double dfdPsx_b = tr11 * dfdPsxl + tr21 * dfdPsyl + tr31 * dfdPszl;
+            const double dfdPsy_b = tr12 * dfdPsxl + tr22 * dfdPsyl + tr32 * dfdPszl;
+            const double dfdPsz_b = tr13 * dfdPsxl + tr23 * dfdPsyl + tr33 * dfdPszl;
+
+            double Ex = 0.0, Ey = 0.0, Ez = 0.0;
+            if (c.dep_flag) {  // E = -grad phi, depol_varEPS.cu:246-273
+                const double* ph = b.phi;
+                const double dx = -(ph[ie] - ph[iw]);
+                const double dy = -(ph[in] - ph[is]);
+                double up_v, dn_v;
+                if (c.zbc == 0) {
+                    up_v = (k == nz - 1) ? c.g_top : ph[vid(i, j, k + 1, nx, ny)];
+                    dn_v = (k == 0) ? c.g_bot : ph[vid(i, j, k - 1, nx, ny)];
+                } else {
+                    const int uu = (k == 0) ? 1 : ((k == nz - 1) ? nz - 2 : k + 1);
+                    const int dd = (k == 0) ? 1 : ((k == nz - 1) ? nz - 2 : k - 1);
+                    up_v = ph[vid(i, j, uu, nx, ny)];
+                    dn_v = ph[vid(i, j, dd, nx, ny)];
+                }
+                const double dz = -(up_v - dn_v);
+                if (c.two_h_pow2) {
+                    Ex = dx * c.inv_two_h;
+                    Ey = dy * c.inv_two_h;
+                    Ez = dz * c.inv_two_h;
+                } else {
+                    Ex = dx / c.two_h;
+                    Ey = dy / c.two_h;
+                    Ez = dz / c.two_h;
+                }
+            }
+            const double dfdPmz_app = -Ez_ext;
+            const double dfdPmx_e = c.dep_flag ? -1.0 * Ex : 0.0;
+            const double dfdPmy_e = c.dep_flag ? -1.0 * Ey : 0.0;
+            const double dfdPmz_e = c.dep_flag ? -1.0 * Ez : 0.0;
+            dfdPmx_b += dfdPmx_e;
+            dfdPmy_b += dfdPmy_e;
+            dfdPmz_b += dfdPmz_app + dfdPmz_e;
+
+            const double A = pc.A, om = pc.one_m6A, cL = pc.cL;
+            b.Pnew[0 * N + id] = pMx0 * om + A * sum[0] - cL * dfdPmx_b;
+            b.Pnew[1 * N + id] = pMy0 * om + A * sum[1] - cL * dfdPmy_b;
+            b.Pnew[2 * N + id] = pMz0 * om + A * sum[2] - cL * dfdPmz_b;
+            b.Pnew[3 * N + id] = pSx0 * om + A * sum[3] - cL * dfdPsx_b;
+            b.Pnew[4 * N + id] = pSy0 * om + A * sum[4] - cL * dfdPsy_b;
+            b.Pnew[5 * N + id] = pSz0 * om + A * sum[5] - cL * dfdPsz_b;
+        }
+    }
+    double tot[2];
+    if (grid_reduce_last<2>(loc, b.partials, nb, b.counters + 0, tot)) {
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            b.diag[2] = tot[0];
+            b.diag[3] = tot[1];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- operator
+struct Nbr {
+    long long c, e, w, n, s, u, d;
+};
+
+__device__ __forceinline__ Nbr op_nbrs(int i, int j, int k, const StepConsts& c) {
+    const int nx = c.nx, ny = c.ny, nz = c.nz;
+    const int e = (i + 1 == nx) ? 0 : i + 1, w = (i == 0) ? nx - 1 : i - 1;
+    const int n = (j + 1 == ny) ? 0 : j + 1, s = (j == 0) ? ny - 1 : j - 1;
+    int up, dn;
+    if (c.zbc == 0) {
+        up = k + 1;
+        dn = k - 1;
+    } else if (k == 0) {
+        up = 1;
+        dn = 1;
+    } else if (k == nz - 1) {
+        up = nz - 2;
+        dn = nz - 2;
+    } else {
+        up = k + 1;
+        dn = k - 1;
+    }
+    Nbr r;
+    r.c = vid(i, j, k, nx, ny);
+    r.e = vid(e, j, k, nx, ny);
+    r.w = vid(w, j, k, nx, ny);
+    r.n = vid(i, n, k, nx, ny);
+    r.s = vid(i, s, k, nx, ny);
+    r.u = vid(i, j, up, nx, ny);
+    r.d = vid(i, j, dn, nx, ny);
+    return r;
+}
+
+// -(sum of harmonic-mean face fluxes)/h^2, depol_varEPS.cu:76-123
+__device__ __forceinline__ double op_row(unsigned code, const double* H, const StepConsts& c,
+                                         double xc, double xe, double xw, double xn, double xs,
+                                         double xu, double xd) {
+    const double* Hr = H + (code & 3u) * 4u;
+    const double ex_p = Hr[(code >> 2) & 3u], ex_m = Hr[(code >> 4) & 3u];
+    const double ey_p = Hr[(code >> 6) & 3u], ey_m = Hr[(code >> 8) & 3u];
+    const double ez_p = Hr[(code >> 10) & 3u], ez_m = Hr[(code >> 12) & 3u];
+    const double num = ex_p * (xe - xc) - ex_m * (xc - xw) + ey_p * (xn - xc) - ey_m * (xc - xs) +
+                       ez_p * (xu - xc) - ez_m * (xc - xd);
+    const double Ax = c.h2_pow2 ? num * c.inv_h2 : num / c.h2;
+    return -Ax;
+}
+
+// identity rows: y = x - g on Dirichlet planes, y = x at the Neumann gauge node
+__device__ __forceinline__ double op_identity(double xc, int k, const StepConsts& c) {
+    if (c.zbc == 0) return xc - ((k == 0) ? c.g_bot : c.g_top);
+    return xc;
+}
+
+__device__ __forceinline__ bool keep_going(const PcgState& st, double rz, double rr) {
+    if (st.iters >= st.max_iter) return false;
+    if (st.rule == 0) return rz > st.tol_abs2 && (st.rz0 <= 0.0 || (rz / st.rz0) > st.tol_rel2);
+    return rr > st.tol_rel2 * st.bb && rr > st.tol_abs2;
+}
+
+__device__ __forceinline__ bool finite(double v) { return __builtin_isfinite(v); }
+
+__device__ __forceinline__ void load_harm(double* sH, const double* lut) {
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+    if (tid < 16) sH[tid] = lut[LUT_HARM + tid];
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------- PCG setup
+__global__ __launch_bounds__(256) void k_pcg_setup_f64(Fp64Bufs b, StepConsts c, int nb) {
+    __shared__ double sH[16];
+    load_harm(sH, b.lut);
+    const int nx = c.nx, ny = c.ny, nz = c.nz;
+    const long long N = (long long)nx * ny * nz;
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k0 = blockIdx.z * KZ;
+    double acc[3] = {0.0, 0.0, 0.0};  // r.z, r.r, b.b
+    if (i < nx && j < ny) {
+        const double* dinv = b.lut + LUT_DINV;
+        const double* Px = b.Pnew;
+        const double* Py = b.Pnew + N;
+        const double* Pz = b.Pnew + 2 * N;
+        const int k1 = min(k0 + KZ, nz);
+        for (int k = k0; k < k1; ++k) {
+            const Nbr nb7 = op_nbrs(i, j, k, c);
+            const unsigned code = b.sc[nb7.c];
+            const double* x = b.phi;
+            double bval, Ax;
+            if (code & SC_IDENTITY) {
+                bval = 0.0;  // rhs_zero_on_planes / gauge pin (depol_varEPS.cu:192-201, 404-414)
+                Ax = op_identity(x[nb7.c], k, c);
+            } else {
+                // divergence with clamped z (depol_varEPS.cu:293-325)
+                const int uz = (k == nz - 1) ? k : k + 1;
+                const int dz = (k == 0) ? k : k - 1;
+                const double v = (Px[nb7.e] - Px[nb7.w]) + (Py[nb7.n] - Py[nb7.s]) +
+                                 (Pz[vid(i, j, uz, nx, ny)] - Pz[vid(i, j, dz, nx, ny)]);
+                double divp = c.two_h_pow2 ? v * c.inv_two_h : v / c.two_h;
+                divp *= c.eps0;
+                bval = -1.0 * divp;
+                Ax = op_row(code, sH, c, x[nb7.c], x[nb7.e], x[nb7.w], x[nb7.n], x[nb7.s], x[nb7.u],
+                            x[nb7.d]);
+            }
+            const double r = -1.0 * Ax + bval;
+            const double z = r * dinv[code];
+            b.r[nb7.c] = r;
+            acc[0] += r * z;
+            acc[1] += r * r;
+            acc[2] += bval * bval;
+        }
+    }
+    double tot[3];
+    if (grid_reduce_last<3>(acc, b.partials, nb, b.counters + 1, tot)) {
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            PcgState& st = *b.st;
+            st.rz0 = tot[0];
+            st.rz_old = tot[0];
+            st.rr0 = tot[1];
+            st.rr = tot[1];
+            st.bb = tot[2];
+            st.iters = 0;
+            st.status = 0;
+            st.done = 0;
+            st.beta = 0.0;
+            if (!finite(tot[0]) || tot[0] < 0.0) {
+                st.status = HZO_PCG_BAD_INITIAL;
+                st.done = 1;
+            } else if (!keep_going(st, tot[0], tot[1])) {
+                st.done = 1;
+                if (st.iters >= st.max_iter) st.status = HZO_PCG_MAXITER;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- PCG direction + SpMV
+// p_new = z + beta*p_old (first iteration: p_new = z), q = A p_new, p_new.q.
+// z = D^-1 r is recomputed from r and the code wherever it is needed
+// (centre and the six neighbours), so z is never stored.
+__device__ __forceinline__ double pnew_at(const Fp64Bufs& b, const double* dinv, long long v,
+                                          const double* pold, double beta, bool first) {
+    const double z = b.r[v] * dinv[b.sc[v]];
+    return first ? z : z + beta * pold[v];
+}
+
+__global__ __launch_bounds__(256) void k_pcg_dir_f64(Fp64Bufs b, StepConsts c, int nb,
+                                                     const double* __restrict__ pold,
+                                                     double* __restrict__ pnew, int first) {
+    if (b.st->done) return;
+    __shared__ double sH[16];
+    load_harm(sH, b.lut);
+    const int nx = c.nx, ny = c.ny, nz = c.nz;
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k0 = blockIdx.z * KZ;
+    const double beta = b.st->beta;
+    const bool fst = first != 0;
+    double acc[1] = {0.0};
+    if (i < nx && j < ny) {
+        const double* dinv = b.lut + LUT_DINV;
+        const int k1 = min(k0 + KZ, nz);
+        for (int k = k0; k < k1; ++k) {
+            const Nbr v = op_nbrs(i, j, k, c);
+            const unsigned code = b.sc[v.c];
+            const double pc = pnew_at(b, dinv, v.c, pold, beta, fst);
+            double q;
+            if (code & SC_IDENTITY) {
+                q = op_identity(pc, k, c);
+            } else {
+                q = op_row(code, sH, c, pc, pnew_at(b, dinv, v.e, pold, beta, fst),
+                           pnew_at(b, dinv, v.w, pold, beta, fst), pnew_at(b, dinv, v.n, pold, beta, fst),
+                           pnew_at(b, dinv, v.s, pold, beta, fst), pnew_at(b, dinv, v.u, pold, beta, fst),
+                           pnew_at(b, dinv, v.d, pold, beta, fst));
+            }
+            pnew[v.c] = pc;
+            acc[0] += pc * q;
+        }
+    }
+    double tot[1];
+    if (grid_reduce_last<1>(acc, b.partials, nb, b.counters + 2, tot)) {
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            PcgState& st = *b.st;
+            st.iters += 1;
+            st.denom = tot[0];
+            if (!finite(tot[0]) || !(tot[0] > 0.0)) {  // depol_varEPS.cu:486-490
+                st.status = HZO_PCG_BREAKDOWN;
+                st.done = 1;
+            } else {
+                st.alpha = st.rz_old / tot[0];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- PCG update
+__global__ __launch_bounds__(256) void k_pcg_update_f64(Fp64Bufs b, StepConsts c, int nb,
+                                                        const double* __restrict__ p) {
+    if (b.st->done) return;
+    __shared__ double sH[16];
+    load_harm(sH, b.lut);
+    const int nx = c.nx, ny = c.ny, nz = c.nz;
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k0 = blockIdx.z * KZ;
+    const double alpha = b.st->alpha;
+    const double nalpha = -alpha;
+    double acc[2] = {0.0, 0.0};  // r.z, r.r
+    if (i < nx && j < ny) {
+        const double* dinv = b.lut + LUT_DINV;
+        const int k1 = min(k0 + KZ, nz);
+        for (int k = k0; k < k1; ++k) {
+            const Nbr v = op_nbrs(i, j, k, c);
+            const unsigned code = b.sc[v.c];
+            const double pc = p[v.c];
+            double q;
+            if (code & SC_IDENTITY)
+                q = op_identity(pc, k, c);
+            else
+                q = op_row(code, sH, c, pc, p[v.e], p[v.w], p[v.n], p[v.s], p[v.u], p[v.d]);
+            b.phi[v.c] = alpha * pc + b.phi[v.c];
+            const double r = nalpha * q + b.r[v.c];
+            b.r[v.c] = r;
+            const double z = r * dinv[code];
+            acc[0] += r * z;
+            acc[1] += r * r;
+        }
+    }
+    double tot[2];
+    if (grid_reduce_last<2>(acc, b.partials, nb, b.counters + 3, tot)) {
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            PcgState& st = *b.st;
+            const double rz_new = tot[0];
+            st.rr = tot[1];
+            if (!finite(rz_new) || rz_new < 0.0) {  // depol_varEPS.cu:508-513
+                st.status = HZO_PCG_BAD_RZ;
+                st.rz_old = rz_new;
+                st.done = 1;
+            } else {
+                st.beta = rz_new / st.rz_old;
+                st.rz_old = rz_new;
+                if (!keep_going(st, rz_new, tot[1])) {
+                    st.done = 1;
+                    if (st.iters >= st.max_iter) st.status = HZO_PCG_MAXITER;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- diagnostics
+// out8 = sum Pmz, sum Psz, then FE, DE, AFE: sum(mask*Pmz), sum(mask)
+// (evolve.cu:259-266, 280-357; build_mat_mask_and_p kernels_evol.cu:23-44).
+__global__ __launch_bounds__(256) void k_diag_f64(const double* __restrict__ P,
+                                                  const uint32_t* __restrict__ cw, Grid g,
+                                                  double* partials, unsigned* counter, double* out,
+                                                  int nb) {
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const long long N = g.N;
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k0 = blockIdx.z * KZ;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    double m[4] = {0.0, 0.0, 0.0, 0.0};
+    if (i < nx && j < ny) {
+        const int k1 = min(k0 + KZ, nz);
+        for (int k = k0; k < k1; ++k) {
+            const long long id = vid(i, j, k, nx, ny);
+            const double pmz = P[2 * N + id];
+            a[0] += pmz;
+            a[1] += P[5 * N + id];
+            const uint32_t w = cw[id];
+            const int phase = (w & CW_VOID) ? -1 : (int)((w >> CW_PHASE_SHIFT) & 3u);
+            const double mfe = (phase == 1) ? 1.0 : 0.0;
+            const double mde = (phase == 2) ? 1.0 : 0.0;
+            const double mafe = (phase == 0) ? 1.0 : 0.0;
+            a[2] += mfe * pmz;
+            a[3] += mde * pmz;
+            m[0] += mafe * pmz;
+            m[1] += mfe;
+            m[2] += mde;
+            m[3] += mafe;
+        }
+    }
+    double v1[4] = {a[0], a[1], a[2], a[3]};
+    double t1[4];
+    if (grid_reduce_last<4>(v1, partials, nb, counter, t1)) {
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            out[0] = t1[0];
+            out[1] = t1[1];
+            out[2] = t1[2];
+            out[4] = t1[3];
+        }
+    }
+    double t2[4];
+    if (grid_reduce_last<4>(m, partials + 4 * (long long)nb, nb, counter + 1, t2)) {
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            out[6] = t2[0];
+            out[3] = t2[1];
+            out[5] = t2[2];
+            out[7] = t2[3];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+void launch_tdgl_f64(hipStream_t s, const Grid& g, const StepConsts& c, double Ez_ext,
+                     const Fp64Bufs& b) {
+    hipLaunchKernelGGL(k_tdgl_f64, grid3(g), dim3(BX, BY, 1), 0, s, b, c, Ez_ext, blocks_for(g));
+}
+
+void launch_pcg_setup_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b) {
+    hipLaunchKernelGGL(k_pcg_setup_f64, grid3(g), dim3(BX, BY, 1), 0, s, b, c, blocks_for(g));
+}
+
+void launch_pcg_dir_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int it) {
+    const double* pold = b.p[it & 1];
+    double* pnew = b.p[(it + 1) & 1];
+    hipLaunchKernelGGL(k_pcg_dir_f64, grid3(g), dim3(BX, BY, 1), 0, s, b, c, blocks_for(g), pold,
+                       pnew, it == 1 ? 1 : 0);
+}
+
+void launch_pcg_update_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b,
+                           int it) {
+    hipLaunchKernelGGL(k_pcg_update_f64, grid3(g), dim3(BX, BY, 1), 0, s, b, c, blocks_for(g),
+                       (const double*)b.p[(it + 1) & 1]);
+}
+
+void launch_diag_f64(hipStream_t s, const Grid& g, const double* P, const uint32_t* cw,
+                     double* partials, unsigned* counters, double* out8) {
+    hipLaunchKernelGGL(k_diag_f64, grid3(g), dim3(BX, BY, 1), 0, s, P, cw, g, partials, counters,
+                       out8, blocks_for(g));
+}
+
+}  // namespace hzo

@@ -1,0 +1,245 @@
+"""Host-side mirror of the reference's per-simulation setup and step loop.
+
+``Film`` owns one device context (one film) behind the C ABI of libhzo.so.
+``make_film_inputs`` builds the seeded inputs of a film the way the reference
+does in SetupAndRunSimulation (HZO_Code_github/src/main.cu:261-359):
+non-dimensionalisation, grain structure, phase assignment (Init_Conf,
+init_conf.cu:108-285), interface flag (main.cu:13-52) and initial
+polarisation (init_conf.cu:14-66).  All compute runs in libhzo.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+
+# Reference base coefficients (SI).  FE: the single-film case (alpha, beta,
+# gamma of the 64x64x16 film); AFE and DE: HZO_Code_github/src/inputs/
+# SingleL_input.txt:4.  L, kappa, eps_r: HZO_Code_github/inputs/InputParams:14-36,
+# with kappa_FE = 1e-10 (the gradient coefficient of the single-film case).
+BASE_RAW = {
+    L.HZO_AFE: dict(alpha=5.022532e8, beta=-9.065448e10, gamma=2.189509e12, g=5.45348e8, a0=8.0e8,
+                    L=0.2, kappa=1.0e-9, eps_r=40.0),
+    L.HZO_FE: dict(alpha=-1.0e9, beta=1.0e10, gamma=1.0e11, g=0.0, a0=1.0e9,
+                   L=0.833, kappa=1.0e-10, eps_r=30.0),
+    L.HZO_DE: dict(alpha=9.5e9, beta=0.0, gamma=0.0, g=0.0, a0=2.372e9,
+                   L=1.0, kappa=2.5e-9, eps_r=15.0),
+}
+
+
+def splitmix64(seed: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.uint64)
+    L.call("hzo_splitmix64", C.c_uint64(seed), C.c_int64(n), L.ptr(out))
+    return out
+
+
+def uniform01(seed: int, n: int) -> np.ndarray:
+    return (splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+
+
+def nondim(raw: dict, ref_type: int = L.HZO_FE):
+    """make_nd_wrt_reference for the three phases; returns (rows, scales)."""
+    arr = (L.PhaseRaw * 3)()
+    for q in range(3):
+        r = raw[q]
+        for k in ("alpha", "beta", "gamma", "g", "a0", "L", "kappa", "eps_r"):
+            setattr(arr[q], k, float(r[k]))
+        arr[q].transition_type = int(r.get("transition_type", 0))
+    out = (L.PhaseND * 3)()
+    sc = np.zeros(4)
+    L.call("hzo_nondim", arr, ref_type, out, L.ptr(sc))
+    rows = [{k: getattr(out[q], k) for k in ("alpha", "beta", "gamma", "g", "a0", "kappa", "L", "eps_r")}
+            for q in range(3)]
+    return rows, {"P0": sc[0], "E0": sc[1], "eps0_nd": sc[2], "alpha_ref": sc[3]}
+
+
+def voronoi(nx, ny, nz, ng, seed, layer_counts=None):
+    layer_counts = list(layer_counts or [nz])
+    lc = np.array(layer_counts, np.int32)
+    omega = np.zeros(nx * ny * nz, np.int32)
+    cent = np.zeros(3 * ng)
+    eul = np.zeros(3 * ng)
+    L.call("hzo_voronoi", nx, ny, nz, len(layer_counts), L.ptr(lc), ng, C.c_uint64(seed),
+           L.ptr(omega), L.ptr(cent), L.ptr(eul))
+    return omega, cent.reshape(ng, 3), eul.reshape(ng, 3)
+
+
+def euler_to_TR(eul: np.ndarray) -> np.ndarray:
+    eul = np.ascontiguousarray(eul, np.float64).reshape(-1)
+    ng = eul.size // 3
+    TR = np.zeros(9 * ng)
+    L.call("hzo_euler_to_TR", ng, L.ptr(eul), L.ptr(TR))
+    return TR
+
+
+def assign_phases(nx, ny, nz, omega, centz, fracs_afe_fe_de, layer_counts=None, seed=123456):
+    layer_counts = list(layer_counts or [nz])
+    lc = np.array(layer_counts, np.int32)
+    fr = np.ascontiguousarray(np.broadcast_to(np.asarray(fracs_afe_fe_de, np.float64),
+                                              (len(layer_counts), 3)))
+    ng = int(np.asarray(centz).size)
+    gm = np.zeros(ng, np.int32)
+    L.call("hzo_assign_phases", nx, ny, nz, L.ptr(np.ascontiguousarray(omega, np.int32)), ng,
+           L.ptr(np.ascontiguousarray(centz, np.float64)), len(layer_counts), L.ptr(lc), L.ptr(fr),
+           C.c_uint(seed), L.ptr(gm))
+    return gm
+
+
+def interface_flag(nx, ny, nz, omega, grain_phase):
+    out = np.zeros(nx * ny * nz, np.int32)
+    L.call("hzo_interface_flag", nx, ny, nz, L.ptr(omega), int(grain_phase.size), L.ptr(grain_phase),
+           L.ptr(out))
+    return out
+
+
+def init_polarization(nx, ny, nz, omega, grain_phase, amp_fe, seed):
+    P = np.zeros(6 * nx * ny * nz)
+    L.call("hzo_init_polarization", nx, ny, nz, L.ptr(omega), int(grain_phase.size), L.ptr(grain_phase),
+           float(amp_fe), C.c_uint64(seed), L.ptr(P))
+    return P
+
+
+@dataclass
+class FilmInputs:
+    nx: int
+    ny: int
+    nz: int
+    del_h: float
+    del_t: float
+    omega: np.ndarray
+    grain_phase: np.ndarray
+    TR: np.ndarray
+    phases: list
+    scales: dict
+    P0: np.ndarray
+    iface: np.ndarray
+    centroids: np.ndarray = None
+    euler: np.ndarray = None
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def N(self):
+        return self.nx * self.ny * self.nz
+
+    @property
+    def ng(self):
+        return int(self.grain_phase.size)
+
+    def field_nd(self, E_MV_per_cm):
+        """Applied field in MV/cm -> non-dimensional (maxEz = Eap/E0_ref, main.cu:347)."""
+        return np.asarray(E_MV_per_cm, np.float64) * 1e8 / self.scales["E0"]
+
+
+def make_film_inputs(nx, ny, nz, ng, fracs=(0.0, 1.0, 0.0), seed=1, del_h=0.5, del_t=0.01,
+                     jitter=0.0, amp_fe_SI=0.01, raw=None, ref_type=L.HZO_FE, layer_counts=None,
+                     phase_seed=123456) -> FilmInputs:
+    """Seeded film: Voronoi grains, phase assignment, Landau jitter, initial P.
+
+    jitter: alpha, beta, gamma of each phase are multiplied by U(1-jitter, 1+jitter)
+    (one draw per phase and coefficient).  amp_fe_SI: FE initial |P| bound in C/m^2.
+    """
+    raw = {q: dict(v) for q, v in (raw or BASE_RAW).items()}
+    if jitter:
+        u = uniform01(seed ^ 0x5EED5EED, 9).reshape(3, 3)
+        for q in range(3):
+            for t, k in enumerate(("alpha", "beta", "gamma")):
+                raw[q][k] = raw[q][k] * (1.0 - jitter + 2.0 * jitter * u[q, t])
+    rows, scales = nondim(raw, ref_type)
+    omega, cent, eul = voronoi(nx, ny, nz, ng, seed, layer_counts)
+    gphase = assign_phases(nx, ny, nz, omega, cent[:, 2].copy(), fracs, layer_counts, phase_seed)
+    TR = euler_to_TR(eul)
+    iface = interface_flag(nx, ny, nz, omega, gphase)
+    P0 = init_polarization(nx, ny, nz, omega, gphase, amp_fe_SI / scales["P0"], seed + 7)
+    return FilmInputs(nx, ny, nz, del_h, del_t, omega, gphase, TR, rows, scales, P0, iface, cent, eul,
+                      meta=dict(seed=seed, fracs=tuple(fracs), jitter=jitter, raw=raw))
+
+
+def triangle_wave(nsteps_period: int, amp: float, nsteps: int, start: int = 0) -> np.ndarray:
+    """E(n) = amp*tri(n/T), tri(s) = 4s | 2-4s | 4s-4 on [0,1/4] | (1/4,3/4] | (3/4,1]."""
+    s = ((np.arange(start, start + nsteps) % nsteps_period) / nsteps_period).astype(np.float64)
+    tri = np.where(s <= 0.25, 4 * s, np.where(s <= 0.75, 2 - 4 * s, 4 * s - 4))
+    return amp * tri
+
+
+class Film:
+    """One film on one GPU: the device context of libhzo.so."""
+
+    def __init__(self, inp: FilmInputs, precision=L.PREC_FP64, pcg_rule=L.PCG_RULE_REFERENCE,
+                 tol_rel=1e-2, tol_abs=1e-3, max_iter=4000, dep_flag=1, zbc=L.ZBC_DIRICHLET,
+                 refine_every=20, phi=None, set_state=True):
+        self.inp = inp
+        cfg = L.default_config(nx=inp.nx, ny=inp.ny, nz=inp.nz, del_h=inp.del_h, del_t=inp.del_t,
+                               dep_flag=dep_flag, zbc=zbc, eps0_nd=inp.scales["eps0_nd"],
+                               precision=precision, pcg_rule=pcg_rule, tol_abs=tol_abs, tol_rel=tol_rel,
+                               max_iter=max_iter, refine_every=refine_every)
+        self.cfg = cfg
+        h = C.c_void_p()
+        L.call("hzo_create", C.byref(cfg), C.byref(h))
+        self.h = h
+        L.call("hzo_set_microstructure", h, L.ptr(inp.omega), inp.ng, L.ptr(inp.grain_phase), L.ptr(inp.TR),
+               L.ptr(inp.iface))
+        L.call("hzo_set_phase_table", h, L.phases_nd(inp.phases))
+        if set_state:
+            self.set_state(inp.P0, phi)
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.lib().hzo_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, P, phi=None):
+        P = np.ascontiguousarray(P, np.float64)
+        L.call("hzo_set_state", self.h, L.ptr(P), L.ptr(None if phi is None else np.ascontiguousarray(phi)))
+
+    def get_state(self):
+        N = self.inp.N
+        P = np.empty(6 * N)
+        phi = np.empty(N)
+        E = np.empty(3 * N)
+        L.call("hzo_get_state", self.h, L.ptr(P), L.ptr(phi), L.ptr(E))
+        return P, phi, E
+
+    def tdgl_step(self, Ez_ext: float):
+        L.call("hzo_tdgl_step", self.h, float(Ez_ext))
+
+    def depol_solve(self) -> L.PcgStats:
+        st = L.PcgStats()
+        L.call("hzo_depol_solve", self.h, C.byref(st))
+        return st
+
+    def step(self, Ez_ext: float) -> L.PcgStats:
+        st = L.PcgStats()
+        L.call("hzo_step", self.h, float(Ez_ext), C.byref(st))
+        return st
+
+    def run(self, Ez) -> np.ndarray:
+        Ez = np.ascontiguousarray(Ez, np.float64)
+        its = np.zeros(Ez.size, np.int32)
+        L.call("hzo_run", self.h, L.ptr(Ez), int(Ez.size), L.ptr(its))
+        return its
+
+    def diagnostics(self) -> np.ndarray:
+        out = np.zeros(10)
+        L.call("hzo_diagnostics", self.h, L.ptr(out))
+        return out
+
+    def synchronize(self):
+        L.call("hzo_synchronize", self.h)
+
+    def set_timing(self, on: bool):
+        L.call("hzo_set_timing", self.h, 1 if on else 0)
+
+    def kernel_times(self):
+        cnt = np.zeros(L.KCLASSES, np.int32)
+        ms = np.zeros(L.KCLASSES)
+        L.call("hzo_kernel_times", self.h, L.ptr(cnt), L.ptr(ms))
+        return {n: (int(cnt[i]), float(ms[i])) for i, n in enumerate(L.KCLASS_NAMES)}

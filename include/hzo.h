@@ -1,0 +1,218 @@
+/*
+ * hzo.h -- C ABI of the MI355X-native HZO phase-field time step.
+ *
+ * The library replaces one simulator step of jhb6617-arch/cmsl (HZO
+ * ferroelectric thin-film phase-field code): the explicit-Euler TDGL update
+ * and the Jacobi-preconditioned CG Poisson solve for the depolarisation
+ * potential.  In the reference the step is three call sites inside the
+ * Evolve() loop (HZO_Code_github/src/evolve.cu:220-237):
+ *   kernelCalc_antiferro<<<>>>(37 pointer/scalar args)   kernels_evol.h:9-18
+ *   kernelUpdate_P<<<>>>(..) x2                          kernels_evol.h:20-21
+ *   void Depol_field(void)                               header.h:32
+ * operating on ~40 process-global device pointers (cuda_deps.h:13-41).
+ * Here one opaque context owns all device state of one film; the caller hands
+ * over host arrays once and then steps.  Every call returns HZO_OK (0) or a
+ * negative status; hzo_last_error() gives the text (the reference prints and
+ * calls exit(1) instead, check.h:9-83).  PCG breakdowns are not errors: they
+ * are reported in hzo_pcg_stats.status, as the reference prints and continues
+ * (depol_varEPS.cu:466-513).
+ *
+ * Layout conventions (identical to the reference, kernels_evol.cu:65):
+ *   cell id = i + nx*(j + ny*k), x fastest ("C order");
+ *   P       = 6 planes of N doubles: Pmx, Pmy, Pmz, Psx, Psy, Psz (non-dimensional);
+ *   E       = 3 planes of N doubles: Ex, Ey, Ez;
+ *   phases  = 0 AFE, 1 FE, 2 DE (types.h:12).
+ *
+ * Threading: a context is used by one host thread at a time.  Different
+ * contexts are independent.
+ */
+#ifndef HZO_H
+#define HZO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HZO_OK 0
+#define HZO_ERR_ARG (-1)      /* invalid argument / inconsistent sizes */
+#define HZO_ERR_HIP (-2)      /* HIP runtime error (no device, launch failure) */
+#define HZO_ERR_IO (-3)       /* file missing, short read, parse error */
+#define HZO_ERR_STATE (-4)    /* call order violated (e.g. step before set_state) */
+#define HZO_ERR_NOMEM (-5)    /* device or host allocation failed */
+#define HZO_ERR_DATA (-6)     /* input data inconsistent (layer sums, grain ids) */
+
+enum { HZO_AFE = 0, HZO_FE = 1, HZO_DE = 2 };
+enum { HZO_ZBC_DIRICHLET = 0, HZO_ZBC_NEUMANN = 1 };
+enum { HZO_PREC_FP64 = 0, HZO_PREC_FP32_REFINED = 1 };
+enum { HZO_PCG_RULE_REFERENCE = 0, HZO_PCG_RULE_RELRES = 1 };
+
+/* PCG status codes (hzo_pcg_stats.status). */
+#define HZO_PCG_CONVERGED 0
+#define HZO_PCG_MAXITER 1
+#define HZO_PCG_BAD_INITIAL (-1)   /* non-finite or negative r.z at start */
+#define HZO_PCG_BREAKDOWN (-2)     /* p.Ap non-finite or <= 0 */
+#define HZO_PCG_BAD_RZ (-3)        /* non-finite or negative r.z during iteration */
+
+/* Non-dimensional coefficients of one phase (reference MaterialNorm,
+ * types.h:22-27, as produced by make_nd_wrt_reference, main.cu:232-256). */
+typedef struct hzo_phase_nd {
+    double alpha, beta, gamma, g, a0, kappa, L, eps_r;
+} hzo_phase_nd;
+
+/* Raw (SI) coefficients of one phase (reference MaterialParamsRaw, types.h:15-20). */
+typedef struct hzo_phase_raw {
+    double alpha, beta, gamma, g, a0;
+    int transition_type;
+    double L, kappa, eps_r;
+} hzo_phase_raw;
+
+/* Step configuration.  hzo_config_default() fills the reference constants
+ * (global.cu:30-31, 140-144). */
+typedef struct hzo_config {
+    int nx, ny, nz;
+    double del_h, del_t;
+    int dep_flag;        /* 1: depolarisation solve each step (InputParams dep_flag) */
+    int zbc;             /* HZO_ZBC_* (InputParams z_bc_type diri|neu) */
+    double eps0_nd;      /* 4/(8.854e-12*|alpha_ref|), main.cu:344 */
+    double g_bot, g_top; /* electrode potentials, global.cu:140-141 */
+    int precision;       /* HZO_PREC_* */
+    int pcg_rule;        /* HZO_PCG_RULE_* */
+    double tol_abs, tol_rel;
+    int max_iter;
+    int refine_every;    /* FP32-refined: FP64 residual recompute period (iterations) */
+} hzo_config;
+
+typedef struct hzo_pcg_stats {
+    int iters;           /* PCG iterations of the last solve */
+    int status;          /* HZO_PCG_* */
+    int refinements;     /* FP64 residual replacements (FP32-refined path) */
+    int reserved;
+    double rz0, rz;      /* preconditioned residual r.z at start / end */
+    double rr0, rr;      /* ||r||^2 at start / end */
+    double bb;           /* ||b||^2 */
+} hzo_pcg_stats;
+
+typedef struct hzo_ctx hzo_ctx;
+
+/* ---------------------------------------------------------------- library */
+const char* hzo_last_error(void);
+int hzo_version(void);
+/* Number of visible HIP devices (0 without a GPU); returns HZO_OK. */
+int hzo_device_count(int* n);
+void hzo_config_default(hzo_config* cfg);
+
+/* ---------------------------------------------------------------- context
+ * Replaces AllocateData (allocateVars.cu:5-113) and the device globals. */
+int hzo_create(const hzo_config* cfg, hzo_ctx** out);
+void hzo_destroy(hzo_ctx* ctx);
+
+/* Grain structure: omega[N] grain id per voxel (C order, <0 = void),
+ * grain_phase[ng] (0 AFE,1 FE,2 DE), TR[9*ng] row-major grain rotations
+ * (grain_io.cu:102-126).  iface[N] may be NULL: it is then built by the
+ * reference rule (main.cu:13-42). */
+int hzo_set_microstructure(hzo_ctx* ctx, const int32_t* omega, int ng,
+                           const int32_t* grain_phase, const double* TR,
+                           const int32_t* iface);
+/* Per-phase non-dimensional coefficients, indexed by phase (replaces the
+ * per-voxel maps of InitConfKernel, init_conf.cu:68-106, whose spread is 0). */
+int hzo_set_phase_table(hzo_ctx* ctx, const hzo_phase_nd ph[3]);
+/* Polarisation P[6N] and potential phi[N] (phi NULL: init_phi_linear,
+ * depol_varEPS.cu:203-221).  E is always -grad phi of the current phi. */
+int hzo_set_state(hzo_ctx* ctx, const double* P, const double* phi);
+/* Any pointer may be NULL. */
+int hzo_get_state(hzo_ctx* ctx, double* P, double* phi, double* E);
+
+/* ---------------------------------------------------------------- step */
+/* kernelCalc_antiferro + kernelUpdate_P (kernels_evol.cu:47-242). */
+int hzo_tdgl_step(hzo_ctx* ctx, double Ez_ext);
+/* Depol_field (depol_varEPS.cu:332-595); stats may be NULL. */
+int hzo_depol_solve(hzo_ctx* ctx, hzo_pcg_stats* stats);
+/* One step in reference order (evolve.cu:220-237); stats may be NULL. */
+int hzo_step(hzo_ctx* ctx, double Ez_ext, hzo_pcg_stats* stats);
+/* nsteps steps with applied field Ez[s]; its[s] (may be NULL) receives the PCG
+ * iteration count of step s.  Issues no host synchronisation per step beyond
+ * what the PCG stopping test needs. */
+int hzo_run(hzo_ctx* ctx, const double* Ez, int nsteps, int* its);
+/* Reference diagnostics (evolve.cu:259-357): out[10] = sum Pmz, sum Psz,
+ * sum Pmz_local, sum Psz_local (grain frame, from the last TDGL call's input),
+ * then FE, DE, AFE: sum(mask*Pmz), sum(mask). */
+int hzo_diagnostics(hzo_ctx* ctx, double out[10]);
+int hzo_synchronize(hzo_ctx* ctx);
+
+/* ---------------------------------------------------------------- timing
+ * Kernel-class timing with HIP events on the context's stream.  While
+ * enabled, the first HZO_TIMING_MAX launches of every class are bracketed by
+ * events; hzo_kernel_times returns, per class, launches timed and their mean
+ * duration in ms.  Classes: 0 tdgl, 1 pcg_setup, 2 pcg_dir_spmv,
+ * 3 pcg_update, 4 refine (FP32-refined residual replacement). */
+#define HZO_TIMING_MAX 8192
+#define HZO_KCLASSES 5
+int hzo_set_timing(hzo_ctx* ctx, int enable);
+int hzo_kernel_times(hzo_ctx* ctx, int counts[HZO_KCLASSES], double mean_ms[HZO_KCLASSES]);
+
+/* ---------------------------------------------------------------- host side
+ * These run on the CPU only (no device needed). */
+
+/* Reference deck "inputs/InputParams" (get_InputParams.cu:8-151). */
+typedef struct hzo_deck {
+    int device_flag, nx, ny, nz, dep_flag, num_steps, out_flag;
+    int dimx, dimy, dimz, do_pre_relax, n_layers;
+    int layer_counts[16];
+    double del_h, del_t, delEap, Eap, PsNoise, G0, omg;
+    double kappa_fe, kappa_afe, kappa_de, L_fe, L_afe, L_de;
+    double eps_fe, eps_afe, eps_de;
+    char crystal_type[20];
+    char z_bc_type[20];
+} hzo_deck;
+int hzo_read_deck(const char* path, hzo_deck* out);
+
+/* Grain count file "gnumb_<tag>.txt" (grain_io.cu:10-25). */
+int hzo_read_grain_count(const char* path, int* ng);
+/* "gdata_<tag>.bin": int32 omega[N] then f64 eta[N], z-fastest; reordered to
+ * C order (grain_io.cu:26-69).  eta may be NULL. */
+int hzo_read_gdata(const char* path, int nx, int ny, int nz, int32_t* omega, double* eta);
+/* "gchar_<tag>.txt": gid cx cy cz phi1 Phi phi2 (radians) per line
+ * (grain_io.cu:85-99).  centroids[3*ng], euler[3*ng]; missing ids keep -1/0. */
+int hzo_read_gchar(const char* path, int ng, double* centroids, double* euler);
+/* Rotation matrices from Bunge Euler angles (grain_io.cu:102-126). */
+int hzo_euler_to_TR(int ng, const double* euler, double* TR);
+/* Writers of the generator formats (voronoi.c:162-223). */
+int hzo_write_gdata(const char* path, int nx, int ny, int nz, const int32_t* omega_c);
+int hzo_write_gchar(const char* path, int ng, const double* centroids, const double* euler);
+
+/* Seeded layered Voronoi structure (voronoi.c:55-153 with the rand() stream
+ * replaced by SplitMix64 and orientations uniform on SO(3)).
+ * layer_counts[n_layers] must sum to nz.  Outputs: omega[N] (C order),
+ * centroids[3*ng], euler[3*ng]. */
+int hzo_voronoi(int nx, int ny, int nz, int n_layers, const int* layer_counts,
+                int ng, uint64_t seed, int32_t* omega, double* centroids, double* euler);
+
+/* Non-dimensionalisation (main.cu:200-256, 268-274, 343-349).
+ * ref_type selects the reference phase.  out_nd[3] indexed by phase;
+ * scales[4] = {P0_ref, E0_ref (V/m), eps0_nd, alpha_ref}. */
+int hzo_nondim(const hzo_phase_raw raw[3], int ref_type, hzo_phase_nd out_nd[3], double scales[4]);
+
+/* Layer-wise greedy phase assignment (init_conf.cu:108-285), glibc srand/rand
+ * seeded with `seed` (the reference uses 123456).  fracs[3*n_layers] =
+ * (afe, fe, de) per layer. */
+int hzo_assign_phases(int nx, int ny, int nz, const int32_t* omega, int ng,
+                      const double* centz, int n_layers, const int* layer_counts,
+                      const double* fracs, unsigned seed, int32_t* grain_phase);
+/* Interface flag (main.cu:13-42). */
+int hzo_interface_flag(int nx, int ny, int nz, const int32_t* omega, int ng,
+                       const int32_t* grain_phase, int32_t* iface);
+/* Initial polarisation by phase (init_conf.cu:14-66) with the per-voxel
+ * uniform draws u = SplitMix64(seed) stream (the reference's cuRAND stream is
+ * not reproducible): FE Pm = amp_fe*(2u-1) on all three components, AFE
+ * Ps = (1,1,1), DE and void 0.  P[6N]. */
+int hzo_init_polarization(int nx, int ny, int nz, const int32_t* omega, int ng,
+                          const int32_t* grain_phase, double amp_fe, uint64_t seed, double* P);
+/* n outputs of SplitMix64 started at `seed` (Steele, Lea, Flood 2014). */
+int hzo_splitmix64(uint64_t seed, int64_t n, uint64_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
