@@ -48,6 +48,11 @@ class PcgStats(C.Structure):
                 ("bb", C.c_double)]
 
 
+class SweepParams(C.Structure):
+    _fields_ = [("maxEz", C.c_double), ("delEz", C.c_double), ("num_steps", C.c_int), ("max_levels", C.c_int),
+                ("Ez_scale", C.c_double), ("P0_scale", C.c_double)]
+
+
 class Deck(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("device_flag", "nx", "ny", "nz", "dep_flag", "num_steps", "out_flag",
                                       "dimx", "dimy", "dimz", "do_pre_relax", "n_layers")] + \
@@ -81,6 +86,9 @@ SIGNATURES = {
     "hzo_run": (_I, [_P, _P, _I, _P]),
     "hzo_diagnostics": (_I, [_P, _P]),
     "hzo_synchronize": (_I, [_P]),
+    "hzo_grid_dims": (_I, [_P, _P]),
+    "hzo_sweep_schedule": (_I, [_P, _I, _P, _P, _P]),
+    "hzo_sweep_run": (_I, [_P, _P, C.c_char_p, _I, _P, _P]),
     "hzo_set_timing": (_I, [_P, _I]),
     "hzo_kernel_times": (_I, [_P, _P, _P]),
     "hzo_read_deck": (_I, [C.c_char_p, _P]),

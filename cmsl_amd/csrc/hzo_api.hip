@@ -28,7 +28,16 @@ struct hzo_ctx {
     hipStream_t stream = nullptr;
     int nb = 0;
 
+    bool fp32 = false;  // HZO_PREC_FP32_REFINED
     double* P[2] = {nullptr, nullptr};
+    float* P32[2] = {nullptr, nullptr};
+    float* xlo = nullptr;
+    float* r32 = nullptr;
+    float* p32[2] = {nullptr, nullptr};
+    double* b64 = nullptr;
+    float* TR32 = nullptr;
+    hzo::PhaseC32* ptab32 = nullptr;
+    float* lut32 = nullptr;
     int cur = 0;
     double* phi = nullptr;
     double* r = nullptr;
@@ -77,6 +86,7 @@ hzo::Fp64Bufs bufs(hzo_ctx* x) {
     hzo::Fp64Bufs b;
     b.Pold = x->P[x->cur];
     b.Pnew = x->P[x->cur ^ 1];
+    b.Pcur = x->P[x->cur];
     b.phi = x->phi;
     b.r = x->r;
     b.p[0] = x->p[0];
@@ -86,6 +96,30 @@ hzo::Fp64Bufs bufs(hzo_ctx* x) {
     b.TR = x->TR;
     b.ptab = x->ptab;
     b.lut = x->lut;
+    b.partials = x->partials;
+    b.counters = x->counters;
+    b.st = x->st;
+    b.diag = x->diag;
+    return b;
+}
+
+hzo::Fp32Bufs bufs32(hzo_ctx* x) {
+    hzo::Fp32Bufs b;
+    b.Pold = x->P32[x->cur];
+    b.Pnew = x->P32[x->cur ^ 1];
+    b.Pcur = x->P32[x->cur];
+    b.xhi = x->phi;
+    b.xlo = x->xlo;
+    b.r = x->r32;
+    b.p[0] = x->p32[0];
+    b.p[1] = x->p32[1];
+    b.b = x->b64;
+    b.cw = x->cw;
+    b.sc = x->sc;
+    b.TR32 = x->TR32;
+    b.ptab32 = x->ptab32;
+    b.lut = x->lut;
+    b.lut32 = x->lut32;
     b.partials = x->partials;
     b.counters = x->counters;
     b.st = x->st;
@@ -230,6 +264,23 @@ int upload_phase(hzo_ctx* x) {
     lut[hzo::LUT_DINV + hzo::SC_IDENTITY] = 1.0;
     HZO_HIP(hipMemcpy(x->ptab, pt, sizeof(pt), hipMemcpyHostToDevice));
     HZO_HIP(hipMemcpy(x->lut, lut.data(), sizeof(double) * hzo::LUT_DOUBLES, hipMemcpyHostToDevice));
+    if (x->fp32) {
+        hzo::PhaseC32 p32[hzo::PTAB_ENTRIES];
+        for (int q = 0; q < hzo::PTAB_ENTRIES; ++q) {
+            p32[q].A = (float)pt[q].A;
+            p32[q].one_m6A = (float)pt[q].one_m6A;
+            p32[q].a0_2 = (float)pt[q].a0_2;
+            p32[q].beta_4 = (float)pt[q].beta_4;
+            p32[q].beta_12 = (float)pt[q].beta_12;
+            p32[q].gamma_6 = (float)pt[q].gamma_6;
+            p32[q].ag = (float)pt[q].ag;
+            p32[q].ags = (float)pt[q].ags;
+            p32[q].cL = (float)pt[q].cL;
+        }
+        std::vector<float> l32(lut.begin(), lut.end());
+        HZO_HIP(hipMemcpy(x->ptab32, p32, sizeof(p32), hipMemcpyHostToDevice));
+        HZO_HIP(hipMemcpy(x->lut32, l32.data(), sizeof(float) * hzo::LUT_DOUBLES, hipMemcpyHostToDevice));
+    }
     return HZO_OK;
 }
 
@@ -240,14 +291,21 @@ int init_pcg_state(hzo_ctx* x) {
     s.rule = x->cfg.pcg_rule;
     s.tol_abs2 = x->cfg.tol_abs * x->cfg.tol_abs;
     s.tol_rel2 = x->cfg.tol_rel * x->cfg.tol_rel;
+    s.refine_every = x->cfg.refine_every;
     s.done = 1;
     HZO_HIP(hipMemcpy(x->st, &s, sizeof(s), hipMemcpyHostToDevice));
     return HZO_OK;
 }
 
 int tdgl(hzo_ctx* x, double Ez_ext) {
-    const hzo::Fp64Bufs b = bufs(x);
-    int rc = timed(x, 0, [&] { hzo::launch_tdgl_f64(x->stream, x->g, x->c, Ez_ext, b); });
+    int rc;
+    if (x->fp32) {
+        const hzo::Fp32Bufs b = bufs32(x);
+        rc = timed(x, 0, [&] { hzo::launch_tdgl_f32(x->stream, x->g, x->c, Ez_ext, b); });
+    } else {
+        const hzo::Fp64Bufs b = bufs(x);
+        rc = timed(x, 0, [&] { hzo::launch_tdgl_f64(x->stream, x->g, x->c, Ez_ext, b); });
+    }
     if (rc) return rc;
     x->cur ^= 1;  // kernelUpdate_P becomes a buffer swap
     return HZO_OK;
@@ -255,26 +313,40 @@ int tdgl(hzo_ctx* x, double Ez_ext) {
 
 int depol(hzo_ctx* x, hzo_pcg_stats* stats) {
     const hzo::Fp64Bufs b = bufs(x);
-    int rc = timed(x, 1, [&] { hzo::launch_pcg_setup_f64(x->stream, x->g, x->c, b); });
+    const hzo::Fp32Bufs b32 = bufs32(x);
+    int rc = x->fp32 ? timed(x, 1, [&] { hzo::launch_pcg_setup_f32(x->stream, x->g, x->c, b32); })
+                     : timed(x, 1, [&] { hzo::launch_pcg_setup_f64(x->stream, x->g, x->c, b); });
     if (rc) return rc;
-    const int maxit = x->cfg.max_iter;
-    int it = 0;
+    // FP64: one (dir, update) pair per iteration.  FP32-refined: the device
+    // may turn a pair into a residual replacement, so up to
+    // max_iter + max_iter/refine_every + 1 pairs can be needed.
+    const int re = x->cfg.refine_every > 0 ? x->cfg.refine_every : x->cfg.max_iter + 1;
+    const long long maxpairs = x->fp32 ? (long long)x->cfg.max_iter + x->cfg.max_iter / re + 2
+                                       : (long long)x->cfg.max_iter;
+    long long it = 0;
     int batch = x->last_iters > 0 ? x->last_iters : 1;
     while (true) {
-        for (int m = 0; m < batch && it < maxit; ++m) {
+        for (int m = 0; m < batch && it < maxpairs; ++m) {
             ++it;
-            rc = timed(x, 2, [&] { hzo::launch_pcg_dir_f64(x->stream, x->g, x->c, b, it); });
-            if (rc) return rc;
-            rc = timed(x, 3, [&] { hzo::launch_pcg_update_f64(x->stream, x->g, x->c, b, it); });
+            if (x->fp32) {
+                rc = timed(x, 2, [&] { hzo::launch_pcg_dir_f32(x->stream, x->g, x->c, b32); });
+                if (rc) return rc;
+                rc = timed(x, 3, [&] { hzo::launch_pcg_update_f32(x->stream, x->g, x->c, b32); });
+            } else {
+                const int iti = (int)it;
+                rc = timed(x, 2, [&] { hzo::launch_pcg_dir_f64(x->stream, x->g, x->c, b, iti); });
+                if (rc) return rc;
+                rc = timed(x, 3, [&] { hzo::launch_pcg_update_f64(x->stream, x->g, x->c, b, iti); });
+            }
             if (rc) return rc;
         }
         HZO_HIP(hipMemcpyAsync(x->st_host, x->st, sizeof(hzo::PcgState), hipMemcpyDeviceToHost, x->stream));
         HZO_HIP(hipStreamSynchronize(x->stream));
-        if (x->st_host->done || it >= maxit) break;
+        if (x->st_host->done || it >= maxpairs) break;
         batch = x->last_iters / 4 > 2 ? x->last_iters / 4 : 2;
     }
     const hzo::PcgState& s = *x->st_host;
-    x->last_iters = s.iters > 0 ? s.iters : 1;
+    x->last_iters = s.iters + s.refinements > 0 ? s.iters + s.refinements : 1;
     if (stats) {
         stats->iters = s.iters;
         stats->status = s.status;
@@ -305,6 +377,16 @@ void free_all(hzo_ctx* x) {
     (void)hipFree(x->counters);
     (void)hipFree(x->st);
     (void)hipFree(x->diag);
+    (void)hipFree(x->P32[0]);
+    (void)hipFree(x->P32[1]);
+    (void)hipFree(x->xlo);
+    (void)hipFree(x->r32);
+    (void)hipFree(x->p32[0]);
+    (void)hipFree(x->p32[1]);
+    (void)hipFree(x->b64);
+    (void)hipFree(x->TR32);
+    (void)hipFree(x->ptab32);
+    (void)hipFree(x->lut32);
     if (x->st_host) (void)hipHostFree(x->st_host);
     if (x->diag_host) (void)hipHostFree(x->diag_host);
     for (int c = 0; c < HZO_KCLASSES; ++c)
@@ -356,8 +438,9 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
         return set_error(HZO_ERR_ARG, "hzo_create: Neumann mirror needs nz >= 3");
     if (cfg->zbc == HZO_ZBC_DIRICHLET && cfg->nz < 2)
         return set_error(HZO_ERR_ARG, "hzo_create: Dirichlet electrodes need nz >= 2");
-    if (cfg->precision != HZO_PREC_FP64)
-        return set_error(HZO_ERR_ARG, "hzo_create: precision %d not available in this build", cfg->precision);
+    if (cfg->precision != HZO_PREC_FP64 && cfg->precision != HZO_PREC_FP32_REFINED)
+        return set_error(HZO_ERR_ARG, "hzo_create: precision %d", cfg->precision);
+    if (cfg->refine_every < 0) return set_error(HZO_ERR_ARG, "hzo_create: refine_every < 0");
     if (cfg->pcg_rule != 0 && cfg->pcg_rule != 1) return set_error(HZO_ERR_ARG, "hzo_create: pcg_rule %d", cfg->pcg_rule);
     if (cfg->max_iter < 0) return set_error(HZO_ERR_ARG, "hzo_create: max_iter < 0");
     const long long N = (long long)cfg->nx * cfg->ny * cfg->nz;
@@ -405,12 +488,25 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
     } while (0)
     if (hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(set_error(HZO_ERR_HIP, "hzo_create: stream creation failed"));
-    HZO_ALLOC(x->P[0], sizeof(double) * 6 * N);
-    HZO_ALLOC(x->P[1], sizeof(double) * 6 * N);
+    x->fp32 = cfg->precision == HZO_PREC_FP32_REFINED;
     HZO_ALLOC(x->phi, sizeof(double) * N);
-    HZO_ALLOC(x->r, sizeof(double) * N);
-    HZO_ALLOC(x->p[0], sizeof(double) * N);
-    HZO_ALLOC(x->p[1], sizeof(double) * N);
+    if (x->fp32) {
+        HZO_ALLOC(x->P32[0], sizeof(float) * 6 * N);
+        HZO_ALLOC(x->P32[1], sizeof(float) * 6 * N);
+        HZO_ALLOC(x->xlo, sizeof(float) * N);
+        HZO_ALLOC(x->r32, sizeof(float) * N);
+        HZO_ALLOC(x->p32[0], sizeof(float) * N);
+        HZO_ALLOC(x->p32[1], sizeof(float) * N);
+        HZO_ALLOC(x->b64, sizeof(double) * N);
+        HZO_ALLOC(x->ptab32, sizeof(hzo::PhaseC32) * hzo::PTAB_ENTRIES);
+        HZO_ALLOC(x->lut32, sizeof(float) * hzo::LUT_DOUBLES);
+    } else {
+        HZO_ALLOC(x->P[0], sizeof(double) * 6 * N);
+        HZO_ALLOC(x->P[1], sizeof(double) * 6 * N);
+        HZO_ALLOC(x->r, sizeof(double) * N);
+        HZO_ALLOC(x->p[0], sizeof(double) * N);
+        HZO_ALLOC(x->p[1], sizeof(double) * N);
+    }
     HZO_ALLOC(x->cw, sizeof(uint32_t) * N);
     HZO_ALLOC(x->sc, sizeof(uint16_t) * N);
     HZO_ALLOC(x->ptab, sizeof(hzo::PhaseC) * hzo::PTAB_ENTRIES);
@@ -424,9 +520,7 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
         hipHostMalloc((void**)&x->diag_host, sizeof(double) * 16, hipHostMallocDefault) != hipSuccess)
         return fail(set_error(HZO_ERR_NOMEM, "hzo_create: pinned host allocation failed"));
     if (hipMemset(x->counters, 0, sizeof(unsigned) * hzo::RED_COUNTERS) != hipSuccess ||
-        hipMemset(x->diag, 0, sizeof(double) * 16) != hipSuccess ||
-        hipMemset(x->p[0], 0, sizeof(double) * N) != hipSuccess ||
-        hipMemset(x->p[1], 0, sizeof(double) * N) != hipSuccess)
+        hipMemset(x->diag, 0, sizeof(double) * 16) != hipSuccess)
         return fail(set_error(HZO_ERR_HIP, "hzo_create: memset failed"));
     int rc = init_pcg_state(x);
     if (rc) return fail(rc);
@@ -464,6 +558,14 @@ int hzo_set_microstructure(hzo_ctx* x, const int32_t* omega, int ng, const int32
     if (hipMalloc((void**)&x->TR, sizeof(double) * 9 * (size_t)ng) != hipSuccess)
         return set_error(HZO_ERR_NOMEM, "hzo_set_microstructure: TR allocation failed");
     HZO_HIP(hipMemcpy(x->TR, TR, sizeof(double) * 9 * (size_t)ng, hipMemcpyHostToDevice));
+    if (x->fp32) {
+        (void)hipFree(x->TR32);
+        x->TR32 = nullptr;
+        if (hipMalloc((void**)&x->TR32, sizeof(float) * 9 * (size_t)ng) != hipSuccess)
+            return set_error(HZO_ERR_NOMEM, "hzo_set_microstructure: TR allocation failed");
+        std::vector<float> t32(TR, TR + 9 * (size_t)ng);
+        HZO_HIP(hipMemcpy(x->TR32, t32.data(), sizeof(float) * 9 * (size_t)ng, hipMemcpyHostToDevice));
+    }
     int rc = upload_micro(x);
     if (rc) return rc;
     x->have_micro = true;
@@ -485,7 +587,13 @@ int hzo_set_state(hzo_ctx* x, const double* P, const double* phi) {
     HZO_HIP(hipSetDevice(x->device));
     const long long N = x->g.N;
     x->cur = 0;
-    HZO_HIP(hipMemcpy(x->P[0], P, sizeof(double) * 6 * N, hipMemcpyHostToDevice));
+    if (x->fp32) {
+        std::vector<float> p32(P, P + 6 * N);
+        HZO_HIP(hipMemcpy(x->P32[0], p32.data(), sizeof(float) * 6 * N, hipMemcpyHostToDevice));
+        HZO_HIP(hipMemset(x->xlo, 0, sizeof(float) * N));
+    } else {
+        HZO_HIP(hipMemcpy(x->P[0], P, sizeof(double) * 6 * N, hipMemcpyHostToDevice));
+    }
     if (phi) {
         HZO_HIP(hipMemcpy(x->phi, phi, sizeof(double) * N, hipMemcpyHostToDevice));
     } else {  // init_phi_linear (depol_varEPS.cu:203-221)
@@ -511,11 +619,24 @@ int hzo_get_state(hzo_ctx* x, double* P, double* phi, double* E) {
     HZO_HIP(hipSetDevice(x->device));
     HZO_HIP(hipStreamSynchronize(x->stream));
     const long long N = x->g.N;
-    if (P) HZO_HIP(hipMemcpy(P, x->P[x->cur], sizeof(double) * 6 * N, hipMemcpyDeviceToHost));
+    if (P) {
+        if (x->fp32) {
+            std::vector<float> p32(6 * N);
+            HZO_HIP(hipMemcpy(p32.data(), x->P32[x->cur], sizeof(float) * 6 * N, hipMemcpyDeviceToHost));
+            for (long long q = 0; q < 6 * N; ++q) P[q] = p32[q];
+        } else {
+            HZO_HIP(hipMemcpy(P, x->P[x->cur], sizeof(double) * 6 * N, hipMemcpyDeviceToHost));
+        }
+    }
     std::vector<double> ph;
     if (phi || E) {
         ph.resize(N);
         HZO_HIP(hipMemcpy(ph.data(), x->phi, sizeof(double) * N, hipMemcpyDeviceToHost));
+        if (x->fp32) {  // phi = x_hi + x_lo
+            std::vector<float> lo(N);
+            HZO_HIP(hipMemcpy(lo.data(), x->xlo, sizeof(float) * N, hipMemcpyDeviceToHost));
+            for (long long q = 0; q < N; ++q) ph[q] += (double)lo[q];
+        }
         if (phi) memcpy(phi, ph.data(), sizeof(double) * N);
     }
     if (E) {  // E = -grad phi, depol_varEPS.cu:223-276 (host evaluation of the same expressions)
@@ -594,7 +715,10 @@ int hzo_diagnostics(hzo_ctx* x, double out[10]) {
     if (rc) return rc;
     if (!out) return set_error(HZO_ERR_ARG, "hzo_diagnostics: null output");
     HZO_HIP(hipSetDevice(x->device));
-    hzo::launch_diag_f64(x->stream, x->g, x->P[x->cur], x->cw, x->partials, x->counters + 4, x->diag + 8);
+    if (x->fp32)
+        hzo::launch_diag_f32(x->stream, x->g, x->P32[x->cur], x->cw, x->partials, x->counters + 4, x->diag + 8);
+    else
+        hzo::launch_diag_f64(x->stream, x->g, x->P[x->cur], x->cw, x->partials, x->counters + 4, x->diag + 8);
     HZO_HIP(hipGetLastError());
     HZO_HIP(hipMemcpyAsync(x->diag_host, x->diag, sizeof(double) * 16, hipMemcpyDeviceToHost, x->stream));
     HZO_HIP(hipStreamSynchronize(x->stream));
@@ -611,6 +735,14 @@ int hzo_synchronize(hzo_ctx* x) {
     if (!x) return set_error(HZO_ERR_ARG, "null context");
     HZO_HIP(hipSetDevice(x->device));
     HZO_HIP(hipStreamSynchronize(x->stream));
+    return HZO_OK;
+}
+
+int hzo_grid_dims(hzo_ctx* x, int dims[3]) {
+    if (!x || !dims) return set_error(HZO_ERR_ARG, "hzo_grid_dims: null argument");
+    dims[0] = x->g.nx;
+    dims[1] = x->g.ny;
+    dims[2] = x->g.nz;
     return HZO_OK;
 }
 

@@ -61,6 +61,11 @@ struct PcgState {
     int iters, status, done, refinements;
     int max_iter, rule, pad0, pad1;
     double tol_abs2, tol_rel2;
+    // FP32-refined path only: mode 0 = CG iteration, 1 = FP64 residual
+    // replacement pending (the next launch pair folds x_lo into x_hi and
+    // recomputes r = b - A x_hi); pbuf selects the current search direction
+    // buffer; refine_every = period of the replacement.
+    int mode, pbuf, refine_every, pad2;
 };
 
 // Reduction scratch: one completion counter per kernel class and the
@@ -89,8 +94,9 @@ struct StepConsts {
 
 // Launch helpers (kernels_fp64.hip).
 struct Fp64Bufs {
-    const double* Pold;
-    double* Pnew;
+    const double* Pold;  // TDGL input
+    double* Pnew;        // TDGL output
+    const double* Pcur;  // current polarisation (PCG right-hand side)
     double* phi;    // PCG x
     double* r;
     double* p[2];   // ping-pong search direction
@@ -105,6 +111,32 @@ struct Fp64Bufs {
     double* diag;       // [16] results of reductions (local sums etc.)
 };
 
+// FP32-refined form: FP32 polarisation and CG vectors, FP64 potential x_hi
+// with an FP32 correction x_lo (phi = x_hi + x_lo), FP64 right-hand side b.
+struct PhaseC32 {
+    float A, one_m6A, a0_2, beta_4, beta_12, gamma_6, ag, ags, cL;
+};
+struct Fp32Bufs {
+    const float* Pold;
+    float* Pnew;
+    const float* Pcur;
+    double* xhi;
+    float* xlo;
+    float* r;
+    float* p[2];
+    double* b;
+    const uint32_t* cw;
+    const uint16_t* sc;
+    const float* TR32;
+    const PhaseC32* ptab32;
+    const double* lut;    // FP64 harm + dinv (residual replacement)
+    const float* lut32;   // FP32 harm + dinv (CG iteration)
+    double* partials;
+    unsigned* counters;
+    PcgState* st;
+    double* diag;
+};
+
 int blocks_for(const Grid& g);
 
 void launch_tdgl_f64(hipStream_t s, const Grid& g, const StepConsts& c, double Ez_ext,
@@ -112,6 +144,12 @@ void launch_tdgl_f64(hipStream_t s, const Grid& g, const StepConsts& c, double E
 void launch_pcg_setup_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b);
 void launch_pcg_dir_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int it);
 void launch_pcg_update_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int it);
+void launch_tdgl_f32(hipStream_t s, const Grid& g, const StepConsts& c, double Ez_ext, const Fp32Bufs& b);
+void launch_pcg_setup_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b);
+void launch_pcg_dir_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b);
+void launch_pcg_update_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b);
+void launch_diag_f32(hipStream_t s, const Grid& g, const float* P, const uint32_t* cw, double* partials,
+                     unsigned* counters, double* out8);
 void launch_diag_f64(hipStream_t s, const Grid& g, const double* P, const uint32_t* cw,
                      double* partials, unsigned* counters, double* out8);
 

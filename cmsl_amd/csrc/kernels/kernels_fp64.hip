@@ -19,22 +19,11 @@
 // contiguous 512-byte x-row segment), each thread walking KZ z-planes.
 #include "../hzo_internal.h"
 #include "reduce.hip.h"
+#include "common.hip.h"
 
 namespace hzo {
 
-constexpr int BX = 64, BY = 4, KZ = 4;
-
-int blocks_for(const Grid& g) {
-    return ((g.nx + BX - 1) / BX) * ((g.ny + BY - 1) / BY) * ((g.nz + KZ - 1) / KZ);
-}
-
-static dim3 grid3(const Grid& g) {
-    return dim3((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, (g.nz + KZ - 1) / KZ);
-}
-
-__device__ __forceinline__ long long vid(int i, int j, int k, int nx, int ny) {
-    return (long long)i + (long long)nx * ((long long)j + (long long)ny * k);
-}
+int blocks_for(const Grid& g) { return blocks_for_impl(g); }
 
 // ---------------------------------------------------------------- TDGL
 __global__ __launch_bounds__(256) void k_tdgl_f64(Fp64Bufs b, StepConsts c, double Ez_ext, int nb) {
@@ -163,40 +152,6 @@ __global__ __launch_bounds__(256) void k_tdgl_f64(Fp64Bufs b, StepConsts c, doub
     }
 }
 
-// ---------------------------------------------------------------- operator
-struct Nbr {
-    long long c, e, w, n, s, u, d;
-};
-
-__device__ __forceinline__ Nbr op_nbrs(int i, int j, int k, const StepConsts& c) {
-    const int nx = c.nx, ny = c.ny, nz = c.nz;
-    const int e = (i + 1 == nx) ? 0 : i + 1, w = (i == 0) ? nx - 1 : i - 1;
-    const int n = (j + 1 == ny) ? 0 : j + 1, s = (j == 0) ? ny - 1 : j - 1;
-    int up, dn;
-    if (c.zbc == 0) {
-        up = k + 1;
-        dn = k - 1;
-    } else if (k == 0) {
-        up = 1;
-        dn = 1;
-    } else if (k == nz - 1) {
-        up = nz - 2;
-        dn = nz - 2;
-    } else {
-        up = k + 1;
-        dn = k - 1;
-    }
-    Nbr r;
-    r.c = vid(i, j, k, nx, ny);
-    r.e = vid(e, j, k, nx, ny);
-    r.w = vid(w, j, k, nx, ny);
-    r.n = vid(i, n, k, nx, ny);
-    r.s = vid(i, s, k, nx, ny);
-    r.u = vid(i, j, up, nx, ny);
-    r.d = vid(i, j, dn, nx, ny);
-    return r;
-}
-
 // -(sum of harmonic-mean face fluxes)/h^2, depol_varEPS.cu:76-123
 __device__ __forceinline__ double op_row(unsigned code, const double* H, const StepConsts& c,
                                          double xc, double xe, double xw, double xn, double xs,
@@ -211,30 +166,10 @@ __device__ __forceinline__ double op_row(unsigned code, const double* H, const S
     return -Ax;
 }
 
-// identity rows: y = x - g on Dirichlet planes, y = x at the Neumann gauge node
-__device__ __forceinline__ double op_identity(double xc, int k, const StepConsts& c) {
-    if (c.zbc == 0) return xc - ((k == 0) ? c.g_bot : c.g_top);
-    return xc;
-}
-
-__device__ __forceinline__ bool keep_going(const PcgState& st, double rz, double rr) {
-    if (st.iters >= st.max_iter) return false;
-    if (st.rule == 0) return rz > st.tol_abs2 && (st.rz0 <= 0.0 || (rz / st.rz0) > st.tol_rel2);
-    return rr > st.tol_rel2 * st.bb && rr > st.tol_abs2;
-}
-
-__device__ __forceinline__ bool finite(double v) { return __builtin_isfinite(v); }
-
-__device__ __forceinline__ void load_harm(double* sH, const double* lut) {
-    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
-    if (tid < 16) sH[tid] = lut[LUT_HARM + tid];
-    __syncthreads();
-}
-
 // ---------------------------------------------------------------- PCG setup
 __global__ __launch_bounds__(256) void k_pcg_setup_f64(Fp64Bufs b, StepConsts c, int nb) {
     __shared__ double sH[16];
-    load_harm(sH, b.lut);
+    load_lut(sH, b.lut);
     const int nx = c.nx, ny = c.ny, nz = c.nz;
     const long long N = (long long)nx * ny * nz;
     const int i = blockIdx.x * BX + threadIdx.x;
@@ -243,9 +178,9 @@ __global__ __launch_bounds__(256) void k_pcg_setup_f64(Fp64Bufs b, StepConsts c,
     double acc[3] = {0.0, 0.0, 0.0};  // r.z, r.r, b.b
     if (i < nx && j < ny) {
         const double* dinv = b.lut + LUT_DINV;
-        const double* Px = b.Pnew;
-        const double* Py = b.Pnew + N;
-        const double* Pz = b.Pnew + 2 * N;
+        const double* Px = b.Pcur;
+        const double* Py = b.Pcur + N;
+        const double* Pz = b.Pcur + 2 * N;
         const int k1 = min(k0 + KZ, nz);
         for (int k = k0; k < k1; ++k) {
             const Nbr nb7 = op_nbrs(i, j, k, c);
@@ -314,7 +249,7 @@ __global__ __launch_bounds__(256) void k_pcg_dir_f64(Fp64Bufs b, StepConsts c, i
                                                      double* __restrict__ pnew, int first) {
     if (b.st->done) return;
     __shared__ double sH[16];
-    load_harm(sH, b.lut);
+    load_lut(sH, b.lut);
     const int nx = c.nx, ny = c.ny, nz = c.nz;
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
@@ -363,7 +298,7 @@ __global__ __launch_bounds__(256) void k_pcg_update_f64(Fp64Bufs b, StepConsts c
                                                         const double* __restrict__ p) {
     if (b.st->done) return;
     __shared__ double sH[16];
-    load_harm(sH, b.lut);
+    load_lut(sH, b.lut);
     const int nx = c.nx, ny = c.ny, nz = c.nz;
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;

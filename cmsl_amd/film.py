@@ -164,6 +164,20 @@ def triangle_wave(nsteps_period: int, amp: float, nsteps: int, start: int = 0) -
     return amp * tri
 
 
+def sweep_params(scales, Eap_SI, delEap_SI, num_steps, max_levels=0) -> L.SweepParams:
+    E0 = scales["E0"]
+    return L.SweepParams(Eap_SI / E0, delEap_SI / E0, int(num_steps), int(max_levels), E0 / 1.0e8,
+                         scales["P0"] * 100.0)
+
+
+def sweep_schedule(p: L.SweepParams, max_rows=100000):
+    E = np.zeros(max_rows)
+    cnt = np.zeros(max_rows, np.int32)
+    n = C.c_int(0)
+    L.call("hzo_sweep_schedule", C.byref(p), max_rows, L.ptr(E), L.ptr(cnt), C.byref(n))
+    return E[: n.value], cnt[: n.value]
+
+
 class Film:
     """One film on one GPU: the device context of libhzo.so."""
 
@@ -231,6 +245,15 @@ class Film:
         out = np.zeros(10)
         L.call("hzo_diagnostics", self.h, L.ptr(out))
         return out
+
+    def sweep(self, Eap_SI, delEap_SI, num_steps, path=None, max_rows=4096, max_levels=0):
+        """Reference staircase P-E sweep (evolve.cu:213-410); returns the rows (n x 12)."""
+        p = sweep_params(self.inp.scales, Eap_SI, delEap_SI, num_steps, max_levels)
+        rows = np.zeros((max_rows, 12))
+        n = C.c_int(0)
+        L.call("hzo_sweep_run", self.h, C.byref(p), None if path is None else str(path).encode(), max_rows,
+               L.ptr(rows), C.byref(n))
+        return rows[: min(n.value, max_rows)]
 
     def synchronize(self):
         L.call("hzo_synchronize", self.h)

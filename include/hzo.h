@@ -140,6 +140,29 @@ int hzo_run(hzo_ctx* ctx, const double* Ez, int nsteps, int* its);
  * then FE, DE, AFE: sum(mask*Pmz), sum(mask). */
 int hzo_diagnostics(hzo_ctx* ctx, double out[10]);
 int hzo_synchronize(hzo_ctx* ctx);
+/* Grid dimensions of a context: dims[3] = nx, ny, nz. */
+int hzo_grid_dims(hzo_ctx* ctx, int dims[3]);
+
+/* ---------------------------------------------------------------- sweep driver
+ * The staircase P-E loop of Evolve() (evolve.cu:213-410): the field starts at
+ * 0, changes by delEz every num_steps steps and turns at +-maxEz; the run ends
+ * after one full loop (peloop == 2).  One 12-column row per level from the
+ * first turn on, in the sweep%03d.txt format (evolve.cu:391-400): E [MV/cm],
+ * <Pmz>, <Psz>, <Pmz_local>, <Psz_local>, <Pmz>_FE, f_FE, <Pmz>_AFE, f_AFE,
+ * <Pmz>_DE, f_DE (P in uC/cm^2), step count. */
+typedef struct hzo_sweep_params {
+    double maxEz, delEz;   /* non-dimensional: Eap/E0_ref, delEap/E0_ref (main.cu:347-348) */
+    int num_steps;         /* steps per field level (time_to_change) */
+    int max_levels;        /* safety cap on levels, 0 = none */
+    double Ez_scale;       /* E0_ref/1e8: non-dimensional -> MV/cm (evolve.cu:117) */
+    double P0_scale;       /* P0_ref*100: non-dimensional -> uC/cm^2 (evolve.cu:118) */
+} hzo_sweep_params;
+/* Host-only: field (MV/cm) and step count of every row the sweep writes. */
+int hzo_sweep_schedule(const hzo_sweep_params* p, int max_rows, double* E_MVcm, int* counts, int* nrows);
+/* Runs the sweep on a context.  path (may be NULL) is opened in append mode;
+ * rows_out (may be NULL) receives up to max_rows rows of 12 doubles. */
+int hzo_sweep_run(hzo_ctx* ctx, const hzo_sweep_params* p, const char* path, int max_rows,
+                  double* rows_out, int* nrows);
 
 /* ---------------------------------------------------------------- timing
  * Kernel-class timing with HIP events on the context's stream.  While
