@@ -39,7 +39,7 @@ class Config(C.Structure):
                 ("eps0_nd", C.c_double), ("g_bot", C.c_double), ("g_top", C.c_double),
                 ("precision", C.c_int), ("pcg_rule", C.c_int),
                 ("tol_abs", C.c_double), ("tol_rel", C.c_double),
-                ("max_iter", C.c_int), ("refine_every", C.c_int)]
+                ("max_iter", C.c_int), ("refine_every", C.c_int), ("tile_kz", C.c_int)]
 
 
 class PcgStats(C.Structure):
@@ -87,6 +87,7 @@ SIGNATURES = {
     "hzo_diagnostics": (_I, [_P, _P]),
     "hzo_synchronize": (_I, [_P]),
     "hzo_grid_dims": (_I, [_P, _P]),
+    "hzo_tile_depth": (_I, [_P]),
     "hzo_sweep_schedule": (_I, [_P, _I, _P, _P, _P]),
     "hzo_sweep_run": (_I, [_P, _P, C.c_char_p, _I, _P, _P]),
     "hzo_set_timing": (_I, [_P, _I]),

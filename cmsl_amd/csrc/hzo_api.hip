@@ -47,12 +47,12 @@ struct hzo_ctx {
     double* TR = nullptr;
     hzo::PhaseC* ptab = nullptr;
     double* lut = nullptr;
-    double* partials = nullptr;
-    unsigned* counters = nullptr;
-    hzo::PcgState* st = nullptr;
-    double* diag = nullptr;
+    double* part = nullptr;       // 2 parity slots x RED_MAXQ x nb
+    double* tdgl_part = nullptr;  // TDGL_Q x nb
+    double* diag_part = nullptr;  // DIAG_Q x nb
+    hzo::PcgState* st = nullptr;  // [2]
     hzo::PcgState* st_host = nullptr;  // pinned
-    double* diag_host = nullptr;       // pinned [16]
+    std::vector<double> diag_host;
 
     int ng = 0;
     std::vector<int32_t> omega, grain_phase, iface;
@@ -96,10 +96,10 @@ hzo::Fp64Bufs bufs(hzo_ctx* x) {
     b.TR = x->TR;
     b.ptab = x->ptab;
     b.lut = x->lut;
-    b.partials = x->partials;
-    b.counters = x->counters;
+    b.part[0] = x->part;
+    b.part[1] = x->part + (size_t)hzo::RED_MAXQ * x->nb;
     b.st = x->st;
-    b.diag = x->diag;
+    b.tdgl_part = x->tdgl_part;
     return b;
 }
 
@@ -120,10 +120,10 @@ hzo::Fp32Bufs bufs32(hzo_ctx* x) {
     b.ptab32 = x->ptab32;
     b.lut = x->lut;
     b.lut32 = x->lut32;
-    b.partials = x->partials;
-    b.counters = x->counters;
+    b.part[0] = x->part;
+    b.part[1] = x->part + (size_t)hzo::RED_MAXQ * x->nb;
     b.st = x->st;
-    b.diag = x->diag;
+    b.tdgl_part = x->tdgl_part;
     return b;
 }
 
@@ -284,7 +284,7 @@ int upload_phase(hzo_ctx* x) {
     return HZO_OK;
 }
 
-int init_pcg_state(hzo_ctx* x) {
+hzo::PcgState pcg_init(const hzo_ctx* x) {
     hzo::PcgState s;
     memset(&s, 0, sizeof(s));
     s.max_iter = x->cfg.max_iter;
@@ -292,9 +292,8 @@ int init_pcg_state(hzo_ctx* x) {
     s.tol_abs2 = x->cfg.tol_abs * x->cfg.tol_abs;
     s.tol_rel2 = x->cfg.tol_rel * x->cfg.tol_rel;
     s.refine_every = x->cfg.refine_every;
-    s.done = 1;
-    HZO_HIP(hipMemcpy(x->st, &s, sizeof(s), hipMemcpyHostToDevice));
-    return HZO_OK;
+    s.kind = hzo::K_SETUP;
+    return s;
 }
 
 int tdgl(hzo_ctx* x, double Ez_ext) {
@@ -314,36 +313,38 @@ int tdgl(hzo_ctx* x, double Ez_ext) {
 int depol(hzo_ctx* x, hzo_pcg_stats* stats) {
     const hzo::Fp64Bufs b = bufs(x);
     const hzo::Fp32Bufs b32 = bufs32(x);
-    int rc = x->fp32 ? timed(x, 1, [&] { hzo::launch_pcg_setup_f32(x->stream, x->g, x->c, b32); })
-                     : timed(x, 1, [&] { hzo::launch_pcg_setup_f64(x->stream, x->g, x->c, b); });
+    const hzo::PcgState init = pcg_init(x);
+    int rc = x->fp32 ? timed(x, 1, [&] { hzo::launch_pcg_setup_f32(x->stream, x->g, x->c, b32, init); })
+                     : timed(x, 1, [&] { hzo::launch_pcg_setup_f64(x->stream, x->g, x->c, b, init); });
     if (rc) return rc;
-    // FP64: one (dir, update) pair per iteration.  FP32-refined: the device
-    // may turn a pair into a residual replacement, so up to
-    // max_iter + max_iter/refine_every + 1 pairs can be needed.
+    // One (dir, update) launch pair per CG iteration; the pair that follows
+    // the last iteration finishes its inner products and raises `done`.  The
+    // FP32-refined form also spends pairs on residual replacements, so up to
+    // max_iter + max_iter/refine_every + 2 pairs can be needed.
     const int re = x->cfg.refine_every > 0 ? x->cfg.refine_every : x->cfg.max_iter + 1;
-    const long long maxpairs = x->fp32 ? (long long)x->cfg.max_iter + x->cfg.max_iter / re + 2
-                                       : (long long)x->cfg.max_iter;
-    long long it = 0;
-    int batch = x->last_iters > 0 ? x->last_iters : 1;
+    const long long maxpairs = (long long)x->cfg.max_iter + (x->fp32 ? x->cfg.max_iter / re + 2 : 0) + 2;
+    long long pairs = 0;
+    int t = 0;  // launch index since the setup
+    int batch = x->last_iters + 1;
     while (true) {
-        for (int m = 0; m < batch && it < maxpairs; ++m) {
-            ++it;
+        for (int m = 0; m < batch && pairs < maxpairs; ++m, ++pairs) {
+            const int ta = ++t, tb = ++t;
             if (x->fp32) {
-                rc = timed(x, 2, [&] { hzo::launch_pcg_dir_f32(x->stream, x->g, x->c, b32); });
+                rc = timed(x, 2, [&] { hzo::launch_pcg_dir_f32(x->stream, x->g, x->c, b32, ta); });
                 if (rc) return rc;
-                rc = timed(x, 3, [&] { hzo::launch_pcg_update_f32(x->stream, x->g, x->c, b32); });
+                rc = timed(x, 3, [&] { hzo::launch_pcg_update_f32(x->stream, x->g, x->c, b32, tb); });
             } else {
-                const int iti = (int)it;
-                rc = timed(x, 2, [&] { hzo::launch_pcg_dir_f64(x->stream, x->g, x->c, b, iti); });
+                rc = timed(x, 2, [&] { hzo::launch_pcg_dir_f64(x->stream, x->g, x->c, b, ta); });
                 if (rc) return rc;
-                rc = timed(x, 3, [&] { hzo::launch_pcg_update_f64(x->stream, x->g, x->c, b, iti); });
+                rc = timed(x, 3, [&] { hzo::launch_pcg_update_f64(x->stream, x->g, x->c, b, tb); });
             }
             if (rc) return rc;
         }
-        HZO_HIP(hipMemcpyAsync(x->st_host, x->st, sizeof(hzo::PcgState), hipMemcpyDeviceToHost, x->stream));
+        HZO_HIP(hipMemcpyAsync(x->st_host, x->st + (t & 1), sizeof(hzo::PcgState), hipMemcpyDeviceToHost,
+                               x->stream));
         HZO_HIP(hipStreamSynchronize(x->stream));
-        if (x->st_host->done || it >= maxpairs) break;
-        batch = x->last_iters / 4 > 2 ? x->last_iters / 4 : 2;
+        if (x->st_host->done || pairs >= maxpairs) break;
+        batch = x->last_iters / 8 > 2 ? x->last_iters / 8 : 2;
     }
     const hzo::PcgState& s = *x->st_host;
     x->last_iters = s.iters + s.refinements > 0 ? s.iters + s.refinements : 1;
@@ -373,10 +374,10 @@ void free_all(hzo_ctx* x) {
     (void)hipFree(x->TR);
     (void)hipFree(x->ptab);
     (void)hipFree(x->lut);
-    (void)hipFree(x->partials);
-    (void)hipFree(x->counters);
+    (void)hipFree(x->part);
+    (void)hipFree(x->tdgl_part);
+    (void)hipFree(x->diag_part);
     (void)hipFree(x->st);
-    (void)hipFree(x->diag);
     (void)hipFree(x->P32[0]);
     (void)hipFree(x->P32[1]);
     (void)hipFree(x->xlo);
@@ -388,7 +389,6 @@ void free_all(hzo_ctx* x) {
     (void)hipFree(x->ptab32);
     (void)hipFree(x->lut32);
     if (x->st_host) (void)hipHostFree(x->st_host);
-    if (x->diag_host) (void)hipHostFree(x->diag_host);
     for (int c = 0; c < HZO_KCLASSES; ++c)
         for (int q = 0; q < 2; ++q)
             for (auto e : x->ev[c][q]) (void)hipEventDestroy(e);
@@ -424,6 +424,7 @@ void hzo_config_default(hzo_config* cfg) {
     cfg->tol_rel = 1.0e-2;
     cfg->max_iter = 4000;
     cfg->refine_every = 20;
+    cfg->tile_kz = 0;
 }
 
 int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
@@ -441,6 +442,7 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
     if (cfg->precision != HZO_PREC_FP64 && cfg->precision != HZO_PREC_FP32_REFINED)
         return set_error(HZO_ERR_ARG, "hzo_create: precision %d", cfg->precision);
     if (cfg->refine_every < 0) return set_error(HZO_ERR_ARG, "hzo_create: refine_every < 0");
+    if (cfg->tile_kz < 0) return set_error(HZO_ERR_ARG, "hzo_create: tile_kz < 0");
     if (cfg->pcg_rule != 0 && cfg->pcg_rule != 1) return set_error(HZO_ERR_ARG, "hzo_create: pcg_rule %d", cfg->pcg_rule);
     if (cfg->max_iter < 0) return set_error(HZO_ERR_ARG, "hzo_create: max_iter < 0");
     const long long N = (long long)cfg->nx * cfg->ny * cfg->nz;
@@ -455,10 +457,21 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
     x->g.ny = cfg->ny;
     x->g.nz = cfg->nz;
     x->g.N = N;
+    // Tile depth: about a thousand 64x8 workgroups over the film unless the
+    // caller fixes it (the inner-product grouping, hence the last bits of the
+    // dot products, follows the tiling).
+    {
+        const long long txy = (long long)((cfg->nx + 63) / 64) * ((cfg->ny + 7) / 8);
+        int kz = cfg->tile_kz > 0 ? cfg->tile_kz : (int)((txy * cfg->nz + 1023) / 1024);
+        if (kz < 1) kz = 1;
+        if (kz > cfg->nz) kz = cfg->nz;
+        x->g.kz = kz;
+    }
     hzo::StepConsts& c = x->c;
     c.nx = cfg->nx;
     c.ny = cfg->ny;
     c.nz = cfg->nz;
+    c.kz = x->g.kz;
     c.dep_flag = cfg->dep_flag;
     c.zbc = cfg->zbc;
     c.del_h = cfg->del_h;
@@ -511,19 +524,15 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
     HZO_ALLOC(x->sc, sizeof(uint16_t) * N);
     HZO_ALLOC(x->ptab, sizeof(hzo::PhaseC) * hzo::PTAB_ENTRIES);
     HZO_ALLOC(x->lut, sizeof(double) * hzo::LUT_DOUBLES);
-    HZO_ALLOC(x->partials, sizeof(double) * hzo::RED_MAXQ * (size_t)x->nb);
-    HZO_ALLOC(x->counters, sizeof(unsigned) * hzo::RED_COUNTERS);
-    HZO_ALLOC(x->st, sizeof(hzo::PcgState));
-    HZO_ALLOC(x->diag, sizeof(double) * 16);
+    HZO_ALLOC(x->part, sizeof(double) * 2 * hzo::RED_MAXQ * (size_t)x->nb);
+    HZO_ALLOC(x->tdgl_part, sizeof(double) * hzo::TDGL_Q * (size_t)x->nb);
+    HZO_ALLOC(x->diag_part, sizeof(double) * hzo::DIAG_Q * (size_t)x->nb);
+    HZO_ALLOC(x->st, sizeof(hzo::PcgState) * 2);
 #undef HZO_ALLOC
-    if (hipHostMalloc((void**)&x->st_host, sizeof(hzo::PcgState), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&x->diag_host, sizeof(double) * 16, hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void**)&x->st_host, sizeof(hzo::PcgState), hipHostMallocDefault) != hipSuccess)
         return fail(set_error(HZO_ERR_NOMEM, "hzo_create: pinned host allocation failed"));
-    if (hipMemset(x->counters, 0, sizeof(unsigned) * hzo::RED_COUNTERS) != hipSuccess ||
-        hipMemset(x->diag, 0, sizeof(double) * 16) != hipSuccess)
+    if (hipMemset(x->tdgl_part, 0, sizeof(double) * hzo::TDGL_Q * (size_t)x->nb) != hipSuccess)
         return fail(set_error(HZO_ERR_HIP, "hzo_create: memset failed"));
-    int rc = init_pcg_state(x);
-    if (rc) return fail(rc);
     *out = x;
     return HZO_OK;
 }
@@ -716,18 +725,29 @@ int hzo_diagnostics(hzo_ctx* x, double out[10]) {
     if (!out) return set_error(HZO_ERR_ARG, "hzo_diagnostics: null output");
     HZO_HIP(hipSetDevice(x->device));
     if (x->fp32)
-        hzo::launch_diag_f32(x->stream, x->g, x->P32[x->cur], x->cw, x->partials, x->counters + 4, x->diag + 8);
+        hzo::launch_diag_f32(x->stream, x->g, x->P32[x->cur], x->cw, x->diag_part);
     else
-        hzo::launch_diag_f64(x->stream, x->g, x->P[x->cur], x->cw, x->partials, x->counters + 4, x->diag + 8);
+        hzo::launch_diag_f64(x->stream, x->g, x->P[x->cur], x->cw, x->diag_part);
     HZO_HIP(hipGetLastError());
-    HZO_HIP(hipMemcpyAsync(x->diag_host, x->diag, sizeof(double) * 16, hipMemcpyDeviceToHost, x->stream));
+    const size_t nb = (size_t)x->nb;
+    x->diag_host.resize((hzo::DIAG_Q + hzo::TDGL_Q) * nb);
+    HZO_HIP(hipMemcpyAsync(x->diag_host.data(), x->diag_part, sizeof(double) * hzo::DIAG_Q * nb,
+                           hipMemcpyDeviceToHost, x->stream));
+    HZO_HIP(hipMemcpyAsync(x->diag_host.data() + hzo::DIAG_Q * nb, x->tdgl_part, sizeof(double) * hzo::TDGL_Q * nb,
+                           hipMemcpyDeviceToHost, x->stream));
     HZO_HIP(hipStreamSynchronize(x->stream));
-    const double* d = x->diag_host;
-    out[0] = d[8];
-    out[1] = d[9];
-    out[2] = d[2];
-    out[3] = d[3];
-    for (int q = 0; q < 6; ++q) out[4 + q] = d[10 + q];
+    // fixed-order sums of the per-workgroup partials
+    double s[hzo::DIAG_Q + hzo::TDGL_Q];
+    for (int q = 0; q < hzo::DIAG_Q + hzo::TDGL_Q; ++q) {
+        double a = 0.0;
+        for (size_t i = 0; i < nb; ++i) a += x->diag_host[q * nb + i];
+        s[q] = a;
+    }
+    out[0] = s[0];
+    out[1] = s[1];
+    out[2] = s[hzo::DIAG_Q];      // grain-frame Pmz of the last TDGL input
+    out[3] = s[hzo::DIAG_Q + 1];  // grain-frame Psz
+    for (int q = 0; q < 6; ++q) out[4 + q] = s[2 + q];
     return HZO_OK;
 }
 
@@ -745,6 +765,8 @@ int hzo_grid_dims(hzo_ctx* x, int dims[3]) {
     dims[2] = x->g.nz;
     return HZO_OK;
 }
+
+int hzo_tile_depth(hzo_ctx* x) { return x ? x->g.kz : 0; }
 
 int hzo_set_timing(hzo_ctx* x, int enable) {
     if (!x) return set_error(HZO_ERR_ARG, "null context");

@@ -55,31 +55,33 @@ struct PhaseC {
 constexpr int PTAB_ENTRIES = 6;
 
 // Device-resident PCG control block.  Scalars never travel to the host
-// inside the iteration; the host polls `done` between launch batches.
+// inside the iteration; the host polls `done` between launch batches.  Two
+// copies alternate by launch parity: the kernel launched t-th after the setup
+// reads copy (t-1)&1 and its workgroup 0 writes copy t&1 (see reduce.hip.h).
+enum : int { K_SETUP = 0, K_DIR = 1, K_FOLD = 2, K_UPDATE = 3, K_RESID = 4 };
 struct PcgState {
     double rz_old, rz0, rr, rr0, bb, alpha, beta, denom;
     int iters, status, done, refinements;
-    int max_iter, rule, pad0, pad1;
+    int max_iter, rule, kind, pbuf;  // kind: K_* of the kernel that wrote this copy
     double tol_abs2, tol_rel2;
-    // FP32-refined path only: mode 0 = CG iteration, 1 = FP64 residual
-    // replacement pending (the next launch pair folds x_lo into x_hi and
-    // recomputes r = b - A x_hi); pbuf selects the current search direction
-    // buffer; refine_every = period of the replacement.
-    int mode, pbuf, refine_every, pad2;
+    int refine_every, pad0, pad1, pad2;  // FP32-refined: residual replacement period
 };
 
-// Reduction scratch: one completion counter per kernel class and the
-// per-block partial sums.
-constexpr int RED_COUNTERS = 8;
-constexpr int RED_MAXQ = 8;
+// Per-workgroup partial sums: two parity slots of RED_MAXQ arrays each for
+// the PCG, plus the TDGL local-frame sums and the diagnostics sums.
+constexpr int RED_MAXQ = 4;
+constexpr int DIAG_Q = 8;
+constexpr int TDGL_Q = 2;
 
 struct Grid {
     int nx, ny, nz;
     long long N;
+    int kz;  // z planes per workgroup (tile depth)
 };
 
 struct StepConsts {
     int nx, ny, nz;
+    int kz;  // z planes per workgroup (tile depth), equal to Grid::kz
     int dep_flag, zbc;
     double del_h, del_t;
     double g_bot, g_top;
@@ -105,10 +107,9 @@ struct Fp64Bufs {
     const double* TR;
     const PhaseC* ptab;
     const double* lut;
-    double* partials;   // RED_MAXQ * nblocks_max
-    unsigned* counters; // RED_COUNTERS
-    PcgState* st;
-    double* diag;       // [16] results of reductions (local sums etc.)
+    double* part[2];    // PCG partials by launch parity, RED_MAXQ * nb each
+    PcgState* st;       // [2] control blocks by launch parity
+    double* tdgl_part;  // TDGL_Q * nb: grain-frame z sums of the last TDGL call
 };
 
 // FP32-refined form: FP32 polarisation and CG vectors, FP64 potential x_hi
@@ -131,26 +132,25 @@ struct Fp32Bufs {
     const PhaseC32* ptab32;
     const double* lut;    // FP64 harm + dinv (residual replacement)
     const float* lut32;   // FP32 harm + dinv (CG iteration)
-    double* partials;
-    unsigned* counters;
+    double* part[2];
     PcgState* st;
-    double* diag;
+    double* tdgl_part;
 };
 
 int blocks_for(const Grid& g);
 
-void launch_tdgl_f64(hipStream_t s, const Grid& g, const StepConsts& c, double Ez_ext,
-                     const Fp64Bufs& b);
-void launch_pcg_setup_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b);
-void launch_pcg_dir_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int it);
-void launch_pcg_update_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int it);
+// Launchers.  `t` is the launch index since the last PCG setup (setup = 0).
+void launch_tdgl_f64(hipStream_t s, const Grid& g, const StepConsts& c, double Ez_ext, const Fp64Bufs& b);
+void launch_pcg_setup_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b,
+                          const PcgState& init);
+void launch_pcg_dir_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t);
+void launch_pcg_update_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t);
+void launch_diag_f64(hipStream_t s, const Grid& g, const double* P, const uint32_t* cw, double* part);
 void launch_tdgl_f32(hipStream_t s, const Grid& g, const StepConsts& c, double Ez_ext, const Fp32Bufs& b);
-void launch_pcg_setup_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b);
-void launch_pcg_dir_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b);
-void launch_pcg_update_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b);
-void launch_diag_f32(hipStream_t s, const Grid& g, const float* P, const uint32_t* cw, double* partials,
-                     unsigned* counters, double* out8);
-void launch_diag_f64(hipStream_t s, const Grid& g, const double* P, const uint32_t* cw,
-                     double* partials, unsigned* counters, double* out8);
+void launch_pcg_setup_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b,
+                          const PcgState& init);
+void launch_pcg_dir_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b, int t);
+void launch_pcg_update_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b, int t);
+void launch_diag_f32(hipStream_t s, const Grid& g, const float* P, const uint32_t* cw, double* part);
 
 }  // namespace hzo

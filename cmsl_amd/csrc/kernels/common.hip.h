@@ -5,14 +5,14 @@
 
 namespace hzo {
 
-constexpr int BX = 64, BY = 4, KZ = 4;
+constexpr int BX = 64, BY = 8, NT = BX * BY;
 
 inline int blocks_for_impl(const Grid& g) {
-    return ((g.nx + BX - 1) / BX) * ((g.ny + BY - 1) / BY) * ((g.nz + KZ - 1) / KZ);
+    return ((g.nx + BX - 1) / BX) * ((g.ny + BY - 1) / BY) * ((g.nz + g.kz - 1) / g.kz);
 }
 
 inline dim3 grid3(const Grid& g) {
-    return dim3((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, (g.nz + KZ - 1) / KZ);
+    return dim3((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, (g.nz + g.kz - 1) / g.kz);
 }
 
 __device__ __forceinline__ long long vid(int i, int j, int k, int nx, int ny) {
@@ -67,6 +67,38 @@ __device__ __forceinline__ bool keep_going(const PcgState& st, double rz, double
 }
 
 __device__ __forceinline__ bool finite(double v) { return __builtin_isfinite(v); }
+
+__device__ __forceinline__ bool block0_thread0() {
+    return blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 && threadIdx.y == 0;
+}
+
+// Copies the control block forward unchanged (used once the solve is done).
+__device__ __forceinline__ void pass_state(const PcgState& s, PcgState* out) {
+    if (block0_thread0()) *out = s;
+}
+
+// The stopping test after a completed reduction of r.z and r.r (the
+// reference loop head, depol_varEPS.cu:472-474, or ||r|| <= tol ||b||).
+__device__ __forceinline__ void judge(PcgState& s, double rz, double rr) {
+    if (!keep_going(s, rz, rr)) {
+        s.done = 1;
+        if (s.iters >= s.max_iter) s.status = HZO_PCG_MAXITER;
+    }
+}
+
+// Jacobi inverse diagonal of a row from its stencil code and the harmonic
+// face table H (depol_varEPS.cu:177-179: D = sum of the six face means / h^2,
+// D^-1 = 1/D, or 1 when D <= 1e-30 and on identity rows).  In FP64 this is
+// the same sequence of IEEE operations as the reference.
+template <class T>
+__device__ __forceinline__ T dinv_of(unsigned code, const T* H, const StepConsts& c) {
+    if (code & SC_IDENTITY) return (T)1;
+    const T* Hr = H + (code & 3u) * 4u;
+    const T sum = Hr[(code >> 2) & 3u] + Hr[(code >> 4) & 3u] + Hr[(code >> 6) & 3u] + Hr[(code >> 8) & 3u] +
+                  Hr[(code >> 10) & 3u] + Hr[(code >> 12) & 3u];
+    const T D = c.h2_pow2 ? sum * (T)c.inv_h2 : sum / (T)c.h2;
+    return D > (T)1e-30 ? (T)1 / D : (T)1;
+}
 
 template <class T>
 __device__ __forceinline__ void load_lut(T* sH, const T* lut) {

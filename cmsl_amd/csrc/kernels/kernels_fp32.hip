@@ -10,30 +10,31 @@
 //
 // The iteration is the same Jacobi PCG as the FP64 form (depol_varEPS.cu:
 // 442-523); the launch pattern is one (dir, update) pair per host step and
-// the device decides, through PcgState.mode, whether a pair is a CG
+// the device decides, from the control block, whether a pair is a CG
 // iteration or a residual replacement (fold x_lo into x_hi, recompute r).
 //
 // This file is compiled with -ffp-contract=fast: FP32 kernels may fuse.
 #include "../hzo_internal.h"
 #include "common.hip.h"
+#include "tile.hip.h"
 #include "reduce.hip.h"
 
 namespace hzo {
 
 // ---------------------------------------------------------------- TDGL
-__global__ __launch_bounds__(256) void k_tdgl_f32(Fp32Bufs b, StepConsts c, float Ez_ext, int nb) {
+__global__ __launch_bounds__(NT) void k_tdgl_f32(Fp32Bufs b, StepConsts c, float Ez_ext, int nb) {
     const int nx = c.nx, ny = c.ny, nz = c.nz;
     const long long N = (long long)nx * ny * nz;
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
-    const int k0 = blockIdx.z * KZ;
+    const int k0 = blockIdx.z * c.kz;
     double loc[2] = {0.0, 0.0};
     if (i < nx && j < ny) {
         const int e = (i + 1 == nx) ? 0 : i + 1;
         const int w = (i == 0) ? nx - 1 : i - 1;
         const int n = (j + 1 == ny) ? 0 : j + 1;
         const int s = (j == 0) ? ny - 1 : j - 1;
-        const int k1 = min(k0 + KZ, nz);
+        const int k1 = min(k0 + c.kz, nz);
         for (int k = k0; k < k1; ++k) {
             const long long id = vid(i, j, k, nx, ny);
             const uint32_t cw = b.cw[id];
@@ -109,13 +110,7 @@ __global__ __launch_bounds__(256) void k_tdgl_f32(Fp32Bufs b, StepConsts c, floa
             b.Pnew[5 * N + id] = sz * om + A * sum[5] - cL * dsz;
         }
     }
-    double tot[2];
-    if (grid_reduce_last<2>(loc, b.partials, nb, b.counters + 0, tot)) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            b.diag[2] = tot[0];
-            b.diag[3] = tot[1];
-        }
-    }
+    write_block_partials<TDGL_Q>(loc, b.tdgl_part, nb);
 }
 
 // ---------------------------------------------------------------- operator rows
@@ -144,21 +139,22 @@ __device__ __forceinline__ double true_residual(const Fp32Bufs& b, const double*
 }
 
 // ---------------------------------------------------------------- PCG setup
-__global__ __launch_bounds__(256) void k_pcg_setup_f32(Fp32Bufs b, StepConsts c, int nb) {
+__global__ __launch_bounds__(NT) void k_pcg_setup_f32(Fp32Bufs b, StepConsts c, int nb, PcgState init) {
     __shared__ double sH[16];
+    __shared__ float sH32[16];
     load_lut(sH, b.lut);
+    load_lut(sH32, b.lut32);
     const int nx = c.nx, ny = c.ny, nz = c.nz;
     const long long N = (long long)nx * ny * nz;
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
-    const int k0 = blockIdx.z * KZ;
+    const int k0 = blockIdx.z * c.kz;
     double acc[3] = {0.0, 0.0, 0.0};
     if (i < nx && j < ny) {
-        const float* dinv = b.lut32 + LUT_DINV;
         const float* Px = b.Pcur;
         const float* Py = b.Pcur + N;
         const float* Pz = b.Pcur + 2 * N;
-        const int k1 = min(k0 + KZ, nz);
+        const int k1 = min(k0 + c.kz, nz);
         for (int k = k0; k < k1; ++k) {
             const Nbr v = op_nbrs(i, j, k, c);
             const unsigned code = b.sc[v.c];
@@ -172,193 +168,299 @@ __global__ __launch_bounds__(256) void k_pcg_setup_f32(Fp32Bufs b, StepConsts c,
             }
             b.b[v.c] = bval;
             const float r = (float)true_residual(b, sH, c, v, code, k, bval, true);
-            const float z = r * dinv[code];
+            const float z = r * dinv_of(code, sH32, c);
             b.r[v.c] = r;
             acc[0] += (double)r * (double)z;
             acc[1] += (double)r * (double)r;
             acc[2] += bval * bval;
         }
     }
-    double tot[3];
-    if (grid_reduce_last<3>(acc, b.partials, nb, b.counters + 1, tot)) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            PcgState& st = *b.st;
-            st.rz0 = st.rz_old = tot[0];
-            st.rr0 = st.rr = tot[1];
-            st.bb = tot[2];
-            st.iters = 0;
-            st.status = 0;
-            st.done = 0;
-            st.beta = 0.0;
-            st.mode = 0;
-            st.refinements = 0;
-            if (!finite(tot[0]) || tot[0] < 0.0) {
-                st.status = HZO_PCG_BAD_INITIAL;
-                st.done = 1;
-            } else if (!keep_going(st, tot[0], tot[1])) {
-                st.done = 1;
-                if (st.iters >= st.max_iter) st.status = HZO_PCG_MAXITER;
-            }
-        }
+    write_block_partials<3>(acc, b.part[0], nb);
+    if (block0_thread0()) {
+        init.kind = K_SETUP;
+        b.st[0] = init;
     }
 }
 
 // ---------------------------------------------------------------- direction / fold
-__global__ __launch_bounds__(256) void k_pcg_dir_f32(Fp32Bufs b, StepConsts c, int nb) {
-    const PcgState* sp = b.st;
-    if (sp->done) return;
+// Finishes the previous kernel's inner products and decides what this launch
+// is: a CG direction update (K_DIR) or the fold x_hi += x_lo that starts a
+// residual replacement (K_FOLD).
+__global__ __launch_bounds__(NT) void k_pcg_dir_f32(Fp32Bufs b, StepConsts c, int nb, int t) {
+    const int ip = (t - 1) & 1, io = t & 1;
+    PcgState s = b.st[ip];
+    if (s.done) {
+        pass_state(s, b.st + io);
+        return;
+    }
+    const bool first = s.kind == K_SETUP;
+    bool fold = false;
     const int nx = c.nx, ny = c.ny, nz = c.nz;
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
-    const int k0 = blockIdx.z * KZ;
-    if (sp->mode == 1) {  // fold the FP32 correction into the FP64 potential
+    const int k0 = blockIdx.z * c.kz;
+    const int k1 = min(k0 + c.kz, nz);
+    __shared__ float sH[16];
+    __shared__ float ring[3][HY][HX];
+    const TilePos tp = tile_pos(c);
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const float* pold = b.p[s.pbuf];
+    float* pnew = b.p[s.pbuf ^ 1];
+    float beta = 0.f;  // set once the previous inner products are finished
+    float rc_ = 0.f, pc_ = 0.f, rh = 0.f, ph = 0.f;
+    unsigned cc_ = 0, ch = 0;
+    auto fetch = [&](int kk) {
+        const long long id = vid(tp.ci, tp.cj, kk, nx, ny);
+        rc_ = b.r[id];
+        cc_ = b.sc[id];
+        if (!first) pc_ = pold[id];
+        if (tp.has_h) {
+            const long long h = vid(tp.gi, tp.gj, kk, nx, ny);
+            rh = b.r[h];
+            ch = b.sc[h];
+            if (!first) ph = pold[h];
+        }
+    };
+    auto make = [&](int kk) -> unsigned {
+        float(*S)[HX] = ring[kk % 3];
+        const float z = rc_ * dinv_of(cc_, sH, c);
+        S[ty + 1][tx + 1] = first ? z : z + beta * pc_;
+        if (tp.has_h) {
+            const float zh = rh * dinv_of(ch, sH, c);
+            S[tp.hy][tp.hx] = first ? zh : zh + beta * ph;
+        }
+        return cc_;
+    };
+    int lo, hi;
+    plane_window(c, k0, k1, lo, hi);
+    double acc[1] = {0.0};
+    fetch(lo);  // the first plane's loads overlap the reduction below
+    if (first) {
+        double tot[3];
+        sum_partials<3>(b.part[ip], nb, tot);
+        s.rz0 = s.rz_old = tot[0];
+        s.rr0 = s.rr = tot[1];
+        s.bb = tot[2];
+        if (!finite(tot[0]) || tot[0] < 0.0) {
+            s.status = HZO_PCG_BAD_INITIAL;
+            s.done = 1;
+        } else {
+            judge(s, tot[0], tot[1]);
+        }
+    } else {
+        double tot[2];
+        sum_partials<2>(b.part[ip], nb, tot);
+        s.rr = tot[1];
+        if (!finite(tot[0]) || tot[0] < 0.0) {
+            s.status = HZO_PCG_BAD_RZ;
+            s.rz_old = tot[0];
+            s.done = 1;
+        } else if (s.kind == K_RESID) {  // true residual: trust the test
+            s.refinements += 1;
+            s.beta = tot[0] / s.rz_old;
+            s.rz_old = tot[0];
+            judge(s, tot[0], tot[1]);
+        } else {  // K_UPDATE: recursively updated residual
+            if (s.iters >= s.max_iter) {
+                s.rz_old = tot[0];
+                s.done = 1;
+                s.status = HZO_PCG_MAXITER;
+            } else if (!keep_going(s, tot[0], tot[1]) ||
+                       (s.refine_every > 0 && s.iters % s.refine_every == 0)) {
+                fold = true;  // replace the residual before trusting or continuing it
+            } else {
+                s.beta = tot[0] / s.rz_old;
+                s.rz_old = tot[0];
+            }
+        }
+    }
+    if (s.done) {
+        pass_state(s, b.st + io);
+        return;
+    }
+    if (fold) {
         if (i < nx && j < ny) {
-            const int k1 = min(k0 + KZ, nz);
             for (int k = k0; k < k1; ++k) {
                 const long long id = vid(i, j, k, nx, ny);
                 b.xhi[id] += (double)b.xlo[id];
                 b.xlo[id] = 0.0f;
             }
         }
+        s.kind = K_FOLD;
+        pass_state(s, b.st + io);
         return;
     }
-    __shared__ float sH[16];
+    beta = (float)s.beta;
     load_lut(sH, b.lut32);
-    const int pb = sp->pbuf;
-    const float* pold = b.p[pb];
-    float* pnew = b.p[pb ^ 1];
-    const bool first = sp->iters == 0;
-    const float beta = (float)sp->beta;
-    double acc[1] = {0.0};
-    if (i < nx && j < ny) {
-        const float* dinv = b.lut32 + LUT_DINV;
-        auto pn = [&](long long q) {
-            const float z = b.r[q] * dinv[b.sc[q]];
-            return first ? z : z + beta * pold[q];
-        };
-        const int k1 = min(k0 + KZ, nz);
-        for (int k = k0; k < k1; ++k) {
-            const Nbr v = op_nbrs(i, j, k, c);
-            const unsigned code = b.sc[v.c];
-            const float pc = pn(v.c);
-            float q;
-            if (code & SC_IDENTITY)
-                q = op_identity(pc, k, c);
-            else
-                q = row_Ax<float>(code, sH, c, pc, pn(v.e), pn(v.w), pn(v.n), pn(v.s), pn(v.u), pn(v.d));
-            pnew[v.c] = pc;
+    unsigned code_k = make(lo), code_next = 0;
+    int kn = lo + 1;
+    if (kn <= hi) fetch(kn);
+    for (int k = k0; k < k1; ++k) {
+        const int need = min(k + 1, hi);
+        while (kn <= need) {
+            const unsigned cm = make(kn);
+            if (kn == k) code_k = cm;
+            else code_next = cm;
+            ++kn;
+            if (kn <= hi) fetch(kn);
+        }
+        __syncthreads();
+        float(*S)[HX] = ring[k % 3];
+        const float pc = S[ty + 1][tx + 1];
+        float q;
+        if (code_k & SC_IDENTITY) {
+            q = op_identity(pc, k, c);
+        } else {
+            int up = k + 1, dn = k - 1;
+            if (c.zbc != 0) {
+                if (k == 0) up = dn = 1;
+                else if (k == nz - 1) up = dn = nz - 2;
+            }
+            q = row_Ax<float>(code_k, sH, c, pc, S[ty + 1][tx + 2], S[ty + 1][tx], S[ty + 2][tx + 1],
+                              S[ty][tx + 1], ring[up % 3][ty + 1][tx + 1], ring[dn % 3][ty + 1][tx + 1]);
+        }
+        if (tp.live) {
+            pnew[vid(tp.i, tp.j, k, nx, ny)] = pc;
             acc[0] += (double)pc * (double)q;
         }
+        __syncthreads();
+        code_k = code_next;
     }
-    double tot[1];
-    if (grid_reduce_last<1>(acc, b.partials, nb, b.counters + 2, tot)) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            PcgState& st = *b.st;
-            st.iters += 1;
-            st.denom = tot[0];
-            st.pbuf = pb ^ 1;
-            if (!finite(tot[0]) || !(tot[0] > 0.0)) {
-                st.status = HZO_PCG_BREAKDOWN;
-                st.done = 1;
-            } else {
-                st.alpha = st.rz_old / tot[0];
-            }
-        }
-    }
+    write_block_partials<1>(acc, b.part[io], nb);
+    s.iters += 1;
+    s.pbuf ^= 1;
+    s.kind = K_DIR;
+    pass_state(s, b.st + io);
 }
 
 // ---------------------------------------------------------------- update / residual replacement
-__global__ __launch_bounds__(256) void k_pcg_update_f32(Fp32Bufs b, StepConsts c, int nb) {
-    const PcgState* sp = b.st;
-    if (sp->done) return;
+__global__ __launch_bounds__(NT) void k_pcg_update_f32(Fp32Bufs b, StepConsts c, int nb, int t) {
+    const int ip = (t - 1) & 1, io = t & 1;
+    PcgState s = b.st[ip];
+    if (s.done) {
+        pass_state(s, b.st + io);
+        return;
+    }
+    const bool resid = s.kind == K_FOLD;
     const int nx = c.nx, ny = c.ny, nz = c.nz;
-    const int i = blockIdx.x * BX + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    const int k0 = blockIdx.z * KZ;
-    const int mode = sp->mode;
-    __shared__ float sH32[16];
-    __shared__ double sH64[16];
-    if (mode == 0)
-        load_lut(sH32, b.lut32);
-    else
-        load_lut(sH64, b.lut);
-    const float alpha = (float)sp->alpha;
-    const float* p = b.p[sp->pbuf];
+    const int k0 = blockIdx.z * c.kz, k1 = min(k0 + c.kz, nz);
     double acc[2] = {0.0, 0.0};
-    if (i < nx && j < ny) {
-        const float* dinv = b.lut32 + LUT_DINV;
-        const int k1 = min(k0 + KZ, nz);
+    if (resid) {  // r = b - A x_hi in FP64 (x_lo was folded by the previous launch)
+        __shared__ double sH64[16];
+        __shared__ float sH32r[16];
+        load_lut(sH64, b.lut);
+        load_lut(sH32r, b.lut32);
+        const int i = blockIdx.x * BX + threadIdx.x;
+        const int j = blockIdx.y * BY + threadIdx.y;
+        if (i < nx && j < ny) {
+            for (int k = k0; k < k1; ++k) {
+                const Nbr v = op_nbrs(i, j, k, c);
+                const unsigned code = b.sc[v.c];
+                const float r = (float)true_residual(b, sH64, c, v, code, k, b.b[v.c], false);
+                b.r[v.c] = r;
+                const float z = r * dinv_of(code, sH32r, c);
+                acc[0] += (double)r * (double)z;
+                acc[1] += (double)r * (double)r;
+            }
+        }
+    } else {
+        __shared__ float sH32[16];
+        __shared__ float ring[3][HY][HX];
+        const TilePos tp = tile_pos(c);
+        const int tx = threadIdx.x, ty = threadIdx.y;
+        const float* p = b.p[s.pbuf];
+        float pcv = 0.f, phv = 0.f;
+        auto fetch = [&](int kk) {
+            pcv = p[vid(tp.ci, tp.cj, kk, nx, ny)];
+            if (tp.has_h) phv = p[vid(tp.gi, tp.gj, kk, nx, ny)];
+        };
+        auto make = [&](int kk) {
+            ring[kk % 3][ty + 1][tx + 1] = pcv;
+            if (tp.has_h) ring[kk % 3][tp.hy][tp.hx] = phv;
+        };
+        struct Ctr {
+            float x, r;
+            unsigned code;
+        };
+        auto fetch_c = [&](int kk) {
+            const long long id = vid(tp.ci, tp.cj, kk, nx, ny);
+            Ctr v;
+            v.x = b.xlo[id];
+            v.r = b.r[id];
+            v.code = b.sc[id];
+            return v;
+        };
+        int lo, hi;
+        plane_window(c, k0, k1, lo, hi);
+        fetch(lo);  // loads of the first planes overlap the reduction below
+        Ctr cur = fetch_c(k0), nxt = cur;
+        double tot[1];
+        sum_partials<1>(b.part[ip], nb, tot);
+        s.denom = tot[0];
+        if (!finite(tot[0]) || !(tot[0] > 0.0)) {
+            s.status = HZO_PCG_BREAKDOWN;
+            s.done = 1;
+            pass_state(s, b.st + io);
+            return;
+        }
+        s.alpha = s.rz_old / tot[0];
+        const float alpha = (float)s.alpha;
+        load_lut(sH32, b.lut32);
+        make(lo);
+        int kn = lo + 1;
+        if (kn <= hi) fetch(kn);
         for (int k = k0; k < k1; ++k) {
-            const Nbr v = op_nbrs(i, j, k, c);
-            const unsigned code = b.sc[v.c];
-            float r;
-            if (mode == 0) {
-                const float pc = p[v.c];
-                float q;
-                if (code & SC_IDENTITY)
-                    q = op_identity(pc, k, c);
-                else
-                    q = row_Ax<float>(code, sH32, c, pc, p[v.e], p[v.w], p[v.n], p[v.s], p[v.u], p[v.d]);
-                b.xlo[v.c] += alpha * pc;
-                r = b.r[v.c] - alpha * q;
-            } else {
-                r = (float)true_residual(b, sH64, c, v, code, k, b.b[v.c], false);
+            const int need = min(k + 1, hi);
+            while (kn <= need) {
+                make(kn);
+                ++kn;
+                if (kn <= hi) fetch(kn);
             }
-            b.r[v.c] = r;
-            const float z = r * dinv[code];
-            acc[0] += (double)r * (double)z;
-            acc[1] += (double)r * (double)r;
+            if (k + 1 < k1) nxt = fetch_c(k + 1);
+            __syncthreads();
+            float(*S)[HX] = ring[k % 3];
+            const float pc = S[ty + 1][tx + 1];
+            float q;
+            if (cur.code & SC_IDENTITY) {
+                q = op_identity(pc, k, c);
+            } else {
+                int up = k + 1, dn = k - 1;
+                if (c.zbc != 0) {
+                    if (k == 0) up = dn = 1;
+                    else if (k == nz - 1) up = dn = nz - 2;
+                }
+                q = row_Ax<float>(cur.code, sH32, c, pc, S[ty + 1][tx + 2], S[ty + 1][tx], S[ty + 2][tx + 1],
+                                  S[ty][tx + 1], ring[up % 3][ty + 1][tx + 1], ring[dn % 3][ty + 1][tx + 1]);
+            }
+            if (tp.live) {
+                const long long id = vid(tp.i, tp.j, k, nx, ny);
+                b.xlo[id] = cur.x + alpha * pc;
+                const float r = cur.r - alpha * q;
+                b.r[id] = r;
+                const float z = r * dinv_of(cur.code, sH32, c);
+                acc[0] += (double)r * (double)z;
+                acc[1] += (double)r * (double)r;
+            }
+            __syncthreads();
+            cur = nxt;
         }
     }
-    double tot[2];
-    if (grid_reduce_last<2>(acc, b.partials, nb, b.counters + 3, tot)) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            PcgState& st = *b.st;
-            const double rz = tot[0];
-            st.rr = tot[1];
-            if (!finite(rz) || rz < 0.0) {
-                st.status = HZO_PCG_BAD_RZ;
-                st.rz_old = rz;
-                st.done = 1;
-            } else if (mode == 0) {
-                const bool conv = !keep_going(st, rz, tot[1]);
-                if (st.iters >= st.max_iter) {
-                    st.rz_old = rz;
-                    st.done = 1;
-                    st.status = HZO_PCG_MAXITER;
-                } else if (conv || (st.refine_every > 0 && st.iters % st.refine_every == 0)) {
-                    st.mode = 1;  // replace the recursive residual before trusting it
-                } else {
-                    st.beta = rz / st.rz_old;
-                    st.rz_old = rz;
-                }
-            } else {
-                st.refinements += 1;
-                st.mode = 0;
-                st.beta = rz / st.rz_old;
-                st.rz_old = rz;
-                if (!keep_going(st, rz, tot[1])) {
-                    st.done = 1;
-                    if (st.iters >= st.max_iter) st.status = HZO_PCG_MAXITER;
-                }
-            }
-        }
-    }
+    write_block_partials<2>(acc, b.part[io], nb);
+    s.kind = resid ? K_RESID : K_UPDATE;
+    pass_state(s, b.st + io);
 }
 
 // ---------------------------------------------------------------- diagnostics
-__global__ __launch_bounds__(256) void k_diag_f32(const float* __restrict__ P, const uint32_t* __restrict__ cw,
-                                                  Grid g, double* partials, unsigned* counter, double* out,
-                                                  int nb) {
+__global__ __launch_bounds__(NT) void k_diag_f32(const float* __restrict__ P, const uint32_t* __restrict__ cw,
+                                                  Grid g, double* part, int nb) {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long N = g.N;
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
-    const int k0 = blockIdx.z * KZ;
-    double a[4] = {0.0, 0.0, 0.0, 0.0};
-    double m[4] = {0.0, 0.0, 0.0, 0.0};
+    const int k0 = blockIdx.z * g.kz;
+    double a[DIAG_Q] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (i < nx && j < ny) {
-        const int k1 = min(k0 + KZ, nz);
+        const int k1 = min(k0 + g.kz, nz);
         for (int k = k0; k < k1; ++k) {
             const long long id = vid(i, j, k, nx, ny);
             const double pmz = P[2 * N + id];
@@ -367,49 +469,31 @@ __global__ __launch_bounds__(256) void k_diag_f32(const float* __restrict__ P, c
             const uint32_t w = cw[id];
             const int phase = (w & CW_VOID) ? -1 : (int)((w >> CW_PHASE_SHIFT) & 3u);
             a[2] += (phase == 1) ? pmz : 0.0;
-            a[3] += (phase == 2) ? pmz : 0.0;
-            m[0] += (phase == 0) ? pmz : 0.0;
-            m[1] += (phase == 1) ? 1.0 : 0.0;
-            m[2] += (phase == 2) ? 1.0 : 0.0;
-            m[3] += (phase == 0) ? 1.0 : 0.0;
+            a[3] += (phase == 1) ? 1.0 : 0.0;
+            a[4] += (phase == 2) ? pmz : 0.0;
+            a[5] += (phase == 2) ? 1.0 : 0.0;
+            a[6] += (phase == 0) ? pmz : 0.0;
+            a[7] += (phase == 0) ? 1.0 : 0.0;
         }
     }
-    double t1[4];
-    if (grid_reduce_last<4>(a, partials, nb, counter, t1)) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            out[0] = t1[0];
-            out[1] = t1[1];
-            out[2] = t1[2];
-            out[4] = t1[3];
-        }
-    }
-    double t2[4];
-    if (grid_reduce_last<4>(m, partials + 4 * (long long)nb, nb, counter + 1, t2)) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            out[6] = t2[0];
-            out[3] = t2[1];
-            out[5] = t2[2];
-            out[7] = t2[3];
-        }
-    }
+    write_block_partials<DIAG_Q>(a, part, nb);
 }
 
 void launch_tdgl_f32(hipStream_t s, const Grid& g, const StepConsts& c, double Ez_ext, const Fp32Bufs& b) {
     hipLaunchKernelGGL(k_tdgl_f32, grid3(g), dim3(BX, BY, 1), 0, s, b, c, (float)Ez_ext, blocks_for_impl(g));
 }
-void launch_pcg_setup_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b) {
-    hipLaunchKernelGGL(k_pcg_setup_f32, grid3(g), dim3(BX, BY, 1), 0, s, b, c, blocks_for_impl(g));
+void launch_pcg_setup_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b,
+                          const PcgState& init) {
+    hipLaunchKernelGGL(k_pcg_setup_f32, grid3(g), dim3(BX, BY, 1), 0, s, b, c, blocks_for_impl(g), init);
 }
-void launch_pcg_dir_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b) {
-    hipLaunchKernelGGL(k_pcg_dir_f32, grid3(g), dim3(BX, BY, 1), 0, s, b, c, blocks_for_impl(g));
+void launch_pcg_dir_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b, int t) {
+    hipLaunchKernelGGL(k_pcg_dir_f32, grid3(g), dim3(BX, BY, 1), 0, s, b, c, blocks_for_impl(g), t);
 }
-void launch_pcg_update_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b) {
-    hipLaunchKernelGGL(k_pcg_update_f32, grid3(g), dim3(BX, BY, 1), 0, s, b, c, blocks_for_impl(g));
+void launch_pcg_update_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b, int t) {
+    hipLaunchKernelGGL(k_pcg_update_f32, grid3(g), dim3(BX, BY, 1), 0, s, b, c, blocks_for_impl(g), t);
 }
-void launch_diag_f32(hipStream_t s, const Grid& g, const float* P, const uint32_t* cw, double* partials,
-                     unsigned* counters, double* out8) {
-    hipLaunchKernelGGL(k_diag_f32, grid3(g), dim3(BX, BY, 1), 0, s, P, cw, g, partials, counters, out8,
-                       blocks_for_impl(g));
+void launch_diag_f32(hipStream_t s, const Grid& g, const float* P, const uint32_t* cw, double* part) {
+    hipLaunchKernelGGL(k_diag_f32, grid3(g), dim3(BX, BY, 1), 0, s, P, cw, g, part, blocks_for_impl(g));
 }
 
 }  // namespace hzo

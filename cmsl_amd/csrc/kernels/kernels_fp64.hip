@@ -15,30 +15,32 @@
 // double the reference expression denotes.  Inner products are
 // deterministic (fixed-order grid reduction, reduce.hip.h).
 //
-// Launch shape: 64x4 workgroups over x-y tiles (x fastest: each wave reads a
-// contiguous 512-byte x-row segment), each thread walking KZ z-planes.
+// Launch shape: 64x8 workgroups over x-y tiles (x fastest: each wave reads a
+// contiguous x-row segment), each walking kz z-planes (tile depth chosen per
+// grid so that about a thousand workgroups cover the film).
 #include "../hzo_internal.h"
 #include "reduce.hip.h"
 #include "common.hip.h"
+#include "tile.hip.h"
 
 namespace hzo {
 
 int blocks_for(const Grid& g) { return blocks_for_impl(g); }
 
 // ---------------------------------------------------------------- TDGL
-__global__ __launch_bounds__(256) void k_tdgl_f64(Fp64Bufs b, StepConsts c, double Ez_ext, int nb) {
+__global__ __launch_bounds__(NT) void k_tdgl_f64(Fp64Bufs b, StepConsts c, double Ez_ext, int nb) {
     const int nx = c.nx, ny = c.ny, nz = c.nz;
     const long long N = (long long)nx * ny * nz;
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
-    const int k0 = blockIdx.z * KZ;
+    const int k0 = blockIdx.z * c.kz;
     double loc[2] = {0.0, 0.0};  // sum Pmz_local, sum Psz_local (evolve.cu:269-276)
     if (i < nx && j < ny) {
         const int e = (i + 1 == nx) ? 0 : i + 1;
         const int w = (i == 0) ? nx - 1 : i - 1;
         const int n = (j + 1 == ny) ? 0 : j + 1;
         const int s = (j == 0) ? ny - 1 : j - 1;
-        const int k1 = min(k0 + KZ, nz);
+        const int k1 = min(k0 + c.kz, nz);
         for (int k = k0; k < k1; ++k) {
             const long long id = vid(i, j, k, nx, ny);
             const uint32_t cw = b.cw[id];
@@ -143,13 +145,7 @@ __global__ __launch_bounds__(256) void k_tdgl_f64(Fp64Bufs b, StepConsts c, doub
             b.Pnew[5 * N + id] = pSz0 * om + A * sum[5] - cL * dfdPsz_b;
         }
     }
-    double tot[2];
-    if (grid_reduce_last<2>(loc, b.partials, nb, b.counters + 0, tot)) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            b.diag[2] = tot[0];
-            b.diag[3] = tot[1];
-        }
-    }
+    write_block_partials<TDGL_Q>(loc, b.tdgl_part, nb);
 }
 
 // -(sum of harmonic-mean face fluxes)/h^2, depol_varEPS.cu:76-123
@@ -167,21 +163,20 @@ __device__ __forceinline__ double op_row(unsigned code, const double* H, const S
 }
 
 // ---------------------------------------------------------------- PCG setup
-__global__ __launch_bounds__(256) void k_pcg_setup_f64(Fp64Bufs b, StepConsts c, int nb) {
+__global__ __launch_bounds__(NT) void k_pcg_setup_f64(Fp64Bufs b, StepConsts c, int nb, PcgState init) {
     __shared__ double sH[16];
     load_lut(sH, b.lut);
     const int nx = c.nx, ny = c.ny, nz = c.nz;
     const long long N = (long long)nx * ny * nz;
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
-    const int k0 = blockIdx.z * KZ;
+    const int k0 = blockIdx.z * c.kz;
     double acc[3] = {0.0, 0.0, 0.0};  // r.z, r.r, b.b
     if (i < nx && j < ny) {
-        const double* dinv = b.lut + LUT_DINV;
         const double* Px = b.Pcur;
         const double* Py = b.Pcur + N;
         const double* Pz = b.Pcur + 2 * N;
-        const int k1 = min(k0 + KZ, nz);
+        const int k1 = min(k0 + c.kz, nz);
         for (int k = k0; k < k1; ++k) {
             const Nbr nb7 = op_nbrs(i, j, k, c);
             const unsigned code = b.sc[nb7.c];
@@ -203,167 +198,258 @@ __global__ __launch_bounds__(256) void k_pcg_setup_f64(Fp64Bufs b, StepConsts c,
                             x[nb7.d]);
             }
             const double r = -1.0 * Ax + bval;
-            const double z = r * dinv[code];
+            const double z = r * dinv_of(code, sH, c);
             b.r[nb7.c] = r;
             acc[0] += r * z;
             acc[1] += r * r;
             acc[2] += bval * bval;
         }
     }
-    double tot[3];
-    if (grid_reduce_last<3>(acc, b.partials, nb, b.counters + 1, tot)) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            PcgState& st = *b.st;
-            st.rz0 = tot[0];
-            st.rz_old = tot[0];
-            st.rr0 = tot[1];
-            st.rr = tot[1];
-            st.bb = tot[2];
-            st.iters = 0;
-            st.status = 0;
-            st.done = 0;
-            st.beta = 0.0;
-            if (!finite(tot[0]) || tot[0] < 0.0) {
-                st.status = HZO_PCG_BAD_INITIAL;
-                st.done = 1;
-            } else if (!keep_going(st, tot[0], tot[1])) {
-                st.done = 1;
-                if (st.iters >= st.max_iter) st.status = HZO_PCG_MAXITER;
-            }
-        }
+    write_block_partials<3>(acc, b.part[0], nb);
+    if (block0_thread0()) {
+        init.kind = K_SETUP;
+        b.st[0] = init;
     }
 }
 
 // ---------------------------------------------------------------- PCG direction + SpMV
 // p_new = z + beta*p_old (first iteration: p_new = z), q = A p_new, p_new.q.
-// z = D^-1 r is recomputed from r and the code wherever it is needed
-// (centre and the six neighbours), so z is never stored.
-__device__ __forceinline__ double pnew_at(const Fp64Bufs& b, const double* dinv, long long v,
-                                          const double* pold, double beta, bool first) {
-    const double z = b.r[v] * dinv[b.sc[v]];
-    return first ? z : z + beta * pold[v];
-}
-
-__global__ __launch_bounds__(256) void k_pcg_dir_f64(Fp64Bufs b, StepConsts c, int nb,
-                                                     const double* __restrict__ pold,
-                                                     double* __restrict__ pnew, int first) {
-    if (b.st->done) return;
+// z = D^-1 r is formed from r and the stencil code wherever it is staged, so
+// z is never stored.
+__global__ __launch_bounds__(NT) void k_pcg_dir_f64(Fp64Bufs b, StepConsts c, int nb, int t) {
+    const int ip = (t - 1) & 1, io = t & 1;
+    PcgState s = b.st[ip];
+    if (s.done) {
+        pass_state(s, b.st + io);
+        return;
+    }
+    const bool first = s.kind == K_SETUP;
     __shared__ double sH[16];
-    load_lut(sH, b.lut);
+    __shared__ double ring[3][HY][HX];
     const int nx = c.nx, ny = c.ny, nz = c.nz;
-    const int i = blockIdx.x * BX + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    const int k0 = blockIdx.z * KZ;
-    const double beta = b.st->beta;
-    const bool fst = first != 0;
+    const int k0 = blockIdx.z * c.kz, k1 = min(k0 + c.kz, nz);
+    const TilePos tp = tile_pos(c);
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const double* pold = b.p[s.pbuf];
+    double* pnew = b.p[s.pbuf ^ 1];
+    double beta = 0;  // set once the previous inner products are finished
+    // staged operand: p_new = z + beta*p_old with z = D^-1 r (first: p_new = z)
+    double rc_ = 0.0, pc_ = 0.0, rh = 0.0, ph = 0.0;
+    unsigned cc_ = 0, ch = 0;
+    auto fetch = [&](int kk) {
+        const long long id = vid(tp.ci, tp.cj, kk, nx, ny);
+        rc_ = b.r[id];
+        cc_ = b.sc[id];
+        if (!first) pc_ = pold[id];
+        if (tp.has_h) {
+            const long long h = vid(tp.gi, tp.gj, kk, nx, ny);
+            rh = b.r[h];
+            ch = b.sc[h];
+            if (!first) ph = pold[h];
+        }
+    };
+    auto make = [&](int kk) -> unsigned {
+        double(*S)[HX] = ring[kk % 3];
+        const double z = rc_ * dinv_of(cc_, sH, c);
+        S[ty + 1][tx + 1] = first ? z : z + beta * pc_;
+        if (tp.has_h) {
+            const double zh = rh * dinv_of(ch, sH, c);
+            S[tp.hy][tp.hx] = first ? zh : zh + beta * ph;
+        }
+        return cc_;
+    };
+    int lo, hi;
+    plane_window(c, k0, k1, lo, hi);
     double acc[1] = {0.0};
-    if (i < nx && j < ny) {
-        const double* dinv = b.lut + LUT_DINV;
-        const int k1 = min(k0 + KZ, nz);
-        for (int k = k0; k < k1; ++k) {
-            const Nbr v = op_nbrs(i, j, k, c);
-            const unsigned code = b.sc[v.c];
-            const double pc = pnew_at(b, dinv, v.c, pold, beta, fst);
-            double q;
-            if (code & SC_IDENTITY) {
-                q = op_identity(pc, k, c);
-            } else {
-                q = op_row(code, sH, c, pc, pnew_at(b, dinv, v.e, pold, beta, fst),
-                           pnew_at(b, dinv, v.w, pold, beta, fst), pnew_at(b, dinv, v.n, pold, beta, fst),
-                           pnew_at(b, dinv, v.s, pold, beta, fst), pnew_at(b, dinv, v.u, pold, beta, fst),
-                           pnew_at(b, dinv, v.d, pold, beta, fst));
+    fetch(lo);  // the first plane's loads overlap the reduction below
+    // finish the previous kernel's inner products (setup: r.z, r.r, b.b;
+    // update: r.z, r.r) and apply the loop test (depol_varEPS.cu:459-474, 506-521)
+    if (first) {
+        double tot[3];
+        sum_partials<3>(b.part[ip], nb, tot);
+        s.rz0 = s.rz_old = tot[0];
+        s.rr0 = s.rr = tot[1];
+        s.bb = tot[2];
+        if (!finite(tot[0]) || tot[0] < 0.0) {  // :466-469
+            s.status = HZO_PCG_BAD_INITIAL;
+            s.done = 1;
+        } else {
+            judge(s, tot[0], tot[1]);
+        }
+    } else {
+        double tot[2];
+        sum_partials<2>(b.part[ip], nb, tot);
+        s.rr = tot[1];
+        if (!finite(tot[0]) || tot[0] < 0.0) {  // :508-513
+            s.status = HZO_PCG_BAD_RZ;
+            s.rz_old = tot[0];
+            s.done = 1;
+        } else {
+            s.beta = tot[0] / s.rz_old;
+            s.rz_old = tot[0];
+            judge(s, tot[0], tot[1]);
+        }
+    }
+    if (s.done) {
+        pass_state(s, b.st + io);
+        return;
+    }
+    beta = (double)s.beta;
+    load_lut(sH, b.lut);
+    unsigned code_k = make(lo), code_next = 0;
+    int kn = lo + 1;  // next plane to stage
+    if (kn <= hi) fetch(kn);
+    for (int k = k0; k < k1; ++k) {
+        const int need = min(k + 1, hi);
+        while (kn <= need) {
+            const unsigned cm = make(kn);
+            if (kn == k) code_k = cm;
+            else code_next = cm;
+            ++kn;
+            if (kn <= hi) fetch(kn);  // in flight while plane k is evaluated
+        }
+        __syncthreads();
+        double(*S)[HX] = ring[k % 3];
+        const double pc = S[ty + 1][tx + 1];
+        double q;
+        if (code_k & SC_IDENTITY) {
+            q = op_identity(pc, k, c);
+        } else {
+            int up = k + 1, dn = k - 1;
+            if (c.zbc != 0) {
+                if (k == 0) up = dn = 1;
+                else if (k == nz - 1) up = dn = nz - 2;
             }
-            pnew[v.c] = pc;
+            q = op_row(code_k, sH, c, pc, S[ty + 1][tx + 2], S[ty + 1][tx], S[ty + 2][tx + 1], S[ty][tx + 1],
+                       ring[up % 3][ty + 1][tx + 1], ring[dn % 3][ty + 1][tx + 1]);
+        }
+        if (tp.live) {
+            pnew[vid(tp.i, tp.j, k, nx, ny)] = pc;
             acc[0] += pc * q;
         }
+        __syncthreads();
+        code_k = code_next;
     }
-    double tot[1];
-    if (grid_reduce_last<1>(acc, b.partials, nb, b.counters + 2, tot)) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            PcgState& st = *b.st;
-            st.iters += 1;
-            st.denom = tot[0];
-            if (!finite(tot[0]) || !(tot[0] > 0.0)) {  // depol_varEPS.cu:486-490
-                st.status = HZO_PCG_BREAKDOWN;
-                st.done = 1;
-            } else {
-                st.alpha = st.rz_old / tot[0];
-            }
-        }
-    }
+    write_block_partials<1>(acc, b.part[io], nb);
+    s.iters += 1;  // ++kiter (:476)
+    s.pbuf ^= 1;
+    s.kind = K_DIR;
+    pass_state(s, b.st + io);
 }
 
 // ---------------------------------------------------------------- PCG update
-__global__ __launch_bounds__(256) void k_pcg_update_f64(Fp64Bufs b, StepConsts c, int nb,
-                                                        const double* __restrict__ p) {
-    if (b.st->done) return;
+__global__ __launch_bounds__(NT) void k_pcg_update_f64(Fp64Bufs b, StepConsts c, int nb, int t) {
+    const int ip = (t - 1) & 1, io = t & 1;
+    PcgState s = b.st[ip];
+    if (s.done) {
+        pass_state(s, b.st + io);
+        return;
+    }
     __shared__ double sH[16];
-    load_lut(sH, b.lut);
+    __shared__ double ring[3][HY][HX];
     const int nx = c.nx, ny = c.ny, nz = c.nz;
-    const int i = blockIdx.x * BX + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    const int k0 = blockIdx.z * KZ;
-    const double alpha = b.st->alpha;
-    const double nalpha = -alpha;
+    const int k0 = blockIdx.z * c.kz, k1 = min(k0 + c.kz, nz);
+    const TilePos tp = tile_pos(c);
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const double* p = b.p[s.pbuf];
+    double pcv = 0.0, phv = 0.0;
+    auto fetch = [&](int kk) {
+        pcv = p[vid(tp.ci, tp.cj, kk, nx, ny)];
+        if (tp.has_h) phv = p[vid(tp.gi, tp.gj, kk, nx, ny)];
+    };
+    auto make = [&](int kk) {
+        ring[kk % 3][ty + 1][tx + 1] = pcv;
+        if (tp.has_h) ring[kk % 3][tp.hy][tp.hx] = phv;
+    };
+    // element-wise operands of the centre voxel, one plane ahead
+    struct Ctr {
+        double x, r;
+        unsigned code;
+    };
+    auto fetch_c = [&](int kk) {
+        const long long id = vid(tp.ci, tp.cj, kk, nx, ny);
+        Ctr v;
+        v.x = b.phi[id];
+        v.r = b.r[id];
+        v.code = b.sc[id];
+        return v;
+    };
+    int lo, hi;
+    plane_window(c, k0, k1, lo, hi);
     double acc[2] = {0.0, 0.0};  // r.z, r.r
-    if (i < nx && j < ny) {
-        const double* dinv = b.lut + LUT_DINV;
-        const int k1 = min(k0 + KZ, nz);
-        for (int k = k0; k < k1; ++k) {
-            const Nbr v = op_nbrs(i, j, k, c);
-            const unsigned code = b.sc[v.c];
-            const double pc = p[v.c];
-            double q;
-            if (code & SC_IDENTITY)
-                q = op_identity(pc, k, c);
-            else
-                q = op_row(code, sH, c, pc, p[v.e], p[v.w], p[v.n], p[v.s], p[v.u], p[v.d]);
-            b.phi[v.c] = alpha * pc + b.phi[v.c];
-            const double r = nalpha * q + b.r[v.c];
-            b.r[v.c] = r;
-            const double z = r * dinv[code];
+    fetch(lo);  // loads of the first planes overlap the reduction below
+    Ctr cur = fetch_c(k0), nxt = cur;
+    double tot[1];
+    sum_partials<1>(b.part[ip], nb, tot);  // p.Ap (:485)
+    s.denom = tot[0];
+    if (!finite(tot[0]) || !(tot[0] > 0.0)) {  // :486-490
+        s.status = HZO_PCG_BREAKDOWN;
+        s.done = 1;
+        pass_state(s, b.st + io);
+        return;
+    }
+    s.alpha = s.rz_old / tot[0];
+    const double alpha = s.alpha;
+    const double nalpha = -alpha;
+    load_lut(sH, b.lut);
+    make(lo);
+    int kn = lo + 1;
+    if (kn <= hi) fetch(kn);
+    for (int k = k0; k < k1; ++k) {
+        const int need = min(k + 1, hi);
+        while (kn <= need) {
+            make(kn);
+            ++kn;
+            if (kn <= hi) fetch(kn);
+        }
+        if (k + 1 < k1) nxt = fetch_c(k + 1);
+        __syncthreads();
+        double(*S)[HX] = ring[k % 3];
+        const double pc = S[ty + 1][tx + 1];
+        double q;
+        if (cur.code & SC_IDENTITY) {
+            q = op_identity(pc, k, c);
+        } else {
+            int up = k + 1, dn = k - 1;
+            if (c.zbc != 0) {
+                if (k == 0) up = dn = 1;
+                else if (k == nz - 1) up = dn = nz - 2;
+            }
+            q = op_row(cur.code, sH, c, pc, S[ty + 1][tx + 2], S[ty + 1][tx], S[ty + 2][tx + 1], S[ty][tx + 1],
+                       ring[up % 3][ty + 1][tx + 1], ring[dn % 3][ty + 1][tx + 1]);
+        }
+        if (tp.live) {
+            const long long id = vid(tp.i, tp.j, k, nx, ny);
+            b.phi[id] = alpha * pc + cur.x;         // Daxpy x += alpha p (:495)
+            const double r = nalpha * q + cur.r;    // Daxpy r -= alpha Ap (:499)
+            b.r[id] = r;
+            const double z = r * dinv_of(cur.code, sH, c);  // :502
             acc[0] += r * z;
             acc[1] += r * r;
         }
+        __syncthreads();
+        cur = nxt;
     }
-    double tot[2];
-    if (grid_reduce_last<2>(acc, b.partials, nb, b.counters + 3, tot)) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            PcgState& st = *b.st;
-            const double rz_new = tot[0];
-            st.rr = tot[1];
-            if (!finite(rz_new) || rz_new < 0.0) {  // depol_varEPS.cu:508-513
-                st.status = HZO_PCG_BAD_RZ;
-                st.rz_old = rz_new;
-                st.done = 1;
-            } else {
-                st.beta = rz_new / st.rz_old;
-                st.rz_old = rz_new;
-                if (!keep_going(st, rz_new, tot[1])) {
-                    st.done = 1;
-                    if (st.iters >= st.max_iter) st.status = HZO_PCG_MAXITER;
-                }
-            }
-        }
-    }
+    write_block_partials<2>(acc, b.part[io], nb);
+    s.kind = K_UPDATE;
+    pass_state(s, b.st + io);
 }
 
 // ---------------------------------------------------------------- diagnostics
 // out8 = sum Pmz, sum Psz, then FE, DE, AFE: sum(mask*Pmz), sum(mask)
 // (evolve.cu:259-266, 280-357; build_mat_mask_and_p kernels_evol.cu:23-44).
-__global__ __launch_bounds__(256) void k_diag_f64(const double* __restrict__ P,
-                                                  const uint32_t* __restrict__ cw, Grid g,
-                                                  double* partials, unsigned* counter, double* out,
+__global__ __launch_bounds__(NT) void k_diag_f64(const double* __restrict__ P,
+                                                  const uint32_t* __restrict__ cw, Grid g, double* part,
                                                   int nb) {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long N = g.N;
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
-    const int k0 = blockIdx.z * KZ;
-    double a[4] = {0.0, 0.0, 0.0, 0.0};
-    double m[4] = {0.0, 0.0, 0.0, 0.0};
+    const int k0 = blockIdx.z * g.kz;
+    // sum Pmz, sum Psz, FE: mask*Pmz, mask; DE: mask*Pmz, mask; AFE: mask*Pmz, mask
+    double a[DIAG_Q] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (i < nx && j < ny) {
-        const int k1 = min(k0 + KZ, nz);
+        const int k1 = min(k0 + g.kz, nz);
         for (int k = k0; k < k1; ++k) {
             const long long id = vid(i, j, k, nx, ny);
             const double pmz = P[2 * N + id];
@@ -375,61 +461,36 @@ __global__ __launch_bounds__(256) void k_diag_f64(const double* __restrict__ P,
             const double mde = (phase == 2) ? 1.0 : 0.0;
             const double mafe = (phase == 0) ? 1.0 : 0.0;
             a[2] += mfe * pmz;
-            a[3] += mde * pmz;
-            m[0] += mafe * pmz;
-            m[1] += mfe;
-            m[2] += mde;
-            m[3] += mafe;
+            a[3] += mfe;
+            a[4] += mde * pmz;
+            a[5] += mde;
+            a[6] += mafe * pmz;
+            a[7] += mafe;
         }
     }
-    double v1[4] = {a[0], a[1], a[2], a[3]};
-    double t1[4];
-    if (grid_reduce_last<4>(v1, partials, nb, counter, t1)) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            out[0] = t1[0];
-            out[1] = t1[1];
-            out[2] = t1[2];
-            out[4] = t1[3];
-        }
-    }
-    double t2[4];
-    if (grid_reduce_last<4>(m, partials + 4 * (long long)nb, nb, counter + 1, t2)) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            out[6] = t2[0];
-            out[3] = t2[1];
-            out[5] = t2[2];
-            out[7] = t2[3];
-        }
-    }
+    write_block_partials<DIAG_Q>(a, part, nb);
 }
 
 // ---------------------------------------------------------------- launchers
-void launch_tdgl_f64(hipStream_t s, const Grid& g, const StepConsts& c, double Ez_ext,
-                     const Fp64Bufs& b) {
+void launch_tdgl_f64(hipStream_t s, const Grid& g, const StepConsts& c, double Ez_ext, const Fp64Bufs& b) {
     hipLaunchKernelGGL(k_tdgl_f64, grid3(g), dim3(BX, BY, 1), 0, s, b, c, Ez_ext, blocks_for(g));
 }
 
-void launch_pcg_setup_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b) {
-    hipLaunchKernelGGL(k_pcg_setup_f64, grid3(g), dim3(BX, BY, 1), 0, s, b, c, blocks_for(g));
+void launch_pcg_setup_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b,
+                          const PcgState& init) {
+    hipLaunchKernelGGL(k_pcg_setup_f64, grid3(g), dim3(BX, BY, 1), 0, s, b, c, blocks_for(g), init);
 }
 
-void launch_pcg_dir_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int it) {
-    const double* pold = b.p[it & 1];
-    double* pnew = b.p[(it + 1) & 1];
-    hipLaunchKernelGGL(k_pcg_dir_f64, grid3(g), dim3(BX, BY, 1), 0, s, b, c, blocks_for(g), pold,
-                       pnew, it == 1 ? 1 : 0);
+void launch_pcg_dir_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t) {
+    hipLaunchKernelGGL(k_pcg_dir_f64, grid3(g), dim3(BX, BY, 1), 0, s, b, c, blocks_for(g), t);
 }
 
-void launch_pcg_update_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b,
-                           int it) {
-    hipLaunchKernelGGL(k_pcg_update_f64, grid3(g), dim3(BX, BY, 1), 0, s, b, c, blocks_for(g),
-                       (const double*)b.p[(it + 1) & 1]);
+void launch_pcg_update_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t) {
+    hipLaunchKernelGGL(k_pcg_update_f64, grid3(g), dim3(BX, BY, 1), 0, s, b, c, blocks_for(g), t);
 }
 
-void launch_diag_f64(hipStream_t s, const Grid& g, const double* P, const uint32_t* cw,
-                     double* partials, unsigned* counters, double* out8) {
-    hipLaunchKernelGGL(k_diag_f64, grid3(g), dim3(BX, BY, 1), 0, s, P, cw, g, partials, counters,
-                       out8, blocks_for(g));
+void launch_diag_f64(hipStream_t s, const Grid& g, const double* P, const uint32_t* cw, double* part) {
+    hipLaunchKernelGGL(k_diag_f64, grid3(g), dim3(BX, BY, 1), 0, s, P, cw, g, part, blocks_for(g));
 }
 
 }  // namespace hzo

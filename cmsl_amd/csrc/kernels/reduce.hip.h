@@ -1,10 +1,13 @@
-// reduce.hip.h -- deterministic wave64 / workgroup / grid reductions.
+// reduce.hip.h -- deterministic wave64 / workgroup reductions and the
+// fence-free grid reduction protocol.
 //
-// A grid reduction is finished by the last workgroup to arrive (completion
-// counter), which adds the per-workgroup partial sums in a fixed order.  The
-// result therefore does not depend on workgroup scheduling, and the scalar
-// never leaves the device.  Partials and the counter use agent-scope atomics
-// so that workgroups on different XCDs (separate L2s) see each other's data.
+// A kernel that produces an inner product writes one partial sum per
+// workgroup (plain stores).  The NEXT kernel on the stream finishes the
+// reduction: every one of its workgroups adds the partials in the same fixed
+// order, so all workgroups obtain bit-identical totals without atomics or
+// memory fences (the kernel boundary makes the partials visible on every
+// XCD).  Partials and the PCG control block are double-buffered by launch
+// parity, so a kernel never overwrites what its own workgroups still read.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -20,17 +23,18 @@ __device__ __forceinline__ int linear_block() {
     return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
 }
 
-// Reduces NQ per-thread values over the whole grid.  Returns true in every
-// thread of the last workgroup, with tot[] holding the grid totals.
+__device__ __forceinline__ int linear_tid() {
+    return threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+}
+
+// Workgroup sum of NQ per-thread values; thread 0 stores them at
+// out[q*nb + block].
 template <int NQ>
-__device__ bool grid_reduce_last(const double (&v)[NQ], double* partials, int nb,
-                                 unsigned* counter, double (&tot)[NQ]) {
+__device__ void write_block_partials(const double (&v)[NQ], double* out, int nb) {
     __shared__ double sred[NQ][16];
-    __shared__ int s_last;
-    const int nthreads = blockDim.x * blockDim.y * blockDim.z;
-    const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
-    const int lane = tid & 63, wid = tid >> 6, nw = nthreads >> 6;
-    const int b = linear_block();
+    const int tid = linear_tid();
+    const int lane = tid & 63, wid = tid >> 6;
+    const int nw = (blockDim.x * blockDim.y * blockDim.z) >> 6;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         const double w = wave_sum(v[q]);
@@ -38,31 +42,31 @@ __device__ bool grid_reduce_last(const double (&v)[NQ], double* partials, int nb
     }
     __syncthreads();
     if (tid == 0) {
+        const int b = linear_block();
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             double s = 0.0;
             for (int w = 0; w < nw; ++w) s += sred[q][w];
-            __hip_atomic_store(&partials[(long long)q * nb + b], s, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+            out[(long long)q * nb + b] = s;
         }
-        __threadfence();
-        const unsigned prev =
-            __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = (prev == (unsigned)nb - 1u);
     }
-    __syncthreads();
-    if (!s_last) return false;
-    __threadfence();
+}
+
+// Totals of the NQ partial arrays written by the previous kernel; identical
+// in every thread of every workgroup.
+template <int NQ>
+__device__ void sum_partials(const double* in, int nb, double (&tot)[NQ]) {
+    __shared__ double sred[NQ][16];
+    const int tid = linear_tid();
+    const int nthreads = blockDim.x * blockDim.y * blockDim.z;
+    const int lane = tid & 63, wid = tid >> 6, nw = nthreads >> 6;
     double acc[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
     for (int t = tid; t < nb; t += nthreads) {
 #pragma unroll
-        for (int q = 0; q < NQ; ++q)
-            acc[q] += __hip_atomic_load(&partials[(long long)q * nb + t], __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT);
+        for (int q = 0; q < NQ; ++q) acc[q] += in[(long long)q * nb + t];
     }
-    __syncthreads();
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         const double w = wave_sum(acc[q]);
@@ -75,8 +79,7 @@ __device__ bool grid_reduce_last(const double (&v)[NQ], double* partials, int nb
         for (int w = 0; w < nw; ++w) s += sred[q][w];
         tot[q] = s;
     }
-    if (tid == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return true;
+    __syncthreads();
 }
 
 }  // namespace hzo

@@ -82,6 +82,7 @@ typedef struct hzo_config {
     double tol_abs, tol_rel;
     int max_iter;
     int refine_every;    /* FP32-refined: FP64 residual recompute period (iterations) */
+    int tile_kz;         /* z planes per workgroup of the stencil kernels; 0 = automatic */
 } hzo_config;
 
 typedef struct hzo_pcg_stats {
@@ -142,6 +143,8 @@ int hzo_diagnostics(hzo_ctx* ctx, double out[10]);
 int hzo_synchronize(hzo_ctx* ctx);
 /* Grid dimensions of a context: dims[3] = nx, ny, nz. */
 int hzo_grid_dims(hzo_ctx* ctx, int dims[3]);
+/* Tile depth (z planes per workgroup) the context launches with. */
+int hzo_tile_depth(hzo_ctx* ctx);
 
 /* ---------------------------------------------------------------- sweep driver
  * The staircase P-E loop of Evolve() (evolve.cu:213-410): the field starts at

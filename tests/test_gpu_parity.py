@@ -117,7 +117,22 @@ def test_run_parity_neumann_gauge():
     assert rel_err(phig, phio) < 1e-9
 
 
-def test_run_parity_odd_sizes_and_void_voxels():
+def test_run_parity_odd_sizes():
+    # grid not a multiple of the 64x4 tile, nz odd
+    inp = film_inputs(37, 29, 7, 11, fracs=(0.3, 0.4, 0.3), seed=31, jitter=0.2)
+    Ez = inp.field_nd(H.triangle_wave(40, 2.0, 12))
+    _say("run 37x29x7 12 steps")
+    its_g, Pg, phig, its_o, Po, phio = _run_pair(inp, Ez)
+    assert np.array_equal(its_g, its_o)
+    assert rel_err(Pg, Po) < TOL
+    assert rel_err(phig, phio) < TOL
+
+
+def test_void_voxels_tdgl_bitexact():
+    # Void voxels (omega < 0) are skipped by the TDGL kernel and read as 0.
+    # With eps = 0 they also make the Poisson matrix singular (zero rows), a
+    # degenerate system the reference cannot solve either, so this case runs
+    # without the depolarisation solve.
     inp = film_inputs(37, 29, 7, 11, fracs=(0.3, 0.4, 0.3), seed=31, jitter=0.2)
     om = inp.omega.copy()
     om[np.random.default_rng(4).random(inp.N) < 0.05] = -1
@@ -125,11 +140,9 @@ def test_run_parity_odd_sizes_and_void_voxels():
     inp.iface = H.film.interface_flag(inp.nx, inp.ny, inp.nz, om, inp.grain_phase)
     inp.P0 = H.film.init_polarization(inp.nx, inp.ny, inp.nz, om, inp.grain_phase, 0.0124, 9)
     Ez = inp.field_nd(H.triangle_wave(40, 2.0, 12))
-    _say("run 37x29x7 void 12 steps")
-    its_g, Pg, phig, its_o, Po, phio = _run_pair(inp, Ez)
-    assert np.array_equal(its_g, its_o)
-    assert rel_err(Pg, Po) < TOL
-    assert rel_err(phig, phio) < TOL
+    _say("run 37x29x7 void dep_flag=0 12 steps")
+    its_g, Pg, phig, its_o, Po, phio = _run_pair(inp, Ez, dep_flag=0)
+    assert np.array_equal(Pg, Po)
     assert np.all(Pg.reshape(6, -1)[:, om < 0] == 0.0)
 
 
