@@ -58,7 +58,7 @@ def run_gpu(args, inp, precision):
 
     prec = L.PREC_FP64 if precision == "fp64" else L.PREC_FP32_REFINED
     film = H.Film(inp, precision=prec, pcg_rule=L.PCG_RULE_RELRES, tol_rel=1e-8, tol_abs=1e-30,
-                  refine_every=20)
+                  refine_every=20, tile_kz=args.tile_kz)
     Ez = inp.field_nd(H.triangle_wave(PERIOD, AMP, args.warmup + args.steps))
     film.run(Ez[: args.warmup])
     film.synchronize()
@@ -104,6 +104,7 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=4)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tile-kz", type=int, default=0, help="z planes per workgroup (0: automatic)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the film fits one MI355X); --gpus {args.gpus} "

@@ -442,7 +442,8 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
     if (cfg->precision != HZO_PREC_FP64 && cfg->precision != HZO_PREC_FP32_REFINED)
         return set_error(HZO_ERR_ARG, "hzo_create: precision %d", cfg->precision);
     if (cfg->refine_every < 0) return set_error(HZO_ERR_ARG, "hzo_create: refine_every < 0");
-    if (cfg->tile_kz < 0) return set_error(HZO_ERR_ARG, "hzo_create: tile_kz < 0");
+    if (cfg->tile_kz != 0 && cfg->tile_kz != 2 && cfg->tile_kz != 4 && cfg->tile_kz != 8)
+        return set_error(HZO_ERR_ARG, "hzo_create: tile_kz must be 0, 2, 4 or 8");
     if (cfg->pcg_rule != 0 && cfg->pcg_rule != 1) return set_error(HZO_ERR_ARG, "hzo_create: pcg_rule %d", cfg->pcg_rule);
     if (cfg->max_iter < 0) return set_error(HZO_ERR_ARG, "hzo_create: max_iter < 0");
     const long long N = (long long)cfg->nx * cfg->ny * cfg->nz;
@@ -462,9 +463,16 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
     // dot products, follows the tiling).
     {
         const long long txy = (long long)((cfg->nx + 63) / 64) * ((cfg->ny + 7) / 8);
-        int kz = cfg->tile_kz > 0 ? cfg->tile_kz : (int)((txy * cfg->nz + 1023) / 1024);
-        if (kz < 1) kz = 1;
-        if (kz > cfg->nz) kz = cfg->nz;
+        int kz = cfg->tile_kz;
+        if (kz == 0) {  // smallest depth giving at most ~1024 workgroups
+            kz = 8;
+            for (int cand : {2, 4}) {
+                if (txy * ((cfg->nz + cand - 1) / cand) <= 1024) {
+                    kz = cand;
+                    break;
+                }
+            }
+        }
         x->g.kz = kz;
     }
     hzo::StepConsts& c = x->c;

@@ -72,9 +72,58 @@ __device__ __forceinline__ bool block0_thread0() {
     return blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 && threadIdx.y == 0;
 }
 
-// Copies the control block forward unchanged (used once the solve is done).
+// Field-wise copies of the control block.  Aggregate copies would be lowered
+// to memcpy through private memory; field by field it stays in registers.
+__device__ __forceinline__ PcgState load_state(const PcgState* p) {
+    PcgState s;
+    s.rz_old = p->rz_old;
+    s.rz0 = p->rz0;
+    s.rr = p->rr;
+    s.rr0 = p->rr0;
+    s.bb = p->bb;
+    s.alpha = p->alpha;
+    s.beta = p->beta;
+    s.denom = p->denom;
+    s.iters = p->iters;
+    s.status = p->status;
+    s.done = p->done;
+    s.refinements = p->refinements;
+    s.max_iter = p->max_iter;
+    s.rule = p->rule;
+    s.kind = p->kind;
+    s.pbuf = p->pbuf;
+    s.tol_abs2 = p->tol_abs2;
+    s.tol_rel2 = p->tol_rel2;
+    s.refine_every = p->refine_every;
+    s.pad0 = s.pad1 = s.pad2 = 0;
+    return s;
+}
+
+__device__ __forceinline__ void store_state(const PcgState& s, PcgState* p) {
+    p->rz_old = s.rz_old;
+    p->rz0 = s.rz0;
+    p->rr = s.rr;
+    p->rr0 = s.rr0;
+    p->bb = s.bb;
+    p->alpha = s.alpha;
+    p->beta = s.beta;
+    p->denom = s.denom;
+    p->iters = s.iters;
+    p->status = s.status;
+    p->done = s.done;
+    p->refinements = s.refinements;
+    p->max_iter = s.max_iter;
+    p->rule = s.rule;
+    p->kind = s.kind;
+    p->pbuf = s.pbuf;
+    p->tol_abs2 = s.tol_abs2;
+    p->tol_rel2 = s.tol_rel2;
+    p->refine_every = s.refine_every;
+}
+
+// Copies the control block forward (workgroup 0, thread 0).
 __device__ __forceinline__ void pass_state(const PcgState& s, PcgState* out) {
-    if (block0_thread0()) *out = s;
+    if (block0_thread0()) store_state(s, out);
 }
 
 // The stopping test after a completed reduction of r.z and r.r (the
@@ -90,14 +139,25 @@ __device__ __forceinline__ void judge(PcgState& s, double rz, double rr) {
 // face table H (depol_varEPS.cu:177-179: D = sum of the six face means / h^2,
 // D^-1 = 1/D, or 1 when D <= 1e-30 and on identity rows).  In FP64 this is
 // the same sequence of IEEE operations as the reference.
-template <class T>
-__device__ __forceinline__ T dinv_of(unsigned code, const T* H, const StepConsts& c) {
-    if (code & SC_IDENTITY) return (T)1;
-    const T* Hr = H + (code & 3u) * 4u;
-    const T sum = Hr[(code >> 2) & 3u] + Hr[(code >> 4) & 3u] + Hr[(code >> 6) & 3u] + Hr[(code >> 8) & 3u] +
-                  Hr[(code >> 10) & 3u] + Hr[(code >> 12) & 3u];
-    const T D = c.h2_pow2 ? sum * (T)c.inv_h2 : sum / (T)c.h2;
-    return D > (T)1e-30 ? (T)1 / D : (T)1;
+__device__ __forceinline__ double dinv_of(unsigned code, const double* H, const StepConsts& c) {
+    if (code & SC_IDENTITY) return 1.0;
+    const double* Hr = H + (code & 3u) * 4u;
+    const double sum = Hr[(code >> 2) & 3u] + Hr[(code >> 4) & 3u] + Hr[(code >> 6) & 3u] + Hr[(code >> 8) & 3u] +
+                       Hr[(code >> 10) & 3u] + Hr[(code >> 12) & 3u];
+    const double D = c.h2_pow2 ? sum * c.inv_h2 : sum / c.h2;
+    return D > 1e-30 ? 1.0 / D : 1.0;
+}
+
+// FP32 form: the hardware reciprocal (1 ulp) replaces the division; every
+// FP32 kernel uses this same function, so the preconditioner is one fixed
+// diagonal matrix throughout a solve.
+__device__ __forceinline__ float dinv_of(unsigned code, const float* H, const StepConsts& c) {
+    if (code & SC_IDENTITY) return 1.0f;
+    const float* Hr = H + (code & 3u) * 4u;
+    const float sum = Hr[(code >> 2) & 3u] + Hr[(code >> 4) & 3u] + Hr[(code >> 6) & 3u] + Hr[(code >> 8) & 3u] +
+                      Hr[(code >> 10) & 3u] + Hr[(code >> 12) & 3u];
+    const float D = sum * (float)c.inv_h2;
+    return D > 1e-30f ? __builtin_amdgcn_rcpf(D) : 1.0f;
 }
 
 template <class T>

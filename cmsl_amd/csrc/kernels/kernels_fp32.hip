@@ -176,65 +176,55 @@ __global__ __launch_bounds__(NT) void k_pcg_setup_f32(Fp32Bufs b, StepConsts c, 
         }
     }
     write_block_partials<3>(acc, b.part[0], nb);
-    if (block0_thread0()) {
-        init.kind = K_SETUP;
-        b.st[0] = init;
-    }
+    init.kind = K_SETUP;
+    pass_state(init, b.st);
 }
 
 // ---------------------------------------------------------------- direction / fold
 // Finishes the previous kernel's inner products and decides what this launch
 // is: a CG direction update (K_DIR) or the fold x_hi += x_lo that starts a
-// residual replacement (K_FOLD).
+// residual replacement (K_FOLD).  Loads of all KZ+2 planes of the tile are
+// issued before the reduction prologue (one memory round trip per workgroup).
+template <int KZ>
 __global__ __launch_bounds__(NT) void k_pcg_dir_f32(Fp32Bufs b, StepConsts c, int nb, int t) {
     const int ip = (t - 1) & 1, io = t & 1;
-    PcgState s = b.st[ip];
+    PcgState s = load_state(b.st + ip);
     if (s.done) {
         pass_state(s, b.st + io);
         return;
     }
     const bool first = s.kind == K_SETUP;
-    bool fold = false;
-    const int nx = c.nx, ny = c.ny, nz = c.nz;
-    const int i = blockIdx.x * BX + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    const int k0 = blockIdx.z * c.kz;
-    const int k1 = min(k0 + c.kz, nz);
     __shared__ float sH[16];
-    __shared__ float ring[3][HY][HX];
+    __shared__ float pl[KZ + 2][HY][HX];
+    const int nx = c.nx, ny = c.ny, nz = c.nz;
+    const int k0 = blockIdx.z * KZ;
     const TilePos tp = tile_pos(c);
     const int tx = threadIdx.x, ty = threadIdx.y;
     const float* pold = b.p[s.pbuf];
     float* pnew = b.p[s.pbuf ^ 1];
-    float beta = 0.f;  // set once the previous inner products are finished
-    float rc_ = 0.f, pc_ = 0.f, rh = 0.f, ph = 0.f;
-    unsigned cc_ = 0, ch = 0;
-    auto fetch = [&](int kk) {
-        const long long id = vid(tp.ci, tp.cj, kk, nx, ny);
-        rc_ = b.r[id];
-        cc_ = b.sc[id];
-        if (!first) pc_ = pold[id];
-        if (tp.has_h) {
-            const long long h = vid(tp.gi, tp.gj, kk, nx, ny);
-            rh = b.r[h];
-            ch = b.sc[h];
-            if (!first) ph = pold[h];
+    float rc[KZ + 2], pc[KZ + 2], rh[KZ + 2], ph[KZ + 2];
+    unsigned cc[KZ + 2], ch[KZ + 2];
+    const long long plane = (long long)nx * ny;
+    const long long cbase = vid(tp.ci, tp.cj, k0 - 1, nx, ny), hbase = vid(tp.gi, tp.gj, k0 - 1, nx, ny);
+#pragma unroll
+    for (int q = 0; q < KZ + 2; ++q) {
+        const int kk = k0 - 1 + q;
+        rc[q] = pc[q] = rh[q] = ph[q] = 0.f;
+        cc[q] = ch[q] = SC_IDENTITY;
+        if (kk >= 0 && kk < nz) {
+            const long long id = cbase + q * plane;
+            rc[q] = b.r[id];
+            cc[q] = b.sc[id];
+            if (!first) pc[q] = pold[id];
+            if (tp.has_h) {
+                const long long h = hbase + q * plane;
+                rh[q] = b.r[h];
+                ch[q] = b.sc[h];
+                if (!first) ph[q] = pold[h];
+            }
         }
-    };
-    auto make = [&](int kk) -> unsigned {
-        float(*S)[HX] = ring[kk % 3];
-        const float z = rc_ * dinv_of(cc_, sH, c);
-        S[ty + 1][tx + 1] = first ? z : z + beta * pc_;
-        if (tp.has_h) {
-            const float zh = rh * dinv_of(ch, sH, c);
-            S[tp.hy][tp.hx] = first ? zh : zh + beta * ph;
-        }
-        return cc_;
-    };
-    int lo, hi;
-    plane_window(c, k0, k1, lo, hi);
-    double acc[1] = {0.0};
-    fetch(lo);  // the first plane's loads overlap the reduction below
+    }
+    bool fold = false;
     if (first) {
         double tot[3];
         sum_partials<3>(b.part[ip], nb, tot);
@@ -279,52 +269,59 @@ __global__ __launch_bounds__(NT) void k_pcg_dir_f32(Fp32Bufs b, StepConsts c, in
         return;
     }
     if (fold) {
-        if (i < nx && j < ny) {
-            for (int k = k0; k < k1; ++k) {
-                const long long id = vid(i, j, k, nx, ny);
-                b.xhi[id] += (double)b.xlo[id];
-                b.xlo[id] = 0.0f;
+        if (tp.live) {
+#pragma unroll
+            for (int q = 0; q < KZ; ++q) {
+                const int k = k0 + q;
+                if (k < nz) {
+                    const long long id = vid(tp.i, tp.j, k, nx, ny);
+                    b.xhi[id] += (double)b.xlo[id];
+                    b.xlo[id] = 0.0f;
+                }
             }
         }
         s.kind = K_FOLD;
         pass_state(s, b.st + io);
         return;
     }
-    beta = (float)s.beta;
+    const float beta = (float)s.beta;
     load_lut(sH, b.lut32);
-    unsigned code_k = make(lo), code_next = 0;
-    int kn = lo + 1;
-    if (kn <= hi) fetch(kn);
-    for (int k = k0; k < k1; ++k) {
-        const int need = min(k + 1, hi);
-        while (kn <= need) {
-            const unsigned cm = make(kn);
-            if (kn == k) code_k = cm;
-            else code_next = cm;
-            ++kn;
-            if (kn <= hi) fetch(kn);
-        }
-        __syncthreads();
-        float(*S)[HX] = ring[k % 3];
-        const float pc = S[ty + 1][tx + 1];
-        float q;
-        if (code_k & SC_IDENTITY) {
-            q = op_identity(pc, k, c);
-        } else {
-            int up = k + 1, dn = k - 1;
-            if (c.zbc != 0) {
-                if (k == 0) up = dn = 1;
-                else if (k == nz - 1) up = dn = nz - 2;
+#pragma unroll
+    for (int q = 0; q < KZ + 2; ++q) {
+        const int kk = k0 - 1 + q;
+        if (kk >= 0 && kk < nz) {
+            const float z = rc[q] * dinv_of(cc[q], sH, c);
+            pl[q][ty + 1][tx + 1] = first ? z : z + beta * pc[q];
+            if (tp.has_h) {
+                const float zh = rh[q] * dinv_of(ch[q], sH, c);
+                pl[q][tp.hy][tp.hx] = first ? zh : zh + beta * ph[q];
             }
-            q = row_Ax<float>(code_k, sH, c, pc, S[ty + 1][tx + 2], S[ty + 1][tx], S[ty + 2][tx + 1],
-                              S[ty][tx + 1], ring[up % 3][ty + 1][tx + 1], ring[dn % 3][ty + 1][tx + 1]);
         }
-        if (tp.live) {
-            pnew[vid(tp.i, tp.j, k, nx, ny)] = pc;
-            acc[0] += (double)pc * (double)q;
+    }
+    __syncthreads();
+    double acc[1] = {0.0};
+#pragma unroll
+    for (int q = 1; q <= KZ; ++q) {
+        const int k = k0 - 1 + q;
+        if (k < nz) {
+            const float xc = pl[q][ty + 1][tx + 1];
+            float qv;
+            if (cc[q] & SC_IDENTITY) {
+                qv = op_identity(xc, k, c);
+            } else {
+                int qu = q + 1, qd = q - 1;
+                if (c.zbc != 0) {
+                    if (k == 0) qu = qd = q + 1;
+                    else if (k == nz - 1) qu = qd = q - 1;
+                }
+                qv = row_Ax<float>(cc[q], sH, c, xc, pl[q][ty + 1][tx + 2], pl[q][ty + 1][tx], pl[q][ty + 2][tx + 1],
+                                   pl[q][ty][tx + 1], pl[qu][ty + 1][tx + 1], pl[qd][ty + 1][tx + 1]);
+            }
+            if (tp.live) {
+                pnew[cbase + q * plane] = xc;
+                acc[0] += (double)xc * (double)qv;
+            }
         }
-        __syncthreads();
-        code_k = code_next;
     }
     write_block_partials<1>(acc, b.part[io], nb);
     s.iters += 1;
@@ -334,16 +331,17 @@ __global__ __launch_bounds__(NT) void k_pcg_dir_f32(Fp32Bufs b, StepConsts c, in
 }
 
 // ---------------------------------------------------------------- update / residual replacement
+template <int KZ>
 __global__ __launch_bounds__(NT) void k_pcg_update_f32(Fp32Bufs b, StepConsts c, int nb, int t) {
     const int ip = (t - 1) & 1, io = t & 1;
-    PcgState s = b.st[ip];
+    PcgState s = load_state(b.st + ip);
     if (s.done) {
         pass_state(s, b.st + io);
         return;
     }
     const bool resid = s.kind == K_FOLD;
     const int nx = c.nx, ny = c.ny, nz = c.nz;
-    const int k0 = blockIdx.z * c.kz, k1 = min(k0 + c.kz, nz);
+    const int k0 = blockIdx.z * KZ;
     double acc[2] = {0.0, 0.0};
     if (resid) {  // r = b - A x_hi in FP64 (x_lo was folded by the previous launch)
         __shared__ double sH64[16];
@@ -353,6 +351,7 @@ __global__ __launch_bounds__(NT) void k_pcg_update_f32(Fp32Bufs b, StepConsts c,
         const int i = blockIdx.x * BX + threadIdx.x;
         const int j = blockIdx.y * BY + threadIdx.y;
         if (i < nx && j < ny) {
+            const int k1 = min(k0 + KZ, nz);
             for (int k = k0; k < k1; ++k) {
                 const Nbr v = op_nbrs(i, j, k, c);
                 const unsigned code = b.sc[v.c];
@@ -365,35 +364,35 @@ __global__ __launch_bounds__(NT) void k_pcg_update_f32(Fp32Bufs b, StepConsts c,
         }
     } else {
         __shared__ float sH32[16];
-        __shared__ float ring[3][HY][HX];
+        __shared__ float pl[KZ + 2][HY][HX];
         const TilePos tp = tile_pos(c);
         const int tx = threadIdx.x, ty = threadIdx.y;
         const float* p = b.p[s.pbuf];
-        float pcv = 0.f, phv = 0.f;
-        auto fetch = [&](int kk) {
-            pcv = p[vid(tp.ci, tp.cj, kk, nx, ny)];
-            if (tp.has_h) phv = p[vid(tp.gi, tp.gj, kk, nx, ny)];
-        };
-        auto make = [&](int kk) {
-            ring[kk % 3][ty + 1][tx + 1] = pcv;
-            if (tp.has_h) ring[kk % 3][tp.hy][tp.hx] = phv;
-        };
-        struct Ctr {
-            float x, r;
-            unsigned code;
-        };
-        auto fetch_c = [&](int kk) {
-            const long long id = vid(tp.ci, tp.cj, kk, nx, ny);
-            Ctr v;
-            v.x = b.xlo[id];
-            v.r = b.r[id];
-            v.code = b.sc[id];
-            return v;
-        };
-        int lo, hi;
-        plane_window(c, k0, k1, lo, hi);
-        fetch(lo);  // loads of the first planes overlap the reduction below
-        Ctr cur = fetch_c(k0), nxt = cur;
+        float pc[KZ + 2], ph[KZ + 2], xv[KZ], rv[KZ];
+        unsigned cv[KZ];
+        const long long plane = (long long)nx * ny;
+        const long long cbase = vid(tp.ci, tp.cj, k0 - 1, nx, ny), hbase = vid(tp.gi, tp.gj, k0 - 1, nx, ny);
+#pragma unroll
+        for (int q = 0; q < KZ + 2; ++q) {
+            const int kk = k0 - 1 + q;
+            pc[q] = ph[q] = 0.f;
+            if (kk >= 0 && kk < nz) {
+                pc[q] = p[cbase + q * plane];
+                if (tp.has_h) ph[q] = p[hbase + q * plane];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < KZ; ++q) {
+            const int kk = k0 + q;
+            xv[q] = rv[q] = 0.f;
+            cv[q] = SC_IDENTITY;
+            if (kk < nz) {
+                const long long id = cbase + (q + 1) * plane;
+                xv[q] = b.xlo[id];
+                rv[q] = b.r[id];
+                cv[q] = b.sc[id];
+            }
+        }
         double tot[1];
         sum_partials<1>(b.part[ip], nb, tot);
         s.denom = tot[0];
@@ -406,43 +405,44 @@ __global__ __launch_bounds__(NT) void k_pcg_update_f32(Fp32Bufs b, StepConsts c,
         s.alpha = s.rz_old / tot[0];
         const float alpha = (float)s.alpha;
         load_lut(sH32, b.lut32);
-        make(lo);
-        int kn = lo + 1;
-        if (kn <= hi) fetch(kn);
-        for (int k = k0; k < k1; ++k) {
-            const int need = min(k + 1, hi);
-            while (kn <= need) {
-                make(kn);
-                ++kn;
-                if (kn <= hi) fetch(kn);
+#pragma unroll
+        for (int q = 0; q < KZ + 2; ++q) {
+            const int kk = k0 - 1 + q;
+            if (kk >= 0 && kk < nz) {
+                pl[q][ty + 1][tx + 1] = pc[q];
+                if (tp.has_h) pl[q][tp.hy][tp.hx] = ph[q];
             }
-            if (k + 1 < k1) nxt = fetch_c(k + 1);
-            __syncthreads();
-            float(*S)[HX] = ring[k % 3];
-            const float pc = S[ty + 1][tx + 1];
-            float q;
-            if (cur.code & SC_IDENTITY) {
-                q = op_identity(pc, k, c);
-            } else {
-                int up = k + 1, dn = k - 1;
-                if (c.zbc != 0) {
-                    if (k == 0) up = dn = 1;
-                    else if (k == nz - 1) up = dn = nz - 2;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 1; q <= KZ; ++q) {
+            const int k = k0 - 1 + q;
+            if (k < nz) {
+                const float xc = pl[q][ty + 1][tx + 1];
+                const unsigned code = cv[q - 1];
+                float qv;
+                if (code & SC_IDENTITY) {
+                    qv = op_identity(xc, k, c);
+                } else {
+                    int qu = q + 1, qd = q - 1;
+                    if (c.zbc != 0) {
+                        if (k == 0) qu = qd = q + 1;
+                        else if (k == nz - 1) qu = qd = q - 1;
+                    }
+                    qv = row_Ax<float>(code, sH32, c, xc, pl[q][ty + 1][tx + 2], pl[q][ty + 1][tx],
+                                       pl[q][ty + 2][tx + 1], pl[q][ty][tx + 1], pl[qu][ty + 1][tx + 1],
+                                       pl[qd][ty + 1][tx + 1]);
                 }
-                q = row_Ax<float>(cur.code, sH32, c, pc, S[ty + 1][tx + 2], S[ty + 1][tx], S[ty + 2][tx + 1],
-                                  S[ty][tx + 1], ring[up % 3][ty + 1][tx + 1], ring[dn % 3][ty + 1][tx + 1]);
+                if (tp.live) {
+                    const long long id = cbase + q * plane;
+                    b.xlo[id] = xv[q - 1] + alpha * xc;
+                    const float r = rv[q - 1] - alpha * qv;
+                    b.r[id] = r;
+                    const float z = r * dinv_of(code, sH32, c);
+                    acc[0] += (double)r * (double)z;
+                    acc[1] += (double)r * (double)r;
+                }
             }
-            if (tp.live) {
-                const long long id = vid(tp.i, tp.j, k, nx, ny);
-                b.xlo[id] = cur.x + alpha * pc;
-                const float r = cur.r - alpha * q;
-                b.r[id] = r;
-                const float z = r * dinv_of(cur.code, sH32, c);
-                acc[0] += (double)r * (double)z;
-                acc[1] += (double)r * (double)r;
-            }
-            __syncthreads();
-            cur = nxt;
         }
     }
     write_block_partials<2>(acc, b.part[io], nb);
@@ -487,10 +487,22 @@ void launch_pcg_setup_f32(hipStream_t s, const Grid& g, const StepConsts& c, con
     hipLaunchKernelGGL(k_pcg_setup_f32, grid3(g), dim3(BX, BY, 1), 0, s, b, c, blocks_for_impl(g), init);
 }
 void launch_pcg_dir_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b, int t) {
-    hipLaunchKernelGGL(k_pcg_dir_f32, grid3(g), dim3(BX, BY, 1), 0, s, b, c, blocks_for_impl(g), t);
+    const dim3 gr = grid3(g), bl(BX, BY, 1);
+    const int nb = blocks_for_impl(g);
+    switch (g.kz) {
+        case 2: hipLaunchKernelGGL(k_pcg_dir_f32<2>, gr, bl, 0, s, b, c, nb, t); break;
+        case 4: hipLaunchKernelGGL(k_pcg_dir_f32<4>, gr, bl, 0, s, b, c, nb, t); break;
+        default: hipLaunchKernelGGL(k_pcg_dir_f32<8>, gr, bl, 0, s, b, c, nb, t); break;
+    }
 }
 void launch_pcg_update_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b, int t) {
-    hipLaunchKernelGGL(k_pcg_update_f32, grid3(g), dim3(BX, BY, 1), 0, s, b, c, blocks_for_impl(g), t);
+    const dim3 gr = grid3(g), bl(BX, BY, 1);
+    const int nb = blocks_for_impl(g);
+    switch (g.kz) {
+        case 2: hipLaunchKernelGGL(k_pcg_update_f32<2>, gr, bl, 0, s, b, c, nb, t); break;
+        case 4: hipLaunchKernelGGL(k_pcg_update_f32<4>, gr, bl, 0, s, b, c, nb, t); break;
+        default: hipLaunchKernelGGL(k_pcg_update_f32<8>, gr, bl, 0, s, b, c, nb, t); break;
+    }
 }
 void launch_diag_f32(hipStream_t s, const Grid& g, const float* P, const uint32_t* cw, double* part) {
     hipLaunchKernelGGL(k_diag_f32, grid3(g), dim3(BX, BY, 1), 0, s, P, cw, g, part, blocks_for_impl(g));

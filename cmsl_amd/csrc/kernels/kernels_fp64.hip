@@ -206,62 +206,57 @@ __global__ __launch_bounds__(NT) void k_pcg_setup_f64(Fp64Bufs b, StepConsts c, 
         }
     }
     write_block_partials<3>(acc, b.part[0], nb);
-    if (block0_thread0()) {
-        init.kind = K_SETUP;
-        b.st[0] = init;
-    }
+    init.kind = K_SETUP;
+    pass_state(init, b.st);
 }
 
 // ---------------------------------------------------------------- PCG direction + SpMV
 // p_new = z + beta*p_old (first iteration: p_new = z), q = A p_new, p_new.q.
 // z = D^-1 r is formed from r and the stencil code wherever it is staged, so
 // z is never stored.
+//
+// Each workgroup issues the loads of all KZ+2 planes of its tile at once (they
+// overlap the reduction prologue), stages the operand in LDS and then
+// evaluates its KZ planes: one memory round trip per workgroup.
+template <int KZ>
 __global__ __launch_bounds__(NT) void k_pcg_dir_f64(Fp64Bufs b, StepConsts c, int nb, int t) {
     const int ip = (t - 1) & 1, io = t & 1;
-    PcgState s = b.st[ip];
+    PcgState s = load_state(b.st + ip);
     if (s.done) {
         pass_state(s, b.st + io);
         return;
     }
     const bool first = s.kind == K_SETUP;
     __shared__ double sH[16];
-    __shared__ double ring[3][HY][HX];
+    __shared__ double pl[KZ + 2][HY][HX];
     const int nx = c.nx, ny = c.ny, nz = c.nz;
-    const int k0 = blockIdx.z * c.kz, k1 = min(k0 + c.kz, nz);
+    const int k0 = blockIdx.z * KZ;
     const TilePos tp = tile_pos(c);
     const int tx = threadIdx.x, ty = threadIdx.y;
     const double* pold = b.p[s.pbuf];
     double* pnew = b.p[s.pbuf ^ 1];
-    double beta = 0;  // set once the previous inner products are finished
-    // staged operand: p_new = z + beta*p_old with z = D^-1 r (first: p_new = z)
-    double rc_ = 0.0, pc_ = 0.0, rh = 0.0, ph = 0.0;
-    unsigned cc_ = 0, ch = 0;
-    auto fetch = [&](int kk) {
-        const long long id = vid(tp.ci, tp.cj, kk, nx, ny);
-        rc_ = b.r[id];
-        cc_ = b.sc[id];
-        if (!first) pc_ = pold[id];
-        if (tp.has_h) {
-            const long long h = vid(tp.gi, tp.gj, kk, nx, ny);
-            rh = b.r[h];
-            ch = b.sc[h];
-            if (!first) ph = pold[h];
+    double rc[KZ + 2], pc[KZ + 2], rh[KZ + 2], ph[KZ + 2];
+    unsigned cc[KZ + 2], ch[KZ + 2];
+    const long long plane = (long long)nx * ny;
+    const long long cbase = vid(tp.ci, tp.cj, k0 - 1, nx, ny), hbase = vid(tp.gi, tp.gj, k0 - 1, nx, ny);
+#pragma unroll
+    for (int q = 0; q < KZ + 2; ++q) {
+        const int kk = k0 - 1 + q;
+        rc[q] = pc[q] = rh[q] = ph[q] = 0.0;
+        cc[q] = ch[q] = SC_IDENTITY;
+        if (kk >= 0 && kk < nz) {
+            const long long id = cbase + q * plane;
+            rc[q] = b.r[id];
+            cc[q] = b.sc[id];
+            if (!first) pc[q] = pold[id];
+            if (tp.has_h) {
+                const long long h = hbase + q * plane;
+                rh[q] = b.r[h];
+                ch[q] = b.sc[h];
+                if (!first) ph[q] = pold[h];
+            }
         }
-    };
-    auto make = [&](int kk) -> unsigned {
-        double(*S)[HX] = ring[kk % 3];
-        const double z = rc_ * dinv_of(cc_, sH, c);
-        S[ty + 1][tx + 1] = first ? z : z + beta * pc_;
-        if (tp.has_h) {
-            const double zh = rh * dinv_of(ch, sH, c);
-            S[tp.hy][tp.hx] = first ? zh : zh + beta * ph;
-        }
-        return cc_;
-    };
-    int lo, hi;
-    plane_window(c, k0, k1, lo, hi);
-    double acc[1] = {0.0};
-    fetch(lo);  // the first plane's loads overlap the reduction below
+    }
     // finish the previous kernel's inner products (setup: r.z, r.r, b.b;
     // update: r.z, r.r) and apply the loop test (depol_varEPS.cu:459-474, 506-521)
     if (first) {
@@ -285,7 +280,7 @@ __global__ __launch_bounds__(NT) void k_pcg_dir_f64(Fp64Bufs b, StepConsts c, in
             s.rz_old = tot[0];
             s.done = 1;
         } else {
-            s.beta = tot[0] / s.rz_old;
+            s.beta = tot[0] / s.rz_old;  // :515
             s.rz_old = tot[0];
             judge(s, tot[0], tot[1]);
         }
@@ -294,41 +289,45 @@ __global__ __launch_bounds__(NT) void k_pcg_dir_f64(Fp64Bufs b, StepConsts c, in
         pass_state(s, b.st + io);
         return;
     }
-    beta = (double)s.beta;
+    const double beta = s.beta;
     load_lut(sH, b.lut);
-    unsigned code_k = make(lo), code_next = 0;
-    int kn = lo + 1;  // next plane to stage
-    if (kn <= hi) fetch(kn);
-    for (int k = k0; k < k1; ++k) {
-        const int need = min(k + 1, hi);
-        while (kn <= need) {
-            const unsigned cm = make(kn);
-            if (kn == k) code_k = cm;
-            else code_next = cm;
-            ++kn;
-            if (kn <= hi) fetch(kn);  // in flight while plane k is evaluated
-        }
-        __syncthreads();
-        double(*S)[HX] = ring[k % 3];
-        const double pc = S[ty + 1][tx + 1];
-        double q;
-        if (code_k & SC_IDENTITY) {
-            q = op_identity(pc, k, c);
-        } else {
-            int up = k + 1, dn = k - 1;
-            if (c.zbc != 0) {
-                if (k == 0) up = dn = 1;
-                else if (k == nz - 1) up = dn = nz - 2;
+    const double* dinv = b.lut + LUT_DINV;  // exact D^-1 per stencil code (host-built table)
+#pragma unroll
+    for (int q = 0; q < KZ + 2; ++q) {
+        const int kk = k0 - 1 + q;
+        if (kk >= 0 && kk < nz) {
+            const double z = rc[q] * dinv[cc[q]];  // :502
+            pl[q][ty + 1][tx + 1] = first ? z : z + beta * pc[q];  // :518-519
+            if (tp.has_h) {
+                const double zh = rh[q] * dinv[ch[q]];
+                pl[q][tp.hy][tp.hx] = first ? zh : zh + beta * ph[q];
             }
-            q = op_row(code_k, sH, c, pc, S[ty + 1][tx + 2], S[ty + 1][tx], S[ty + 2][tx + 1], S[ty][tx + 1],
-                       ring[up % 3][ty + 1][tx + 1], ring[dn % 3][ty + 1][tx + 1]);
         }
-        if (tp.live) {
-            pnew[vid(tp.i, tp.j, k, nx, ny)] = pc;
-            acc[0] += pc * q;
+    }
+    __syncthreads();
+    double acc[1] = {0.0};
+#pragma unroll
+    for (int q = 1; q <= KZ; ++q) {
+        const int k = k0 - 1 + q;
+        if (k < nz) {
+            const double xc = pl[q][ty + 1][tx + 1];
+            double qv;
+            if (cc[q] & SC_IDENTITY) {
+                qv = op_identity(xc, k, c);
+            } else {
+                int qu = q + 1, qd = q - 1;
+                if (c.zbc != 0) {
+                    if (k == 0) qu = qd = q + 1;
+                    else if (k == nz - 1) qu = qd = q - 1;
+                }
+                qv = op_row(cc[q], sH, c, xc, pl[q][ty + 1][tx + 2], pl[q][ty + 1][tx], pl[q][ty + 2][tx + 1],
+                            pl[q][ty][tx + 1], pl[qu][ty + 1][tx + 1], pl[qd][ty + 1][tx + 1]);
+            }
+            if (tp.live) {
+                pnew[cbase + q * plane] = xc;
+                acc[0] += xc * qv;  // p.Ap (:485)
+            }
         }
-        __syncthreads();
-        code_k = code_next;
     }
     write_block_partials<1>(acc, b.part[io], nb);
     s.iters += 1;  // ++kiter (:476)
@@ -338,47 +337,46 @@ __global__ __launch_bounds__(NT) void k_pcg_dir_f64(Fp64Bufs b, StepConsts c, in
 }
 
 // ---------------------------------------------------------------- PCG update
+template <int KZ>
 __global__ __launch_bounds__(NT) void k_pcg_update_f64(Fp64Bufs b, StepConsts c, int nb, int t) {
     const int ip = (t - 1) & 1, io = t & 1;
-    PcgState s = b.st[ip];
+    PcgState s = load_state(b.st + ip);
     if (s.done) {
         pass_state(s, b.st + io);
         return;
     }
     __shared__ double sH[16];
-    __shared__ double ring[3][HY][HX];
+    __shared__ double pl[KZ + 2][HY][HX];
     const int nx = c.nx, ny = c.ny, nz = c.nz;
-    const int k0 = blockIdx.z * c.kz, k1 = min(k0 + c.kz, nz);
+    const int k0 = blockIdx.z * KZ;
     const TilePos tp = tile_pos(c);
     const int tx = threadIdx.x, ty = threadIdx.y;
     const double* p = b.p[s.pbuf];
-    double pcv = 0.0, phv = 0.0;
-    auto fetch = [&](int kk) {
-        pcv = p[vid(tp.ci, tp.cj, kk, nx, ny)];
-        if (tp.has_h) phv = p[vid(tp.gi, tp.gj, kk, nx, ny)];
-    };
-    auto make = [&](int kk) {
-        ring[kk % 3][ty + 1][tx + 1] = pcv;
-        if (tp.has_h) ring[kk % 3][tp.hy][tp.hx] = phv;
-    };
-    // element-wise operands of the centre voxel, one plane ahead
-    struct Ctr {
-        double x, r;
-        unsigned code;
-    };
-    auto fetch_c = [&](int kk) {
-        const long long id = vid(tp.ci, tp.cj, kk, nx, ny);
-        Ctr v;
-        v.x = b.phi[id];
-        v.r = b.r[id];
-        v.code = b.sc[id];
-        return v;
-    };
-    int lo, hi;
-    plane_window(c, k0, k1, lo, hi);
-    double acc[2] = {0.0, 0.0};  // r.z, r.r
-    fetch(lo);  // loads of the first planes overlap the reduction below
-    Ctr cur = fetch_c(k0), nxt = cur;
+    double pc[KZ + 2], ph[KZ + 2], xv[KZ], rv[KZ];
+    unsigned cv[KZ];
+    const long long plane = (long long)nx * ny;
+    const long long cbase = vid(tp.ci, tp.cj, k0 - 1, nx, ny), hbase = vid(tp.gi, tp.gj, k0 - 1, nx, ny);
+#pragma unroll
+    for (int q = 0; q < KZ + 2; ++q) {
+        const int kk = k0 - 1 + q;
+        pc[q] = ph[q] = 0.0;
+        if (kk >= 0 && kk < nz) {
+            pc[q] = p[cbase + q * plane];
+            if (tp.has_h) ph[q] = p[hbase + q * plane];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < KZ; ++q) {
+        const int kk = k0 + q;
+        xv[q] = rv[q] = 0.0;
+        cv[q] = SC_IDENTITY;
+        if (kk < nz) {
+            const long long id = cbase + (q + 1) * plane;
+            xv[q] = b.phi[id];
+            rv[q] = b.r[id];
+            cv[q] = b.sc[id];
+        }
+    }
     double tot[1];
     sum_partials<1>(b.part[ip], nb, tot);  // p.Ap (:485)
     s.denom = tot[0];
@@ -388,47 +386,49 @@ __global__ __launch_bounds__(NT) void k_pcg_update_f64(Fp64Bufs b, StepConsts c,
         pass_state(s, b.st + io);
         return;
     }
-    s.alpha = s.rz_old / tot[0];
+    s.alpha = s.rz_old / tot[0];  // :492
     const double alpha = s.alpha;
     const double nalpha = -alpha;
     load_lut(sH, b.lut);
-    make(lo);
-    int kn = lo + 1;
-    if (kn <= hi) fetch(kn);
-    for (int k = k0; k < k1; ++k) {
-        const int need = min(k + 1, hi);
-        while (kn <= need) {
-            make(kn);
-            ++kn;
-            if (kn <= hi) fetch(kn);
+    const double* dinv = b.lut + LUT_DINV;
+#pragma unroll
+    for (int q = 0; q < KZ + 2; ++q) {
+        const int kk = k0 - 1 + q;
+        if (kk >= 0 && kk < nz) {
+            pl[q][ty + 1][tx + 1] = pc[q];
+            if (tp.has_h) pl[q][tp.hy][tp.hx] = ph[q];
         }
-        if (k + 1 < k1) nxt = fetch_c(k + 1);
-        __syncthreads();
-        double(*S)[HX] = ring[k % 3];
-        const double pc = S[ty + 1][tx + 1];
-        double q;
-        if (cur.code & SC_IDENTITY) {
-            q = op_identity(pc, k, c);
-        } else {
-            int up = k + 1, dn = k - 1;
-            if (c.zbc != 0) {
-                if (k == 0) up = dn = 1;
-                else if (k == nz - 1) up = dn = nz - 2;
+    }
+    __syncthreads();
+    double acc[2] = {0.0, 0.0};  // r.z, r.r
+#pragma unroll
+    for (int q = 1; q <= KZ; ++q) {
+        const int k = k0 - 1 + q;
+        if (k < nz) {
+            const double xc = pl[q][ty + 1][tx + 1];
+            const unsigned code = cv[q - 1];
+            double qv;
+            if (code & SC_IDENTITY) {
+                qv = op_identity(xc, k, c);
+            } else {
+                int qu = q + 1, qd = q - 1;
+                if (c.zbc != 0) {
+                    if (k == 0) qu = qd = q + 1;
+                    else if (k == nz - 1) qu = qd = q - 1;
+                }
+                qv = op_row(code, sH, c, xc, pl[q][ty + 1][tx + 2], pl[q][ty + 1][tx], pl[q][ty + 2][tx + 1],
+                            pl[q][ty][tx + 1], pl[qu][ty + 1][tx + 1], pl[qd][ty + 1][tx + 1]);
             }
-            q = op_row(cur.code, sH, c, pc, S[ty + 1][tx + 2], S[ty + 1][tx], S[ty + 2][tx + 1], S[ty][tx + 1],
-                       ring[up % 3][ty + 1][tx + 1], ring[dn % 3][ty + 1][tx + 1]);
+            if (tp.live) {
+                const long long id = cbase + q * plane;
+                b.phi[id] = alpha * xc + xv[q - 1];         // Daxpy x += alpha p (:495)
+                const double r = nalpha * qv + rv[q - 1];    // Daxpy r -= alpha Ap (:499)
+                b.r[id] = r;
+                const double z = r * dinv[code];            // :502
+                acc[0] += r * z;
+                acc[1] += r * r;
+            }
         }
-        if (tp.live) {
-            const long long id = vid(tp.i, tp.j, k, nx, ny);
-            b.phi[id] = alpha * pc + cur.x;         // Daxpy x += alpha p (:495)
-            const double r = nalpha * q + cur.r;    // Daxpy r -= alpha Ap (:499)
-            b.r[id] = r;
-            const double z = r * dinv_of(cur.code, sH, c);  // :502
-            acc[0] += r * z;
-            acc[1] += r * r;
-        }
-        __syncthreads();
-        cur = nxt;
     }
     write_block_partials<2>(acc, b.part[io], nb);
     s.kind = K_UPDATE;
@@ -482,11 +482,23 @@ void launch_pcg_setup_f64(hipStream_t s, const Grid& g, const StepConsts& c, con
 }
 
 void launch_pcg_dir_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t) {
-    hipLaunchKernelGGL(k_pcg_dir_f64, grid3(g), dim3(BX, BY, 1), 0, s, b, c, blocks_for(g), t);
+    const dim3 gr = grid3(g), bl(BX, BY, 1);
+    const int nb = blocks_for(g);
+    switch (g.kz) {
+        case 2: hipLaunchKernelGGL(k_pcg_dir_f64<2>, gr, bl, 0, s, b, c, nb, t); break;
+        case 4: hipLaunchKernelGGL(k_pcg_dir_f64<4>, gr, bl, 0, s, b, c, nb, t); break;
+        default: hipLaunchKernelGGL(k_pcg_dir_f64<8>, gr, bl, 0, s, b, c, nb, t); break;
+    }
 }
 
 void launch_pcg_update_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t) {
-    hipLaunchKernelGGL(k_pcg_update_f64, grid3(g), dim3(BX, BY, 1), 0, s, b, c, blocks_for(g), t);
+    const dim3 gr = grid3(g), bl(BX, BY, 1);
+    const int nb = blocks_for(g);
+    switch (g.kz) {
+        case 2: hipLaunchKernelGGL(k_pcg_update_f64<2>, gr, bl, 0, s, b, c, nb, t); break;
+        case 4: hipLaunchKernelGGL(k_pcg_update_f64<4>, gr, bl, 0, s, b, c, nb, t); break;
+        default: hipLaunchKernelGGL(k_pcg_update_f64<8>, gr, bl, 0, s, b, c, nb, t); break;
+    }
 }
 
 void launch_diag_f64(hipStream_t s, const Grid& g, const double* P, const uint32_t* cw, double* part) {

@@ -30,7 +30,7 @@ __device__ __forceinline__ int linear_tid() {
 // Workgroup sum of NQ per-thread values; thread 0 stores them at
 // out[q*nb + block].
 template <int NQ>
-__device__ void write_block_partials(const double (&v)[NQ], double* out, int nb) {
+__device__ __forceinline__ void write_block_partials(const double (&v)[NQ], double* out, int nb) {
     __shared__ double sred[NQ][16];
     const int tid = linear_tid();
     const int lane = tid & 63, wid = tid >> 6;
@@ -55,7 +55,7 @@ __device__ void write_block_partials(const double (&v)[NQ], double* out, int nb)
 // Totals of the NQ partial arrays written by the previous kernel; identical
 // in every thread of every workgroup.
 template <int NQ>
-__device__ void sum_partials(const double* in, int nb, double (&tot)[NQ]) {
+__device__ __forceinline__ void sum_partials(const double* in, int nb, double (&tot)[NQ]) {
     __shared__ double sred[NQ][16];
     const int tid = linear_tid();
     const int nthreads = blockDim.x * blockDim.y * blockDim.z;

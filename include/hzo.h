@@ -82,7 +82,7 @@ typedef struct hzo_config {
     double tol_abs, tol_rel;
     int max_iter;
     int refine_every;    /* FP32-refined: FP64 residual recompute period (iterations) */
-    int tile_kz;         /* z planes per workgroup of the stencil kernels; 0 = automatic */
+    int tile_kz;         /* z planes per workgroup of the stencil kernels: 2, 4, 8; 0 = automatic */
 } hzo_config;
 
 typedef struct hzo_pcg_stats {
