@@ -47,7 +47,8 @@ struct hzo_ctx {
     double* TR = nullptr;
     hzo::PhaseC* ptab = nullptr;
     double* lut = nullptr;
-    double* part = nullptr;       // 2 parity slots x RED_MAXQ x nb
+    double* part = nullptr;       // 2 parity slots x RED_MAXQ x ps (zero tail beyond nb)
+    int ps = 0;                   // stride of one partial array
     double* tdgl_part = nullptr;  // TDGL_Q x nb
     double* diag_part = nullptr;  // DIAG_Q x nb
     hzo::PcgState* st = nullptr;  // [2]
@@ -97,7 +98,7 @@ hzo::Fp64Bufs bufs(hzo_ctx* x) {
     b.ptab = x->ptab;
     b.lut = x->lut;
     b.part[0] = x->part;
-    b.part[1] = x->part + (size_t)hzo::RED_MAXQ * x->nb;
+    b.part[1] = x->part + (size_t)hzo::RED_MAXQ * x->ps;
     b.st = x->st;
     b.tdgl_part = x->tdgl_part;
     return b;
@@ -121,7 +122,7 @@ hzo::Fp32Bufs bufs32(hzo_ctx* x) {
     b.lut = x->lut;
     b.lut32 = x->lut32;
     b.part[0] = x->part;
-    b.part[1] = x->part + (size_t)hzo::RED_MAXQ * x->nb;
+    b.part[1] = x->part + (size_t)hzo::RED_MAXQ * x->ps;
     b.st = x->st;
     b.tdgl_part = x->tdgl_part;
     return b;
@@ -262,6 +263,10 @@ int upload_phase(hzo_ctx* x) {
         lut[hzo::LUT_DINV + code] = (D > 1e-30 ? 1.0 / D : 1.0);
     }
     lut[hzo::LUT_DINV + hzo::SC_IDENTITY] = 1.0;
+    for (int q = 0; q < 16; ++q) {
+        x->c.harm[q] = lut[hzo::LUT_HARM + q];
+        x->c.harm32[q] = (float)lut[hzo::LUT_HARM + q];
+    }
     HZO_HIP(hipMemcpy(x->ptab, pt, sizeof(pt), hipMemcpyHostToDevice));
     HZO_HIP(hipMemcpy(x->lut, lut.data(), sizeof(double) * hzo::LUT_DOUBLES, hipMemcpyHostToDevice));
     if (x->fp32) {
@@ -447,7 +452,7 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
     if (cfg->pcg_rule != 0 && cfg->pcg_rule != 1) return set_error(HZO_ERR_ARG, "hzo_create: pcg_rule %d", cfg->pcg_rule);
     if (cfg->max_iter < 0) return set_error(HZO_ERR_ARG, "hzo_create: max_iter < 0");
     const long long N = (long long)cfg->nx * cfg->ny * cfg->nz;
-    if (N > (1LL << 31)) return set_error(HZO_ERR_ARG, "hzo_create: grid too large");
+    if (N > (1LL << 28)) return set_error(HZO_ERR_ARG, "hzo_create: grid too large (at most 2^28 voxels)");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return set_error(HZO_ERR_HIP, "hzo_create: no HIP device available");
@@ -493,10 +498,13 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
     c.h2 = cfg->del_h * cfg->del_h;
     c.inv_h2 = 1.0 / c.h2;
     c.h2_pow2 = is_pow2(c.h2);
+    c.plane = (unsigned)cfg->nx * (unsigned)cfg->ny;
+    c.nvox = (unsigned)N;
     c.ref_id = (long long)(cfg->nx / 2) + (long long)cfg->nx * ((long long)(cfg->ny / 2) + (long long)cfg->ny * (cfg->nz / 2));
 
     (void)hipGetDevice(&x->device);
     x->nb = hzo::blocks_for(x->g);
+    x->ps = hzo::part_stride(x->g);
     auto fail = [&](int rc) {
         free_all(x);
         delete x;
@@ -532,13 +540,15 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
     HZO_ALLOC(x->sc, sizeof(uint16_t) * N);
     HZO_ALLOC(x->ptab, sizeof(hzo::PhaseC) * hzo::PTAB_ENTRIES);
     HZO_ALLOC(x->lut, sizeof(double) * hzo::LUT_DOUBLES);
-    HZO_ALLOC(x->part, sizeof(double) * 2 * hzo::RED_MAXQ * (size_t)x->nb);
+    HZO_ALLOC(x->part, sizeof(double) * 2 * hzo::RED_MAXQ * (size_t)x->ps);
     HZO_ALLOC(x->tdgl_part, sizeof(double) * hzo::TDGL_Q * (size_t)x->nb);
     HZO_ALLOC(x->diag_part, sizeof(double) * hzo::DIAG_Q * (size_t)x->nb);
     HZO_ALLOC(x->st, sizeof(hzo::PcgState) * 2);
 #undef HZO_ALLOC
     if (hipHostMalloc((void**)&x->st_host, sizeof(hzo::PcgState), hipHostMallocDefault) != hipSuccess)
         return fail(set_error(HZO_ERR_NOMEM, "hzo_create: pinned host allocation failed"));
+    if (hipMemset(x->part, 0, sizeof(double) * 2 * hzo::RED_MAXQ * (size_t)x->ps) != hipSuccess)
+        return fail(set_error(HZO_ERR_HIP, "hzo_create: memset failed"));
     if (hipMemset(x->tdgl_part, 0, sizeof(double) * hzo::TDGL_Q * (size_t)x->nb) != hipSuccess)
         return fail(set_error(HZO_ERR_HIP, "hzo_create: memset failed"));
     *out = x;

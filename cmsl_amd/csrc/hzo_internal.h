@@ -92,6 +92,13 @@ struct StepConsts {
     double two_h, inv_two_h, h2, inv_h2;
     int two_h_pow2, h2_pow2;
     long long ref_id;  // Neumann gauge node
+    unsigned plane;    // nx*ny
+    unsigned nvox;     // nx*ny*nz (at most 2^28)
+    // harmonic face means H[a*4+b] of the permittivity classes (the same
+    // values as the lut's harm table), carried in the kernel arguments so
+    // a kernel builds its LDS copy without a vector-memory round trip
+    double harm[16];
+    float harm32[16];
 };
 
 // Launch helpers (kernels_fp64.hip).
@@ -138,6 +145,8 @@ struct Fp32Bufs {
 };
 
 int blocks_for(const Grid& g);
+// Stride of one PCG partial-sum array (>= blocks_for; the tail is kept zero).
+int part_stride(const Grid& g);
 
 // Launchers.  `t` is the launch index since the last PCG setup (setup = 0).
 void launch_tdgl_f64(hipStream_t s, const Grid& g, const StepConsts& c, double Ez_ext, const Fp64Bufs& b);

@@ -11,8 +11,63 @@ inline int blocks_for_impl(const Grid& g) {
     return ((g.nx + BX - 1) / BX) * ((g.ny + BY - 1) / BY) * ((g.nz + g.kz - 1) / g.kz);
 }
 
+// PCG partial sums: PM entries per thread of a consuming workgroup, over a
+// zero-padded stride of PM*NT (PM = 0: plain stride nb, loop reduction).
+inline int part_pm(int nb) { return nb <= NT ? 1 : nb <= 2 * NT ? 2 : nb <= 4 * NT ? 4 : 0; }
+inline int part_stride_impl(int nb) {
+    const int pm = part_pm(nb);
+    return pm ? pm * NT : nb;
+}
+
+// Launch of a PCG kernel template <int KZ, int PM> with the instance that
+// matches the grid's tile depth and partial layout.
+#define HZO_PCG_LAUNCH_PM(kern, KZ, pm, gr, bl, s, ...)                               \
+    switch (pm) {                                                                     \
+        case 1: hipLaunchKernelGGL((kern<KZ, 1>), gr, bl, 0, s, __VA_ARGS__); break;  \
+        case 2: hipLaunchKernelGGL((kern<KZ, 2>), gr, bl, 0, s, __VA_ARGS__); break;  \
+        case 4: hipLaunchKernelGGL((kern<KZ, 4>), gr, bl, 0, s, __VA_ARGS__); break;  \
+        default: hipLaunchKernelGGL((kern<KZ, 0>), gr, bl, 0, s, __VA_ARGS__); break; \
+    }
+#define HZO_PCG_LAUNCH(kern, g, s, ...)                                                   \
+    do {                                                                                  \
+        const int pm_ = part_pm(blocks_for_impl(g));                                      \
+        const dim3 gr_ = grid3(g), bl_(BX, BY, 1);                                        \
+        switch ((g).kz) {                                                                 \
+            case 2: HZO_PCG_LAUNCH_PM(kern, 2, pm_, gr_, bl_, s, __VA_ARGS__); break;     \
+            case 4: HZO_PCG_LAUNCH_PM(kern, 4, pm_, gr_, bl_, s, __VA_ARGS__); break;     \
+            default: HZO_PCG_LAUNCH_PM(kern, 8, pm_, gr_, bl_, s, __VA_ARGS__); break;    \
+        }                                                                                 \
+    } while (0)
+
 inline dim3 grid3(const Grid& g) {
     return dim3((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, (g.nz + g.kz - 1) / g.kz);
+}
+
+// XCD-aware tile order.  MI355X dispatches workgroup w (linear index) to XCD
+// w % 8, and each XCD has its own L2.  Tiles are numbered x fastest, then z,
+// then y, and each XCD receives one contiguous run of that numbering (a slab
+// in y), so the halo columns, rows and planes a tile shares with its
+// neighbours are mostly fetched by workgroups of the same XCD and served from
+// that L2 instead of crossing the fabric once per neighbour.
+constexpr int N_XCD = 8;
+
+struct TileId {
+    int bx, by, bz;
+};
+
+__device__ __forceinline__ TileId tile_id() {
+    const int gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+    const int nbk = gx * gy * gz;
+    const int w = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const int xcd = w % N_XCD, loc = w / N_XCD;
+    const int per = nbk / N_XCD, rem = nbk % N_XCD;
+    const int t = xcd * per + min(xcd, rem) + loc;
+    TileId T;
+    T.bx = t % gx;
+    const int r = t / gx;
+    T.bz = r % gz;
+    T.by = r / gz;
+    return T;
 }
 
 __device__ __forceinline__ long long vid(int i, int j, int k, int nx, int ny) {

@@ -25,9 +25,10 @@ namespace hzo {
 __global__ __launch_bounds__(NT) void k_tdgl_f32(Fp32Bufs b, StepConsts c, float Ez_ext, int nb) {
     const int nx = c.nx, ny = c.ny, nz = c.nz;
     const long long N = (long long)nx * ny * nz;
-    const int i = blockIdx.x * BX + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    const int k0 = blockIdx.z * c.kz;
+    const TileId T = tile_id();
+    const int i = T.bx * BX + threadIdx.x;
+    const int j = T.by * BY + threadIdx.y;
+    const int k0 = T.bz * c.kz;
     double loc[2] = {0.0, 0.0};
     if (i < nx && j < ny) {
         const int e = (i + 1 == nx) ? 0 : i + 1;
@@ -124,6 +125,18 @@ __device__ __forceinline__ T row_Ax(unsigned code, const T* H, const StepConsts&
     return -(num * (T)c.inv_h2);
 }
 
+// FP32 row from the paired face table: -(sum_d H_d (x_d - x_c)) / h^2.
+__device__ __forceinline__ float row_Ax_pairs(unsigned code, const HPair<float>* sHP, float ih2, float xc,
+                                              float xe, float xw, float xn, float xs, float xu, float xd) {
+    const unsigned cb = (code & 3u) << 4;
+    const HPair<float> hx = sHP[cb | ((code >> 2) & 15u)];
+    const HPair<float> hy = sHP[cb | ((code >> 6) & 15u)];
+    const HPair<float> hz = sHP[cb | ((code >> 10) & 15u)];
+    const float num = hx.p * (xe - xc) - hx.m * (xc - xw) + hy.p * (xn - xc) - hy.m * (xc - xs) +
+                      hz.p * (xu - xc) - hz.m * (xc - xd);
+    return -(num * ih2);
+}
+
 // r = b - A x with x = x_hi (+ x_lo), all in FP64; shared by the PCG setup and
 // the residual replacement.
 __device__ __forceinline__ double true_residual(const Fp32Bufs& b, const double* sH, const StepConsts& c,
@@ -139,16 +152,16 @@ __device__ __forceinline__ double true_residual(const Fp32Bufs& b, const double*
 }
 
 // ---------------------------------------------------------------- PCG setup
-__global__ __launch_bounds__(NT) void k_pcg_setup_f32(Fp32Bufs b, StepConsts c, int nb, PcgState init) {
+__global__ __launch_bounds__(NT) void k_pcg_setup_f32(Fp32Bufs b, StepConsts c, int ps, PcgState init) {
     __shared__ double sH[16];
-    __shared__ float sH32[16];
     load_lut(sH, b.lut);
-    load_lut(sH32, b.lut32);
+    const float* dinv = b.lut32 + LUT_DINV;
     const int nx = c.nx, ny = c.ny, nz = c.nz;
     const long long N = (long long)nx * ny * nz;
-    const int i = blockIdx.x * BX + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    const int k0 = blockIdx.z * c.kz;
+    const TileId T = tile_id();
+    const int i = T.bx * BX + threadIdx.x;
+    const int j = T.by * BY + threadIdx.y;
+    const int k0 = T.bz * c.kz;
     double acc[3] = {0.0, 0.0, 0.0};
     if (i < nx && j < ny) {
         const float* Px = b.Pcur;
@@ -168,25 +181,26 @@ __global__ __launch_bounds__(NT) void k_pcg_setup_f32(Fp32Bufs b, StepConsts c, 
             }
             b.b[v.c] = bval;
             const float r = (float)true_residual(b, sH, c, v, code, k, bval, true);
-            const float z = r * dinv_of(code, sH32, c);
+            const float z = r * dinv[code];
             b.r[v.c] = r;
             acc[0] += (double)r * (double)z;
             acc[1] += (double)r * (double)r;
             acc[2] += bval * bval;
         }
     }
-    write_block_partials<3>(acc, b.part[0], nb);
+    write_block_partials<3>(acc, b.part[0], ps);
     init.kind = K_SETUP;
     pass_state(init, b.st);
 }
 
 // ---------------------------------------------------------------- direction / fold
 // Finishes the previous kernel's inner products and decides what this launch
-// is: a CG direction update (K_DIR) or the fold x_hi += x_lo that starts a
-// residual replacement (K_FOLD).  Loads of all KZ+2 planes of the tile are
-// issued before the reduction prologue (one memory round trip per workgroup).
-template <int KZ>
-__global__ __launch_bounds__(NT) void k_pcg_dir_f32(Fp32Bufs b, StepConsts c, int nb, int t) {
+// is: a CG direction update (K_DIR: p = z + beta p, q = A p, p.q) or the fold
+// x_hi += x_lo that starts a residual replacement (K_FOLD).  Load order and
+// tile staging as in the FP64 form; D^-1 is the FP32 rounding of the exact
+// per-code table, the same matrix in every FP32 kernel.
+template <int KZ, int PM>
+__global__ __launch_bounds__(NT, 4) void k_pcg_dir_f32(Fp32Bufs b, StepConsts c, int nb, int ps, int t) {
     const int ip = (t - 1) & 1, io = t & 1;
     PcgState s = load_state(b.st + ip);
     if (s.done) {
@@ -194,40 +208,71 @@ __global__ __launch_bounds__(NT) void k_pcg_dir_f32(Fp32Bufs b, StepConsts c, in
         return;
     }
     const bool first = s.kind == K_SETUP;
-    __shared__ float sH[16];
-    __shared__ float pl[KZ + 2][HY][HX];
-    const int nx = c.nx, ny = c.ny, nz = c.nz;
-    const int k0 = blockIdx.z * KZ;
-    const TilePos tp = tile_pos(c);
+    __shared__ HPair<float> sHP[64];
+    __shared__ float pl[KZ][HY][HX];
+    const int nz = c.nz;
+    const TileId T = tile_id();
+    const int k0 = T.bz * KZ;
+    const TilePos tp = tile_pos(c, T);
+    const TileCols G = tile_cols(tp, c);
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const float* pold = b.p[s.pbuf];
-    float* pnew = b.p[s.pbuf ^ 1];
-    float rc[KZ + 2], pc[KZ + 2], rh[KZ + 2], ph[KZ + 2];
-    unsigned cc[KZ + 2], ch[KZ + 2];
-    const long long plane = (long long)nx * ny;
-    const long long cbase = vid(tp.ci, tp.cj, k0 - 1, nx, ny), hbase = vid(tp.gi, tp.gj, k0 - 1, nx, ny);
+    const unsigned pb4 = c.plane * 4u, pb2 = c.plane * 2u;
+    const rsrc_t Rr = make_rsrc(b.r, c.nvox * 4u);
+    const rsrc_t Rpo = make_rsrc(s.pbuf ? b.p[1] : b.p[0], c.nvox * 4u);
+    const rsrc_t Rpn = make_rsrc(s.pbuf ? b.p[0] : b.p[1], c.nvox * 4u);
+    const rsrc_t Rsc = make_rsrc(b.sc, c.nvox * 2u);
+    const rsrc_t Rdi = make_rsrc(b.lut32 + LUT_DINV, SC_LUT_SIZE * 4u);
+    const double* pin = ip ? b.part[1] : b.part[0];
+
+    Partials<3, PM> P;
+    if (first) {
+        load_partials<3, PM>(pin, nb, ps, P);
+    } else {
+        Partials<2, PM> P2;
+        load_partials<2, PM>(pin, nb, ps, P2);
 #pragma unroll
-    for (int q = 0; q < KZ + 2; ++q) {
-        const int kk = k0 - 1 + q;
-        rc[q] = pc[q] = rh[q] = ph[q] = 0.f;
-        cc[q] = ch[q] = SC_IDENTITY;
-        if (kk >= 0 && kk < nz) {
-            const long long id = cbase + q * plane;
-            rc[q] = b.r[id];
-            cc[q] = b.sc[id];
-            if (!first) pc[q] = pold[id];
-            if (tp.has_h) {
-                const long long h = hbase + q * plane;
-                rh[q] = b.r[h];
-                ch[q] = b.sc[h];
-                if (!first) ph[q] = pold[h];
-            }
+        for (int m = 0; m < (PM > 0 ? PM : 1); ++m) {
+            P.v[0][m] = P2.v[0][m];
+            P.v[1][m] = P2.v[1][m];
+            P.v[2][m] = 0.0;
         }
     }
+    unsigned so[KZ + 2];
+#pragma unroll
+    for (int q = 0; q < KZ + 2; ++q) so[q] = (unsigned)min(max(k0 - 1 + q, 0), nz - 1);
+    unsigned cc[KZ + 2], ch[KZ];
+#pragma unroll
+    for (int q = 0; q < KZ + 2; ++q) cc[q] = bload_code(Rsc, G.cofs * 2u, so[q] * pb2);
+    {
+#pragma unroll
+        for (int q = 0; q < KZ; ++q) ch[q] = bload_code(Rsc, G.hofs * 2u, so[q + 1] * pb2);
+    }
+    float rc[KZ + 2], pc[KZ + 2], rh[KZ], ph[KZ];
+#pragma unroll
+    for (int q = 0; q < KZ + 2; ++q) {
+        rc[q] = bload<float>(Rr, G.cofs * 4u, so[q] * pb4);
+        pc[q] = bload<float>(Rpo, G.cofs * 4u, so[q] * pb4);
+    }
+    {
+#pragma unroll
+        for (int q = 0; q < KZ; ++q) {
+            rh[q] = bload<float>(Rr, G.hofs * 4u, so[q + 1] * pb4);
+            ph[q] = bload<float>(Rpo, G.hofs * 4u, so[q + 1] * pb4);
+        }
+    }
+    float dc[KZ + 2], dh[KZ];
+#pragma unroll
+    for (int q = 0; q < KZ + 2; ++q) dc[q] = bload<float>(Rdi, cc[q] * 4u, 0u);
+    {
+#pragma unroll
+        for (int q = 0; q < KZ; ++q) dh[q] = bload<float>(Rdi, ch[q] * 4u, 0u);
+    }
+    build_hpairs(sHP, c.harm32);
+
+    double tot[3];
+    finish_partials<3, PM>(P, tot);
     bool fold = false;
     if (first) {
-        double tot[3];
-        sum_partials<3>(b.part[ip], nb, tot);
         s.rz0 = s.rz_old = tot[0];
         s.rr0 = s.rr = tot[1];
         s.bb = tot[2];
@@ -238,8 +283,6 @@ __global__ __launch_bounds__(NT) void k_pcg_dir_f32(Fp32Bufs b, StepConsts c, in
             judge(s, tot[0], tot[1]);
         }
     } else {
-        double tot[2];
-        sum_partials<2>(b.part[ip], nb, tot);
         s.rr = tot[1];
         if (!finite(tot[0]) || tot[0] < 0.0) {
             s.status = HZO_PCG_BAD_RZ;
@@ -274,7 +317,7 @@ __global__ __launch_bounds__(NT) void k_pcg_dir_f32(Fp32Bufs b, StepConsts c, in
             for (int q = 0; q < KZ; ++q) {
                 const int k = k0 + q;
                 if (k < nz) {
-                    const long long id = vid(tp.i, tp.j, k, nx, ny);
+                    const long long id = (long long)G.cofs + (long long)c.plane * k;
                     b.xhi[id] += (double)b.xlo[id];
                     b.xlo[id] = 0.0f;
                 }
@@ -285,45 +328,49 @@ __global__ __launch_bounds__(NT) void k_pcg_dir_f32(Fp32Bufs b, StepConsts c, in
         return;
     }
     const float beta = (float)s.beta;
-    load_lut(sH, b.lut32);
+    float pcol[KZ + 2];
 #pragma unroll
     for (int q = 0; q < KZ + 2; ++q) {
-        const int kk = k0 - 1 + q;
-        if (kk >= 0 && kk < nz) {
-            const float z = rc[q] * dinv_of(cc[q], sH, c);
-            pl[q][ty + 1][tx + 1] = first ? z : z + beta * pc[q];
-            if (tp.has_h) {
-                const float zh = rh[q] * dinv_of(ch[q], sH, c);
-                pl[q][tp.hy][tp.hx] = first ? zh : zh + beta * ph[q];
-            }
+        const float z = rc[q] * dc[q];
+        pcol[q] = first ? z : z + beta * pc[q];
+        if (q >= 1 && q <= KZ) pl[q - 1][ty + 1][tx + 1] = pcol[q];
+    }
+    {
+#pragma unroll
+        for (int q = 0; q < KZ; ++q) {
+            const float zh = rh[q] * dh[q];
+            const float v = first ? zh : zh + beta * ph[q];
+            if (tp.has_h) pl[q][tp.hy][tp.hx] = v;
         }
     }
     __syncthreads();
+    const float ih2 = (float)c.inv_h2;
     double acc[1] = {0.0};
 #pragma unroll
     for (int q = 1; q <= KZ; ++q) {
         const int k = k0 - 1 + q;
         if (k < nz) {
-            const float xc = pl[q][ty + 1][tx + 1];
+            const float xc = pcol[q];
+            const unsigned code = cc[q];
             float qv;
-            if (cc[q] & SC_IDENTITY) {
+            if (code & SC_IDENTITY) {
                 qv = op_identity(xc, k, c);
             } else {
-                int qu = q + 1, qd = q - 1;
+                float xu = pcol[q + 1], xd = pcol[q - 1];
                 if (c.zbc != 0) {
-                    if (k == 0) qu = qd = q + 1;
-                    else if (k == nz - 1) qu = qd = q - 1;
+                    if (k == 0) xd = xu;
+                    else if (k == nz - 1) xu = xd;
                 }
-                qv = row_Ax<float>(cc[q], sH, c, xc, pl[q][ty + 1][tx + 2], pl[q][ty + 1][tx], pl[q][ty + 2][tx + 1],
-                                   pl[q][ty][tx + 1], pl[qu][ty + 1][tx + 1], pl[qd][ty + 1][tx + 1]);
+                qv = row_Ax_pairs(code, sHP, ih2, xc, pl[q - 1][ty + 1][tx + 2], pl[q - 1][ty + 1][tx],
+                                  pl[q - 1][ty + 2][tx + 1], pl[q - 1][ty][tx + 1], xu, xd);
             }
             if (tp.live) {
-                pnew[cbase + q * plane] = xc;
+                bstore(Rpn, G.cofs * 4u, (unsigned)k * pb4, xc);
                 acc[0] += (double)xc * (double)qv;
             }
         }
     }
-    write_block_partials<1>(acc, b.part[io], nb);
+    write_block_partials<1>(acc, io ? b.part[1] : b.part[0], ps);
     s.iters += 1;
     s.pbuf ^= 1;
     s.kind = K_DIR;
@@ -331,8 +378,8 @@ __global__ __launch_bounds__(NT) void k_pcg_dir_f32(Fp32Bufs b, StepConsts c, in
 }
 
 // ---------------------------------------------------------------- update / residual replacement
-template <int KZ>
-__global__ __launch_bounds__(NT) void k_pcg_update_f32(Fp32Bufs b, StepConsts c, int nb, int t) {
+template <int KZ, int PM>
+__global__ __launch_bounds__(NT, 4) void k_pcg_update_f32(Fp32Bufs b, StepConsts c, int nb, int ps, int t) {
     const int ip = (t - 1) & 1, io = t & 1;
     PcgState s = load_state(b.st + ip);
     if (s.done) {
@@ -341,15 +388,15 @@ __global__ __launch_bounds__(NT) void k_pcg_update_f32(Fp32Bufs b, StepConsts c,
     }
     const bool resid = s.kind == K_FOLD;
     const int nx = c.nx, ny = c.ny, nz = c.nz;
-    const int k0 = blockIdx.z * KZ;
+    const TileId T = tile_id();
+    const int k0 = T.bz * KZ;
     double acc[2] = {0.0, 0.0};
     if (resid) {  // r = b - A x_hi in FP64 (x_lo was folded by the previous launch)
         __shared__ double sH64[16];
-        __shared__ float sH32r[16];
         load_lut(sH64, b.lut);
-        load_lut(sH32r, b.lut32);
-        const int i = blockIdx.x * BX + threadIdx.x;
-        const int j = blockIdx.y * BY + threadIdx.y;
+        const float* dinv = b.lut32 + LUT_DINV;
+        const int i = T.bx * BX + threadIdx.x;
+        const int j = T.by * BY + threadIdx.y;
         if (i < nx && j < ny) {
             const int k1 = min(k0 + KZ, nz);
             for (int k = k0; k < k1; ++k) {
@@ -357,44 +404,51 @@ __global__ __launch_bounds__(NT) void k_pcg_update_f32(Fp32Bufs b, StepConsts c,
                 const unsigned code = b.sc[v.c];
                 const float r = (float)true_residual(b, sH64, c, v, code, k, b.b[v.c], false);
                 b.r[v.c] = r;
-                const float z = r * dinv_of(code, sH32r, c);
+                const float z = r * dinv[code];
                 acc[0] += (double)r * (double)z;
                 acc[1] += (double)r * (double)r;
             }
         }
     } else {
-        __shared__ float sH32[16];
-        __shared__ float pl[KZ + 2][HY][HX];
-        const TilePos tp = tile_pos(c);
+        __shared__ HPair<float> sHP[64];
+        __shared__ float pl[KZ][HY][HX];
+        const TilePos tp = tile_pos(c, T);
+        const TileCols G = tile_cols(tp, c);
         const int tx = threadIdx.x, ty = threadIdx.y;
-        const float* p = b.p[s.pbuf];
-        float pc[KZ + 2], ph[KZ + 2], xv[KZ], rv[KZ];
-        unsigned cv[KZ];
-        const long long plane = (long long)nx * ny;
-        const long long cbase = vid(tp.ci, tp.cj, k0 - 1, nx, ny), hbase = vid(tp.gi, tp.gj, k0 - 1, nx, ny);
+        const unsigned pb4 = c.plane * 4u, pb2 = c.plane * 2u;
+        const rsrc_t Rp = make_rsrc(s.pbuf ? b.p[1] : b.p[0], c.nvox * 4u);
+        const rsrc_t Rr = make_rsrc(b.r, c.nvox * 4u);
+        const rsrc_t Rx = make_rsrc(b.xlo, c.nvox * 4u);
+        const rsrc_t Rsc = make_rsrc(b.sc, c.nvox * 2u);
+        const rsrc_t Rdi = make_rsrc(b.lut32 + LUT_DINV, SC_LUT_SIZE * 4u);
+
+        Partials<1, PM> P;
+        load_partials<1, PM>(ip ? b.part[1] : b.part[0], nb, ps, P);
+        unsigned so[KZ + 2];
 #pragma unroll
-        for (int q = 0; q < KZ + 2; ++q) {
-            const int kk = k0 - 1 + q;
-            pc[q] = ph[q] = 0.f;
-            if (kk >= 0 && kk < nz) {
-                pc[q] = p[cbase + q * plane];
-                if (tp.has_h) ph[q] = p[hbase + q * plane];
-            }
+        for (int q = 0; q < KZ + 2; ++q) so[q] = (unsigned)min(max(k0 - 1 + q, 0), nz - 1);
+        unsigned cv[KZ];
+#pragma unroll
+        for (int q = 0; q < KZ; ++q) cv[q] = bload_code(Rsc, G.cofs * 2u, so[q + 1] * pb2);
+        float pcol[KZ + 2], ph[KZ];
+#pragma unroll
+        for (int q = 0; q < KZ + 2; ++q) pcol[q] = bload<float>(Rp, G.cofs * 4u, so[q] * pb4);
+        {
+#pragma unroll
+            for (int q = 0; q < KZ; ++q) ph[q] = bload<float>(Rp, G.hofs * 4u, so[q + 1] * pb4);
         }
+        float xv[KZ], rv[KZ];
 #pragma unroll
         for (int q = 0; q < KZ; ++q) {
-            const int kk = k0 + q;
-            xv[q] = rv[q] = 0.f;
-            cv[q] = SC_IDENTITY;
-            if (kk < nz) {
-                const long long id = cbase + (q + 1) * plane;
-                xv[q] = b.xlo[id];
-                rv[q] = b.r[id];
-                cv[q] = b.sc[id];
-            }
+            xv[q] = bload<float>(Rx, G.cofs * 4u, so[q + 1] * pb4);
+            rv[q] = bload<float>(Rr, G.cofs * 4u, so[q + 1] * pb4);
         }
+        float dv[KZ];
+#pragma unroll
+        for (int q = 0; q < KZ; ++q) dv[q] = bload<float>(Rdi, cv[q] * 4u, 0u);
+        build_hpairs(sHP, c.harm32);
         double tot[1];
-        sum_partials<1>(b.part[ip], nb, tot);
+        finish_partials<1, PM>(P, tot);
         s.denom = tot[0];
         if (!finite(tot[0]) || !(tot[0] > 0.0)) {
             s.status = HZO_PCG_BREAKDOWN;
@@ -404,48 +458,44 @@ __global__ __launch_bounds__(NT) void k_pcg_update_f32(Fp32Bufs b, StepConsts c,
         }
         s.alpha = s.rz_old / tot[0];
         const float alpha = (float)s.alpha;
-        load_lut(sH32, b.lut32);
 #pragma unroll
-        for (int q = 0; q < KZ + 2; ++q) {
-            const int kk = k0 - 1 + q;
-            if (kk >= 0 && kk < nz) {
-                pl[q][ty + 1][tx + 1] = pc[q];
-                if (tp.has_h) pl[q][tp.hy][tp.hx] = ph[q];
-            }
+        for (int q = 0; q < KZ; ++q) pl[q][ty + 1][tx + 1] = pcol[q + 1];
+        if (tp.has_h) {
+#pragma unroll
+            for (int q = 0; q < KZ; ++q) pl[q][tp.hy][tp.hx] = ph[q];
         }
         __syncthreads();
+        const float ih2 = (float)c.inv_h2;
 #pragma unroll
         for (int q = 1; q <= KZ; ++q) {
             const int k = k0 - 1 + q;
             if (k < nz) {
-                const float xc = pl[q][ty + 1][tx + 1];
+                const float xc = pcol[q];
                 const unsigned code = cv[q - 1];
                 float qv;
                 if (code & SC_IDENTITY) {
                     qv = op_identity(xc, k, c);
                 } else {
-                    int qu = q + 1, qd = q - 1;
+                    float xu = pcol[q + 1], xd = pcol[q - 1];
                     if (c.zbc != 0) {
-                        if (k == 0) qu = qd = q + 1;
-                        else if (k == nz - 1) qu = qd = q - 1;
+                        if (k == 0) xd = xu;
+                        else if (k == nz - 1) xu = xd;
                     }
-                    qv = row_Ax<float>(code, sH32, c, xc, pl[q][ty + 1][tx + 2], pl[q][ty + 1][tx],
-                                       pl[q][ty + 2][tx + 1], pl[q][ty][tx + 1], pl[qu][ty + 1][tx + 1],
-                                       pl[qd][ty + 1][tx + 1]);
+                    qv = row_Ax_pairs(code, sHP, ih2, xc, pl[q - 1][ty + 1][tx + 2], pl[q - 1][ty + 1][tx],
+                                      pl[q - 1][ty + 2][tx + 1], pl[q - 1][ty][tx + 1], xu, xd);
                 }
                 if (tp.live) {
-                    const long long id = cbase + q * plane;
-                    b.xlo[id] = xv[q - 1] + alpha * xc;
+                    bstore(Rx, G.cofs * 4u, (unsigned)k * pb4, xv[q - 1] + alpha * xc);
                     const float r = rv[q - 1] - alpha * qv;
-                    b.r[id] = r;
-                    const float z = r * dinv_of(code, sH32, c);
+                    bstore(Rr, G.cofs * 4u, (unsigned)k * pb4, r);
+                    const float z = r * dv[q - 1];
                     acc[0] += (double)r * (double)z;
                     acc[1] += (double)r * (double)r;
                 }
             }
         }
     }
-    write_block_partials<2>(acc, b.part[io], nb);
+    write_block_partials<2>(acc, io ? b.part[1] : b.part[0], ps);
     s.kind = resid ? K_RESID : K_UPDATE;
     pass_state(s, b.st + io);
 }
@@ -455,9 +505,10 @@ __global__ __launch_bounds__(NT) void k_diag_f32(const float* __restrict__ P, co
                                                   Grid g, double* part, int nb) {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long N = g.N;
-    const int i = blockIdx.x * BX + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    const int k0 = blockIdx.z * g.kz;
+    const TileId T = tile_id();
+    const int i = T.bx * BX + threadIdx.x;
+    const int j = T.by * BY + threadIdx.y;
+    const int k0 = T.bz * g.kz;
     double a[DIAG_Q] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (i < nx && j < ny) {
         const int k1 = min(k0 + g.kz, nz);
@@ -484,25 +535,13 @@ void launch_tdgl_f32(hipStream_t s, const Grid& g, const StepConsts& c, double E
 }
 void launch_pcg_setup_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b,
                           const PcgState& init) {
-    hipLaunchKernelGGL(k_pcg_setup_f32, grid3(g), dim3(BX, BY, 1), 0, s, b, c, blocks_for_impl(g), init);
+    hipLaunchKernelGGL(k_pcg_setup_f32, grid3(g), dim3(BX, BY, 1), 0, s, b, c, part_stride(g), init);
 }
 void launch_pcg_dir_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b, int t) {
-    const dim3 gr = grid3(g), bl(BX, BY, 1);
-    const int nb = blocks_for_impl(g);
-    switch (g.kz) {
-        case 2: hipLaunchKernelGGL(k_pcg_dir_f32<2>, gr, bl, 0, s, b, c, nb, t); break;
-        case 4: hipLaunchKernelGGL(k_pcg_dir_f32<4>, gr, bl, 0, s, b, c, nb, t); break;
-        default: hipLaunchKernelGGL(k_pcg_dir_f32<8>, gr, bl, 0, s, b, c, nb, t); break;
-    }
+    HZO_PCG_LAUNCH(k_pcg_dir_f32, g, s, b, c, blocks_for(g), part_stride(g), t);
 }
 void launch_pcg_update_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b, int t) {
-    const dim3 gr = grid3(g), bl(BX, BY, 1);
-    const int nb = blocks_for_impl(g);
-    switch (g.kz) {
-        case 2: hipLaunchKernelGGL(k_pcg_update_f32<2>, gr, bl, 0, s, b, c, nb, t); break;
-        case 4: hipLaunchKernelGGL(k_pcg_update_f32<4>, gr, bl, 0, s, b, c, nb, t); break;
-        default: hipLaunchKernelGGL(k_pcg_update_f32<8>, gr, bl, 0, s, b, c, nb, t); break;
-    }
+    HZO_PCG_LAUNCH(k_pcg_update_f32, g, s, b, c, blocks_for(g), part_stride(g), t);
 }
 void launch_diag_f32(hipStream_t s, const Grid& g, const float* P, const uint32_t* cw, double* part) {
     hipLaunchKernelGGL(k_diag_f32, grid3(g), dim3(BX, BY, 1), 0, s, P, cw, g, part, blocks_for_impl(g));

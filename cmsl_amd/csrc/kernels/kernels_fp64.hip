@@ -26,14 +26,16 @@
 namespace hzo {
 
 int blocks_for(const Grid& g) { return blocks_for_impl(g); }
+int part_stride(const Grid& g) { return part_stride_impl(blocks_for_impl(g)); }
 
 // ---------------------------------------------------------------- TDGL
 __global__ __launch_bounds__(NT) void k_tdgl_f64(Fp64Bufs b, StepConsts c, double Ez_ext, int nb) {
     const int nx = c.nx, ny = c.ny, nz = c.nz;
     const long long N = (long long)nx * ny * nz;
-    const int i = blockIdx.x * BX + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    const int k0 = blockIdx.z * c.kz;
+    const TileId T = tile_id();
+    const int i = T.bx * BX + threadIdx.x;
+    const int j = T.by * BY + threadIdx.y;
+    const int k0 = T.bz * c.kz;
     double loc[2] = {0.0, 0.0};  // sum Pmz_local, sum Psz_local (evolve.cu:269-276)
     if (i < nx && j < ny) {
         const int e = (i + 1 == nx) ? 0 : i + 1;
@@ -163,14 +165,15 @@ __device__ __forceinline__ double op_row(unsigned code, const double* H, const S
 }
 
 // ---------------------------------------------------------------- PCG setup
-__global__ __launch_bounds__(NT) void k_pcg_setup_f64(Fp64Bufs b, StepConsts c, int nb, PcgState init) {
+__global__ __launch_bounds__(NT) void k_pcg_setup_f64(Fp64Bufs b, StepConsts c, int ps, PcgState init) {
     __shared__ double sH[16];
     load_lut(sH, b.lut);
     const int nx = c.nx, ny = c.ny, nz = c.nz;
     const long long N = (long long)nx * ny * nz;
-    const int i = blockIdx.x * BX + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    const int k0 = blockIdx.z * c.kz;
+    const TileId T = tile_id();
+    const int i = T.bx * BX + threadIdx.x;
+    const int j = T.by * BY + threadIdx.y;
+    const int k0 = T.bz * c.kz;
     double acc[3] = {0.0, 0.0, 0.0};  // r.z, r.r, b.b
     if (i < nx && j < ny) {
         const double* Px = b.Pcur;
@@ -205,21 +208,26 @@ __global__ __launch_bounds__(NT) void k_pcg_setup_f64(Fp64Bufs b, StepConsts c, 
             acc[2] += bval * bval;
         }
     }
-    write_block_partials<3>(acc, b.part[0], nb);
+    write_block_partials<3>(acc, b.part[0], ps);
     init.kind = K_SETUP;
     pass_state(init, b.st);
 }
 
 // ---------------------------------------------------------------- PCG direction + SpMV
 // p_new = z + beta*p_old (first iteration: p_new = z), q = A p_new, p_new.q.
-// z = D^-1 r is formed from r and the stencil code wherever it is staged, so
-// z is never stored.
+// z = D^-1 r is formed from r and the exact per-code D^-1 table wherever it
+// is staged, so z is never stored.
 //
-// Each workgroup issues the loads of all KZ+2 planes of its tile at once (they
-// overlap the reduction prologue), stages the operand in LDS and then
-// evaluates its KZ planes: one memory round trip per workgroup.
-template <int KZ>
-__global__ __launch_bounds__(NT) void k_pcg_dir_f64(Fp64Bufs b, StepConsts c, int nb, int t) {
+// Issue order per workgroup: the previous kernel's partials, the stencil
+// codes, r and p_old of the KZ planes and of the two z-ghost planes (centre
+// column only: ghost planes feed just the u/d neighbours), the x-y halo of
+// the KZ planes, then the D^-1 gathers (which need only the codes).  The
+// reduction is finished and the loop test applied while the tile loads are
+// in flight.  Plane indices are clamped into the film instead of branched
+// on; values of clamped planes are never used by a live row.  The thread's
+// own column of p stays in registers; e/w/n/s come from LDS.
+template <int KZ, int PM>
+__global__ __launch_bounds__(NT, 4) void k_pcg_dir_f64(Fp64Bufs b, StepConsts c, int nb, int ps, int t) {
     const int ip = (t - 1) & 1, io = t & 1;
     PcgState s = load_state(b.st + ip);
     if (s.done) {
@@ -227,41 +235,72 @@ __global__ __launch_bounds__(NT) void k_pcg_dir_f64(Fp64Bufs b, StepConsts c, in
         return;
     }
     const bool first = s.kind == K_SETUP;
-    __shared__ double sH[16];
-    __shared__ double pl[KZ + 2][HY][HX];
-    const int nx = c.nx, ny = c.ny, nz = c.nz;
-    const int k0 = blockIdx.z * KZ;
-    const TilePos tp = tile_pos(c);
+    __shared__ HPair<double> sHP[64];
+    __shared__ double pl[KZ][HY][HX];
+    const int nz = c.nz;
+    const TileId T = tile_id();
+    const int k0 = T.bz * KZ;
+    const TilePos tp = tile_pos(c, T);
+    const TileCols G = tile_cols(tp, c);
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const double* pold = b.p[s.pbuf];
-    double* pnew = b.p[s.pbuf ^ 1];
-    double rc[KZ + 2], pc[KZ + 2], rh[KZ + 2], ph[KZ + 2];
-    unsigned cc[KZ + 2], ch[KZ + 2];
-    const long long plane = (long long)nx * ny;
-    const long long cbase = vid(tp.ci, tp.cj, k0 - 1, nx, ny), hbase = vid(tp.gi, tp.gj, k0 - 1, nx, ny);
+    const unsigned pb8 = c.plane * 8u, pb2 = c.plane * 2u;
+    const rsrc_t Rr = make_rsrc(b.r, c.nvox * 8u);
+    const rsrc_t Rpo = make_rsrc(s.pbuf ? b.p[1] : b.p[0], c.nvox * 8u);
+    const rsrc_t Rpn = make_rsrc(s.pbuf ? b.p[0] : b.p[1], c.nvox * 8u);
+    const rsrc_t Rsc = make_rsrc(b.sc, c.nvox * 2u);
+    const rsrc_t Rdi = make_rsrc(b.lut + LUT_DINV, SC_LUT_SIZE * 8u);
+    const double* pin = ip ? b.part[1] : b.part[0];
+
+    Partials<3, PM> P;
+    if (first) {
+        load_partials<3, PM>(pin, nb, ps, P);
+    } else {
+        Partials<2, PM> P2;
+        load_partials<2, PM>(pin, nb, ps, P2);
 #pragma unroll
-    for (int q = 0; q < KZ + 2; ++q) {
-        const int kk = k0 - 1 + q;
-        rc[q] = pc[q] = rh[q] = ph[q] = 0.0;
-        cc[q] = ch[q] = SC_IDENTITY;
-        if (kk >= 0 && kk < nz) {
-            const long long id = cbase + q * plane;
-            rc[q] = b.r[id];
-            cc[q] = b.sc[id];
-            if (!first) pc[q] = pold[id];
-            if (tp.has_h) {
-                const long long h = hbase + q * plane;
-                rh[q] = b.r[h];
-                ch[q] = b.sc[h];
-                if (!first) ph[q] = pold[h];
-            }
+        for (int m = 0; m < (PM > 0 ? PM : 1); ++m) {
+            P.v[0][m] = P2.v[0][m];
+            P.v[1][m] = P2.v[1][m];
+            P.v[2][m] = 0.0;
         }
     }
+    unsigned so[KZ + 2];  // byte offset of each staged plane (clamped)
+#pragma unroll
+    for (int q = 0; q < KZ + 2; ++q) so[q] = (unsigned)min(max(k0 - 1 + q, 0), nz - 1);
+    unsigned cc[KZ + 2], ch[KZ];
+#pragma unroll
+    for (int q = 0; q < KZ + 2; ++q) cc[q] = bload_code(Rsc, G.cofs * 2u, so[q] * pb2);
+    {
+#pragma unroll
+        for (int q = 0; q < KZ; ++q) ch[q] = bload_code(Rsc, G.hofs * 2u, so[q + 1] * pb2);
+    }
+    double rc[KZ + 2], pc[KZ + 2], rh[KZ], ph[KZ];
+#pragma unroll
+    for (int q = 0; q < KZ + 2; ++q) {
+        rc[q] = bload<double>(Rr, G.cofs * 8u, so[q] * pb8);
+        pc[q] = bload<double>(Rpo, G.cofs * 8u, so[q] * pb8);
+    }
+    {
+#pragma unroll
+        for (int q = 0; q < KZ; ++q) {
+            rh[q] = bload<double>(Rr, G.hofs * 8u, so[q + 1] * pb8);
+            ph[q] = bload<double>(Rpo, G.hofs * 8u, so[q + 1] * pb8);
+        }
+    }
+    double dc[KZ + 2], dh[KZ];
+#pragma unroll
+    for (int q = 0; q < KZ + 2; ++q) dc[q] = bload<double>(Rdi, cc[q] * 8u, 0u);
+    {
+#pragma unroll
+        for (int q = 0; q < KZ; ++q) dh[q] = bload<double>(Rdi, ch[q] * 8u, 0u);
+    }
+    build_hpairs(sHP, c.harm);
+
     // finish the previous kernel's inner products (setup: r.z, r.r, b.b;
     // update: r.z, r.r) and apply the loop test (depol_varEPS.cu:459-474, 506-521)
+    double tot[3];
+    finish_partials<3, PM>(P, tot);
     if (first) {
-        double tot[3];
-        sum_partials<3>(b.part[ip], nb, tot);
         s.rz0 = s.rz_old = tot[0];
         s.rr0 = s.rr = tot[1];
         s.bb = tot[2];
@@ -272,8 +311,6 @@ __global__ __launch_bounds__(NT) void k_pcg_dir_f64(Fp64Bufs b, StepConsts c, in
             judge(s, tot[0], tot[1]);
         }
     } else {
-        double tot[2];
-        sum_partials<2>(b.part[ip], nb, tot);
         s.rr = tot[1];
         if (!finite(tot[0]) || tot[0] < 0.0) {  // :508-513
             s.status = HZO_PCG_BAD_RZ;
@@ -290,18 +327,19 @@ __global__ __launch_bounds__(NT) void k_pcg_dir_f64(Fp64Bufs b, StepConsts c, in
         return;
     }
     const double beta = s.beta;
-    load_lut(sH, b.lut);
-    const double* dinv = b.lut + LUT_DINV;  // exact D^-1 per stencil code (host-built table)
+    double pcol[KZ + 2];
 #pragma unroll
     for (int q = 0; q < KZ + 2; ++q) {
-        const int kk = k0 - 1 + q;
-        if (kk >= 0 && kk < nz) {
-            const double z = rc[q] * dinv[cc[q]];  // :502
-            pl[q][ty + 1][tx + 1] = first ? z : z + beta * pc[q];  // :518-519
-            if (tp.has_h) {
-                const double zh = rh[q] * dinv[ch[q]];
-                pl[q][tp.hy][tp.hx] = first ? zh : zh + beta * ph[q];
-            }
+        const double z = rc[q] * dc[q];  // :502
+        pcol[q] = first ? z : z + beta * pc[q];  // :518-519
+        if (q >= 1 && q <= KZ) pl[q - 1][ty + 1][tx + 1] = pcol[q];
+    }
+    {
+#pragma unroll
+        for (int q = 0; q < KZ; ++q) {
+            const double zh = rh[q] * dh[q];
+            const double v = first ? zh : zh + beta * ph[q];
+            if (tp.has_h) pl[q][tp.hy][tp.hx] = v;
         }
     }
     __syncthreads();
@@ -310,26 +348,35 @@ __global__ __launch_bounds__(NT) void k_pcg_dir_f64(Fp64Bufs b, StepConsts c, in
     for (int q = 1; q <= KZ; ++q) {
         const int k = k0 - 1 + q;
         if (k < nz) {
-            const double xc = pl[q][ty + 1][tx + 1];
+            const double xc = pcol[q];
+            const unsigned code = cc[q];
             double qv;
-            if (cc[q] & SC_IDENTITY) {
+            if (code & SC_IDENTITY) {
                 qv = op_identity(xc, k, c);
             } else {
-                int qu = q + 1, qd = q - 1;
-                if (c.zbc != 0) {
-                    if (k == 0) qu = qd = q + 1;
-                    else if (k == nz - 1) qu = qd = q - 1;
+                double xu = pcol[q + 1], xd = pcol[q - 1];
+                if (c.zbc != 0) {  // mirror ghosts (depol_varEPS.cu:94-98)
+                    if (k == 0) xd = xu;
+                    else if (k == nz - 1) xu = xd;
                 }
-                qv = op_row(cc[q], sH, c, xc, pl[q][ty + 1][tx + 2], pl[q][ty + 1][tx], pl[q][ty + 2][tx + 1],
-                            pl[q][ty][tx + 1], pl[qu][ty + 1][tx + 1], pl[qd][ty + 1][tx + 1]);
+                const unsigned cb = (code & 3u) << 4;
+                const HPair<double> hx = sHP[cb | ((code >> 2) & 15u)];
+                const HPair<double> hy = sHP[cb | ((code >> 6) & 15u)];
+                const HPair<double> hz = sHP[cb | ((code >> 10) & 15u)];
+                const double xe = pl[q - 1][ty + 1][tx + 2], xw = pl[q - 1][ty + 1][tx];
+                const double xn = pl[q - 1][ty + 2][tx + 1], xs = pl[q - 1][ty][tx + 1];
+                const double num = hx.p * (xe - xc) - hx.m * (xc - xw) + hy.p * (xn - xc) - hy.m * (xc - xs) +
+                                   hz.p * (xu - xc) - hz.m * (xc - xd);
+                const double Ax = c.h2_pow2 ? num * c.inv_h2 : num / c.h2;
+                qv = -Ax;
             }
             if (tp.live) {
-                pnew[cbase + q * plane] = xc;
+                bstore(Rpn, G.cofs * 8u, (unsigned)k * pb8, xc);
                 acc[0] += xc * qv;  // p.Ap (:485)
             }
         }
     }
-    write_block_partials<1>(acc, b.part[io], nb);
+    write_block_partials<1>(acc, io ? b.part[1] : b.part[0], ps);
     s.iters += 1;  // ++kiter (:476)
     s.pbuf ^= 1;
     s.kind = K_DIR;
@@ -337,48 +384,58 @@ __global__ __launch_bounds__(NT) void k_pcg_dir_f64(Fp64Bufs b, StepConsts c, in
 }
 
 // ---------------------------------------------------------------- PCG update
-template <int KZ>
-__global__ __launch_bounds__(NT) void k_pcg_update_f64(Fp64Bufs b, StepConsts c, int nb, int t) {
+// x += alpha p, r -= alpha A p (A p re-evaluated from p instead of stored),
+// z = D^-1 r, r.z and r.r (depol_varEPS.cu:492-507).
+template <int KZ, int PM>
+__global__ __launch_bounds__(NT, 4) void k_pcg_update_f64(Fp64Bufs b, StepConsts c, int nb, int ps, int t) {
     const int ip = (t - 1) & 1, io = t & 1;
     PcgState s = load_state(b.st + ip);
     if (s.done) {
         pass_state(s, b.st + io);
         return;
     }
-    __shared__ double sH[16];
-    __shared__ double pl[KZ + 2][HY][HX];
-    const int nx = c.nx, ny = c.ny, nz = c.nz;
-    const int k0 = blockIdx.z * KZ;
-    const TilePos tp = tile_pos(c);
+    __shared__ HPair<double> sHP[64];
+    __shared__ double pl[KZ][HY][HX];
+    const int nz = c.nz;
+    const TileId T = tile_id();
+    const int k0 = T.bz * KZ;
+    const TilePos tp = tile_pos(c, T);
+    const TileCols G = tile_cols(tp, c);
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const double* p = b.p[s.pbuf];
-    double pc[KZ + 2], ph[KZ + 2], xv[KZ], rv[KZ];
-    unsigned cv[KZ];
-    const long long plane = (long long)nx * ny;
-    const long long cbase = vid(tp.ci, tp.cj, k0 - 1, nx, ny), hbase = vid(tp.gi, tp.gj, k0 - 1, nx, ny);
+    const unsigned pb8 = c.plane * 8u, pb2 = c.plane * 2u;
+    const rsrc_t Rp = make_rsrc(s.pbuf ? b.p[1] : b.p[0], c.nvox * 8u);
+    const rsrc_t Rr = make_rsrc(b.r, c.nvox * 8u);
+    const rsrc_t Rx = make_rsrc(b.phi, c.nvox * 8u);
+    const rsrc_t Rsc = make_rsrc(b.sc, c.nvox * 2u);
+    const rsrc_t Rdi = make_rsrc(b.lut + LUT_DINV, SC_LUT_SIZE * 8u);
+
+    Partials<1, PM> P;
+    load_partials<1, PM>(ip ? b.part[1] : b.part[0], nb, ps, P);
+    unsigned so[KZ + 2];
 #pragma unroll
-    for (int q = 0; q < KZ + 2; ++q) {
-        const int kk = k0 - 1 + q;
-        pc[q] = ph[q] = 0.0;
-        if (kk >= 0 && kk < nz) {
-            pc[q] = p[cbase + q * plane];
-            if (tp.has_h) ph[q] = p[hbase + q * plane];
-        }
+    for (int q = 0; q < KZ + 2; ++q) so[q] = (unsigned)min(max(k0 - 1 + q, 0), nz - 1);
+    unsigned cv[KZ];
+#pragma unroll
+    for (int q = 0; q < KZ; ++q) cv[q] = bload_code(Rsc, G.cofs * 2u, so[q + 1] * pb2);
+    double pcol[KZ + 2], ph[KZ];
+#pragma unroll
+    for (int q = 0; q < KZ + 2; ++q) pcol[q] = bload<double>(Rp, G.cofs * 8u, so[q] * pb8);
+    {
+#pragma unroll
+        for (int q = 0; q < KZ; ++q) ph[q] = bload<double>(Rp, G.hofs * 8u, so[q + 1] * pb8);
     }
+    double xv[KZ], rv[KZ];
 #pragma unroll
     for (int q = 0; q < KZ; ++q) {
-        const int kk = k0 + q;
-        xv[q] = rv[q] = 0.0;
-        cv[q] = SC_IDENTITY;
-        if (kk < nz) {
-            const long long id = cbase + (q + 1) * plane;
-            xv[q] = b.phi[id];
-            rv[q] = b.r[id];
-            cv[q] = b.sc[id];
-        }
+        xv[q] = bload<double>(Rx, G.cofs * 8u, so[q + 1] * pb8);
+        rv[q] = bload<double>(Rr, G.cofs * 8u, so[q + 1] * pb8);
     }
+    double dv[KZ];
+#pragma unroll
+    for (int q = 0; q < KZ; ++q) dv[q] = bload<double>(Rdi, cv[q] * 8u, 0u);
+    build_hpairs(sHP, c.harm);
     double tot[1];
-    sum_partials<1>(b.part[ip], nb, tot);  // p.Ap (:485)
+    finish_partials<1, PM>(P, tot);  // p.Ap (:485)
     s.denom = tot[0];
     if (!finite(tot[0]) || !(tot[0] > 0.0)) {  // :486-490
         s.status = HZO_PCG_BREAKDOWN;
@@ -389,15 +446,11 @@ __global__ __launch_bounds__(NT) void k_pcg_update_f64(Fp64Bufs b, StepConsts c,
     s.alpha = s.rz_old / tot[0];  // :492
     const double alpha = s.alpha;
     const double nalpha = -alpha;
-    load_lut(sH, b.lut);
-    const double* dinv = b.lut + LUT_DINV;
 #pragma unroll
-    for (int q = 0; q < KZ + 2; ++q) {
-        const int kk = k0 - 1 + q;
-        if (kk >= 0 && kk < nz) {
-            pl[q][ty + 1][tx + 1] = pc[q];
-            if (tp.has_h) pl[q][tp.hy][tp.hx] = ph[q];
-        }
+    for (int q = 0; q < KZ; ++q) pl[q][ty + 1][tx + 1] = pcol[q + 1];
+    if (tp.has_h) {
+#pragma unroll
+        for (int q = 0; q < KZ; ++q) pl[q][tp.hy][tp.hx] = ph[q];
     }
     __syncthreads();
     double acc[2] = {0.0, 0.0};  // r.z, r.r
@@ -405,32 +458,39 @@ __global__ __launch_bounds__(NT) void k_pcg_update_f64(Fp64Bufs b, StepConsts c,
     for (int q = 1; q <= KZ; ++q) {
         const int k = k0 - 1 + q;
         if (k < nz) {
-            const double xc = pl[q][ty + 1][tx + 1];
+            const double xc = pcol[q];
             const unsigned code = cv[q - 1];
             double qv;
             if (code & SC_IDENTITY) {
                 qv = op_identity(xc, k, c);
             } else {
-                int qu = q + 1, qd = q - 1;
+                double xu = pcol[q + 1], xd = pcol[q - 1];
                 if (c.zbc != 0) {
-                    if (k == 0) qu = qd = q + 1;
-                    else if (k == nz - 1) qu = qd = q - 1;
+                    if (k == 0) xd = xu;
+                    else if (k == nz - 1) xu = xd;
                 }
-                qv = op_row(code, sH, c, xc, pl[q][ty + 1][tx + 2], pl[q][ty + 1][tx], pl[q][ty + 2][tx + 1],
-                            pl[q][ty][tx + 1], pl[qu][ty + 1][tx + 1], pl[qd][ty + 1][tx + 1]);
+                const unsigned cb = (code & 3u) << 4;
+                const HPair<double> hx = sHP[cb | ((code >> 2) & 15u)];
+                const HPair<double> hy = sHP[cb | ((code >> 6) & 15u)];
+                const HPair<double> hz = sHP[cb | ((code >> 10) & 15u)];
+                const double xe = pl[q - 1][ty + 1][tx + 2], xw = pl[q - 1][ty + 1][tx];
+                const double xn = pl[q - 1][ty + 2][tx + 1], xs = pl[q - 1][ty][tx + 1];
+                const double num = hx.p * (xe - xc) - hx.m * (xc - xw) + hy.p * (xn - xc) - hy.m * (xc - xs) +
+                                   hz.p * (xu - xc) - hz.m * (xc - xd);
+                const double Ax = c.h2_pow2 ? num * c.inv_h2 : num / c.h2;
+                qv = -Ax;
             }
             if (tp.live) {
-                const long long id = cbase + q * plane;
-                b.phi[id] = alpha * xc + xv[q - 1];         // Daxpy x += alpha p (:495)
-                const double r = nalpha * qv + rv[q - 1];    // Daxpy r -= alpha Ap (:499)
-                b.r[id] = r;
-                const double z = r * dinv[code];            // :502
+                bstore(Rx, G.cofs * 8u, (unsigned)k * pb8, alpha * xc + xv[q - 1]);  // Daxpy x += alpha p (:495)
+                const double r = nalpha * qv + rv[q - 1];                            // Daxpy r -= alpha Ap (:499)
+                bstore(Rr, G.cofs * 8u, (unsigned)k * pb8, r);
+                const double z = r * dv[q - 1];  // :502
                 acc[0] += r * z;
                 acc[1] += r * r;
             }
         }
     }
-    write_block_partials<2>(acc, b.part[io], nb);
+    write_block_partials<2>(acc, io ? b.part[1] : b.part[0], ps);
     s.kind = K_UPDATE;
     pass_state(s, b.st + io);
 }
@@ -443,9 +503,10 @@ __global__ __launch_bounds__(NT) void k_diag_f64(const double* __restrict__ P,
                                                   int nb) {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long N = g.N;
-    const int i = blockIdx.x * BX + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    const int k0 = blockIdx.z * g.kz;
+    const TileId T = tile_id();
+    const int i = T.bx * BX + threadIdx.x;
+    const int j = T.by * BY + threadIdx.y;
+    const int k0 = T.bz * g.kz;
     // sum Pmz, sum Psz, FE: mask*Pmz, mask; DE: mask*Pmz, mask; AFE: mask*Pmz, mask
     double a[DIAG_Q] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (i < nx && j < ny) {
@@ -478,27 +539,15 @@ void launch_tdgl_f64(hipStream_t s, const Grid& g, const StepConsts& c, double E
 
 void launch_pcg_setup_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b,
                           const PcgState& init) {
-    hipLaunchKernelGGL(k_pcg_setup_f64, grid3(g), dim3(BX, BY, 1), 0, s, b, c, blocks_for(g), init);
+    hipLaunchKernelGGL(k_pcg_setup_f64, grid3(g), dim3(BX, BY, 1), 0, s, b, c, part_stride(g), init);
 }
 
 void launch_pcg_dir_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t) {
-    const dim3 gr = grid3(g), bl(BX, BY, 1);
-    const int nb = blocks_for(g);
-    switch (g.kz) {
-        case 2: hipLaunchKernelGGL(k_pcg_dir_f64<2>, gr, bl, 0, s, b, c, nb, t); break;
-        case 4: hipLaunchKernelGGL(k_pcg_dir_f64<4>, gr, bl, 0, s, b, c, nb, t); break;
-        default: hipLaunchKernelGGL(k_pcg_dir_f64<8>, gr, bl, 0, s, b, c, nb, t); break;
-    }
+    HZO_PCG_LAUNCH(k_pcg_dir_f64, g, s, b, c, blocks_for(g), part_stride(g), t);
 }
 
 void launch_pcg_update_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t) {
-    const dim3 gr = grid3(g), bl(BX, BY, 1);
-    const int nb = blocks_for(g);
-    switch (g.kz) {
-        case 2: hipLaunchKernelGGL(k_pcg_update_f64<2>, gr, bl, 0, s, b, c, nb, t); break;
-        case 4: hipLaunchKernelGGL(k_pcg_update_f64<4>, gr, bl, 0, s, b, c, nb, t); break;
-        default: hipLaunchKernelGGL(k_pcg_update_f64<8>, gr, bl, 0, s, b, c, nb, t); break;
-    }
+    HZO_PCG_LAUNCH(k_pcg_update_f64, g, s, b, c, blocks_for(g), part_stride(g), t);
 }
 
 void launch_diag_f64(hipStream_t s, const Grid& g, const double* P, const uint32_t* cw, double* part) {

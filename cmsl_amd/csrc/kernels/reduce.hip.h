@@ -28,9 +28,9 @@ __device__ __forceinline__ int linear_tid() {
 }
 
 // Workgroup sum of NQ per-thread values; thread 0 stores them at
-// out[q*nb + block].
+// out[q*stride + block].
 template <int NQ>
-__device__ __forceinline__ void write_block_partials(const double (&v)[NQ], double* out, int nb) {
+__device__ __forceinline__ void write_block_partials(const double (&v)[NQ], double* out, int stride) {
     __shared__ double sred[NQ][16];
     const int tid = linear_tid();
     const int lane = tid & 63, wid = tid >> 6;
@@ -47,29 +47,56 @@ __device__ __forceinline__ void write_block_partials(const double (&v)[NQ], doub
         for (int q = 0; q < NQ; ++q) {
             double s = 0.0;
             for (int w = 0; w < nw; ++w) s += sred[q][w];
-            out[(long long)q * nb + b] = s;
+            out[(long long)q * stride + b] = s;
         }
     }
 }
 
-// Totals of the NQ partial arrays written by the previous kernel; identical
-// in every thread of every workgroup.
-template <int NQ>
-__device__ __forceinline__ void sum_partials(const double* in, int nb, double (&tot)[NQ]) {
+// Cross-kernel reduction of the PCG partials.  Partial arrays have a padded
+// stride `ps` = PM * (threads per workgroup) whose tail beyond the number of
+// producing workgroups is zero for the life of the buffer, so thread t of
+// every consuming workgroup loads entries t, t + NT, ... (PM of them) with
+// no bounds test, and every workgroup adds them in the same fixed order.
+// PM = 0 selects a loop over `nb` entries for very large films.
+//
+// The raw values are loaded first (load_partials) and summed later
+// (finish_partials), so the loads stay in flight behind the tile loads.
+template <int NQ, int PM>
+struct Partials {
+    double v[NQ][PM > 0 ? PM : 1];
+};
+
+template <int NQ, int PM>
+__device__ __forceinline__ void load_partials(const double* in, int nb, int ps, Partials<NQ, PM>& P) {
+    const int tid = linear_tid();
+    const int nthreads = blockDim.x * blockDim.y * blockDim.z;
+    if (PM > 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int m = 0; m < (PM > 0 ? PM : 1); ++m) P.v[q][m] = in[(long long)q * ps + tid + m * nthreads];
+    } else {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) P.v[q][0] = 0.0;
+        for (int t = tid; t < nb; t += nthreads) {
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) P.v[q][0] += in[(long long)q * ps + t];
+        }
+    }
+}
+
+template <int NQ, int PM>
+__device__ __forceinline__ void finish_partials(const Partials<NQ, PM>& P, double (&tot)[NQ]) {
     __shared__ double sred[NQ][16];
     const int tid = linear_tid();
     const int nthreads = blockDim.x * blockDim.y * blockDim.z;
     const int lane = tid & 63, wid = tid >> 6, nw = nthreads >> 6;
-    double acc[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
-    for (int t = tid; t < nb; t += nthreads) {
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) acc[q] += in[(long long)q * nb + t];
-    }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        const double w = wave_sum(acc[q]);
+        double a = P.v[q][0];
+#pragma unroll
+        for (int m = 1; m < (PM > 0 ? PM : 1); ++m) a += P.v[q][m];
+        const double w = wave_sum(a);
         if (lane == 0) sred[q][wid] = w;
     }
     __syncthreads();
@@ -80,6 +107,13 @@ __device__ __forceinline__ void sum_partials(const double* in, int nb, double (&
         tot[q] = s;
     }
     __syncthreads();
+}
+
+template <int NQ, int PM>
+__device__ __forceinline__ void sum_partials(const double* in, int nb, int ps, double (&tot)[NQ]) {
+    Partials<NQ, PM> P;
+    load_partials<NQ, PM>(in, nb, ps, P);
+    finish_partials<NQ, PM>(P, tot);
 }
 
 }  // namespace hzo

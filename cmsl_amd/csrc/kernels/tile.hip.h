@@ -35,10 +35,10 @@ struct TilePos {
     int gi, gj;   // its grid column (wrapped)
 };
 
-__device__ __forceinline__ TilePos tile_pos(const StepConsts& c) {
+__device__ __forceinline__ TilePos tile_pos(const StepConsts& c, const TileId& T) {
     TilePos t;
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+    const int i0 = T.bx * TX, j0 = T.by * TY;
     t.i = i0 + tx;
     t.j = j0 + ty;
     t.live = t.i < c.nx && t.j < c.ny;
@@ -62,6 +62,79 @@ __device__ __forceinline__ TilePos tile_pos(const StepConsts& c) {
     t.gi = wrapi(i0 + t.hx - 1, c.nx);
     t.gj = wrapi(j0 + t.hy - 1, c.ny);
     return t;
+}
+
+// ---------------------------------------------------------------- buffer access
+// Raw buffer loads/stores: a 128-bit resource (base, byte size) in SGPRs, a
+// 32-bit per-lane byte offset (constant down a column) and a workgroup-
+// uniform byte offset (the z plane) in an SGPR, so walking planes costs no
+// per-lane address arithmetic.  The host limits a film to 2^28 voxels, which
+// keeps every byte offset of an FP64 array below 2^31.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void* p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+template <class T>
+__device__ __forceinline__ T bload(rsrc_t r, unsigned voff, unsigned soff);
+template <>
+__device__ __forceinline__ float bload<float>(rsrc_t r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+template <>
+__device__ __forceinline__ double bload<double>(rsrc_t r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ unsigned bload_code(rsrc_t r, unsigned voff, unsigned soff) {
+    return (unsigned)__builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0);
+}
+__device__ __forceinline__ void bstore(rsrc_t r, unsigned voff, unsigned soff, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, 0);
+}
+__device__ __forceinline__ void bstore(rsrc_t r, unsigned voff, unsigned soff, double v) {
+    using u2 = unsigned __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, (int)voff, (int)soff, 0);
+}
+
+// Column geometry of one thread of a tile: element offsets within a z plane
+// of its own column and of the halo column it stages.  Every wave issues the
+// halo loads (no branch, so the compiler keeps precise memory-counter waits);
+// lanes that stage no halo voxel get the offset nvox, one past the end of
+// every per-voxel array, which the buffer unit answers with 0 and no memory
+// access.
+struct TileCols {
+    unsigned cofs, hofs;
+};
+
+__device__ __forceinline__ TileCols tile_cols(const TilePos& tp, const StepConsts& c) {
+    TileCols g;
+    g.cofs = (unsigned)tp.ci + (unsigned)c.nx * (unsigned)tp.cj;
+    g.hofs = tp.has_h ? (unsigned)tp.gi + (unsigned)c.nx * (unsigned)tp.gj : c.nvox;
+    return g;
+}
+
+// Paired harmonic face means: entry (c << 4) | (a | b << 2) holds
+// (H[c][a], H[c][b]), so one LDS read serves the + and - face of one axis
+// (stencil-code bits 2-5: e,w; 6-9: n,s; 10-13: u,d).  Thread 0 writes all
+// 64 pairs from the 16 values in the kernel arguments with constant indices
+// (uniform registers, no vector-memory access); the kernel's next barrier
+// publishes them.
+template <class T>
+struct HPair {
+    T p, m;
+};
+
+template <class T>
+__device__ __forceinline__ void build_hpairs(HPair<T>* sHP, const T (&h)[16]) {
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+#pragma unroll
+        for (int e = 0; e < 64; ++e) {
+            const int row = (e >> 4) * 4;
+            sHP[e].p = h[row + (e & 3)];
+            sHP[e].m = h[row + ((e >> 2) & 3)];
+        }
+    }
 }
 
 // The z planes whose operand values the tile's planes [k0, k1) need: k0-1 and
