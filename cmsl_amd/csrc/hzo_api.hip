@@ -263,10 +263,7 @@ int upload_phase(hzo_ctx* x) {
         lut[hzo::LUT_DINV + code] = (D > 1e-30 ? 1.0 / D : 1.0);
     }
     lut[hzo::LUT_DINV + hzo::SC_IDENTITY] = 1.0;
-    for (int q = 0; q < 16; ++q) {
-        x->c.harm[q] = lut[hzo::LUT_HARM + q];
-        x->c.harm32[q] = (float)lut[hzo::LUT_HARM + q];
-    }
+
     HZO_HIP(hipMemcpy(x->ptab, pt, sizeof(pt), hipMemcpyHostToDevice));
     HZO_HIP(hipMemcpy(x->lut, lut.data(), sizeof(double) * hzo::LUT_DOUBLES, hipMemcpyHostToDevice));
     if (x->fp32) {

@@ -94,11 +94,6 @@ struct StepConsts {
     long long ref_id;  // Neumann gauge node
     unsigned plane;    // nx*ny
     unsigned nvox;     // nx*ny*nz (at most 2^28)
-    // harmonic face means H[a*4+b] of the permittivity classes (the same
-    // values as the lut's harm table), carried in the kernel arguments so
-    // a kernel builds its LDS copy without a vector-memory round trip
-    double harm[16];
-    float harm32[16];
 };
 
 // Launch helpers (kernels_fp64.hip).

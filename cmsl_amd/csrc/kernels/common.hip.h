@@ -19,16 +19,17 @@ inline int part_stride_impl(int nb) {
     return pm ? pm * NT : nb;
 }
 
-// Launch of a PCG kernel template <int KZ, int PM> with the instance that
-// matches the grid's tile depth and partial layout.
-#define HZO_PCG_LAUNCH_PM(kern, KZ, pm, gr, bl, s, ...)                               \
-    switch (pm) {                                                                     \
-        case 1: hipLaunchKernelGGL((kern<KZ, 1>), gr, bl, 0, s, __VA_ARGS__); break;  \
-        case 2: hipLaunchKernelGGL((kern<KZ, 2>), gr, bl, 0, s, __VA_ARGS__); break;  \
-        case 4: hipLaunchKernelGGL((kern<KZ, 4>), gr, bl, 0, s, __VA_ARGS__); break;  \
-        default: hipLaunchKernelGGL((kern<KZ, 0>), gr, bl, 0, s, __VA_ARGS__); break; \
+// Launch of a PCG kernel template <int KZ, int PM, EXTRA...> with the
+// instance that matches the grid's tile depth and partial layout.  `X` is the
+// kernel's trailing template argument list (or nothing).
+#define HZO_PCG_LAUNCH_PM(kern, KZ, pm, gr, bl, s, ...)                                   \
+    switch (pm) {                                                                         \
+        case 1: hipLaunchKernelGGL((kern<KZ, 1 HZO_X>), gr, bl, 0, s, __VA_ARGS__); break;  \
+        case 2: hipLaunchKernelGGL((kern<KZ, 2 HZO_X>), gr, bl, 0, s, __VA_ARGS__); break;  \
+        case 4: hipLaunchKernelGGL((kern<KZ, 4 HZO_X>), gr, bl, 0, s, __VA_ARGS__); break;  \
+        default: hipLaunchKernelGGL((kern<KZ, 0 HZO_X>), gr, bl, 0, s, __VA_ARGS__); break; \
     }
-#define HZO_PCG_LAUNCH(kern, g, s, ...)                                                   \
+#define HZO_PCG_LAUNCH_X(kern, g, s, ...)                                                 \
     do {                                                                                  \
         const int pm_ = part_pm(blocks_for_impl(g));                                      \
         const dim3 gr_ = grid3(g), bl_(BX, BY, 1);                                        \

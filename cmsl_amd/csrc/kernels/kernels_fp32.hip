@@ -193,12 +193,19 @@ __global__ __launch_bounds__(NT) void k_pcg_setup_f32(Fp32Bufs b, StepConsts c, 
     pass_state(init, b.st);
 }
 
+// Scalars wave 0 hands to the rest of the workgroup after the loop test.
+struct Bcast {
+    double coef;  // beta (direction kernels) or alpha (update kernels)
+    int flag;     // 0 continue, 1 done, 2 fold
+};
+
 // ---------------------------------------------------------------- direction / fold
 // Finishes the previous kernel's inner products and decides what this launch
 // is: a CG direction update (K_DIR: p = z + beta p, q = A p, p.q) or the fold
-// x_hi += x_lo that starts a residual replacement (K_FOLD).  Load order and
-// tile staging as in the FP64 form; D^-1 is the FP32 rounding of the exact
-// per-code table, the same matrix in every FP32 kernel.
+// x_hi += x_lo that starts a residual replacement (K_FOLD).  Load order, the
+// wave-0 control logic and the tile staging are those of the FP64 form; D^-1
+// is the FP32 rounding of the exact per-code table, the same matrix in every
+// FP32 kernel.
 template <int KZ, int PM>
 __global__ __launch_bounds__(NT, 4) void k_pcg_dir_f32(Fp32Bufs b, StepConsts c, int nb, int ps, int t) {
     const int ip = (t - 1) & 1, io = t & 1;
@@ -210,7 +217,10 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_dir_f32(Fp32Bufs b, StepConsts c,
     const bool first = s.kind == K_SETUP;
     __shared__ HPair<float> sHP[64];
     __shared__ float pl[KZ][HY][HX];
+    __shared__ RawLDS<3> sraw;
+    __shared__ Bcast bc;
     const int nz = c.nz;
+    const int tid = linear_tid();
     const TileId T = tile_id();
     const int k0 = T.bz * KZ;
     const TilePos tp = tile_pos(c, T);
@@ -224,16 +234,17 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_dir_f32(Fp32Bufs b, StepConsts c,
     const rsrc_t Rdi = make_rsrc(b.lut32 + LUT_DINV, SC_LUT_SIZE * 4u);
     const double* pin = ip ? b.part[1] : b.part[0];
 
+    const HPair<float> hpv = load_hpair(b.lut32);
     Partials<3, PM> P;
     if (first) {
         load_partials<3, PM>(pin, nb, ps, P);
     } else {
-        Partials<2, PM> P2;
-        load_partials<2, PM>(pin, nb, ps, P2);
+        Partials<2, PM> Q2;
+        load_partials<2, PM>(pin, nb, ps, Q2);
 #pragma unroll
         for (int m = 0; m < (PM > 0 ? PM : 1); ++m) {
-            P.v[0][m] = P2.v[0][m];
-            P.v[1][m] = P2.v[1][m];
+            P.v[0][m] = Q2.v[0][m];
+            P.v[1][m] = Q2.v[1][m];
             P.v[2][m] = 0.0;
         }
     }
@@ -243,75 +254,86 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_dir_f32(Fp32Bufs b, StepConsts c,
     unsigned cc[KZ + 2], ch[KZ];
 #pragma unroll
     for (int q = 0; q < KZ + 2; ++q) cc[q] = bload_code(Rsc, G.cofs * 2u, so[q] * pb2);
-    {
 #pragma unroll
-        for (int q = 0; q < KZ; ++q) ch[q] = bload_code(Rsc, G.hofs * 2u, so[q + 1] * pb2);
-    }
+    for (int q = 0; q < KZ; ++q) ch[q] = bload_code(Rsc, G.hofs * 2u, so[q + 1] * pb2);
     float rc[KZ + 2], pc[KZ + 2], rh[KZ], ph[KZ];
 #pragma unroll
     for (int q = 0; q < KZ + 2; ++q) {
         rc[q] = bload<float>(Rr, G.cofs * 4u, so[q] * pb4);
         pc[q] = bload<float>(Rpo, G.cofs * 4u, so[q] * pb4);
     }
-    {
 #pragma unroll
-        for (int q = 0; q < KZ; ++q) {
-            rh[q] = bload<float>(Rr, G.hofs * 4u, so[q + 1] * pb4);
-            ph[q] = bload<float>(Rpo, G.hofs * 4u, so[q + 1] * pb4);
-        }
+    for (int q = 0; q < KZ; ++q) {
+        rh[q] = bload<float>(Rr, G.hofs * 4u, so[q + 1] * pb4);
+        ph[q] = bload<float>(Rpo, G.hofs * 4u, so[q + 1] * pb4);
     }
     float dc[KZ + 2], dh[KZ];
 #pragma unroll
     for (int q = 0; q < KZ + 2; ++q) dc[q] = bload<float>(Rdi, cc[q] * 4u, 0u);
-    {
 #pragma unroll
-        for (int q = 0; q < KZ; ++q) dh[q] = bload<float>(Rdi, ch[q] * 4u, 0u);
-    }
-    build_hpairs(sHP, c.harm32);
+    for (int q = 0; q < KZ; ++q) dh[q] = bload<float>(Rdi, ch[q] * 4u, 0u);
 
-    double tot[3];
-    finish_partials<3, PM>(P, tot);
-    bool fold = false;
-    if (first) {
-        s.rz0 = s.rz_old = tot[0];
-        s.rr0 = s.rr = tot[1];
-        s.bb = tot[2];
-        if (!finite(tot[0]) || tot[0] < 0.0) {
-            s.status = HZO_PCG_BAD_INITIAL;
-            s.done = 1;
+    stage_partials<3, PM>(P, sraw);
+    __syncthreads();
+    if (tid < 64) {
+        sHP[tid] = hpv;
+        double tot[3];
+        wave0_total<3>(sraw, tot);
+        bool fold = false;
+        if (first) {
+            s.rz0 = s.rz_old = tot[0];
+            s.rr0 = s.rr = tot[1];
+            s.bb = tot[2];
+            if (!finite(tot[0]) || tot[0] < 0.0) {
+                s.status = HZO_PCG_BAD_INITIAL;
+                s.done = 1;
+            } else {
+                judge(s, tot[0], tot[1]);
+            }
         } else {
-            judge(s, tot[0], tot[1]);
-        }
-    } else {
-        s.rr = tot[1];
-        if (!finite(tot[0]) || tot[0] < 0.0) {
-            s.status = HZO_PCG_BAD_RZ;
-            s.rz_old = tot[0];
-            s.done = 1;
-        } else if (s.kind == K_RESID) {  // true residual: trust the test
-            s.refinements += 1;
-            s.beta = tot[0] / s.rz_old;
-            s.rz_old = tot[0];
-            judge(s, tot[0], tot[1]);
-        } else {  // K_UPDATE: recursively updated residual
-            if (s.iters >= s.max_iter) {
+            s.rr = tot[1];
+            if (!finite(tot[0]) || tot[0] < 0.0) {
+                s.status = HZO_PCG_BAD_RZ;
                 s.rz_old = tot[0];
                 s.done = 1;
-                s.status = HZO_PCG_MAXITER;
-            } else if (!keep_going(s, tot[0], tot[1]) ||
-                       (s.refine_every > 0 && s.iters % s.refine_every == 0)) {
-                fold = true;  // replace the residual before trusting or continuing it
-            } else {
+            } else if (s.kind == K_RESID) {  // true residual: trust the test
+                s.refinements += 1;
                 s.beta = tot[0] / s.rz_old;
                 s.rz_old = tot[0];
+                judge(s, tot[0], tot[1]);
+            } else {  // K_UPDATE: recursively updated residual
+                if (s.iters >= s.max_iter) {
+                    s.rz_old = tot[0];
+                    s.done = 1;
+                    s.status = HZO_PCG_MAXITER;
+                } else if (!keep_going(s, tot[0], tot[1]) ||
+                           (s.refine_every > 0 && s.iters % s.refine_every == 0)) {
+                    fold = true;  // replace the residual before trusting or continuing it
+                } else {
+                    s.beta = tot[0] / s.rz_old;
+                    s.rz_old = tot[0];
+                }
             }
         }
+        if (s.done) {
+            pass_state(s, b.st + io);
+        } else if (fold) {
+            s.kind = K_FOLD;
+            pass_state(s, b.st + io);
+        } else {
+            s.iters += 1;
+            s.pbuf ^= 1;
+            s.kind = K_DIR;
+        }
+        if (tid == 0) {
+            bc.coef = s.beta;
+            bc.flag = s.done ? 1 : (fold ? 2 : 0);
+        }
     }
-    if (s.done) {
-        pass_state(s, b.st + io);
-        return;
-    }
-    if (fold) {
+    __syncthreads();
+    const int flag = bc.flag;
+    if (flag == 1) return;
+    if (flag == 2) {
         if (tp.live) {
 #pragma unroll
             for (int q = 0; q < KZ; ++q) {
@@ -323,11 +345,9 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_dir_f32(Fp32Bufs b, StepConsts c,
                 }
             }
         }
-        s.kind = K_FOLD;
-        pass_state(s, b.st + io);
         return;
     }
-    const float beta = (float)s.beta;
+    const float beta = (float)bc.coef;
     float pcol[KZ + 2];
 #pragma unroll
     for (int q = 0; q < KZ + 2; ++q) {
@@ -335,45 +355,35 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_dir_f32(Fp32Bufs b, StepConsts c,
         pcol[q] = first ? z : z + beta * pc[q];
         if (q >= 1 && q <= KZ) pl[q - 1][ty + 1][tx + 1] = pcol[q];
     }
-    {
 #pragma unroll
-        for (int q = 0; q < KZ; ++q) {
-            const float zh = rh[q] * dh[q];
-            const float v = first ? zh : zh + beta * ph[q];
-            if (tp.has_h) pl[q][tp.hy][tp.hx] = v;
-        }
+    for (int q = 0; q < KZ; ++q) {
+        const float zh = rh[q] * dh[q];
+        const float v = first ? zh : zh + beta * ph[q];
+        if (tp.has_h) pl[q][tp.hy][tp.hx] = v;
     }
     __syncthreads();
     const float ih2 = (float)c.inv_h2;
+    const unsigned oob = c.nvox * 4u;  // store offset of masked lanes (dropped by the buffer unit)
     double acc[1] = {0.0};
 #pragma unroll
     for (int q = 1; q <= KZ; ++q) {
         const int k = k0 - 1 + q;
-        if (k < nz) {
-            const float xc = pcol[q];
-            const unsigned code = cc[q];
-            float qv;
-            if (code & SC_IDENTITY) {
-                qv = op_identity(xc, k, c);
-            } else {
-                float xu = pcol[q + 1], xd = pcol[q - 1];
-                if (c.zbc != 0) {
-                    if (k == 0) xd = xu;
-                    else if (k == nz - 1) xu = xd;
-                }
-                qv = row_Ax_pairs(code, sHP, ih2, xc, pl[q - 1][ty + 1][tx + 2], pl[q - 1][ty + 1][tx],
-                                  pl[q - 1][ty + 2][tx + 1], pl[q - 1][ty][tx + 1], xu, xd);
-            }
-            if (tp.live) {
-                bstore(Rpn, G.cofs * 4u, (unsigned)k * pb4, xc);
-                acc[0] += (double)xc * (double)qv;
-            }
+        const float xc = pcol[q];
+        const unsigned code = cc[q];
+        float xu = pcol[q + 1], xd = pcol[q - 1];
+        if (c.zbc != 0) {
+            if (k == 0) xd = xu;
+            else if (k == nz - 1) xu = xd;
         }
+        const float qs = row_Ax_pairs(code, sHP, ih2, xc, pl[q - 1][ty + 1][tx + 2], pl[q - 1][ty + 1][tx],
+                                      pl[q - 1][ty + 2][tx + 1], pl[q - 1][ty][tx + 1], xu, xd);
+        const float qv = (code & SC_IDENTITY) ? op_identity(xc, k, c) : qs;
+        const bool ok = tp.live && k < nz;
+        bstore(Rpn, ok ? G.cofs * 4u : oob, so[q] * pb4, xc);
+        if (ok) acc[0] += (double)xc * (double)qv;
     }
-    write_block_partials<1>(acc, io ? b.part[1] : b.part[0], ps);
-    s.iters += 1;
-    s.pbuf ^= 1;
-    s.kind = K_DIR;
+    RawLDS<1>& sacc = *reinterpret_cast<RawLDS<1>*>(&sraw);
+    write_block_partials<1>(acc, io ? b.part[1] : b.part[0], ps, sacc);
     pass_state(s, b.st + io);
 }
 
@@ -386,6 +396,7 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_update_f32(Fp32Bufs b, StepConsts
         pass_state(s, b.st + io);
         return;
     }
+    __shared__ RawLDS<2> sraw;
     const bool resid = s.kind == K_FOLD;
     const int nx = c.nx, ny = c.ny, nz = c.nz;
     const TileId T = tile_id();
@@ -409,9 +420,12 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_update_f32(Fp32Bufs b, StepConsts
                 acc[1] += (double)r * (double)r;
             }
         }
+        s.kind = K_RESID;
     } else {
         __shared__ HPair<float> sHP[64];
         __shared__ float pl[KZ][HY][HX];
+        __shared__ Bcast bc;
+        const int tid = linear_tid();
         const TilePos tp = tile_pos(c, T);
         const TileCols G = tile_cols(tp, c);
         const int tx = threadIdx.x, ty = threadIdx.y;
@@ -422,6 +436,7 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_update_f32(Fp32Bufs b, StepConsts
         const rsrc_t Rsc = make_rsrc(b.sc, c.nvox * 2u);
         const rsrc_t Rdi = make_rsrc(b.lut32 + LUT_DINV, SC_LUT_SIZE * 4u);
 
+        const HPair<float> hpv = load_hpair(b.lut32);
         Partials<1, PM> P;
         load_partials<1, PM>(ip ? b.part[1] : b.part[0], nb, ps, P);
         unsigned so[KZ + 2];
@@ -433,10 +448,8 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_update_f32(Fp32Bufs b, StepConsts
         float pcol[KZ + 2], ph[KZ];
 #pragma unroll
         for (int q = 0; q < KZ + 2; ++q) pcol[q] = bload<float>(Rp, G.cofs * 4u, so[q] * pb4);
-        {
 #pragma unroll
-            for (int q = 0; q < KZ; ++q) ph[q] = bload<float>(Rp, G.hofs * 4u, so[q + 1] * pb4);
-        }
+        for (int q = 0; q < KZ; ++q) ph[q] = bload<float>(Rp, G.hofs * 4u, so[q + 1] * pb4);
         float xv[KZ], rv[KZ];
 #pragma unroll
         for (int q = 0; q < KZ; ++q) {
@@ -446,18 +459,29 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_update_f32(Fp32Bufs b, StepConsts
         float dv[KZ];
 #pragma unroll
         for (int q = 0; q < KZ; ++q) dv[q] = bload<float>(Rdi, cv[q] * 4u, 0u);
-        build_hpairs(sHP, c.harm32);
-        double tot[1];
-        finish_partials<1, PM>(P, tot);
-        s.denom = tot[0];
-        if (!finite(tot[0]) || !(tot[0] > 0.0)) {
-            s.status = HZO_PCG_BREAKDOWN;
-            s.done = 1;
-            pass_state(s, b.st + io);
-            return;
+
+        {
+            RawLDS<1>& s1 = *reinterpret_cast<RawLDS<1>*>(&sraw);
+            stage_partials<1, PM>(P, s1);
         }
-        s.alpha = s.rz_old / tot[0];
-        const float alpha = (float)s.alpha;
+        __syncthreads();
+        if (tid < 64) {
+            sHP[tid] = hpv;
+            double tot[1];
+            wave0_total<1>(*reinterpret_cast<RawLDS<1>*>(&sraw), tot);
+            s.denom = tot[0];
+            if (!finite(tot[0]) || !(tot[0] > 0.0)) {
+                s.status = HZO_PCG_BREAKDOWN;
+                s.done = 1;
+                pass_state(s, b.st + io);
+            } else {
+                s.alpha = s.rz_old / tot[0];
+            }
+            if (tid == 0) {
+                bc.coef = s.alpha;
+                bc.flag = s.done;
+            }
+        }
 #pragma unroll
         for (int q = 0; q < KZ; ++q) pl[q][ty + 1][tx + 1] = pcol[q + 1];
         if (tp.has_h) {
@@ -465,38 +489,37 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_update_f32(Fp32Bufs b, StepConsts
             for (int q = 0; q < KZ; ++q) pl[q][tp.hy][tp.hx] = ph[q];
         }
         __syncthreads();
+        if (bc.flag) return;
+        const float alpha = (float)bc.coef;
         const float ih2 = (float)c.inv_h2;
+        const unsigned oob = c.nvox * 4u;
 #pragma unroll
         for (int q = 1; q <= KZ; ++q) {
             const int k = k0 - 1 + q;
-            if (k < nz) {
-                const float xc = pcol[q];
-                const unsigned code = cv[q - 1];
-                float qv;
-                if (code & SC_IDENTITY) {
-                    qv = op_identity(xc, k, c);
-                } else {
-                    float xu = pcol[q + 1], xd = pcol[q - 1];
-                    if (c.zbc != 0) {
-                        if (k == 0) xd = xu;
-                        else if (k == nz - 1) xu = xd;
-                    }
-                    qv = row_Ax_pairs(code, sHP, ih2, xc, pl[q - 1][ty + 1][tx + 2], pl[q - 1][ty + 1][tx],
-                                      pl[q - 1][ty + 2][tx + 1], pl[q - 1][ty][tx + 1], xu, xd);
-                }
-                if (tp.live) {
-                    bstore(Rx, G.cofs * 4u, (unsigned)k * pb4, xv[q - 1] + alpha * xc);
-                    const float r = rv[q - 1] - alpha * qv;
-                    bstore(Rr, G.cofs * 4u, (unsigned)k * pb4, r);
-                    const float z = r * dv[q - 1];
-                    acc[0] += (double)r * (double)z;
-                    acc[1] += (double)r * (double)r;
-                }
+            const float xc = pcol[q];
+            const unsigned code = cv[q - 1];
+            float xu = pcol[q + 1], xd = pcol[q - 1];
+            if (c.zbc != 0) {
+                if (k == 0) xd = xu;
+                else if (k == nz - 1) xu = xd;
+            }
+            const float qs = row_Ax_pairs(code, sHP, ih2, xc, pl[q - 1][ty + 1][tx + 2], pl[q - 1][ty + 1][tx],
+                                          pl[q - 1][ty + 2][tx + 1], pl[q - 1][ty][tx + 1], xu, xd);
+            const float qv = (code & SC_IDENTITY) ? op_identity(xc, k, c) : qs;
+            const bool ok = tp.live && k < nz;
+            const unsigned vo = ok ? G.cofs * 4u : oob;
+            bstore(Rx, vo, so[q] * pb4, xv[q - 1] + alpha * xc);
+            const float r = rv[q - 1] - alpha * qv;
+            bstore(Rr, vo, so[q] * pb4, r);
+            const float z = r * dv[q - 1];
+            if (ok) {
+                acc[0] += (double)r * (double)z;
+                acc[1] += (double)r * (double)r;
             }
         }
+        s.kind = K_UPDATE;
     }
-    write_block_partials<2>(acc, io ? b.part[1] : b.part[0], ps);
-    s.kind = resid ? K_RESID : K_UPDATE;
+    write_block_partials<2>(acc, io ? b.part[1] : b.part[0], ps, sraw);
     pass_state(s, b.st + io);
 }
 
@@ -537,12 +560,14 @@ void launch_pcg_setup_f32(hipStream_t s, const Grid& g, const StepConsts& c, con
                           const PcgState& init) {
     hipLaunchKernelGGL(k_pcg_setup_f32, grid3(g), dim3(BX, BY, 1), 0, s, b, c, part_stride(g), init);
 }
+#define HZO_X
 void launch_pcg_dir_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b, int t) {
-    HZO_PCG_LAUNCH(k_pcg_dir_f32, g, s, b, c, blocks_for(g), part_stride(g), t);
+    HZO_PCG_LAUNCH_X(k_pcg_dir_f32, g, s, b, c, blocks_for(g), part_stride(g), t);
 }
 void launch_pcg_update_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b, int t) {
-    HZO_PCG_LAUNCH(k_pcg_update_f32, g, s, b, c, blocks_for(g), part_stride(g), t);
+    HZO_PCG_LAUNCH_X(k_pcg_update_f32, g, s, b, c, blocks_for(g), part_stride(g), t);
 }
+#undef HZO_X
 void launch_diag_f32(hipStream_t s, const Grid& g, const float* P, const uint32_t* cw, double* part) {
     hipLaunchKernelGGL(k_diag_f32, grid3(g), dim3(BX, BY, 1), 0, s, P, cw, g, part, blocks_for_impl(g));
 }

@@ -213,20 +213,43 @@ __global__ __launch_bounds__(NT) void k_pcg_setup_f64(Fp64Bufs b, StepConsts c, 
     pass_state(init, b.st);
 }
 
+// A x of one non-identity row from the paired face table, in the reference's
+// operand order (depol_varEPS.cu:76-123): num / h^2, as a multiplication by
+// the exact reciprocal when h^2 is a power of two (P2).
+template <bool P2>
+__device__ __forceinline__ double row_num(unsigned code, const HPair<double>* sHP, const StepConsts& c, double xc,
+                                          double xe, double xw, double xn, double xs, double xu, double xd) {
+    const unsigned cb = (code & 3u) << 4;
+    const HPair<double> hx = sHP[cb | ((code >> 2) & 15u)];
+    const HPair<double> hy = sHP[cb | ((code >> 6) & 15u)];
+    const HPair<double> hz = sHP[cb | ((code >> 10) & 15u)];
+    const double num = hx.p * (xe - xc) - hx.m * (xc - xw) + hy.p * (xn - xc) - hy.m * (xc - xs) +
+                       hz.p * (xu - xc) - hz.m * (xc - xd);
+    return P2 ? num * c.inv_h2 : num / c.h2;
+}
+
+// Scalars wave 0 hands to the rest of the workgroup after the loop test.
+struct Bcast {
+    double coef;  // beta (direction kernels) or alpha (update kernels)
+    int flag;     // 0 continue, 1 done, 2 fold (FP32-refined)
+};
+
 // ---------------------------------------------------------------- PCG direction + SpMV
 // p_new = z + beta*p_old (first iteration: p_new = z), q = A p_new, p_new.q.
 // z = D^-1 r is formed from r and the exact per-code D^-1 table wherever it
 // is staged, so z is never stored.
 //
-// Issue order per workgroup: the previous kernel's partials, the stencil
-// codes, r and p_old of the KZ planes and of the two z-ghost planes (centre
-// column only: ghost planes feed just the u/d neighbours), the x-y halo of
-// the KZ planes, then the D^-1 gathers (which need only the codes).  The
-// reduction is finished and the loop test applied while the tile loads are
-// in flight.  Plane indices are clamped into the film instead of branched
-// on; values of clamped planes are never used by a live row.  The thread's
-// own column of p stays in registers; e/w/n/s come from LDS.
-template <int KZ, int PM>
+// Issue order per workgroup: the face-pair table, the previous kernel's
+// partials, the stencil codes, r and p_old of the KZ planes and of the two
+// z-ghost planes (centre column only: ghost planes feed just the u/d
+// neighbours), the x-y halo of the KZ planes, then the D^-1 gathers (which
+// need only the codes).  Wave 0 finishes the reduction and applies the loop
+// test while the tile loads are in flight and broadcasts beta through LDS.
+// Plane indices are clamped into the film instead of branched on (values of
+// clamped planes feed no live row) and masked stores use an out-of-range
+// buffer offset, so the tile code is straight-line.  The thread's own
+// column of p stays in registers; e/w/n/s come from LDS.
+template <int KZ, int PM, bool P2>
 __global__ __launch_bounds__(NT, 4) void k_pcg_dir_f64(Fp64Bufs b, StepConsts c, int nb, int ps, int t) {
     const int ip = (t - 1) & 1, io = t & 1;
     PcgState s = load_state(b.st + ip);
@@ -237,7 +260,10 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_dir_f64(Fp64Bufs b, StepConsts c,
     const bool first = s.kind == K_SETUP;
     __shared__ HPair<double> sHP[64];
     __shared__ double pl[KZ][HY][HX];
+    __shared__ RawLDS<3> sraw;
+    __shared__ Bcast bc;
     const int nz = c.nz;
+    const int tid = linear_tid();
     const TileId T = tile_id();
     const int k0 = T.bz * KZ;
     const TilePos tp = tile_pos(c, T);
@@ -251,82 +277,90 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_dir_f64(Fp64Bufs b, StepConsts c,
     const rsrc_t Rdi = make_rsrc(b.lut + LUT_DINV, SC_LUT_SIZE * 8u);
     const double* pin = ip ? b.part[1] : b.part[0];
 
+    const HPair<double> hpv = load_hpair(b.lut);
     Partials<3, PM> P;
     if (first) {
         load_partials<3, PM>(pin, nb, ps, P);
     } else {
-        Partials<2, PM> P2;
-        load_partials<2, PM>(pin, nb, ps, P2);
+        Partials<2, PM> Q2;
+        load_partials<2, PM>(pin, nb, ps, Q2);
 #pragma unroll
         for (int m = 0; m < (PM > 0 ? PM : 1); ++m) {
-            P.v[0][m] = P2.v[0][m];
-            P.v[1][m] = P2.v[1][m];
+            P.v[0][m] = Q2.v[0][m];
+            P.v[1][m] = Q2.v[1][m];
             P.v[2][m] = 0.0;
         }
     }
-    unsigned so[KZ + 2];  // byte offset of each staged plane (clamped)
+    unsigned so[KZ + 2];  // clamped plane of each staged slot
 #pragma unroll
     for (int q = 0; q < KZ + 2; ++q) so[q] = (unsigned)min(max(k0 - 1 + q, 0), nz - 1);
     unsigned cc[KZ + 2], ch[KZ];
 #pragma unroll
     for (int q = 0; q < KZ + 2; ++q) cc[q] = bload_code(Rsc, G.cofs * 2u, so[q] * pb2);
-    {
 #pragma unroll
-        for (int q = 0; q < KZ; ++q) ch[q] = bload_code(Rsc, G.hofs * 2u, so[q + 1] * pb2);
-    }
+    for (int q = 0; q < KZ; ++q) ch[q] = bload_code(Rsc, G.hofs * 2u, so[q + 1] * pb2);
     double rc[KZ + 2], pc[KZ + 2], rh[KZ], ph[KZ];
 #pragma unroll
     for (int q = 0; q < KZ + 2; ++q) {
         rc[q] = bload<double>(Rr, G.cofs * 8u, so[q] * pb8);
         pc[q] = bload<double>(Rpo, G.cofs * 8u, so[q] * pb8);
     }
-    {
 #pragma unroll
-        for (int q = 0; q < KZ; ++q) {
-            rh[q] = bload<double>(Rr, G.hofs * 8u, so[q + 1] * pb8);
-            ph[q] = bload<double>(Rpo, G.hofs * 8u, so[q + 1] * pb8);
-        }
+    for (int q = 0; q < KZ; ++q) {
+        rh[q] = bload<double>(Rr, G.hofs * 8u, so[q + 1] * pb8);
+        ph[q] = bload<double>(Rpo, G.hofs * 8u, so[q + 1] * pb8);
     }
     double dc[KZ + 2], dh[KZ];
 #pragma unroll
     for (int q = 0; q < KZ + 2; ++q) dc[q] = bload<double>(Rdi, cc[q] * 8u, 0u);
-    {
 #pragma unroll
-        for (int q = 0; q < KZ; ++q) dh[q] = bload<double>(Rdi, ch[q] * 8u, 0u);
-    }
-    build_hpairs(sHP, c.harm);
+    for (int q = 0; q < KZ; ++q) dh[q] = bload<double>(Rdi, ch[q] * 8u, 0u);
 
-    // finish the previous kernel's inner products (setup: r.z, r.r, b.b;
-    // update: r.z, r.r) and apply the loop test (depol_varEPS.cu:459-474, 506-521)
-    double tot[3];
-    finish_partials<3, PM>(P, tot);
-    if (first) {
-        s.rz0 = s.rz_old = tot[0];
-        s.rr0 = s.rr = tot[1];
-        s.bb = tot[2];
-        if (!finite(tot[0]) || tot[0] < 0.0) {  // :466-469
-            s.status = HZO_PCG_BAD_INITIAL;
-            s.done = 1;
+    stage_partials<3, PM>(P, sraw);
+    __syncthreads();
+    if (tid < 64) {
+        // the previous kernel's inner products (setup: r.z, r.r, b.b; update:
+        // r.z, r.r) and the loop test (depol_varEPS.cu:459-474, 506-521)
+        sHP[tid] = hpv;
+        double tot[3];
+        wave0_total<3>(sraw, tot);
+        if (first) {
+            s.rz0 = s.rz_old = tot[0];
+            s.rr0 = s.rr = tot[1];
+            s.bb = tot[2];
+            if (!finite(tot[0]) || tot[0] < 0.0) {  // :466-469
+                s.status = HZO_PCG_BAD_INITIAL;
+                s.done = 1;
+            } else {
+                judge(s, tot[0], tot[1]);
+            }
         } else {
-            judge(s, tot[0], tot[1]);
+            s.rr = tot[1];
+            if (!finite(tot[0]) || tot[0] < 0.0) {  // :508-513
+                s.status = HZO_PCG_BAD_RZ;
+                s.rz_old = tot[0];
+                s.done = 1;
+            } else {
+                s.beta = tot[0] / s.rz_old;  // :515
+                s.rz_old = tot[0];
+                judge(s, tot[0], tot[1]);
+            }
         }
-    } else {
-        s.rr = tot[1];
-        if (!finite(tot[0]) || tot[0] < 0.0) {  // :508-513
-            s.status = HZO_PCG_BAD_RZ;
-            s.rz_old = tot[0];
-            s.done = 1;
+        if (s.done) {
+            pass_state(s, b.st + io);
         } else {
-            s.beta = tot[0] / s.rz_old;  // :515
-            s.rz_old = tot[0];
-            judge(s, tot[0], tot[1]);
+            s.iters += 1;  // ++kiter (:476)
+            s.pbuf ^= 1;
+            s.kind = K_DIR;
+        }
+        if (tid == 0) {
+            bc.coef = s.beta;
+            bc.flag = s.done;
         }
     }
-    if (s.done) {
-        pass_state(s, b.st + io);
-        return;
-    }
-    const double beta = s.beta;
+    __syncthreads();
+    if (bc.flag) return;
+    const double beta = bc.coef;
     double pcol[KZ + 2];
 #pragma unroll
     for (int q = 0; q < KZ + 2; ++q) {
@@ -334,59 +368,41 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_dir_f64(Fp64Bufs b, StepConsts c,
         pcol[q] = first ? z : z + beta * pc[q];  // :518-519
         if (q >= 1 && q <= KZ) pl[q - 1][ty + 1][tx + 1] = pcol[q];
     }
-    {
 #pragma unroll
-        for (int q = 0; q < KZ; ++q) {
-            const double zh = rh[q] * dh[q];
-            const double v = first ? zh : zh + beta * ph[q];
-            if (tp.has_h) pl[q][tp.hy][tp.hx] = v;
-        }
+    for (int q = 0; q < KZ; ++q) {
+        const double zh = rh[q] * dh[q];
+        const double v = first ? zh : zh + beta * ph[q];
+        if (tp.has_h) pl[q][tp.hy][tp.hx] = v;
     }
     __syncthreads();
+    const unsigned oob = c.nvox * 8u;  // store offset of masked lanes (dropped by the buffer unit)
     double acc[1] = {0.0};
 #pragma unroll
     for (int q = 1; q <= KZ; ++q) {
         const int k = k0 - 1 + q;
-        if (k < nz) {
-            const double xc = pcol[q];
-            const unsigned code = cc[q];
-            double qv;
-            if (code & SC_IDENTITY) {
-                qv = op_identity(xc, k, c);
-            } else {
-                double xu = pcol[q + 1], xd = pcol[q - 1];
-                if (c.zbc != 0) {  // mirror ghosts (depol_varEPS.cu:94-98)
-                    if (k == 0) xd = xu;
-                    else if (k == nz - 1) xu = xd;
-                }
-                const unsigned cb = (code & 3u) << 4;
-                const HPair<double> hx = sHP[cb | ((code >> 2) & 15u)];
-                const HPair<double> hy = sHP[cb | ((code >> 6) & 15u)];
-                const HPair<double> hz = sHP[cb | ((code >> 10) & 15u)];
-                const double xe = pl[q - 1][ty + 1][tx + 2], xw = pl[q - 1][ty + 1][tx];
-                const double xn = pl[q - 1][ty + 2][tx + 1], xs = pl[q - 1][ty][tx + 1];
-                const double num = hx.p * (xe - xc) - hx.m * (xc - xw) + hy.p * (xn - xc) - hy.m * (xc - xs) +
-                                   hz.p * (xu - xc) - hz.m * (xc - xd);
-                const double Ax = c.h2_pow2 ? num * c.inv_h2 : num / c.h2;
-                qv = -Ax;
-            }
-            if (tp.live) {
-                bstore(Rpn, G.cofs * 8u, (unsigned)k * pb8, xc);
-                acc[0] += xc * qv;  // p.Ap (:485)
-            }
+        const double xc = pcol[q];
+        const unsigned code = cc[q];
+        double xu = pcol[q + 1], xd = pcol[q - 1];
+        if (c.zbc != 0) {  // mirror ghosts (depol_varEPS.cu:94-98)
+            if (k == 0) xd = xu;
+            else if (k == nz - 1) xu = xd;
         }
+        const double Ax = row_num<P2>(code, sHP, c, xc, pl[q - 1][ty + 1][tx + 2], pl[q - 1][ty + 1][tx],
+                                      pl[q - 1][ty + 2][tx + 1], pl[q - 1][ty][tx + 1], xu, xd);
+        const double qv = (code & SC_IDENTITY) ? op_identity(xc, k, c) : -Ax;
+        const bool ok = tp.live && k < nz;
+        bstore(Rpn, ok ? G.cofs * 8u : oob, so[q] * pb8, xc);
+        if (ok) acc[0] += xc * qv;  // p.Ap (:485)
     }
-    write_block_partials<1>(acc, io ? b.part[1] : b.part[0], ps);
-    s.iters += 1;  // ++kiter (:476)
-    s.pbuf ^= 1;
-    s.kind = K_DIR;
+    RawLDS<1>& sacc = *reinterpret_cast<RawLDS<1>*>(&sraw);
+    write_block_partials<1>(acc, io ? b.part[1] : b.part[0], ps, sacc);
     pass_state(s, b.st + io);
 }
 
 // ---------------------------------------------------------------- PCG update
 // x += alpha p, r -= alpha A p (A p re-evaluated from p instead of stored),
 // z = D^-1 r, r.z and r.r (depol_varEPS.cu:492-507).
-template <int KZ, int PM>
+template <int KZ, int PM, bool P2>
 __global__ __launch_bounds__(NT, 4) void k_pcg_update_f64(Fp64Bufs b, StepConsts c, int nb, int ps, int t) {
     const int ip = (t - 1) & 1, io = t & 1;
     PcgState s = load_state(b.st + ip);
@@ -396,7 +412,10 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_update_f64(Fp64Bufs b, StepConsts
     }
     __shared__ HPair<double> sHP[64];
     __shared__ double pl[KZ][HY][HX];
+    __shared__ RawLDS<2> sraw;
+    __shared__ Bcast bc;
     const int nz = c.nz;
+    const int tid = linear_tid();
     const TileId T = tile_id();
     const int k0 = T.bz * KZ;
     const TilePos tp = tile_pos(c, T);
@@ -409,6 +428,7 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_update_f64(Fp64Bufs b, StepConsts
     const rsrc_t Rsc = make_rsrc(b.sc, c.nvox * 2u);
     const rsrc_t Rdi = make_rsrc(b.lut + LUT_DINV, SC_LUT_SIZE * 8u);
 
+    const HPair<double> hpv = load_hpair(b.lut);
     Partials<1, PM> P;
     load_partials<1, PM>(ip ? b.part[1] : b.part[0], nb, ps, P);
     unsigned so[KZ + 2];
@@ -420,10 +440,8 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_update_f64(Fp64Bufs b, StepConsts
     double pcol[KZ + 2], ph[KZ];
 #pragma unroll
     for (int q = 0; q < KZ + 2; ++q) pcol[q] = bload<double>(Rp, G.cofs * 8u, so[q] * pb8);
-    {
 #pragma unroll
-        for (int q = 0; q < KZ; ++q) ph[q] = bload<double>(Rp, G.hofs * 8u, so[q + 1] * pb8);
-    }
+    for (int q = 0; q < KZ; ++q) ph[q] = bload<double>(Rp, G.hofs * 8u, so[q + 1] * pb8);
     double xv[KZ], rv[KZ];
 #pragma unroll
     for (int q = 0; q < KZ; ++q) {
@@ -433,19 +451,30 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_update_f64(Fp64Bufs b, StepConsts
     double dv[KZ];
 #pragma unroll
     for (int q = 0; q < KZ; ++q) dv[q] = bload<double>(Rdi, cv[q] * 8u, 0u);
-    build_hpairs(sHP, c.harm);
-    double tot[1];
-    finish_partials<1, PM>(P, tot);  // p.Ap (:485)
-    s.denom = tot[0];
-    if (!finite(tot[0]) || !(tot[0] > 0.0)) {  // :486-490
-        s.status = HZO_PCG_BREAKDOWN;
-        s.done = 1;
-        pass_state(s, b.st + io);
-        return;
+
+    {
+        RawLDS<1>& s1 = *reinterpret_cast<RawLDS<1>*>(&sraw);
+        stage_partials<1, PM>(P, s1);
     }
-    s.alpha = s.rz_old / tot[0];  // :492
-    const double alpha = s.alpha;
-    const double nalpha = -alpha;
+    __syncthreads();
+    if (tid < 64) {
+        sHP[tid] = hpv;
+        double tot[1];
+        wave0_total<1>(*reinterpret_cast<RawLDS<1>*>(&sraw), tot);  // p.Ap (:485)
+        s.denom = tot[0];
+        if (!finite(tot[0]) || !(tot[0] > 0.0)) {  // :486-490
+            s.status = HZO_PCG_BREAKDOWN;
+            s.done = 1;
+            pass_state(s, b.st + io);
+        } else {
+            s.alpha = s.rz_old / tot[0];  // :492
+            s.kind = K_UPDATE;
+        }
+        if (tid == 0) {
+            bc.coef = s.alpha;
+            bc.flag = s.done;
+        }
+    }
 #pragma unroll
     for (int q = 0; q < KZ; ++q) pl[q][ty + 1][tx + 1] = pcol[q + 1];
     if (tp.has_h) {
@@ -453,45 +482,36 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_update_f64(Fp64Bufs b, StepConsts
         for (int q = 0; q < KZ; ++q) pl[q][tp.hy][tp.hx] = ph[q];
     }
     __syncthreads();
+    if (bc.flag) return;
+    const double alpha = bc.coef;
+    const double nalpha = -alpha;
+    const unsigned oob = c.nvox * 8u;
     double acc[2] = {0.0, 0.0};  // r.z, r.r
 #pragma unroll
     for (int q = 1; q <= KZ; ++q) {
         const int k = k0 - 1 + q;
-        if (k < nz) {
-            const double xc = pcol[q];
-            const unsigned code = cv[q - 1];
-            double qv;
-            if (code & SC_IDENTITY) {
-                qv = op_identity(xc, k, c);
-            } else {
-                double xu = pcol[q + 1], xd = pcol[q - 1];
-                if (c.zbc != 0) {
-                    if (k == 0) xd = xu;
-                    else if (k == nz - 1) xu = xd;
-                }
-                const unsigned cb = (code & 3u) << 4;
-                const HPair<double> hx = sHP[cb | ((code >> 2) & 15u)];
-                const HPair<double> hy = sHP[cb | ((code >> 6) & 15u)];
-                const HPair<double> hz = sHP[cb | ((code >> 10) & 15u)];
-                const double xe = pl[q - 1][ty + 1][tx + 2], xw = pl[q - 1][ty + 1][tx];
-                const double xn = pl[q - 1][ty + 2][tx + 1], xs = pl[q - 1][ty][tx + 1];
-                const double num = hx.p * (xe - xc) - hx.m * (xc - xw) + hy.p * (xn - xc) - hy.m * (xc - xs) +
-                                   hz.p * (xu - xc) - hz.m * (xc - xd);
-                const double Ax = c.h2_pow2 ? num * c.inv_h2 : num / c.h2;
-                qv = -Ax;
-            }
-            if (tp.live) {
-                bstore(Rx, G.cofs * 8u, (unsigned)k * pb8, alpha * xc + xv[q - 1]);  // Daxpy x += alpha p (:495)
-                const double r = nalpha * qv + rv[q - 1];                            // Daxpy r -= alpha Ap (:499)
-                bstore(Rr, G.cofs * 8u, (unsigned)k * pb8, r);
-                const double z = r * dv[q - 1];  // :502
-                acc[0] += r * z;
-                acc[1] += r * r;
-            }
+        const double xc = pcol[q];
+        const unsigned code = cv[q - 1];
+        double xu = pcol[q + 1], xd = pcol[q - 1];
+        if (c.zbc != 0) {
+            if (k == 0) xd = xu;
+            else if (k == nz - 1) xu = xd;
+        }
+        const double Ax = row_num<P2>(code, sHP, c, xc, pl[q - 1][ty + 1][tx + 2], pl[q - 1][ty + 1][tx],
+                                      pl[q - 1][ty + 2][tx + 1], pl[q - 1][ty][tx + 1], xu, xd);
+        const double qv = (code & SC_IDENTITY) ? op_identity(xc, k, c) : -Ax;
+        const bool ok = tp.live && k < nz;
+        const unsigned vo = ok ? G.cofs * 8u : oob;
+        bstore(Rx, vo, so[q] * pb8, alpha * xc + xv[q - 1]);  // Daxpy x += alpha p (:495)
+        const double r = nalpha * qv + rv[q - 1];             // Daxpy r -= alpha Ap (:499)
+        bstore(Rr, vo, so[q] * pb8, r);
+        const double z = r * dv[q - 1];  // :502
+        if (ok) {
+            acc[0] += r * z;
+            acc[1] += r * r;
         }
     }
-    write_block_partials<2>(acc, io ? b.part[1] : b.part[0], ps);
-    s.kind = K_UPDATE;
+    write_block_partials<2>(acc, io ? b.part[1] : b.part[0], ps, sraw);
     pass_state(s, b.st + io);
 }
 
@@ -542,12 +562,26 @@ void launch_pcg_setup_f64(hipStream_t s, const Grid& g, const StepConsts& c, con
     hipLaunchKernelGGL(k_pcg_setup_f64, grid3(g), dim3(BX, BY, 1), 0, s, b, c, part_stride(g), init);
 }
 
+#define HZO_X , P2
+template <bool P2>
+static void launch_dir64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t) {
+    HZO_PCG_LAUNCH_X(k_pcg_dir_f64, g, s, b, c, blocks_for(g), part_stride(g), t);
+}
+template <bool P2>
+static void launch_update64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t) {
+    HZO_PCG_LAUNCH_X(k_pcg_update_f64, g, s, b, c, blocks_for(g), part_stride(g), t);
+}
+
+#undef HZO_X
+
 void launch_pcg_dir_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t) {
-    HZO_PCG_LAUNCH(k_pcg_dir_f64, g, s, b, c, blocks_for(g), part_stride(g), t);
+    if (c.h2_pow2) launch_dir64<true>(s, g, c, b, t);
+    else launch_dir64<false>(s, g, c, b, t);
 }
 
 void launch_pcg_update_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t) {
-    HZO_PCG_LAUNCH(k_pcg_update_f64, g, s, b, c, blocks_for(g), part_stride(g), t);
+    if (c.h2_pow2) launch_update64<true>(s, g, c, b, t);
+    else launch_update64<false>(s, g, c, b, t);
 }
 
 void launch_diag_f64(hipStream_t s, const Grid& g, const double* P, const uint32_t* cw, double* part) {

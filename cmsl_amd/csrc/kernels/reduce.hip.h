@@ -13,10 +13,33 @@
 
 namespace hzo {
 
+// One 32-bit DPP move of each half of a double (all lanes active).
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ double lane_d(double v, int lane) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), lane);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Wave64 sum, the same bits in every lane: butterfly within each 16-lane row
+// by DPP (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror,
+// row_mirror; every lane of a row ends with the identical row sum because
+// IEEE addition is commutative), then the four row sums in a fixed order.
+// Must be called with the whole wave active.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    v += dpp_d<0xB1>(v);
+    v += dpp_d<0x4E>(v);
+    v += dpp_d<0x141>(v);
+    v += dpp_d<0x140>(v);
+    return ((lane_d(v, 0) + lane_d(v, 16)) + lane_d(v, 32)) + lane_d(v, 48);
 }
 
 __device__ __forceinline__ int linear_block() {
@@ -27,40 +50,61 @@ __device__ __forceinline__ int linear_tid() {
     return threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
 }
 
-// Workgroup sum of NQ per-thread values; thread 0 stores them at
-// out[q*stride + block].
+// Per-thread values of a 512-thread workgroup staged in LDS for wave 0.
 template <int NQ>
-__device__ __forceinline__ void write_block_partials(const double (&v)[NQ], double* out, int stride) {
-    __shared__ double sred[NQ][16];
-    const int tid = linear_tid();
-    const int lane = tid & 63, wid = tid >> 6;
-    const int nw = (blockDim.x * blockDim.y * blockDim.z) >> 6;
+struct RawLDS {
+    double v[NQ][512];
+};
+
+// Wave 0 sums the NQ columns of a RawLDS: lane l adds entries l, l+64, ...,
+// l+448 in that order, then the wave sum; the same bits in every lane of
+// wave 0 (the other waves must not call it).
+template <int NQ>
+__device__ __forceinline__ void wave0_total(const RawLDS<NQ>& R, double (&tot)[NQ]) {
+    const int lane = linear_tid() & 63;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        const double w = wave_sum(v[q]);
-        if (lane == 0) sred[q][wid] = w;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        const int b = linear_block();
+        double a = R.v[q][lane];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            double s = 0.0;
-            for (int w = 0; w < nw; ++w) s += sred[q][w];
-            out[(long long)q * stride + b] = s;
+        for (int m = 1; m < 8; ++m) a += R.v[q][lane + 64 * m];
+        tot[q] = wave_sum(a);
+    }
+}
+
+// Workgroup sum of NQ per-thread values, stored by thread 0 at
+// out[q*stride + block].  Uses R as scratch (one barrier inside); the caller
+// must not reuse R before a later barrier.
+template <int NQ>
+__device__ __forceinline__ void write_block_partials(const double (&v)[NQ], double* out, int stride, RawLDS<NQ>& R) {
+    const int tid = linear_tid();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) R.v[q][tid] = v[q];
+    __syncthreads();
+    if (tid < 64) {
+        double tot[NQ];
+        wave0_total<NQ>(R, tot);
+        if (tid == 0) {
+            const int b = linear_block();
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) out[(long long)q * stride + b] = tot[q];
         }
     }
 }
 
+template <int NQ>
+__device__ __forceinline__ void write_block_partials(const double (&v)[NQ], double* out, int stride) {
+    __shared__ RawLDS<NQ> R;
+    write_block_partials<NQ>(v, out, stride, R);
+}
+
 // Cross-kernel reduction of the PCG partials.  Partial arrays have a padded
-// stride `ps` = PM * (threads per workgroup) whose tail beyond the number of
-// producing workgroups is zero for the life of the buffer, so thread t of
-// every consuming workgroup loads entries t, t + NT, ... (PM of them) with
-// no bounds test, and every workgroup adds them in the same fixed order.
-// PM = 0 selects a loop over `nb` entries for very large films.
-//
-// The raw values are loaded first (load_partials) and summed later
-// (finish_partials), so the loads stay in flight behind the tile loads.
+// stride `ps` = PM * 512 whose tail beyond the number of producing
+// workgroups is zero for the life of the buffer, so thread t of every
+// consuming workgroup loads entries t, t + 512, ... (PM of them) with no
+// bounds test; PM = 0 selects a loop over `nb` entries for very large films.
+// The raw values are loaded first (load_partials) and staged for wave 0
+// later (stage_partials), so the loads stay in flight behind the tile loads.
+// Every workgroup adds the partials in the same fixed order.
 template <int NQ, int PM>
 struct Partials {
     double v[NQ][PM > 0 ? PM : 1];
@@ -69,16 +113,15 @@ struct Partials {
 template <int NQ, int PM>
 __device__ __forceinline__ void load_partials(const double* in, int nb, int ps, Partials<NQ, PM>& P) {
     const int tid = linear_tid();
-    const int nthreads = blockDim.x * blockDim.y * blockDim.z;
     if (PM > 0) {
 #pragma unroll
         for (int q = 0; q < NQ; ++q)
 #pragma unroll
-            for (int m = 0; m < (PM > 0 ? PM : 1); ++m) P.v[q][m] = in[(long long)q * ps + tid + m * nthreads];
+            for (int m = 0; m < (PM > 0 ? PM : 1); ++m) P.v[q][m] = in[(long long)q * ps + tid + m * 512];
     } else {
 #pragma unroll
         for (int q = 0; q < NQ; ++q) P.v[q][0] = 0.0;
-        for (int t = tid; t < nb; t += nthreads) {
+        for (int t = tid; t < nb; t += 512) {
 #pragma unroll
             for (int q = 0; q < NQ; ++q) P.v[q][0] += in[(long long)q * ps + t];
         }
@@ -86,34 +129,15 @@ __device__ __forceinline__ void load_partials(const double* in, int nb, int ps, 
 }
 
 template <int NQ, int PM>
-__device__ __forceinline__ void finish_partials(const Partials<NQ, PM>& P, double (&tot)[NQ]) {
-    __shared__ double sred[NQ][16];
+__device__ __forceinline__ void stage_partials(const Partials<NQ, PM>& P, RawLDS<NQ>& R) {
     const int tid = linear_tid();
-    const int nthreads = blockDim.x * blockDim.y * blockDim.z;
-    const int lane = tid & 63, wid = tid >> 6, nw = nthreads >> 6;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         double a = P.v[q][0];
 #pragma unroll
         for (int m = 1; m < (PM > 0 ? PM : 1); ++m) a += P.v[q][m];
-        const double w = wave_sum(a);
-        if (lane == 0) sred[q][wid] = w;
+        R.v[q][tid] = a;
     }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        double s = 0.0;
-        for (int w = 0; w < nw; ++w) s += sred[q][w];
-        tot[q] = s;
-    }
-    __syncthreads();
-}
-
-template <int NQ, int PM>
-__device__ __forceinline__ void sum_partials(const double* in, int nb, int ps, double (&tot)[NQ]) {
-    Partials<NQ, PM> P;
-    load_partials<NQ, PM>(in, nb, ps, P);
-    finish_partials<NQ, PM>(P, tot);
 }
 
 }  // namespace hzo

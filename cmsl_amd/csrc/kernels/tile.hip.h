@@ -116,25 +116,23 @@ __device__ __forceinline__ TileCols tile_cols(const TilePos& tp, const StepConst
 
 // Paired harmonic face means: entry (c << 4) | (a | b << 2) holds
 // (H[c][a], H[c][b]), so one LDS read serves the + and - face of one axis
-// (stencil-code bits 2-5: e,w; 6-9: n,s; 10-13: u,d).  Thread 0 writes all
-// 64 pairs from the 16 values in the kernel arguments with constant indices
-// (uniform registers, no vector-memory access); the kernel's next barrier
-// publishes them.
+// (stencil-code bits 2-5: e,w; 6-9: n,s; 10-13: u,d).  Every wave loads the
+// 64 pairs (lane e loads entry e; two L2-resident values) as its first
+// memory operation, so waiting for them never waits for a tile load; wave 0
+// writes them to LDS after the kernel's first barrier point.
 template <class T>
 struct HPair {
     T p, m;
 };
 
 template <class T>
-__device__ __forceinline__ void build_hpairs(HPair<T>* sHP, const T (&h)[16]) {
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-#pragma unroll
-        for (int e = 0; e < 64; ++e) {
-            const int row = (e >> 4) * 4;
-            sHP[e].p = h[row + (e & 3)];
-            sHP[e].m = h[row + ((e >> 2) & 3)];
-        }
-    }
+__device__ __forceinline__ HPair<T> load_hpair(const T* lut) {
+    const unsigned e = (threadIdx.x + BX * threadIdx.y) & 63u;
+    const unsigned row = (e >> 4) * 4u;
+    HPair<T> h;
+    h.p = lut[LUT_HARM + row + (e & 3u)];
+    h.m = lut[LUT_HARM + row + ((e >> 2) & 3u)];
+    return h;
 }
 
 // The z planes whose operand values the tile's planes [k0, k1) need: k0-1 and
