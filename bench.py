@@ -102,14 +102,27 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--timing-steps", type=int, default=4)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-kz", type=int, default=0, help="z planes per workgroup (0: automatic)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a device-to-device copy of 1 GiB three times (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the film fits one MI355X); --gpus {args.gpus} "
               "is not supported", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        import torch
+        n = 1 << 30  # larger than the 256 MiB last-level cache: every byte crosses HBM
+        a = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_per_launch": n, "launches": 3}))
+        return
 
     inp = build_inputs()
     N = inp.N

@@ -1,6 +1,7 @@
 // hzo_api.hip -- C-ABI context layer: owns the device state of one film and
 // drives the step (replaces AllocateData/Cleanup, the device globals, the
 // step loop body of evolve.cu:220-237 and the host side of Depol_field).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -142,13 +143,14 @@ int timed(hzo_ctx* x, int cls, F launch) {
             s.push_back(a);
             e.push_back(b);
         }
-        HZO_HIP(hipEventRecord(s[i], x->stream));
-        launch();
+        hzo::LaunchEv ev;
+        ev.start = s[i];
+        ev.stop = e[i];
+        launch(ev);
         HZO_HIP(hipGetLastError());
-        HZO_HIP(hipEventRecord(e[i], x->stream));
         x->nt[cls]++;
     } else {
-        launch();
+        launch(hzo::LaunchEv{});
         HZO_HIP(hipGetLastError());
     }
     return HZO_OK;
@@ -302,10 +304,10 @@ int tdgl(hzo_ctx* x, double Ez_ext) {
     int rc;
     if (x->fp32) {
         const hzo::Fp32Bufs b = bufs32(x);
-        rc = timed(x, 0, [&] { hzo::launch_tdgl_f32(x->stream, x->g, x->c, Ez_ext, b); });
+        rc = timed(x, 0, [&](const hzo::LaunchEv& ev) { hzo::launch_tdgl_f32(x->stream, x->g, x->c, Ez_ext, b, ev); });
     } else {
         const hzo::Fp64Bufs b = bufs(x);
-        rc = timed(x, 0, [&] { hzo::launch_tdgl_f64(x->stream, x->g, x->c, Ez_ext, b); });
+        rc = timed(x, 0, [&](const hzo::LaunchEv& ev) { hzo::launch_tdgl_f64(x->stream, x->g, x->c, Ez_ext, b, ev); });
     }
     if (rc) return rc;
     x->cur ^= 1;  // kernelUpdate_P becomes a buffer swap
@@ -316,8 +318,8 @@ int depol(hzo_ctx* x, hzo_pcg_stats* stats) {
     const hzo::Fp64Bufs b = bufs(x);
     const hzo::Fp32Bufs b32 = bufs32(x);
     const hzo::PcgState init = pcg_init(x);
-    int rc = x->fp32 ? timed(x, 1, [&] { hzo::launch_pcg_setup_f32(x->stream, x->g, x->c, b32, init); })
-                     : timed(x, 1, [&] { hzo::launch_pcg_setup_f64(x->stream, x->g, x->c, b, init); });
+    int rc = x->fp32 ? timed(x, 1, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_setup_f32(x->stream, x->g, x->c, b32, init, ev); })
+                     : timed(x, 1, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_setup_f64(x->stream, x->g, x->c, b, init, ev); });
     if (rc) return rc;
     // One (dir, update) launch pair per CG iteration; the pair that follows
     // the last iteration finishes its inner products and raises `done`.  The
@@ -332,13 +334,13 @@ int depol(hzo_ctx* x, hzo_pcg_stats* stats) {
         for (int m = 0; m < batch && pairs < maxpairs; ++m, ++pairs) {
             const int ta = ++t, tb = ++t;
             if (x->fp32) {
-                rc = timed(x, 2, [&] { hzo::launch_pcg_dir_f32(x->stream, x->g, x->c, b32, ta); });
+                rc = timed(x, 2, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_dir_f32(x->stream, x->g, x->c, b32, ta, ev); });
                 if (rc) return rc;
-                rc = timed(x, 3, [&] { hzo::launch_pcg_update_f32(x->stream, x->g, x->c, b32, tb); });
+                rc = timed(x, 3, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_update_f32(x->stream, x->g, x->c, b32, tb, ev); });
             } else {
-                rc = timed(x, 2, [&] { hzo::launch_pcg_dir_f64(x->stream, x->g, x->c, b, ta); });
+                rc = timed(x, 2, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_dir_f64(x->stream, x->g, x->c, b, ta, ev); });
                 if (rc) return rc;
-                rc = timed(x, 3, [&] { hzo::launch_pcg_update_f64(x->stream, x->g, x->c, b, tb); });
+                rc = timed(x, 3, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_update_f64(x->stream, x->g, x->c, b, tb, ev); });
             }
             if (rc) return rc;
         }
@@ -460,16 +462,22 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
     x->g.ny = cfg->ny;
     x->g.nz = cfg->nz;
     x->g.N = N;
-    // Tile depth: about a thousand 64x8 workgroups over the film unless the
-    // caller fixes it (the inner-product grouping, hence the last bits of the
-    // dot products, follows the tiling).
+    // Tile depth (z planes per 64x8 workgroup) unless the caller fixes it;
+    // the inner-product grouping, hence the last bits of the dot products,
+    // follows the tiling.  FP32: the smallest depth giving at most 512
+    // workgroups (one partial per consuming thread, two workgroups per CU),
+    // else 8.  FP64: the smallest of {2, 4} giving at most 1024 workgroups,
+    // else 4 (depth 8 exceeds the 128-VGPR budget of 4 waves per SIMD).
     {
         const long long txy = (long long)((cfg->nx + 63) / 64) * ((cfg->ny + 7) / 8);
+        const bool f32 = cfg->precision == HZO_PREC_FP32_REFINED;
         int kz = cfg->tile_kz;
-        if (kz == 0) {  // smallest depth giving at most ~1024 workgroups
-            kz = 8;
-            for (int cand : {2, 4}) {
-                if (txy * ((cfg->nz + cand - 1) / cand) <= 1024) {
+        if (kz == 0) {
+            kz = f32 ? 8 : 4;
+            const long long cap = f32 ? 512 : 1024;
+            for (int cand : {2, 4, 8}) {
+                if (!f32 && cand == 8) break;
+                if (txy * ((cfg->nz + cand - 1) / cand) <= cap) {
                     kz = cand;
                     break;
                 }
@@ -797,14 +805,29 @@ int hzo_kernel_times(hzo_ctx* x, int counts[HZO_KCLASSES], double mean_ms[HZO_KC
     HZO_HIP(hipSetDevice(x->device));
     HZO_HIP(hipStreamSynchronize(x->stream));
     for (int c = 0; c < HZO_KCLASSES; ++c) {
-        double tot = 0.0;
+        std::vector<double> t(x->nt[c]);
         for (int i = 0; i < x->nt[c]; ++i) {
             float ms = 0.f;
             HZO_HIP(hipEventElapsedTime(&ms, x->ev[c][0][i], x->ev[c][1][i]));
-            tot += ms;
+            t[i] = ms;
         }
-        counts[c] = x->nt[c];
-        mean_ms[c] = x->nt[c] ? tot / x->nt[c] : 0.0;
+        // launches that found the solve already finished return at once;
+        // they take under a quarter of the median and are left out
+        double med = 0.0;
+        if (!t.empty()) {
+            std::vector<double> srt(t);
+            std::nth_element(srt.begin(), srt.begin() + srt.size() / 2, srt.end());
+            med = srt[srt.size() / 2];
+        }
+        double tot = 0.0;
+        int n = 0;
+        for (double v : t)
+            if (v >= 0.25 * med) {
+                tot += v;
+                ++n;
+            }
+        counts[c] = n;
+        mean_ms[c] = n ? tot / n : 0.0;
     }
     return HZO_OK;
 }

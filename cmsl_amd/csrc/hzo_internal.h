@@ -143,18 +143,35 @@ int blocks_for(const Grid& g);
 // Stride of one PCG partial-sum array (>= blocks_for; the tail is kept zero).
 int part_stride(const Grid& g);
 
+// Optional start/stop events attached to a kernel's own dispatch
+// (hipExtLaunchKernelGGL): their elapsed time is the kernel's execution
+// time, without the gaps separate event records would add.
+struct LaunchEv {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+
 // Launchers.  `t` is the launch index since the last PCG setup (setup = 0).
-void launch_tdgl_f64(hipStream_t s, const Grid& g, const StepConsts& c, double Ez_ext, const Fp64Bufs& b);
+void launch_tdgl_f64(hipStream_t s, const Grid& g, const StepConsts& c, double Ez_ext, const Fp64Bufs& b,
+                 const LaunchEv& ev = {});
 void launch_pcg_setup_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b,
-                          const PcgState& init);
-void launch_pcg_dir_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t);
-void launch_pcg_update_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t);
-void launch_diag_f64(hipStream_t s, const Grid& g, const double* P, const uint32_t* cw, double* part);
-void launch_tdgl_f32(hipStream_t s, const Grid& g, const StepConsts& c, double Ez_ext, const Fp32Bufs& b);
+                          const PcgState& init,
+                 const LaunchEv& ev = {});
+void launch_pcg_dir_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t,
+                 const LaunchEv& ev = {});
+void launch_pcg_update_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t,
+                 const LaunchEv& ev = {});
+void launch_diag_f64(hipStream_t s, const Grid& g, const double* P, const uint32_t* cw, double* part,
+                 const LaunchEv& ev = {});
+void launch_tdgl_f32(hipStream_t s, const Grid& g, const StepConsts& c, double Ez_ext, const Fp32Bufs& b,
+                 const LaunchEv& ev = {});
 void launch_pcg_setup_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b,
-                          const PcgState& init);
-void launch_pcg_dir_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b, int t);
-void launch_pcg_update_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b, int t);
-void launch_diag_f32(hipStream_t s, const Grid& g, const float* P, const uint32_t* cw, double* part);
+                          const PcgState& init,
+                 const LaunchEv& ev = {});
+void launch_pcg_dir_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b, int t,
+                 const LaunchEv& ev = {});
+void launch_pcg_update_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b, int t,
+                 const LaunchEv& ev = {});
+void launch_diag_f32(hipStream_t s, const Grid& g, const float* P, const uint32_t* cw, double* part,
+                 const LaunchEv& ev = {});
 
 }  // namespace hzo

@@ -2,6 +2,7 @@
 // FP32-refined kernels.
 #pragma once
 #include "../hzo_internal.h"
+#include <hip/hip_ext.h>
 
 namespace hzo {
 
@@ -24,10 +25,10 @@ inline int part_stride_impl(int nb) {
 // kernel's trailing template argument list (or nothing).
 #define HZO_PCG_LAUNCH_PM(kern, KZ, pm, gr, bl, s, ...)                                   \
     switch (pm) {                                                                         \
-        case 1: hipLaunchKernelGGL((kern<KZ, 1 HZO_X>), gr, bl, 0, s, __VA_ARGS__); break;  \
-        case 2: hipLaunchKernelGGL((kern<KZ, 2 HZO_X>), gr, bl, 0, s, __VA_ARGS__); break;  \
-        case 4: hipLaunchKernelGGL((kern<KZ, 4 HZO_X>), gr, bl, 0, s, __VA_ARGS__); break;  \
-        default: hipLaunchKernelGGL((kern<KZ, 0 HZO_X>), gr, bl, 0, s, __VA_ARGS__); break; \
+        case 1: hipExtLaunchKernelGGL((kern<KZ, 1 HZO_X>), gr, bl, 0, s, ev.start, ev.stop, 0, __VA_ARGS__); break;  \
+        case 2: hipExtLaunchKernelGGL((kern<KZ, 2 HZO_X>), gr, bl, 0, s, ev.start, ev.stop, 0, __VA_ARGS__); break;  \
+        case 4: hipExtLaunchKernelGGL((kern<KZ, 4 HZO_X>), gr, bl, 0, s, ev.start, ev.stop, 0, __VA_ARGS__); break;  \
+        default: hipExtLaunchKernelGGL((kern<KZ, 0 HZO_X>), gr, bl, 0, s, ev.start, ev.stop, 0, __VA_ARGS__); break; \
     }
 #define HZO_PCG_LAUNCH_X(kern, g, s, ...)                                                 \
     do {                                                                                  \

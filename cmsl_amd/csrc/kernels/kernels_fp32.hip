@@ -553,23 +553,28 @@ __global__ __launch_bounds__(NT) void k_diag_f32(const float* __restrict__ P, co
     write_block_partials<DIAG_Q>(a, part, nb);
 }
 
-void launch_tdgl_f32(hipStream_t s, const Grid& g, const StepConsts& c, double Ez_ext, const Fp32Bufs& b) {
-    hipLaunchKernelGGL(k_tdgl_f32, grid3(g), dim3(BX, BY, 1), 0, s, b, c, (float)Ez_ext, blocks_for_impl(g));
+void launch_tdgl_f32(hipStream_t s, const Grid& g, const StepConsts& c, double Ez_ext, const Fp32Bufs& b,
+                  const LaunchEv& ev) {
+    hipExtLaunchKernelGGL(k_tdgl_f32, grid3(g), dim3(BX, BY, 1), 0, s, ev.start, ev.stop, 0, b, c, (float)Ez_ext, blocks_for_impl(g));
 }
 void launch_pcg_setup_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b,
-                          const PcgState& init) {
-    hipLaunchKernelGGL(k_pcg_setup_f32, grid3(g), dim3(BX, BY, 1), 0, s, b, c, part_stride(g), init);
+                          const PcgState& init,
+                  const LaunchEv& ev) {
+    hipExtLaunchKernelGGL(k_pcg_setup_f32, grid3(g), dim3(BX, BY, 1), 0, s, ev.start, ev.stop, 0, b, c, part_stride(g), init);
 }
 #define HZO_X
-void launch_pcg_dir_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b, int t) {
+void launch_pcg_dir_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b, int t,
+                  const LaunchEv& ev) {
     HZO_PCG_LAUNCH_X(k_pcg_dir_f32, g, s, b, c, blocks_for(g), part_stride(g), t);
 }
-void launch_pcg_update_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b, int t) {
+void launch_pcg_update_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b, int t,
+                  const LaunchEv& ev) {
     HZO_PCG_LAUNCH_X(k_pcg_update_f32, g, s, b, c, blocks_for(g), part_stride(g), t);
 }
 #undef HZO_X
-void launch_diag_f32(hipStream_t s, const Grid& g, const float* P, const uint32_t* cw, double* part) {
-    hipLaunchKernelGGL(k_diag_f32, grid3(g), dim3(BX, BY, 1), 0, s, P, cw, g, part, blocks_for_impl(g));
+void launch_diag_f32(hipStream_t s, const Grid& g, const float* P, const uint32_t* cw, double* part,
+                  const LaunchEv& ev) {
+    hipExtLaunchKernelGGL(k_diag_f32, grid3(g), dim3(BX, BY, 1), 0, s, ev.start, ev.stop, 0, P, cw, g, part, blocks_for_impl(g));
 }
 
 }  // namespace hzo

@@ -553,39 +553,44 @@ __global__ __launch_bounds__(NT) void k_diag_f64(const double* __restrict__ P,
 }
 
 // ---------------------------------------------------------------- launchers
-void launch_tdgl_f64(hipStream_t s, const Grid& g, const StepConsts& c, double Ez_ext, const Fp64Bufs& b) {
-    hipLaunchKernelGGL(k_tdgl_f64, grid3(g), dim3(BX, BY, 1), 0, s, b, c, Ez_ext, blocks_for(g));
+void launch_tdgl_f64(hipStream_t s, const Grid& g, const StepConsts& c, double Ez_ext, const Fp64Bufs& b,
+                  const LaunchEv& ev) {
+    hipExtLaunchKernelGGL(k_tdgl_f64, grid3(g), dim3(BX, BY, 1), 0, s, ev.start, ev.stop, 0, b, c, Ez_ext, blocks_for(g));
 }
 
 void launch_pcg_setup_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b,
-                          const PcgState& init) {
-    hipLaunchKernelGGL(k_pcg_setup_f64, grid3(g), dim3(BX, BY, 1), 0, s, b, c, part_stride(g), init);
+                          const PcgState& init,
+                  const LaunchEv& ev) {
+    hipExtLaunchKernelGGL(k_pcg_setup_f64, grid3(g), dim3(BX, BY, 1), 0, s, ev.start, ev.stop, 0, b, c, part_stride(g), init);
 }
 
 #define HZO_X , P2
 template <bool P2>
-static void launch_dir64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t) {
+static void launch_dir64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t, const LaunchEv& ev) {
     HZO_PCG_LAUNCH_X(k_pcg_dir_f64, g, s, b, c, blocks_for(g), part_stride(g), t);
 }
 template <bool P2>
-static void launch_update64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t) {
+static void launch_update64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t, const LaunchEv& ev) {
     HZO_PCG_LAUNCH_X(k_pcg_update_f64, g, s, b, c, blocks_for(g), part_stride(g), t);
 }
 
 #undef HZO_X
 
-void launch_pcg_dir_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t) {
-    if (c.h2_pow2) launch_dir64<true>(s, g, c, b, t);
-    else launch_dir64<false>(s, g, c, b, t);
+void launch_pcg_dir_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t,
+                  const LaunchEv& ev) {
+    if (c.h2_pow2) launch_dir64<true>(s, g, c, b, t, ev);
+    else launch_dir64<false>(s, g, c, b, t, ev);
 }
 
-void launch_pcg_update_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t) {
-    if (c.h2_pow2) launch_update64<true>(s, g, c, b, t);
-    else launch_update64<false>(s, g, c, b, t);
+void launch_pcg_update_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t,
+                  const LaunchEv& ev) {
+    if (c.h2_pow2) launch_update64<true>(s, g, c, b, t, ev);
+    else launch_update64<false>(s, g, c, b, t, ev);
 }
 
-void launch_diag_f64(hipStream_t s, const Grid& g, const double* P, const uint32_t* cw, double* part) {
-    hipLaunchKernelGGL(k_diag_f64, grid3(g), dim3(BX, BY, 1), 0, s, P, cw, g, part, blocks_for(g));
+void launch_diag_f64(hipStream_t s, const Grid& g, const double* P, const uint32_t* cw, double* part,
+                  const LaunchEv& ev) {
+    hipExtLaunchKernelGGL(k_diag_f64, grid3(g), dim3(BX, BY, 1), 0, s, ev.start, ev.stop, 0, P, cw, g, part, blocks_for(g));
 }
 
 }  // namespace hzo

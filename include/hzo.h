@@ -168,10 +168,12 @@ int hzo_sweep_run(hzo_ctx* ctx, const hzo_sweep_params* p, const char* path, int
                   double* rows_out, int* nrows);
 
 /* ---------------------------------------------------------------- timing
- * Kernel-class timing with HIP events on the context's stream.  While
- * enabled, the first HZO_TIMING_MAX launches of every class are bracketed by
- * events; hzo_kernel_times returns, per class, launches timed and their mean
- * duration in ms.  Classes: 0 tdgl, 1 pcg_setup, 2 pcg_dir_spmv,
+ * Kernel-class timing with HIP events attached to the kernels' own
+ * dispatches on the context's stream.  While enabled, the first
+ * HZO_TIMING_MAX launches of every class are timed; hzo_kernel_times returns,
+ * per class, the number of working launches and their mean duration in ms
+ * (launches that found the solve already finished, under a quarter of the
+ * class median, are left out).  Classes: 0 tdgl, 1 pcg_setup, 2 pcg_dir_spmv,
  * 3 pcg_update, 4 refine (FP32-refined residual replacement). */
 #define HZO_TIMING_MAX 8192
 #define HZO_KCLASSES 5
