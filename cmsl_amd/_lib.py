@@ -92,6 +92,9 @@ SIGNATURES = {
     "hzo_sweep_run": (_I, [_P, _P, C.c_char_p, _I, _P, _P]),
     "hzo_set_timing": (_I, [_P, _I]),
     "hzo_kernel_times": (_I, [_P, _P, _P]),
+    "hzo_adjoint_reserve": (_I, [_P, _I, _I, _D]),
+    "hzo_forward_record": (_I, [_P, _P, _I, _P]),
+    "hzo_backward": (_I, [_P, _P, _P, _P]),
     "hzo_read_deck": (_I, [C.c_char_p, _P]),
     "hzo_read_grain_count": (_I, [C.c_char_p, _P]),
     "hzo_read_gdata": (_I, [C.c_char_p, _I, _I, _I, _P, _P]),

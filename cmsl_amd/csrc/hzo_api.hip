@@ -65,6 +65,30 @@ struct hzo_ctx {
     bool timing = false;
     std::vector<hipEvent_t> ev[HZO_KCLASSES][2];
     int nt[HZO_KCLASSES] = {0, 0, 0, 0, 0};
+
+    // reverse mode: checkpoints, recompute segment and adjoint vectors
+    // (hzo_adjoint_reserve / hzo_forward_record / hzo_backward)
+    struct Adjoint {
+        int nsteps = 0;    // steps reserved
+        int K = 0;         // checkpoint period
+        int nck = 0;       // checkpoint slots (the last one holds the final state)
+        int recorded = 0;  // steps of the last forward_record
+        int last_iters = 8;
+        double tol = 1e-8;
+        std::vector<double> Ez;
+        void* ckP = nullptr;      // nck x 6N polarisation (float or double)
+        double* ckphi = nullptr;  // nck x N potential (x_hi)
+        float* cklo = nullptr;    // nck x N potential correction x_lo (FP32-refined)
+        void* seg = nullptr;      // K x 6N forward inputs of one segment
+        void* a = nullptr;        // 6N adjoint of the state
+        void* w = nullptr;        // 6N adjoint through the TDGL map
+        double* mu = nullptr;     // N adjoint potential (x_hi)
+        float* mulo = nullptr;    // N (FP32-refined x_lo)
+        double* rhs = nullptr;    // N adjoint Poisson right-hand side (FP64; FP32 uses b64)
+        double* gpart = nullptr;  // 12 x nb
+        double* grad = nullptr;   // 12
+        double* m = nullptr;      // nsteps loop samples (sum of Pmz)
+    } adj;
 };
 
 namespace {
@@ -314,33 +338,35 @@ int tdgl(hzo_ctx* x, double Ez_ext) {
     return HZO_OK;
 }
 
-int depol(hzo_ctx* x, hzo_pcg_stats* stats) {
-    const hzo::Fp64Bufs b = bufs(x);
-    const hzo::Fp32Bufs b32 = bufs32(x);
-    const hzo::PcgState init = pcg_init(x);
-    int rc = x->fp32 ? timed(x, 1, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_setup_f32(x->stream, x->g, x->c, b32, init, ev); })
-                     : timed(x, 1, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_setup_f64(x->stream, x->g, x->c, b, init, ev); });
+// One PCG solve on the given vectors: the forward depolarisation solve
+// (rhs == nullptr: b = -eps0 div P) or an adjoint solve of the same operator
+// (rhs = the right-hand side, 0 on identity rows).  last_iters sizes the
+// launch batches from the previous solve of the same kind.
+int pcg_solve(hzo_ctx* x, const hzo::Fp64Bufs& b, const hzo::Fp32Bufs& b32, const hzo::StepConsts& c,
+              const hzo::PcgState& init, const double* rhs, int& last_iters, hzo_pcg_stats* stats) {
+    int rc = x->fp32 ? timed(x, 1, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_setup_f32(x->stream, x->g, c, b32, init, rhs, ev); })
+                     : timed(x, 1, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_setup_f64(x->stream, x->g, c, b, init, rhs, ev); });
     if (rc) return rc;
     // One (dir, update) launch pair per CG iteration; the pair that follows
     // the last iteration finishes its inner products and raises `done`.  The
     // FP32-refined form also spends pairs on residual replacements, so up to
     // max_iter + max_iter/refine_every + 2 pairs can be needed.
-    const int re = x->cfg.refine_every > 0 ? x->cfg.refine_every : x->cfg.max_iter + 1;
-    const long long maxpairs = (long long)x->cfg.max_iter + (x->fp32 ? x->cfg.max_iter / re + 2 : 0) + 2;
+    const int re = init.refine_every > 0 ? init.refine_every : init.max_iter + 1;
+    const long long maxpairs = (long long)init.max_iter + (x->fp32 ? init.max_iter / re + 2 : 0) + 2;
     long long pairs = 0;
     int t = 0;  // launch index since the setup
-    int batch = x->last_iters + 1;
+    int batch = last_iters + 1;
     while (true) {
         for (int m = 0; m < batch && pairs < maxpairs; ++m, ++pairs) {
             const int ta = ++t, tb = ++t;
             if (x->fp32) {
-                rc = timed(x, 2, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_dir_f32(x->stream, x->g, x->c, b32, ta, ev); });
+                rc = timed(x, 2, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_dir_f32(x->stream, x->g, c, b32, ta, ev); });
                 if (rc) return rc;
-                rc = timed(x, 3, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_update_f32(x->stream, x->g, x->c, b32, tb, ev); });
+                rc = timed(x, 3, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_update_f32(x->stream, x->g, c, b32, tb, ev); });
             } else {
-                rc = timed(x, 2, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_dir_f64(x->stream, x->g, x->c, b, ta, ev); });
+                rc = timed(x, 2, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_dir_f64(x->stream, x->g, c, b, ta, ev); });
                 if (rc) return rc;
-                rc = timed(x, 3, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_update_f64(x->stream, x->g, x->c, b, tb, ev); });
+                rc = timed(x, 3, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_update_f64(x->stream, x->g, c, b, tb, ev); });
             }
             if (rc) return rc;
         }
@@ -348,10 +374,10 @@ int depol(hzo_ctx* x, hzo_pcg_stats* stats) {
                                x->stream));
         HZO_HIP(hipStreamSynchronize(x->stream));
         if (x->st_host->done || pairs >= maxpairs) break;
-        batch = x->last_iters / 8 > 2 ? x->last_iters / 8 : 2;
+        batch = last_iters / 8 > 2 ? last_iters / 8 : 2;
     }
     const hzo::PcgState& s = *x->st_host;
-    x->last_iters = s.iters + s.refinements > 0 ? s.iters + s.refinements : 1;
+    last_iters = s.iters + s.refinements > 0 ? s.iters + s.refinements : 1;
     if (stats) {
         stats->iters = s.iters;
         stats->status = s.status;
@@ -366,7 +392,29 @@ int depol(hzo_ctx* x, hzo_pcg_stats* stats) {
     return HZO_OK;
 }
 
+int depol(hzo_ctx* x, hzo_pcg_stats* stats) {
+    return pcg_solve(x, bufs(x), bufs32(x), x->c, pcg_init(x), nullptr, x->last_iters, stats);
+}
+
+void free_adj(hzo_ctx* x) {
+    hzo_ctx::Adjoint& A = x->adj;
+    (void)hipFree(A.ckP);
+    (void)hipFree(A.ckphi);
+    (void)hipFree(A.cklo);
+    (void)hipFree(A.seg);
+    (void)hipFree(A.a);
+    (void)hipFree(A.w);
+    (void)hipFree(A.mu);
+    (void)hipFree(A.mulo);
+    (void)hipFree(A.rhs);
+    (void)hipFree(A.gpart);
+    (void)hipFree(A.grad);
+    (void)hipFree(A.m);
+    A = hzo_ctx::Adjoint();
+}
+
 void free_all(hzo_ctx* x) {
+    free_adj(x);
     (void)hipFree(x->P[0]);
     (void)hipFree(x->P[1]);
     (void)hipFree(x->phi);
@@ -829,6 +877,210 @@ int hzo_kernel_times(hzo_ctx* x, int counts[HZO_KCLASSES], double mean_ms[HZO_KC
         counts[c] = n;
         mean_ms[c] = n ? tot / n : 0.0;
     }
+    return HZO_OK;
+}
+
+
+/* ---------------------------------------------------------------- reverse mode */
+
+int hzo_adjoint_reserve(hzo_ctx* x, int nsteps, int ckpt_every, double tol) {
+    if (!x) return set_error(HZO_ERR_ARG, "hzo_adjoint_reserve: null context");
+    if (nsteps < 1 || ckpt_every < 1 || !(tol > 0.0))
+        return set_error(HZO_ERR_ARG, "hzo_adjoint_reserve: nsteps, ckpt_every and tol must be positive");
+    if (x->cfg.dep_flag && x->c.zbc != 0)
+        return set_error(HZO_ERR_ARG,
+                         "hzo_adjoint_reserve: the adjoint needs Dirichlet electrodes (the Neumann gauge row makes "
+                         "the Poisson operator non-symmetric)");
+    HZO_HIP(hipSetDevice(x->device));
+    HZO_HIP(hipStreamSynchronize(x->stream));
+    free_adj(x);
+    hzo_ctx::Adjoint& A = x->adj;
+    const long long N = x->g.N;
+    const size_t ts = x->fp32 ? sizeof(float) : sizeof(double);
+    const int K = ckpt_every < nsteps ? ckpt_every : nsteps;
+    A.nsteps = nsteps;
+    A.K = K;
+    A.nck = (nsteps + K - 1) / K + 1;
+    A.tol = tol;
+#define HZO_AALLOC(ptr, bytes)                                                                         \
+    do {                                                                                               \
+        if (hipMalloc((void**)&(ptr), (bytes)) != hipSuccess) {                                        \
+            free_adj(x);                                                                               \
+            return set_error(HZO_ERR_NOMEM, "hzo_adjoint_reserve: hipMalloc of %zu bytes failed", (size_t)(bytes)); \
+        }                                                                                              \
+    } while (0)
+    HZO_AALLOC(A.ckP, ts * 6 * N * A.nck);
+    HZO_AALLOC(A.ckphi, sizeof(double) * N * A.nck);
+    if (x->fp32) HZO_AALLOC(A.cklo, sizeof(float) * N * A.nck);
+    HZO_AALLOC(A.seg, ts * 6 * N * K);
+    HZO_AALLOC(A.a, ts * 6 * N);
+    HZO_AALLOC(A.w, ts * 6 * N);
+    HZO_AALLOC(A.mu, sizeof(double) * N);
+    if (x->fp32) HZO_AALLOC(A.mulo, sizeof(float) * N);
+    else HZO_AALLOC(A.rhs, sizeof(double) * N);
+    HZO_AALLOC(A.gpart, sizeof(double) * 12 * (size_t)x->nb);
+    HZO_AALLOC(A.grad, sizeof(double) * 12);
+    HZO_AALLOC(A.m, sizeof(double) * nsteps);
+#undef HZO_AALLOC
+    return HZO_OK;
+}
+
+namespace {
+
+const void* cur_P(hzo_ctx* x) { return x->fp32 ? (const void*)x->P32[x->cur] : (const void*)x->P[x->cur]; }
+
+int save_ck(hzo_ctx* x, int slot) {
+    hzo_ctx::Adjoint& A = x->adj;
+    const long long N = x->g.N;
+    const size_t ts = x->fp32 ? sizeof(float) : sizeof(double);
+    HZO_HIP(hipMemcpyAsync((char*)A.ckP + ts * 6 * N * slot, cur_P(x), ts * 6 * N, hipMemcpyDeviceToDevice, x->stream));
+    HZO_HIP(hipMemcpyAsync(A.ckphi + N * slot, x->phi, sizeof(double) * N, hipMemcpyDeviceToDevice, x->stream));
+    if (x->fp32)
+        HZO_HIP(hipMemcpyAsync(A.cklo + N * slot, x->xlo, sizeof(float) * N, hipMemcpyDeviceToDevice, x->stream));
+    return HZO_OK;
+}
+
+int load_ck(hzo_ctx* x, int slot) {
+    hzo_ctx::Adjoint& A = x->adj;
+    const long long N = x->g.N;
+    const size_t ts = x->fp32 ? sizeof(float) : sizeof(double);
+    x->cur = 0;
+    void* P = x->fp32 ? (void*)x->P32[0] : (void*)x->P[0];
+    HZO_HIP(hipMemcpyAsync(P, (char*)A.ckP + ts * 6 * N * slot, ts * 6 * N, hipMemcpyDeviceToDevice, x->stream));
+    HZO_HIP(hipMemcpyAsync(x->phi, A.ckphi + N * slot, sizeof(double) * N, hipMemcpyDeviceToDevice, x->stream));
+    if (x->fp32)
+        HZO_HIP(hipMemcpyAsync(x->xlo, A.cklo + N * slot, sizeof(float) * N, hipMemcpyDeviceToDevice, x->stream));
+    return HZO_OK;
+}
+
+int fwd_step(hzo_ctx* x, double Ez) {
+    int rc = tdgl(x, Ez);
+    if (rc) return rc;
+    if (x->cfg.dep_flag) return depol(x, nullptr);
+    return HZO_OK;
+}
+
+}  // namespace
+
+int hzo_forward_record(hzo_ctx* x, const double* Ez, int nsteps, double* m_out) {
+    int rc = check_ready(x);
+    if (rc) return rc;
+    hzo_ctx::Adjoint& A = x->adj;
+    if (!A.ckP) return set_error(HZO_ERR_STATE, "hzo_forward_record: call hzo_adjoint_reserve first");
+    if (nsteps < 1 || nsteps > A.nsteps || !Ez)
+        return set_error(HZO_ERR_ARG, "hzo_forward_record: nsteps must be 1..%d (reserved)", A.nsteps);
+    HZO_HIP(hipSetDevice(x->device));
+    A.Ez.assign(Ez, Ez + nsteps);
+    A.recorded = 0;
+    for (int n = 0; n < nsteps; ++n) {
+        if (n % A.K == 0) {
+            rc = save_ck(x, n / A.K);
+            if (rc) return rc;
+        }
+        rc = fwd_step(x, Ez[n]);
+        if (rc) return rc;
+        if (x->fp32)
+            hzo::launch_diag_f32(x->stream, x->g, x->P32[x->cur], x->cw, x->diag_part);
+        else
+            hzo::launch_diag_f64(x->stream, x->g, x->P[x->cur], x->cw, x->diag_part);
+        hzo::launch_sum_column(x->stream, x->g, x->diag_part, A.m + n);
+        HZO_HIP(hipGetLastError());
+    }
+    rc = save_ck(x, A.nck - 1);
+    if (rc) return rc;
+    std::vector<double> m(nsteps);
+    HZO_HIP(hipMemcpyAsync(m.data(), A.m, sizeof(double) * nsteps, hipMemcpyDeviceToHost, x->stream));
+    HZO_HIP(hipStreamSynchronize(x->stream));
+    const double N = (double)x->g.N;
+    if (m_out)
+        for (int n = 0; n < nsteps; ++n) m_out[n] = m[n] / N;
+    A.recorded = nsteps;
+    return HZO_OK;
+}
+
+int hzo_backward(hzo_ctx* x, const double* dL_dm, double grad[12], long long* adj_pcg_iters) {
+    int rc = check_ready(x);
+    if (rc) return rc;
+    hzo_ctx::Adjoint& A = x->adj;
+    if (!A.recorded) return set_error(HZO_ERR_STATE, "hzo_backward: call hzo_forward_record first");
+    if (!dL_dm || !grad) return set_error(HZO_ERR_ARG, "hzo_backward: null argument");
+    HZO_HIP(hipSetDevice(x->device));
+    const int S = A.recorded;
+    const long long N = x->g.N;
+    const size_t ts = x->fp32 ? sizeof(float) : sizeof(double);
+    const bool dep = x->cfg.dep_flag != 0;
+    // the adjoint operator: the same matrix without the electrode offsets
+    hzo::StepConsts ca = x->c;
+    ca.g_bot = 0.0;
+    ca.g_top = 0.0;
+    hzo::PcgState init = pcg_init(x);
+    init.rule = HZO_PCG_RULE_RELRES;
+    init.tol_rel2 = A.tol * A.tol;
+    init.tol_abs2 = 0.0;
+    long long iters = 0;
+
+    HZO_HIP(hipMemsetAsync(A.grad, 0, sizeof(double) * 12, x->stream));
+    HZO_HIP(hipMemsetAsync(A.mu, 0, sizeof(double) * N, x->stream));
+    if (x->fp32) HZO_HIP(hipMemsetAsync(A.mulo, 0, sizeof(float) * N, x->stream));
+    hzo::launch_adj_seed(x->stream, x->g, x->fp32, A.a, dL_dm[S - 1] / (double)N);
+    HZO_HIP(hipGetLastError());
+
+    hzo::AdjTDGLBufsAny kb;
+    kb.fp32 = x->fp32;
+    kb.a = A.a;
+    kb.w = A.w;
+    kb.v = x->fp32 ? x->b64 : A.rhs;
+    kb.cw = x->cw;
+    kb.sc = x->sc;
+    kb.TR = x->fp32 ? (const void*)x->TR32 : (const void*)x->TR;
+    kb.ptab = x->fp32 ? (const void*)x->ptab32 : (const void*)x->ptab;
+    kb.gpart = A.gpart;
+    kb.dAdk = x->cfg.del_t / (x->cfg.del_h * x->cfg.del_h);
+
+    const int nseg = (S + A.K - 1) / A.K;
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int s0 = sg * A.K, s1 = (s0 + A.K < S) ? s0 + A.K : S;
+        // recompute the forward inputs P_n, n in [s0, s1), from the checkpoint
+        rc = load_ck(x, sg);
+        if (rc) return rc;
+        HZO_HIP(hipMemcpyAsync(A.seg, cur_P(x), ts * 6 * N, hipMemcpyDeviceToDevice, x->stream));
+        for (int n = s0; n < s1 - 1; ++n) {
+            rc = fwd_step(x, A.Ez[n]);
+            if (rc) return rc;
+            HZO_HIP(hipMemcpyAsync((char*)A.seg + ts * 6 * N * (n + 1 - s0), cur_P(x), ts * 6 * N,
+                                   hipMemcpyDeviceToDevice, x->stream));
+        }
+        for (int n = s1 - 1; n >= s0; --n) {
+            kb.P = (const char*)A.seg + ts * 6 * N * (n - s0);
+            hzo::launch_adj_tdgl(x->stream, x->g, x->c, kb);
+            hzo::launch_adj_gradacc(x->stream, x->g, A.gpart, A.grad);
+            HZO_HIP(hipGetLastError());
+            if (n == 0) break;  // P_0 and phi_0 are inputs: no adjoint needed
+            if (dep) {
+                hzo_pcg_stats st;
+                if (x->fp32) {
+                    hzo::Fp32Bufs b32 = bufs32(x);
+                    b32.xhi = A.mu;
+                    b32.xlo = A.mulo;
+                    rc = pcg_solve(x, bufs(x), b32, ca, init, x->b64, A.last_iters, &st);
+                } else {
+                    hzo::Fp64Bufs b = bufs(x);
+                    b.phi = A.mu;
+                    rc = pcg_solve(x, b, bufs32(x), ca, init, A.rhs, A.last_iters, &st);
+                }
+                if (rc) return rc;
+                iters += st.iters;
+            }
+            hzo::launch_adj_divt(x->stream, x->g, x->c, x->fp32, A.w, A.mu, A.mulo, A.a,
+                                 dL_dm[n - 1] / (double)N, dep);
+            HZO_HIP(hipGetLastError());
+        }
+    }
+    rc = load_ck(x, A.nck - 1);  // leave the context at the recorded final state
+    if (rc) return rc;
+    HZO_HIP(hipMemcpyAsync(grad, A.grad, sizeof(double) * 12, hipMemcpyDeviceToHost, x->stream));
+    HZO_HIP(hipStreamSynchronize(x->stream));
+    if (adj_pcg_iters) *adj_pcg_iters = iters;
     return HZO_OK;
 }
 

@@ -154,7 +154,7 @@ struct LaunchEv {
 void launch_tdgl_f64(hipStream_t s, const Grid& g, const StepConsts& c, double Ez_ext, const Fp64Bufs& b,
                  const LaunchEv& ev = {});
 void launch_pcg_setup_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b,
-                          const PcgState& init,
+                          const PcgState& init, const double* rhs,
                  const LaunchEv& ev = {});
 void launch_pcg_dir_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t,
                  const LaunchEv& ev = {});
@@ -165,7 +165,7 @@ void launch_diag_f64(hipStream_t s, const Grid& g, const double* P, const uint32
 void launch_tdgl_f32(hipStream_t s, const Grid& g, const StepConsts& c, double Ez_ext, const Fp32Bufs& b,
                  const LaunchEv& ev = {});
 void launch_pcg_setup_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b,
-                          const PcgState& init,
+                          const PcgState& init, const double* rhs,
                  const LaunchEv& ev = {});
 void launch_pcg_dir_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b, int t,
                  const LaunchEv& ev = {});
@@ -173,5 +173,27 @@ void launch_pcg_update_f32(hipStream_t s, const Grid& g, const StepConsts& c, co
                  const LaunchEv& ev = {});
 void launch_diag_f32(hipStream_t s, const Grid& g, const float* P, const uint32_t* cw, double* part,
                  const LaunchEv& ev = {});
+
+// Adjoint kernels (kernels_adjoint.hip).  Pointers are float or double per
+// fp32; ptab is PhaseC32 or PhaseC accordingly.
+struct AdjTDGLBufsAny {
+    bool fp32;
+    const void* P;   // forward input state of the step (6N)
+    const void* a;   // adjoint of the step's output state (6N)
+    void* w;         // out: adjoint through the TDGL map (6N)
+    double* v;       // out: adjoint Poisson right-hand side (N)
+    const uint32_t* cw;
+    const uint16_t* sc;
+    const void* TR;
+    const void* ptab;
+    double* gpart;   // 12 x nb parameter-gradient partials
+    double dAdk;     // dA/dkappa = del_t / del_h^2
+};
+void launch_adj_tdgl(hipStream_t s, const Grid& g, const StepConsts& c, const AdjTDGLBufsAny& b);
+void launch_adj_gradacc(hipStream_t s, const Grid& g, const double* gpart, double* grad);
+void launch_adj_divt(hipStream_t s, const Grid& g, const StepConsts& c, bool fp32, const void* w,
+                     const double* mu_hi, const float* mu_lo, void* a, double seed_over_N, bool with_mu);
+void launch_adj_seed(hipStream_t s, const Grid& g, bool fp32, void* a, double seed_over_N);
+void launch_sum_column(hipStream_t s, const Grid& g, const double* part, double* out);
 
 }  // namespace hzo

@@ -152,7 +152,10 @@ __device__ __forceinline__ double true_residual(const Fp32Bufs& b, const double*
 }
 
 // ---------------------------------------------------------------- PCG setup
-__global__ __launch_bounds__(NT) void k_pcg_setup_f32(Fp32Bufs b, StepConsts c, int ps, PcgState init) {
+// rhs == nullptr: b = -eps0 div P (the forward solve); otherwise b = rhs
+// (0 on identity rows), the adjoint solve of the same operator.
+__global__ __launch_bounds__(NT) void k_pcg_setup_f32(Fp32Bufs b, StepConsts c, int ps, PcgState init,
+                                                      const double* rhs) {
     __shared__ double sH[16];
     load_lut(sH, b.lut);
     const float* dinv = b.lut32 + LUT_DINV;
@@ -172,7 +175,9 @@ __global__ __launch_bounds__(NT) void k_pcg_setup_f32(Fp32Bufs b, StepConsts c, 
             const Nbr v = op_nbrs(i, j, k, c);
             const unsigned code = b.sc[v.c];
             double bval = 0.0;
-            if (!(code & SC_IDENTITY)) {
+            if (rhs) {
+                if (!(code & SC_IDENTITY)) bval = rhs[v.c];
+            } else if (!(code & SC_IDENTITY)) {
                 const int uz = (k == nz - 1) ? k : k + 1;
                 const int dz = (k == 0) ? k : k - 1;
                 const double dv = ((double)Px[v.e] - (double)Px[v.w]) + ((double)Py[v.n] - (double)Py[v.s]) +
@@ -558,9 +563,10 @@ void launch_tdgl_f32(hipStream_t s, const Grid& g, const StepConsts& c, double E
     hipExtLaunchKernelGGL(k_tdgl_f32, grid3(g), dim3(BX, BY, 1), 0, s, ev.start, ev.stop, 0, b, c, (float)Ez_ext, blocks_for_impl(g));
 }
 void launch_pcg_setup_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b,
-                          const PcgState& init,
+                          const PcgState& init, const double* rhs,
                   const LaunchEv& ev) {
-    hipExtLaunchKernelGGL(k_pcg_setup_f32, grid3(g), dim3(BX, BY, 1), 0, s, ev.start, ev.stop, 0, b, c, part_stride(g), init);
+    hipExtLaunchKernelGGL(k_pcg_setup_f32, grid3(g), dim3(BX, BY, 1), 0, s, ev.start, ev.stop, 0, b, c, part_stride(g), init,
+                          rhs);
 }
 #define HZO_X
 void launch_pcg_dir_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b, int t,

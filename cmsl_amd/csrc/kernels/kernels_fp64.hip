@@ -165,7 +165,10 @@ __device__ __forceinline__ double op_row(unsigned code, const double* H, const S
 }
 
 // ---------------------------------------------------------------- PCG setup
-__global__ __launch_bounds__(NT) void k_pcg_setup_f64(Fp64Bufs b, StepConsts c, int ps, PcgState init) {
+// rhs == nullptr: b = -eps0 div P (the forward solve); otherwise b = rhs
+// (0 on identity rows), the adjoint solve of the same operator.
+__global__ __launch_bounds__(NT) void k_pcg_setup_f64(Fp64Bufs b, StepConsts c, int ps, PcgState init,
+                                                      const double* rhs) {
     __shared__ double sH[16];
     load_lut(sH, b.lut);
     const int nx = c.nx, ny = c.ny, nz = c.nz;
@@ -188,6 +191,10 @@ __global__ __launch_bounds__(NT) void k_pcg_setup_f64(Fp64Bufs b, StepConsts c, 
             if (code & SC_IDENTITY) {
                 bval = 0.0;  // rhs_zero_on_planes / gauge pin (depol_varEPS.cu:192-201, 404-414)
                 Ax = op_identity(x[nb7.c], k, c);
+            } else if (rhs) {
+                bval = rhs[nb7.c];
+                Ax = op_row(code, sH, c, x[nb7.c], x[nb7.e], x[nb7.w], x[nb7.n], x[nb7.s], x[nb7.u],
+                            x[nb7.d]);
             } else {
                 // divergence with clamped z (depol_varEPS.cu:293-325)
                 const int uz = (k == nz - 1) ? k : k + 1;
@@ -559,9 +566,10 @@ void launch_tdgl_f64(hipStream_t s, const Grid& g, const StepConsts& c, double E
 }
 
 void launch_pcg_setup_f64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b,
-                          const PcgState& init,
+                          const PcgState& init, const double* rhs,
                   const LaunchEv& ev) {
-    hipExtLaunchKernelGGL(k_pcg_setup_f64, grid3(g), dim3(BX, BY, 1), 0, s, ev.start, ev.stop, 0, b, c, part_stride(g), init);
+    hipExtLaunchKernelGGL(k_pcg_setup_f64, grid3(g), dim3(BX, BY, 1), 0, s, ev.start, ev.stop, 0, b, c, part_stride(g), init,
+                          rhs);
 }
 
 #define HZO_X , P2

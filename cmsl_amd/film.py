@@ -214,6 +214,34 @@ class Film:
         except Exception:
             pass
 
+    def set_phase_table(self, rows):
+        """Per-phase non-dimensional coefficients (rows of dicts or 8-tuples, AFE/FE/DE)."""
+        L.call("hzo_set_phase_table", self.h, L.phases_nd(rows))
+
+    # ---------------------------------------------------------- reverse mode
+    def adjoint_reserve(self, nsteps: int, ckpt_every: int = 50, tol: float = 1e-8):
+        """Device buffers for a recorded run of up to nsteps steps."""
+        L.call("hzo_adjoint_reserve", self.h, int(nsteps), int(ckpt_every), float(tol))
+        self._adj_steps = int(nsteps)
+
+    def forward_record(self, Ez) -> np.ndarray:
+        """Runs len(Ez) steps from the current state, checkpointing; returns
+        the loop samples m_n = <Pmz> after each step."""
+        Ez = np.ascontiguousarray(Ez, np.float64)
+        m = np.zeros(Ez.size)
+        L.call("hzo_forward_record", self.h, L.ptr(Ez), int(Ez.size), L.ptr(m))
+        return m
+
+    def backward(self, dL_dm):
+        """Gradient of L with respect to (alpha, beta, gamma, kappa) of each
+        phase, shape (3, 4) [AFE, FE, DE], for the last recorded run; also
+        returns the adjoint PCG iteration count."""
+        dL_dm = np.ascontiguousarray(dL_dm, np.float64)
+        g = np.zeros(12)
+        its = C.c_longlong(0)
+        L.call("hzo_backward", self.h, L.ptr(dL_dm), L.ptr(g), C.byref(its))
+        return g.reshape(3, 4), int(its.value)
+
     def set_state(self, P, phi=None):
         P = np.ascontiguousarray(P, np.float64)
         L.call("hzo_set_state", self.h, L.ptr(P), L.ptr(None if phi is None else np.ascontiguousarray(phi)))

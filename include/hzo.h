@@ -180,6 +180,31 @@ int hzo_sweep_run(hzo_ctx* ctx, const hzo_sweep_params* p, const char* path, int
 int hzo_set_timing(hzo_ctx* ctx, int enable);
 int hzo_kernel_times(hzo_ctx* ctx, int counts[HZO_KCLASSES], double mean_ms[HZO_KCLASSES]);
 
+/* ---------------------------------------------------------------- reverse mode
+ * Gradients of a P-E-loop loss with respect to the per-phase Landau and
+ * gradient coefficients (the alpha, beta, gamma, kappa of the hzo_phase_nd
+ * table), by hand-written adjoint kernels of the TDGL step and the same
+ * (self-adjoint) PCG for the Poisson solve.  No counterpart in the
+ * reference, which has no derivative path; it serves the Landau-coefficient
+ * fitting side (torch.autograd.Function in cmsl_amd/adjoint.py).
+ *
+ * Loop sample of step n: m_n = <Pmz> (mean over all voxels) after step n.
+ * Forward states are checkpointed every ckpt_every steps (P and phi) and the
+ * steps of each segment are recomputed during the backward pass, so device
+ * memory is (nsteps/ckpt_every + 1 + ckpt_every) states.  Needs Dirichlet
+ * electrodes when the depolarisation solve is on. */
+/* Allocates the checkpoint, segment and adjoint buffers for up to nsteps
+ * steps; tol is the relative residual of the adjoint Poisson solves. */
+int hzo_adjoint_reserve(hzo_ctx* ctx, int nsteps, int ckpt_every, double tol);
+/* Runs nsteps steps from the current state with applied field Ez[n],
+ * records checkpoints and writes m_out[n] (may be NULL). */
+int hzo_forward_record(hzo_ctx* ctx, const double* Ez, int nsteps, double* m_out);
+/* dL_dm[n] = dL/dm_n for the recorded run.  grad[p*4 + q]: phase p (0 AFE,
+ * 1 FE, 2 DE), q = 0 alpha, 1 beta, 2 gamma, 3 kappa (non-dimensional).
+ * adj_pcg_iters (may be NULL) receives the adjoint PCG iterations.  Leaves
+ * the context at the recorded final state. */
+int hzo_backward(hzo_ctx* ctx, const double* dL_dm, double grad[12], long long* adj_pcg_iters);
+
 /* ---------------------------------------------------------------- host side
  * These run on the CPU only (no device needed). */
 
