@@ -96,6 +96,107 @@ def run_cpu_baseline(inp, nsteps):
     return dt, its
 
 
+def loop_observables(E, m):
+    """Remanent <Pmz> (descending branch at E = 0) and coercive field
+    (descending branch, <Pmz> = 0, linear interpolation) of a sampled loop."""
+    n = len(E)
+    pr, ec = None, None
+    for i in range(n // 4, 3 * n // 4):
+        if E[i] >= 0.0 > E[i + 1] or E[i] == 0.0:
+            f = 0.0 if E[i] == 0.0 else E[i] / (E[i] - E[i + 1])
+            pr = m[i] + f * (m[i + 1] - m[i])
+            break
+    for i in range(n // 4, 3 * n // 4):
+        if m[i] >= 0.0 > m[i + 1]:
+            f = m[i] / (m[i] - m[i + 1])
+            ec = E[i] + f * (E[i + 1] - E[i])
+            break
+    return pr, ec
+
+
+def run_loop_errors(inp, every=10):
+    """One full period of the triangle field at 256x256x32 in both precisions,
+    <Pmz> sampled every `every` steps: errors of the FP32-refined loop, final P,
+    phi, remanent polarisation and coercive field against the FP64 run (the
+    FP64 GPU path is held to the oracle within 1e-10 by the parity tests)."""
+    import cmsl_amd as H
+    from cmsl_amd import _lib as L
+
+    Ez = inp.field_nd(H.triangle_wave(PERIOD, AMP, PERIOD))
+    E_MV = H.triangle_wave(PERIOD, AMP, PERIOD)
+    out = {}
+    for name, prec in (("fp64", L.PREC_FP64), ("fp32", L.PREC_FP32_REFINED)):
+        film = H.Film(inp, precision=prec, pcg_rule=L.PCG_RULE_RELRES, tol_rel=1e-8, tol_abs=1e-30,
+                      refine_every=20)
+        ms, Es = [], []
+        for s0 in range(0, PERIOD, every):
+            film.run(Ez[s0:s0 + every])
+            ms.append(film.diagnostics()[0] / inp.N)
+            Es.append(E_MV[s0 + every - 1])
+        P, phi, _ = film.get_state()
+        film.close()
+        out[name] = (np.array(ms), np.array(Es), P, phi)
+    m64, E, P64, phi64 = out["fp64"]
+    m32, _, P32, phi32 = out["fp32"]
+    pr64, ec64 = loop_observables(E, m64)
+    pr32, ec32 = loop_observables(E, m32)
+    rel = lambda a, b: float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+    return {"steps": PERIOD, "sample_every": every,
+            "rel_err_P_final": rel(P32, P64), "rel_err_phi_final": rel(phi32, phi64),
+            "rel_err_loop_max": rel(m32, m64),
+            "remanent_Pmz_fp64": pr64, "remanent_Pmz_fp32": pr32,
+            "rel_err_remanent": None if pr64 in (None, 0.0) else abs(pr32 - pr64) / abs(pr64),
+            "coercive_field_MVcm_fp64": ec64, "coercive_field_MVcm_fp32": ec32,
+            "rel_err_coercive": None if ec64 in (None, 0.0) or ec32 is None else abs(ec32 - ec64) / abs(ec64)}
+
+
+def run_adjoint(args):
+    """Reverse-mode cost on the differentiable workload: 128x128x24 film, 100
+    grains FE/AFE 0.7/0.3, 500-step triangle +-3 MV/cm, FP32-refined with FP64
+    reductions, checkpoints every 50 steps.  Loss: MSE between the loop and a
+    target loop simulated with hidden seeded coefficients (+-10 %)."""
+    import cmsl_amd as H
+    from cmsl_amd import _lib as L
+    from cmsl_amd.adjoint import coeff_matrix, rows_with
+
+    S, K = args.adj_steps, 50
+    inp = H.make_film_inputs(128, 128, 24, 100, fracs=(0.3, 0.7, 0.0), seed=5, jitter=0.2)
+    Ez = inp.field_nd(H.triangle_wave(S, AMP, S))
+    film = H.Film(inp, precision=L.PREC_FP32_REFINED, pcg_rule=L.PCG_RULE_RELRES, tol_rel=1e-8,
+                  tol_abs=1e-30, refine_every=20)
+    film.adjoint_reserve(S, K, 1e-8)
+    base = coeff_matrix(inp.phases)
+    rng = np.random.default_rng(12345)
+    hidden = base * (1.0 + 0.1 * rng.uniform(-1.0, 1.0, base.shape))
+    film.set_phase_table(rows_with(inp.phases, hidden))
+    film.set_state(inp.P0)
+    target = film.forward_record(Ez)
+    film.set_phase_table(rows_with(inp.phases, base))
+    film.set_state(inp.P0)
+    film.run(Ez[:10])  # warm-up
+    film.set_state(inp.P0)
+    film.synchronize()
+    t0 = time.perf_counter()
+    its = film.run(Ez)
+    film.synchronize()
+    t_plain = time.perf_counter() - t0
+    film.set_state(inp.P0)
+    film.synchronize()
+    t0 = time.perf_counter()
+    m = film.forward_record(Ez)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g, adj_its = film.backward(2.0 * (m - target) / S)
+    t_bwd = time.perf_counter() - t0
+    film.close()
+    return {"workload": "128x128x24 film, 100 grains FE/AFE 0.7/0.3, %d-step triangle +-3 MV/cm, FP32-refined, "
+                        "checkpoints every %d steps" % (S, K),
+            "t_forward_s": t_fwd, "t_backward_incl_recompute_s": t_bwd, "t_plain_run_s": t_plain,
+            "adjoint_fwd_cost": t_bwd / t_fwd, "loss": float(np.mean((m - target) ** 2)),
+            "fwd_pcg_iters_mean": float(np.mean(its)), "adj_pcg_iters_mean": adj_its / max(S - 1, 1),
+            "grad_norm": float(np.linalg.norm(g))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +206,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-kz", type=int, default=0, help="z planes per workgroup (0: automatic)")
+    ap.add_argument("--adj-steps", type=int, default=500, help="steps of the reverse-mode workload")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-loop", action="store_true", help="skip the full-loop FP32-vs-FP64 error run")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a device-to-device copy of 1 GiB three times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -148,6 +252,9 @@ def main():
             pm = json.load(fh)
         traffic = pm.get("fp32", {}).get(dom)
 
+    adj = None if args.no_adjoint else run_adjoint(args)
+    loop = None if args.no_loop else run_loop_errors(inp)
+
     cpu = None
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
@@ -180,7 +287,9 @@ def main():
                        "pcg_iters_mean": float(np.mean(res["fp64"]["its"])),
                        "kernel_ms": {k: v[1] for k, v in res["fp64"]["kt"].items()}},
         "speedup_fp32_refined_vs_fp64": value / res["fp64"]["value"],
-        "adjoint_fwd_cost": None,
+        "adjoint_fwd_cost": None if adj is None else adj["adjoint_fwd_cost"],
+        "adjoint": adj,
+        "fp32_vs_fp64_loop": loop,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_cell": bpc, "launch_ms": ms, "launches_timed": cnt},
