@@ -19,7 +19,7 @@ ZBC_DIRICHLET, ZBC_NEUMANN = 0, 1
 PREC_FP64, PREC_FP32_REFINED = 0, 1
 PCG_RULE_REFERENCE, PCG_RULE_RELRES = 0, 1
 KCLASSES = 5
-KCLASS_NAMES = ("tdgl", "pcg_setup", "pcg_dir_spmv", "pcg_update", "refine")
+KCLASS_NAMES = ("tdgl", "pcg_setup", "pcg_dir_spmv", "pcg_update", "pcg_fused")
 
 
 class PhaseND(C.Structure):
@@ -39,7 +39,8 @@ class Config(C.Structure):
                 ("eps0_nd", C.c_double), ("g_bot", C.c_double), ("g_top", C.c_double),
                 ("precision", C.c_int), ("pcg_rule", C.c_int),
                 ("tol_abs", C.c_double), ("tol_rel", C.c_double),
-                ("max_iter", C.c_int), ("refine_every", C.c_int), ("tile_kz", C.c_int)]
+                ("max_iter", C.c_int), ("refine_every", C.c_int), ("tile_kz", C.c_int),
+                ("cg_fused", C.c_int)]
 
 
 class PcgStats(C.Structure):

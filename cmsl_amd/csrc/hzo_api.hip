@@ -36,6 +36,9 @@ struct hzo_ctx {
     float* r32 = nullptr;
     float* p32[2] = {nullptr, nullptr};
     double* b64 = nullptr;
+    float* cg_r1 = nullptr;  // single-reduction CG: second r buffer, w and s pairs
+    float* cg_w[2] = {nullptr, nullptr};
+    float* cg_s[2] = {nullptr, nullptr};
     float* TR32 = nullptr;
     hzo::PhaseC32* ptab32 = nullptr;
     float* lut32 = nullptr;
@@ -342,8 +345,75 @@ int tdgl(hzo_ctx* x, double Ez_ext) {
 // (rhs == nullptr: b = -eps0 div P) or an adjoint solve of the same operator
 // (rhs = the right-hand side, 0 on identity rows).  last_iters sizes the
 // launch batches from the previous solve of the same kind.
+int copy_stats(const hzo::PcgState& s, hzo_pcg_stats* stats) {
+    if (stats) {
+        stats->iters = s.iters;
+        stats->status = s.status;
+        stats->refinements = s.refinements;
+        stats->reserved = 0;
+        stats->rz0 = s.rz0;
+        stats->rz = s.rz_old;
+        stats->rr0 = s.rr0;
+        stats->rr = s.rr;
+        stats->bb = s.bb;
+    }
+    return HZO_OK;
+}
+
+// FP32-refined solve by the single-reduction CG: the setup (r = b - A x,
+// partials r.z, r.r, b.b) and then one fused launch per iteration, the
+// device deciding from the control block whether a launch iterates,
+// recomputes the true residual in FP64, or re-forms w after it.
+int cg32_solve(hzo_ctx* x, const hzo::Fp32Bufs& b32in, const hzo::StepConsts& c, const hzo::PcgState& init,
+               const double* rhs, int& last_iters, hzo_pcg_stats* stats) {
+    hzo::Fp32Bufs b32 = b32in;
+    b32.r = x->r32;
+    hzo::CG32Bufs cb;
+    cb.r[0] = x->r32;
+    cb.r[1] = x->cg_r1;
+    cb.w[0] = x->cg_w[0];
+    cb.w[1] = x->cg_w[1];
+    cb.s[0] = x->cg_s[0];
+    cb.s[1] = x->cg_s[1];
+    cb.p = x->p32[0];
+    cb.xlo = b32.xlo;
+    cb.xhi = b32.xhi;
+    cb.b = b32.b;
+    cb.sc = x->sc;
+    cb.lut = x->lut;
+    cb.lut32 = x->lut32;
+    cb.part[0] = b32.part[0];
+    cb.part[1] = b32.part[1];
+    cb.st = b32.st;
+    int rc = timed(x, 1, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_setup_f32(x->stream, x->g, c, b32, init, rhs, ev); });
+    if (rc) return rc;
+    // launches: one WCOMP, one per iteration, two per residual replacement
+    // at most three launches per iteration (replacement, re-form, iterate)
+    const long long maxl = 3LL * init.max_iter + 8;
+    long long launches = 0;
+    int t = 0;
+    int batch = last_iters + 2;
+    while (true) {
+        for (int m = 0; m < batch && launches < maxl; ++m, ++launches) {
+            const int tt = ++t;
+            rc = timed(x, 4, [&](const hzo::LaunchEv& ev) { hzo::launch_cg32(x->stream, x->g, c, cb, tt, ev); });
+            if (rc) return rc;
+        }
+        HZO_HIP(hipMemcpyAsync(x->st_host, x->st + (t & 1), sizeof(hzo::PcgState), hipMemcpyDeviceToHost,
+                               x->stream));
+        HZO_HIP(hipStreamSynchronize(x->stream));
+        if (x->st_host->done || launches >= maxl) break;
+        batch = last_iters / 8 > 2 ? last_iters / 8 : 2;
+    }
+    hzo::PcgState& s = *x->st_host;
+    if (!s.done) s.status = HZO_PCG_MAXITER;
+    last_iters = s.iters + 2 * s.refinements + 1;
+    return copy_stats(s, stats);
+}
+
 int pcg_solve(hzo_ctx* x, const hzo::Fp64Bufs& b, const hzo::Fp32Bufs& b32, const hzo::StepConsts& c,
               const hzo::PcgState& init, const double* rhs, int& last_iters, hzo_pcg_stats* stats) {
+    if (x->fp32 && x->cfg.cg_fused) return cg32_solve(x, b32, c, init, rhs, last_iters, stats);
     int rc = x->fp32 ? timed(x, 1, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_setup_f32(x->stream, x->g, c, b32, init, rhs, ev); })
                      : timed(x, 1, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_setup_f64(x->stream, x->g, c, b, init, rhs, ev); });
     if (rc) return rc;
@@ -351,8 +421,8 @@ int pcg_solve(hzo_ctx* x, const hzo::Fp64Bufs& b, const hzo::Fp32Bufs& b32, cons
     // the last iteration finishes its inner products and raises `done`.  The
     // FP32-refined form also spends pairs on residual replacements, so up to
     // max_iter + max_iter/refine_every + 2 pairs can be needed.
-    const int re = init.refine_every > 0 ? init.refine_every : init.max_iter + 1;
-    const long long maxpairs = (long long)init.max_iter + (x->fp32 ? init.max_iter / re + 2 : 0) + 2;
+    // FP32-refined: a replacement costs a pair too, at most one per iteration
+    const long long maxpairs = (long long)init.max_iter * (x->fp32 ? 2 : 1) + 4;
     long long pairs = 0;
     int t = 0;  // launch index since the setup
     int batch = last_iters + 1;
@@ -376,20 +446,10 @@ int pcg_solve(hzo_ctx* x, const hzo::Fp64Bufs& b, const hzo::Fp32Bufs& b32, cons
         if (x->st_host->done || pairs >= maxpairs) break;
         batch = last_iters / 8 > 2 ? last_iters / 8 : 2;
     }
-    const hzo::PcgState& s = *x->st_host;
+    hzo::PcgState& s = *x->st_host;
+    if (!s.done) s.status = HZO_PCG_MAXITER;
     last_iters = s.iters + s.refinements > 0 ? s.iters + s.refinements : 1;
-    if (stats) {
-        stats->iters = s.iters;
-        stats->status = s.status;
-        stats->refinements = s.refinements;
-        stats->reserved = 0;
-        stats->rz0 = s.rz0;
-        stats->rz = s.rz_old;
-        stats->rr0 = s.rr0;
-        stats->rr = s.rr;
-        stats->bb = s.bb;
-    }
-    return HZO_OK;
+    return copy_stats(s, stats);
 }
 
 int depol(hzo_ctx* x, hzo_pcg_stats* stats) {
@@ -437,6 +497,11 @@ void free_all(hzo_ctx* x) {
     (void)hipFree(x->p32[0]);
     (void)hipFree(x->p32[1]);
     (void)hipFree(x->b64);
+    (void)hipFree(x->cg_r1);
+    (void)hipFree(x->cg_w[0]);
+    (void)hipFree(x->cg_w[1]);
+    (void)hipFree(x->cg_s[0]);
+    (void)hipFree(x->cg_s[1]);
     (void)hipFree(x->TR32);
     (void)hipFree(x->ptab32);
     (void)hipFree(x->lut32);
@@ -477,6 +542,7 @@ void hzo_config_default(hzo_config* cfg) {
     cfg->max_iter = 4000;
     cfg->refine_every = 20;
     cfg->tile_kz = 0;
+    cfg->cg_fused = 1;
 }
 
 int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
@@ -494,8 +560,9 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
     if (cfg->precision != HZO_PREC_FP64 && cfg->precision != HZO_PREC_FP32_REFINED)
         return set_error(HZO_ERR_ARG, "hzo_create: precision %d", cfg->precision);
     if (cfg->refine_every < 0) return set_error(HZO_ERR_ARG, "hzo_create: refine_every < 0");
-    if (cfg->tile_kz != 0 && cfg->tile_kz != 2 && cfg->tile_kz != 4 && cfg->tile_kz != 8)
-        return set_error(HZO_ERR_ARG, "hzo_create: tile_kz must be 0, 2, 4 or 8");
+    if (cfg->tile_kz != 0 && cfg->tile_kz != 2 && cfg->tile_kz != 4 && cfg->tile_kz != 8 &&
+        !(cfg->tile_kz == 16 && cfg->precision == HZO_PREC_FP32_REFINED && cfg->cg_fused))
+        return set_error(HZO_ERR_ARG, "hzo_create: tile_kz must be 0, 2, 4 or 8 (16: fused FP32 CG only)");
     if (cfg->pcg_rule != 0 && cfg->pcg_rule != 1) return set_error(HZO_ERR_ARG, "hzo_create: pcg_rule %d", cfg->pcg_rule);
     if (cfg->max_iter < 0) return set_error(HZO_ERR_ARG, "hzo_create: max_iter < 0");
     const long long N = (long long)cfg->nx * cfg->ny * cfg->nz;
@@ -512,13 +579,14 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
     x->g.N = N;
     // Tile depth (z planes per 64x8 workgroup) unless the caller fixes it;
     // the inner-product grouping, hence the last bits of the dot products,
-    // follows the tiling.  FP32: the smallest depth giving at most 512
-    // workgroups (one partial per consuming thread, two workgroups per CU),
-    // else 8.  FP64: the smallest of {2, 4} giving at most 1024 workgroups,
-    // else 4 (depth 8 exceeds the 128-VGPR budget of 4 waves per SIMD).
+    // follows the tiling.  Two-kernel FP32 PCG: the smallest depth giving at
+    // most 512 workgroups (one partial per consuming thread, two workgroups
+    // per CU), else 8.  FP64 and the single-reduction FP32 CG: the smallest of
+    // {2, 4} giving at most 1024 workgroups, else 4 (their depth-8 tiles
+    // exceed the 128-VGPR budget of 4 waves per SIMD).
     {
         const long long txy = (long long)((cfg->nx + 63) / 64) * ((cfg->ny + 7) / 8);
-        const bool f32 = cfg->precision == HZO_PREC_FP32_REFINED;
+        const bool f32 = cfg->precision == HZO_PREC_FP32_REFINED && !cfg->cg_fused;
         int kz = cfg->tile_kz;
         if (kz == 0) {
             kz = f32 ? 8 : 4;
@@ -580,6 +648,13 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
         HZO_ALLOC(x->p32[0], sizeof(float) * N);
         HZO_ALLOC(x->p32[1], sizeof(float) * N);
         HZO_ALLOC(x->b64, sizeof(double) * N);
+        if (cfg->cg_fused) {
+            HZO_ALLOC(x->cg_r1, sizeof(float) * N);
+            for (int q = 0; q < 2; ++q) {
+                HZO_ALLOC(x->cg_w[q], sizeof(float) * N);
+                HZO_ALLOC(x->cg_s[q], sizeof(float) * N);
+            }
+        }
         HZO_ALLOC(x->ptab32, sizeof(hzo::PhaseC32) * hzo::PTAB_ENTRIES);
         HZO_ALLOC(x->lut32, sizeof(float) * hzo::LUT_DOUBLES);
     } else {

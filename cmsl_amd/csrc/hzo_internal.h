@@ -58,7 +58,7 @@ constexpr int PTAB_ENTRIES = 6;
 // inside the iteration; the host polls `done` between launch batches.  Two
 // copies alternate by launch parity: the kernel launched t-th after the setup
 // reads copy (t-1)&1 and its workgroup 0 writes copy t&1 (see reduce.hip.h).
-enum : int { K_SETUP = 0, K_DIR = 1, K_FOLD = 2, K_UPDATE = 3, K_RESID = 4 };
+enum : int { K_SETUP = 0, K_DIR = 1, K_FOLD = 2, K_UPDATE = 3, K_RESID = 4, K_WCOMP = 5, K_ITER = 6 };
 struct PcgState {
     double rz_old, rz0, rr, rr0, bb, alpha, beta, denom;
     int iters, status, done, refinements;
@@ -139,6 +139,23 @@ struct Fp32Bufs {
     double* tdgl_part;
 };
 
+// Single-reduction CG of the FP32-refined form (kernels_cg32.hip): r, w = A M r
+// and s = A p double-buffered by the control block's parity.
+struct CG32Bufs {
+    float* r[2];
+    float* w[2];
+    float* s[2];
+    float* p;
+    float* xlo;
+    double* xhi;
+    const double* b;
+    const uint16_t* sc;
+    const double* lut;
+    const float* lut32;
+    double* part[2];
+    PcgState* st;
+};
+
 int blocks_for(const Grid& g);
 // Stride of one PCG partial-sum array (>= blocks_for; the tail is kept zero).
 int part_stride(const Grid& g);
@@ -172,6 +189,8 @@ void launch_pcg_dir_f32(hipStream_t s, const Grid& g, const StepConsts& c, const
 void launch_pcg_update_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b, int t,
                  const LaunchEv& ev = {});
 void launch_diag_f32(hipStream_t s, const Grid& g, const float* P, const uint32_t* cw, double* part,
+                 const LaunchEv& ev = {});
+void launch_cg32(hipStream_t s, const Grid& g, const StepConsts& c, const CG32Bufs& b, int t,
                  const LaunchEv& ev = {});
 
 // Adjoint kernels (kernels_adjoint.hip).  Pointers are float or double per

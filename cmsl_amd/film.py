@@ -183,12 +183,13 @@ class Film:
 
     def __init__(self, inp: FilmInputs, precision=L.PREC_FP64, pcg_rule=L.PCG_RULE_REFERENCE,
                  tol_rel=1e-2, tol_abs=1e-3, max_iter=4000, dep_flag=1, zbc=L.ZBC_DIRICHLET,
-                 refine_every=20, phi=None, set_state=True, tile_kz=0):
+                 refine_every=20, phi=None, set_state=True, tile_kz=0, cg_fused=1):
         self.inp = inp
         cfg = L.default_config(nx=inp.nx, ny=inp.ny, nz=inp.nz, del_h=inp.del_h, del_t=inp.del_t,
                                dep_flag=dep_flag, zbc=zbc, eps0_nd=inp.scales["eps0_nd"],
                                precision=precision, pcg_rule=pcg_rule, tol_abs=tol_abs, tol_rel=tol_rel,
-                               max_iter=max_iter, refine_every=refine_every, tile_kz=tile_kz)
+                               max_iter=max_iter, refine_every=refine_every, tile_kz=tile_kz,
+                               cg_fused=cg_fused)
         self.cfg = cfg
         h = C.c_void_p()
         L.call("hzo_create", C.byref(cfg), C.byref(h))

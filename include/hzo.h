@@ -83,6 +83,8 @@ typedef struct hzo_config {
     int max_iter;
     int refine_every;    /* FP32-refined: FP64 residual recompute period (iterations) */
     int tile_kz;         /* z planes per workgroup of the stencil kernels: 2, 4, 8; 0 = automatic */
+    int cg_fused;        /* FP32-refined: 1 = single-reduction CG, one fused kernel per iteration
+                            (Chronopoulos-Gear); 0 = the two-kernel PCG of the FP64 form */
 } hzo_config;
 
 typedef struct hzo_pcg_stats {
@@ -174,7 +176,8 @@ int hzo_sweep_run(hzo_ctx* ctx, const hzo_sweep_params* p, const char* path, int
  * per class, the number of working launches and their mean duration in ms
  * (launches that found the solve already finished, under a quarter of the
  * class median, are left out).  Classes: 0 tdgl, 1 pcg_setup, 2 pcg_dir_spmv,
- * 3 pcg_update, 4 refine (FP32-refined residual replacement). */
+ * 3 pcg_update (two-kernel PCG), 4 pcg_fused (single-reduction CG launch of
+ * the FP32-refined form). */
 #define HZO_TIMING_MAX 8192
 #define HZO_KCLASSES 5
 int hzo_set_timing(hzo_ctx* ctx, int enable);

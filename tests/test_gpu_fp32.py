@@ -21,11 +21,12 @@ def _say(*a):
     print("[gpu]", *a, flush=True)
 
 
-def test_fp32_refined_solve_meets_fp64_tolerance():
+@pytest.mark.parametrize("fused", [1, 0])
+def test_fp32_refined_solve_meets_fp64_tolerance(fused):
     inp = film_inputs(64, 48, 16, 12, fracs=(0.2, 0.6, 0.2), seed=6, jitter=0.2)
-    _say("fp32 solve 64x48x16")
+    _say(f"fp32 solve 64x48x16 fused={fused}")
     g = H.Film(inp, precision=L.PREC_FP32_REFINED, pcg_rule=L.PCG_RULE_RELRES, tol_rel=1e-8, tol_abs=1e-30,
-               refine_every=20)
+               refine_every=20, cg_fused=fused)
     o = oracle_film(inp, pcg_rule=1, tol_rel=1e-8, tol_abs=1e-30)
     Ez = inp.field_nd(H.triangle_wave(200, 3.0, 5))
     its = g.run(Ez)
@@ -41,12 +42,14 @@ def test_fp32_refined_solve_meets_fp64_tolerance():
     g.close()
 
 
-def test_fp32_refined_tracks_fp64_oracle_over_a_sweep():
+@pytest.mark.parametrize("fused", [1, 0])
+def test_fp32_refined_tracks_fp64_oracle_over_a_sweep(fused):
     inp = film_inputs(32, 32, 10, 8, fracs=(0.2, 0.6, 0.2), seed=9, jitter=0.2)
     nsteps = 160
     Ez = inp.field_nd(H.triangle_wave(nsteps, 3.0, nsteps))
-    _say("fp32 sweep 32x32x10 160 steps")
-    g = H.Film(inp, precision=L.PREC_FP32_REFINED, pcg_rule=L.PCG_RULE_RELRES, tol_rel=1e-8, tol_abs=1e-30)
+    _say(f"fp32 sweep 32x32x10 160 steps fused={fused}")
+    g = H.Film(inp, precision=L.PREC_FP32_REFINED, pcg_rule=L.PCG_RULE_RELRES, tol_rel=1e-8, tol_abs=1e-30,
+               cg_fused=fused)
     o = oracle_film(inp, pcg_rule=1, tol_rel=1e-8, tol_abs=1e-30)
     P = inp.P0.copy()
     phi = np.zeros(inp.N)
