@@ -6,7 +6,7 @@ setup and step loop.
 """
 from . import _lib
 from ._lib import HzoError, device_count, lib
-from .film import Film, FilmInputs, make_film_inputs, nondim, triangle_wave
+from .film import Batch, Film, FilmInputs, make_film_inputs, nondim, triangle_wave
 
-__all__ = ["Film", "FilmInputs", "HzoError", "device_count", "lib", "make_film_inputs", "nondim",
+__all__ = ["Batch", "Film", "FilmInputs", "HzoError", "device_count", "lib", "make_film_inputs", "nondim",
            "triangle_wave", "_lib"]

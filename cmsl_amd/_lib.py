@@ -96,6 +96,13 @@ SIGNATURES = {
     "hzo_adjoint_reserve": (_I, [_P, _I, _I, _D]),
     "hzo_forward_record": (_I, [_P, _P, _I, _P]),
     "hzo_backward": (_I, [_P, _P, _P, _P]),
+    "hzo_batch_create": (_I, [_P, _I, _I, _P]),
+    "hzo_batch_destroy": (None, [_P]),
+    "hzo_batch_config": (_I, [_P, _I, _I]),
+    "hzo_batch_tile_depth": (_I, [_P]),
+    "hzo_batch_run": (_I, [_P, _P, _I, _P, _P]),
+    "hzo_batch_autotune": (_I, [_P, _P, _I, _P, _P]),
+    "hzo_batch_counters": (_I, [_P, _P]),
     "hzo_read_deck": (_I, [C.c_char_p, _P]),
     "hzo_read_grain_count": (_I, [C.c_char_p, _P]),
     "hzo_read_gdata": (_I, [C.c_char_p, _I, _I, _I, _P, _P]),

@@ -2,6 +2,7 @@
 // drives the step (replaces AllocateData/Cleanup, the device globals, the
 // step loop body of evolve.cu:220-237 and the host side of Depol_field).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -1156,6 +1157,391 @@ int hzo_backward(hzo_ctx* x, const double* dL_dm, double grad[12], long long* ad
     HZO_HIP(hipMemcpyAsync(grad, A.grad, sizeof(double) * 12, hipMemcpyDeviceToHost, x->stream));
     HZO_HIP(hipStreamSynchronize(x->stream));
     if (adj_pcg_iters) *adj_pcg_iters = iters;
+    return HZO_OK;
+}
+
+/* ---------------------------------------------------------------- batched films */
+
+struct hzo_batch {
+    std::vector<hzo_ctx*> films;
+    int n = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int kz = 2, pm = 1, ps = 0, ntiles = 0;
+    std::vector<hzo::BatchFilm> hf;
+    std::vector<double*> part, tdgl_part, diag_part;
+    hzo::BatchFilm* dfilms = nullptr;
+    int2* dtiles = nullptr;
+    double* dez = nullptr;
+    double* hez = nullptr;  // pinned
+    int* dinfo = nullptr;   // 4 ints per film: done, iters, status, refinements
+    int* hinfo = nullptr;   // pinned
+    double* dm = nullptr;
+    double* hm = nullptr;   // pinned
+    int last_launches = 16;
+    long long launches_total = 0;  // CG launches of all batched runs so far
+    long long unconverged = 0;     // film solves that hit the launch cap
+    int chunk = 16;         // CG launches per graph replay (even)
+    bool graphs = true;
+    hipGraphExec_t gexec = nullptr;
+    int g_chunk = -1;
+};
+
+namespace {
+
+void batch_free_tables(hzo_batch* B) {
+    if (B->gexec) (void)hipGraphExecDestroy(B->gexec);  // it holds the table addresses
+    B->gexec = nullptr;
+    for (double* p : B->part) (void)hipFree(p);
+    for (double* p : B->tdgl_part) (void)hipFree(p);
+    for (double* p : B->diag_part) (void)hipFree(p);
+    B->part.clear();
+    B->tdgl_part.clear();
+    B->diag_part.clear();
+    (void)hipFree(B->dfilms);
+    (void)hipFree(B->dtiles);
+    B->dfilms = nullptr;
+    B->dtiles = nullptr;
+}
+
+void batch_free(hzo_batch* B) {
+    batch_free_tables(B);
+    (void)hipFree(B->dez);
+    (void)hipFree(B->dinfo);
+    (void)hipFree(B->dm);
+    if (B->hez) (void)hipHostFree(B->hez);
+    if (B->hinfo) (void)hipHostFree(B->hinfo);
+    if (B->hm) (void)hipHostFree(B->hm);
+    if (B->stream) (void)hipStreamDestroy(B->stream);
+}
+
+hzo::Fp32Bufs film_bufs32(hzo_ctx* x, int cur) {
+    const int keep = x->cur;
+    x->cur = cur;
+    hzo::Fp32Bufs b = bufs32(x);
+    x->cur = keep;
+    return b;
+}
+
+// Descriptor table, tile table and batch-owned partial arrays for tile depth
+// kz; P parity of film f at batch step s is (cur_f + s) & 1.
+int batch_tables(hzo_batch* B, int kz) {
+    batch_free_tables(B);
+    B->kz = kz;
+    const int n = B->n;
+    int maxnb = 0;
+    std::vector<int> nb(n);
+    for (int f = 0; f < n; ++f) {
+        hzo::Grid g = B->films[f]->g;
+        g.kz = kz;
+        nb[f] = hzo::blocks_for(g);
+        maxnb = nb[f] > maxnb ? nb[f] : maxnb;
+    }
+    B->pm = maxnb <= 512 ? 1 : maxnb <= 1024 ? 2 : maxnb <= 2048 ? 4 : 0;
+    B->ps = B->pm ? B->pm * 512 : maxnb;
+    B->hf.assign(n, hzo::BatchFilm());
+    std::vector<int2> tiles;
+    for (int f = 0; f < n; ++f) {
+        hzo_ctx* x = B->films[f];
+        double *pa = nullptr, *tp = nullptr, *dp = nullptr;
+        const size_t pbytes = sizeof(double) * 2 * hzo::RED_MAXQ * (size_t)B->ps;
+        if (hipMalloc((void**)&pa, pbytes) != hipSuccess || hipMemset(pa, 0, pbytes) != hipSuccess ||
+            hipMalloc((void**)&tp, sizeof(double) * hzo::TDGL_Q * nb[f]) != hipSuccess ||
+            hipMalloc((void**)&dp, sizeof(double) * hzo::DIAG_Q * nb[f]) != hipSuccess) {
+            (void)hipFree(pa);
+            (void)hipFree(tp);
+            return set_error(HZO_ERR_NOMEM, "hzo_batch: partial-sum allocation failed");
+        }
+        B->part.push_back(pa);
+        B->tdgl_part.push_back(tp);
+        B->diag_part.push_back(dp);
+        hzo::BatchFilm& F = B->hf[f];
+        for (int q = 0; q < 2; ++q) {
+            F.b[q] = film_bufs32(x, (x->cur + q) & 1);
+            F.b[q].part[0] = pa;
+            F.b[q].part[1] = pa + (size_t)hzo::RED_MAXQ * B->ps;
+            F.b[q].tdgl_part = tp;
+        }
+        F.cg.r[0] = x->r32;
+        F.cg.r[1] = x->cg_r1;
+        F.cg.w[0] = x->cg_w[0];
+        F.cg.w[1] = x->cg_w[1];
+        F.cg.s[0] = x->cg_s[0];
+        F.cg.s[1] = x->cg_s[1];
+        F.cg.p = x->p32[0];
+        F.cg.xlo = x->xlo;
+        F.cg.xhi = x->phi;
+        F.cg.b = x->b64;
+        F.cg.sc = x->sc;
+        F.cg.lut = x->lut;
+        F.cg.lut32 = x->lut32;
+        F.cg.part[0] = pa;
+        F.cg.part[1] = pa + (size_t)hzo::RED_MAXQ * B->ps;
+        F.cg.st = x->st;
+        F.c = x->c;
+        F.c.kz = kz;
+        F.g = x->g;
+        F.g.kz = kz;
+        F.init = pcg_init(x);
+        F.gx = (x->g.nx + hzo::BX - 1) / hzo::BX;
+        F.gy = (x->g.ny + hzo::BY - 1) / hzo::BY;
+        F.gz = (x->g.nz + kz - 1) / kz;
+        F.nb = nb[f];
+        F.ps = B->ps;
+        F.Ez = 0.f;
+        F.diag_part = dp;
+        for (int t = 0; t < nb[f]; ++t) tiles.push_back(make_int2(f, t));
+    }
+    B->ntiles = (int)tiles.size();
+    HZO_HIP(hipMalloc((void**)&B->dfilms, sizeof(hzo::BatchFilm) * n));
+    HZO_HIP(hipMalloc((void**)&B->dtiles, sizeof(int2) * tiles.size()));
+    HZO_HIP(hipMemcpy(B->dfilms, B->hf.data(), sizeof(hzo::BatchFilm) * n, hipMemcpyHostToDevice));
+    HZO_HIP(hipMemcpy(B->dtiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice));
+    return HZO_OK;
+}
+
+// Re-reads each film's P parity into the descriptor table (contexts may have
+// been stepped on their own since the last batched run).
+int batch_sync_parity(hzo_batch* B) {
+    for (int f = 0; f < B->n; ++f) {
+        hzo_ctx* x = B->films[f];
+        hzo::BatchFilm& F = B->hf[f];
+        for (int q = 0; q < 2; ++q) {
+            const hzo::Fp32Bufs b = film_bufs32(x, (x->cur + q) & 1);
+            F.b[q].Pold = b.Pold;
+            F.b[q].Pnew = b.Pnew;
+            F.b[q].Pcur = b.Pcur;
+        }
+    }
+    HZO_HIP(hipMemcpyAsync(B->dfilms, B->hf.data(), sizeof(hzo::BatchFilm) * B->n, hipMemcpyHostToDevice,
+                           B->stream));
+    return HZO_OK;
+}
+
+int batch_cg_chunk(hzo_batch* B) {
+    const int K = B->chunk;
+    if (B->graphs) {
+        if (!B->gexec || B->g_chunk != K) {
+            if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
+            B->gexec = nullptr;
+            hipGraph_t g;
+            HZO_HIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+            for (int t = 1; t <= K; ++t)
+                hzo::launch_cg32_batch(B->stream, B->ntiles, B->kz, B->pm, B->dfilms, B->dtiles, t);
+            HZO_HIP(hipStreamEndCapture(B->stream, &g));
+            const hipError_t e = hipGraphInstantiate(&B->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HZO_HIP(e);
+            B->g_chunk = K;
+        }
+        HZO_HIP(hipGraphLaunch(B->gexec, B->stream));
+    } else {
+        for (int t = 1; t <= K; ++t)
+            hzo::launch_cg32_batch(B->stream, B->ntiles, B->kz, B->pm, B->dfilms, B->dtiles, t);
+        HZO_HIP(hipGetLastError());
+    }
+    return HZO_OK;
+}
+
+// One batched step: TDGL of every film, PCG setup, then chunks of
+// single-reduction CG launches until every film's solve has finished.
+// Launch index t restarts at 1 in each chunk (chunks are even, so the
+// control-block parity continues).  Returns the CG launches used.
+int batch_step(hzo_batch* B, const double* ez, int s, bool dep, int& launches) {
+    HZO_HIP(hipStreamSynchronize(B->stream));  // the previous step's copy out of hez has run
+    memcpy(B->hez, ez, sizeof(double) * B->n);
+    HZO_HIP(hipMemcpyAsync(B->dez, B->hez, sizeof(double) * B->n, hipMemcpyHostToDevice, B->stream));
+    hzo::launch_batch_set_ez(B->stream, B->dfilms, B->dez, B->n);
+    hzo::launch_tdgl_f32_batch(B->stream, B->ntiles, B->dfilms, B->dtiles, s & 1);
+    HZO_HIP(hipGetLastError());
+    launches = 0;
+    if (!dep) {
+        memset(B->hinfo, 0, sizeof(int) * 4 * B->n);
+        return HZO_OK;
+    }
+    hzo::launch_setup_f32_batch(B->stream, B->ntiles, B->dfilms, B->dtiles, (s + 1) & 1);
+    HZO_HIP(hipGetLastError());
+    // the most launches any film may need: three per iteration
+    int maxl = 0;
+    for (hzo_ctx* x : B->films) maxl = 3 * x->cfg.max_iter + 8 > maxl ? 3 * x->cfg.max_iter + 8 : maxl;
+    int nchunks = (B->last_launches + B->chunk - 1) / B->chunk;
+    while (true) {
+        for (int q = 0; q < nchunks; ++q) {
+            int rc = batch_cg_chunk(B);
+            if (rc) return rc;
+            launches += B->chunk;
+        }
+        hzo::launch_batch_done(B->stream, B->dfilms, B->n, B->chunk, B->dinfo);
+        HZO_HIP(hipMemcpyAsync(B->hinfo, B->dinfo, sizeof(int) * 4 * B->n, hipMemcpyDeviceToHost, B->stream));
+        HZO_HIP(hipStreamSynchronize(B->stream));
+        bool all = true;
+        for (int f = 0; f < B->n; ++f) all = all && B->hinfo[4 * f];
+        if (all || launches >= maxl) break;
+        nchunks = 1;
+    }
+    B->last_launches = launches;
+    return HZO_OK;
+}
+
+}  // namespace
+
+int hzo_batch_create(hzo_ctx* const* films, int n, int tile_kz, hzo_batch** out) {
+    if (!films || n < 1 || !out) return set_error(HZO_ERR_ARG, "hzo_batch_create: bad argument");
+    if (tile_kz != 2 && tile_kz != 4) return set_error(HZO_ERR_ARG, "hzo_batch_create: tile_kz must be 2 or 4");
+    for (int f = 0; f < n; ++f) {
+        hzo_ctx* x = films[f];
+        int rc = check_ready(x);
+        if (rc) return rc;
+        if (!x->fp32 || !x->cfg.cg_fused)
+            return set_error(HZO_ERR_ARG, "hzo_batch_create: films must be FP32-refined with the single-reduction CG");
+        if (x->device != films[0]->device || x->cfg.dep_flag != films[0]->cfg.dep_flag)
+            return set_error(HZO_ERR_ARG, "hzo_batch_create: films must share the device and dep_flag");
+    }
+    hzo_batch* B = new hzo_batch();
+    B->films.assign(films, films + n);
+    B->n = n;
+    B->device = films[0]->device;
+    auto fail = [&](int rc) {
+        batch_free(B);
+        delete B;
+        return rc;
+    };
+    if (hipSetDevice(B->device) != hipSuccess || hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(set_error(HZO_ERR_HIP, "hzo_batch_create: stream creation failed"));
+    if (hipMalloc((void**)&B->dez, sizeof(double) * n) != hipSuccess ||
+        hipMalloc((void**)&B->dinfo, sizeof(int) * 4 * n) != hipSuccess ||
+        hipMalloc((void**)&B->dm, sizeof(double) * n) != hipSuccess ||
+        hipHostMalloc((void**)&B->hez, sizeof(double) * n, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&B->hinfo, sizeof(int) * 4 * n, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&B->hm, sizeof(double) * n, hipHostMallocDefault) != hipSuccess)
+        return fail(set_error(HZO_ERR_NOMEM, "hzo_batch_create: allocation failed"));
+    for (hzo_ctx* x : B->films)
+        if (hipStreamSynchronize(x->stream) != hipSuccess)
+            return fail(set_error(HZO_ERR_HIP, "hzo_batch_create: film stream synchronisation failed"));
+    int rc = batch_tables(B, tile_kz);
+    if (rc) return fail(rc);
+    *out = B;
+    return HZO_OK;
+}
+
+void hzo_batch_destroy(hzo_batch* B) {
+    if (!B) return;
+    (void)hipSetDevice(B->device);
+    (void)hipStreamSynchronize(B->stream);
+    batch_free(B);
+    delete B;
+}
+
+int hzo_batch_config(hzo_batch* B, int use_graphs, int chunk) {
+    if (!B || chunk < 2 || (chunk & 1)) return set_error(HZO_ERR_ARG, "hzo_batch_config: chunk must be even and >= 2");
+    B->graphs = use_graphs != 0;
+    B->chunk = chunk;
+    return HZO_OK;
+}
+
+int hzo_batch_tile_depth(hzo_batch* B) { return B ? B->kz : 0; }
+
+int hzo_batch_run(hzo_batch* B, const double* Ez, int nsteps, int* iters, double* m_out) {
+    if (!B || nsteps < 0 || (nsteps > 0 && !Ez)) return set_error(HZO_ERR_ARG, "hzo_batch_run: bad argument");
+    HZO_HIP(hipSetDevice(B->device));
+    for (hzo_ctx* x : B->films) HZO_HIP(hipStreamSynchronize(x->stream));
+    int rc = batch_sync_parity(B);
+    if (rc) return rc;
+    const bool dep = B->films[0]->cfg.dep_flag != 0;
+    for (int s = 0; s < nsteps; ++s) {
+        int l = 0;
+        rc = batch_step(B, Ez + (size_t)s * B->n, s, dep, l);
+        if (rc) return rc;
+        if (iters)
+            for (int f = 0; f < B->n; ++f) iters[(size_t)s * B->n + f] = B->hinfo[4 * f + 1];
+        B->launches_total += l;
+        for (int f = 0; f < B->n; ++f) B->unconverged += dep && !B->hinfo[4 * f];
+        if (m_out) {
+            hzo::launch_diag_f32_batch(B->stream, B->ntiles, B->dfilms, B->dtiles, (s + 1) & 1);
+            hzo::launch_batch_sum_pmz(B->stream, B->dfilms, B->n, B->dm);
+            HZO_HIP(hipGetLastError());
+            HZO_HIP(hipMemcpyAsync(B->hm, B->dm, sizeof(double) * B->n, hipMemcpyDeviceToHost, B->stream));
+            HZO_HIP(hipStreamSynchronize(B->stream));
+            for (int f = 0; f < B->n; ++f) m_out[(size_t)s * B->n + f] = B->hm[f] / (double)B->films[f]->g.N;
+        }
+    }
+    HZO_HIP(hipStreamSynchronize(B->stream));
+    for (hzo_ctx* x : B->films) x->cur = (x->cur + nsteps) & 1;
+    return HZO_OK;
+}
+
+int hzo_batch_autotune(hzo_batch* B, const double* Ez, int reps, int* kz_out, double ms[2]) {
+    if (!B || !Ez || reps < 1) return set_error(HZO_ERR_ARG, "hzo_batch_autotune: bad argument");
+    HZO_HIP(hipSetDevice(B->device));
+    for (hzo_ctx* x : B->films) HZO_HIP(hipStreamSynchronize(x->stream));
+    // A trial step reads P[cur] and writes only P[cur^1], the potential and
+    // the solver's work vectors, so restoring the potential (x_hi, x_lo)
+    // returns every film to its state before the trial.
+    std::vector<double*> hi(B->n, nullptr);
+    std::vector<float*> lo(B->n, nullptr);
+    auto release = [&]() {
+        for (int f = 0; f < B->n; ++f) {
+            (void)hipFree(hi[f]);
+            (void)hipFree(lo[f]);
+        }
+    };
+    for (int f = 0; f < B->n; ++f) {
+        const size_t N = (size_t)B->films[f]->g.N;
+        if (hipMalloc((void**)&hi[f], sizeof(double) * N) != hipSuccess ||
+            hipMalloc((void**)&lo[f], sizeof(float) * N) != hipSuccess) {
+            release();
+            return set_error(HZO_ERR_NOMEM, "hzo_batch_autotune: snapshot allocation failed");
+        }
+    }
+    auto copy = [&](bool save) -> int {
+        for (int f = 0; f < B->n; ++f) {
+            hzo_ctx* x = B->films[f];
+            const size_t N = (size_t)x->g.N;
+            HZO_HIP(hipMemcpyAsync(save ? hi[f] : x->phi, save ? x->phi : hi[f], sizeof(double) * N,
+                                   hipMemcpyDeviceToDevice, B->stream));
+            HZO_HIP(hipMemcpyAsync(save ? lo[f] : x->xlo, save ? x->xlo : lo[f], sizeof(float) * N,
+                                   hipMemcpyDeviceToDevice, B->stream));
+        }
+        HZO_HIP(hipStreamSynchronize(B->stream));
+        return HZO_OK;
+    };
+    const int cand[2] = {2, 4};
+    double best_ms[2] = {1e300, 1e300};
+    const bool dep = B->films[0]->cfg.dep_flag != 0;
+    int rc = copy(true);
+    for (int w = 0; w < 2 && rc == HZO_OK; ++w) {
+        if ((rc = batch_tables(B, cand[w])) != HZO_OK) break;
+        if ((rc = batch_sync_parity(B)) != HZO_OK) break;
+        for (int r = 0; r <= reps && rc == HZO_OK; ++r) {  // r = 0 warms up (code objects, graph)
+            if ((rc = copy(false)) != HZO_OK) break;
+            const auto t0 = std::chrono::steady_clock::now();
+            int l = 0;
+            if ((rc = batch_step(B, Ez, 0, dep, l)) != HZO_OK) break;
+            if (hipStreamSynchronize(B->stream) != hipSuccess) {
+                rc = set_error(HZO_ERR_HIP, "hzo_batch_autotune: synchronisation failed");
+                break;
+            }
+            const double t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (r > 0 && t < best_ms[w]) best_ms[w] = t;
+        }
+    }
+    if (rc == HZO_OK) rc = copy(false);
+    release();
+    if (rc) return rc;
+    const int best = best_ms[1] < best_ms[0] ? 1 : 0;
+    if ((rc = batch_tables(B, cand[best])) != HZO_OK) return rc;
+    if (kz_out) *kz_out = cand[best];
+    if (ms) {
+        ms[0] = best_ms[0];
+        ms[1] = best_ms[1];
+    }
+    return HZO_OK;
+}
+
+int hzo_batch_counters(const hzo_batch* B, long long out[2]) {
+    if (!B || !out) return set_error(HZO_ERR_ARG, "hzo_batch_counters: null argument");
+    out[0] = B->launches_total;
+    out[1] = B->unconverged;
     return HZO_OK;
 }
 

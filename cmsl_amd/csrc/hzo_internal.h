@@ -70,6 +70,8 @@ struct PcgState {
 // Per-workgroup partial sums: two parity slots of RED_MAXQ arrays each for
 // the PCG, plus the TDGL local-frame sums and the diagnostics sums.
 constexpr int RED_MAXQ = 4;
+// Workgroup tile: BX x BY columns (one thread each) x kz planes.
+constexpr int BX = 64, BY = 8, NT = BX * BY;
 constexpr int DIAG_Q = 8;
 constexpr int TDGL_Q = 2;
 
@@ -156,6 +158,21 @@ struct CG32Bufs {
     PcgState* st;
 };
 
+// One film of a batch (device table, hzo_batch_*): the buffers of its
+// context by P parity, its single-reduction CG vectors, step constants, tile
+// grid and partial layout (batch-owned partial arrays, common stride).
+struct BatchFilm {
+    Fp32Bufs b[2];
+    CG32Bufs cg;
+    StepConsts c;
+    Grid g;
+    PcgState init;
+    int gx, gy, gz;  // tile grid (x, y, z)
+    int nb, ps;      // tiles and partial stride
+    float Ez;        // applied field of the current step
+    double* diag_part;
+};
+
 int blocks_for(const Grid& g);
 // Stride of one PCG partial-sum array (>= blocks_for; the tail is kept zero).
 int part_stride(const Grid& g);
@@ -192,6 +209,16 @@ void launch_diag_f32(hipStream_t s, const Grid& g, const float* P, const uint32_
                  const LaunchEv& ev = {});
 void launch_cg32(hipStream_t s, const Grid& g, const StepConsts& c, const CG32Bufs& b, int t,
                  const LaunchEv& ev = {});
+// Batched FP32-refined launches over every tile of every film (1-D grid of
+// ntiles workgroups; tiles[] maps each to (film, logical tile)).
+void launch_tdgl_f32_batch(hipStream_t s, int ntiles, const BatchFilm* films, const int2* tiles, int parity);
+void launch_setup_f32_batch(hipStream_t s, int ntiles, const BatchFilm* films, const int2* tiles, int parity);
+void launch_diag_f32_batch(hipStream_t s, int ntiles, const BatchFilm* films, const int2* tiles, int parity);
+void launch_cg32_batch(hipStream_t s, int ntiles, int kz, int pm, const BatchFilm* films, const int2* tiles, int t);
+void launch_batch_set_ez(hipStream_t s, BatchFilm* films, const double* ez, int n);
+// info: done, iters, status, refinements per film
+void launch_batch_done(hipStream_t s, const BatchFilm* films, int n, int t, int* info);
+void launch_batch_sum_pmz(hipStream_t s, const BatchFilm* films, int n, double* m);
 
 // Adjoint kernels (kernels_adjoint.hip).  Pointers are float or double per
 // fp32; ptab is PhaseC32 or PhaseC accordingly.

@@ -6,8 +6,6 @@
 
 namespace hzo {
 
-constexpr int BX = 64, BY = 8, NT = BX * BY;
-
 inline int blocks_for_impl(const Grid& g) {
     return ((g.nx + BX - 1) / BX) * ((g.ny + BY - 1) / BY) * ((g.nz + g.kz - 1) / g.kz);
 }
@@ -55,7 +53,23 @@ constexpr int N_XCD = 8;
 
 struct TileId {
     int bx, by, bz;
+    int blk;    // logical tile index within the film: its partial-sum slot
+    bool lead;  // the film's tile 0, which passes the control block on
 };
+
+// Tile coordinates of logical tile t of a film with gx x gy x gz tiles,
+// numbered x fastest, then z, then y.
+__device__ __forceinline__ TileId tile_of(int t, int gx, int gy, int gz) {
+    TileId T;
+    T.bx = t % gx;
+    const int r = t / gx;
+    T.bz = r % gz;
+    T.by = r / gz;
+    T.blk = t;
+    T.lead = t == 0;
+    (void)gy;
+    return T;
+}
 
 __device__ __forceinline__ TileId tile_id() {
     const int gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
@@ -64,12 +78,22 @@ __device__ __forceinline__ TileId tile_id() {
     const int xcd = w % N_XCD, loc = w / N_XCD;
     const int per = nbk / N_XCD, rem = nbk % N_XCD;
     const int t = xcd * per + min(xcd, rem) + loc;
-    TileId T;
-    T.bx = t % gx;
-    const int r = t / gx;
-    T.bz = r % gz;
-    T.by = r / gz;
-    return T;
+    return tile_of(t, gx, gy, gz);
+}
+
+// Batched launches: workgroup w of a 1-D grid over all tiles of all films
+// takes tile t of the XCD-aware order (contiguous runs per XCD, as above);
+// tiles[t] = (film, logical tile within the film).  Returns the film index.
+__device__ __forceinline__ int batch_tile(const int2* tiles, const BatchFilm* films, TileId& T) {
+    const int nbk = gridDim.x;
+    const int w = blockIdx.x;
+    const int xcd = w % N_XCD, loc = w / N_XCD;
+    const int per = nbk / N_XCD, rem = nbk % N_XCD;
+    const int t = xcd * per + min(xcd, rem) + loc;
+    const int2 e = tiles[t];
+    const BatchFilm& F = films[e.x];
+    T = tile_of(e.y, F.gx, F.gy, F.gz);
+    return e.x;
 }
 
 __device__ __forceinline__ long long vid(int i, int j, int k, int nx, int ny) {
@@ -178,9 +202,9 @@ __device__ __forceinline__ void store_state(const PcgState& s, PcgState* p) {
     p->refine_every = s.refine_every;
 }
 
-// Copies the control block forward (workgroup 0, thread 0).
-__device__ __forceinline__ void pass_state(const PcgState& s, PcgState* out) {
-    if (block0_thread0()) store_state(s, out);
+// Copies the control block forward (thread 0 of the film's lead tile).
+__device__ __forceinline__ void pass_state(const PcgState& s, PcgState* out, bool lead) {
+    if (lead && threadIdx.x == 0 && threadIdx.y == 0) store_state(s, out);
 }
 
 // The stopping test after a completed reduction of r.z and r.r (the

@@ -127,11 +127,12 @@ __device__ __forceinline__ int cg_decide(PcgState& s, const double (&tot)[3], do
 }
 
 template <int KZ, int PM>
-__global__ __launch_bounds__(NT, KZ >= 16 ? 2 : 4) void k_cg32(CG32Bufs b, StepConsts c, int nb, int ps, int t) {
+__device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c, int nb, int ps, int t,
+                                          const TileId& T) {
     const int ip = (t - 1) & 1, io = t & 1;
     PcgState s = load_state(b.st + ip);
     if (s.done) {
-        pass_state(s, b.st + io);
+        pass_state(s, b.st + io, T.lead);
         return;
     }
     __shared__ HPair<float> sHP[64];
@@ -140,7 +141,7 @@ __global__ __launch_bounds__(NT, KZ >= 16 ? 2 : 4) void k_cg32(CG32Bufs b, StepC
     __shared__ Bcast32 bc;
     const int nz = c.nz;
     const int tid = linear_tid();
-    const TileId T = tile_id();
+    
     const int k0 = T.bz * KZ;
     const TilePos tp = tile_pos(c, T);
     const TileCols G = tile_cols(tp, c);
@@ -202,7 +203,7 @@ __global__ __launch_bounds__(NT, KZ >= 16 ? 2 : 4) void k_cg32(CG32Bufs b, StepC
         double alpha = 0.0, beta = 0.0;
         const int mode = cg_decide(s, tot, alpha, beta);
         if (mode == 1) {
-            pass_state(s, b.st + io);
+            pass_state(s, b.st + io, T.lead);
         } else if (mode == 2) {
             s.kind = K_RESID;
         } else if (mode == 3) {
@@ -353,8 +354,42 @@ __global__ __launch_bounds__(NT, KZ >= 16 ? 2 : 4) void k_cg32(CG32Bufs b, StepC
             bstore(Rx, vo, so[q] * pb4, xn);
         }
     }
-    write_block_partials<3>(acc, io ? b.part[1] : b.part[0], ps, sraw);
-    pass_state(s, b.st + io);
+    write_block_partials<3>(acc, io ? b.part[1] : b.part[0], ps, T.blk, sraw);
+    pass_state(s, b.st + io, T.lead);
+}
+
+template <int KZ, int PM>
+__global__ __launch_bounds__(NT, KZ >= 16 ? 2 : 4) void k_cg32(CG32Bufs b, StepConsts c, int nb, int ps, int t) {
+    cg32_body<KZ, PM>(b, c, nb, ps, t, tile_id());
+}
+
+// All films of a batch in one launch: the tile table maps each workgroup to
+// (film, logical tile); every film runs its own control block, and the tiles
+// of a film whose solve has finished return at once.
+template <int KZ, int PM>
+__global__ __launch_bounds__(NT, KZ >= 16 ? 2 : 4) void k_cg32_batch(const BatchFilm* films, const int2* tiles,
+                                                                      int t) {
+    TileId T;
+    const BatchFilm& F = films[batch_tile(tiles, films, T)];
+    cg32_body<KZ, PM>(F.cg, F.c, F.nb, F.ps, t, T);
+}
+
+void launch_cg32_batch(hipStream_t s, int ntiles, int kz, int pm, const BatchFilm* films, const int2* tiles,
+                       int t) {
+    const dim3 gr(ntiles), bl(BX, BY, 1);
+#define HZO_CGB(KZ)                                                                                   \
+    switch (pm) {                                                                                     \
+        case 1: hipLaunchKernelGGL((k_cg32_batch<KZ, 1>), gr, bl, 0, s, films, tiles, t); break;       \
+        case 2: hipLaunchKernelGGL((k_cg32_batch<KZ, 2>), gr, bl, 0, s, films, tiles, t); break;       \
+        case 4: hipLaunchKernelGGL((k_cg32_batch<KZ, 4>), gr, bl, 0, s, films, tiles, t); break;       \
+        default: hipLaunchKernelGGL((k_cg32_batch<KZ, 0>), gr, bl, 0, s, films, tiles, t); break;      \
+    }
+    if (kz == 2) {
+        HZO_CGB(2)
+    } else {
+        HZO_CGB(4)
+    }
+#undef HZO_CGB
 }
 
 #define HZO_X
@@ -369,5 +404,46 @@ void launch_cg32(hipStream_t s, const Grid& g, const StepConsts& c, const CG32Bu
     HZO_PCG_LAUNCH_X(k_cg32, g, s, b, c, blocks_for_impl(g), part_stride_impl(blocks_for_impl(g)), t);
 }
 #undef HZO_X
+
+}  // namespace hzo
+
+namespace hzo {
+
+// Batch bookkeeping kernels (one small launch each).
+__global__ void k_batch_set_ez(BatchFilm* films, const double* ez, int n) {
+    for (int f = threadIdx.x; f < n; f += blockDim.x) films[f].Ez = (float)ez[f];
+}
+
+// info[4f..4f+3] = done flag, iterations, status and residual replacements of
+// film f's control block after launch t.
+__global__ void k_batch_done(const BatchFilm* films, int n, int t, int* info) {
+    for (int f = threadIdx.x; f < n; f += blockDim.x) {
+        const PcgState& st = films[f].cg.st[t & 1];
+        info[4 * f + 0] = st.done;
+        info[4 * f + 1] = st.iters;
+        info[4 * f + 2] = st.status;
+        info[4 * f + 3] = st.refinements;
+    }
+}
+
+// m[f] = sum of the first diagnostics column of film f (one wave per film).
+__global__ void k_batch_sum_pmz(const BatchFilm* films, double* m) {
+    const BatchFilm& F = films[blockIdx.x];
+    const int lane = threadIdx.x;
+    double s = 0.0;
+    for (int t = lane; t < F.nb; t += 64) s += F.diag_part[t];
+    s = wave_sum(s);
+    if (lane == 0) m[blockIdx.x] = s;
+}
+
+void launch_batch_set_ez(hipStream_t s, BatchFilm* films, const double* ez, int n) {
+    hipLaunchKernelGGL(k_batch_set_ez, dim3(1), dim3(256), 0, s, films, ez, n);
+}
+void launch_batch_done(hipStream_t s, const BatchFilm* films, int n, int t, int* info) {
+    hipLaunchKernelGGL(k_batch_done, dim3(1), dim3(256), 0, s, films, n, t, info);
+}
+void launch_batch_sum_pmz(hipStream_t s, const BatchFilm* films, int n, double* m) {
+    hipLaunchKernelGGL(k_batch_sum_pmz, dim3(n), dim3(64), 0, s, films, m);
+}
 
 }  // namespace hzo

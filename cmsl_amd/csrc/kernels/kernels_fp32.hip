@@ -22,10 +22,10 @@
 namespace hzo {
 
 // ---------------------------------------------------------------- TDGL
-__global__ __launch_bounds__(NT) void k_tdgl_f32(Fp32Bufs b, StepConsts c, float Ez_ext, int nb) {
+__device__ __forceinline__ void tdgl_f32_body(const Fp32Bufs& b, const StepConsts& c, float Ez_ext, int nb,
+                                              const TileId& T) {
     const int nx = c.nx, ny = c.ny, nz = c.nz;
     const long long N = (long long)nx * ny * nz;
-    const TileId T = tile_id();
     const int i = T.bx * BX + threadIdx.x;
     const int j = T.by * BY + threadIdx.y;
     const int k0 = T.bz * c.kz;
@@ -111,7 +111,17 @@ __global__ __launch_bounds__(NT) void k_tdgl_f32(Fp32Bufs b, StepConsts c, float
             b.Pnew[5 * N + id] = sz * om + A * sum[5] - cL * dsz;
         }
     }
-    write_block_partials<TDGL_Q>(loc, b.tdgl_part, nb);
+    write_block_partials<TDGL_Q>(loc, b.tdgl_part, nb, T.blk);
+}
+
+__global__ __launch_bounds__(NT) void k_tdgl_f32(Fp32Bufs b, StepConsts c, float Ez_ext, int nb) {
+    tdgl_f32_body(b, c, Ez_ext, nb, tile_id());
+}
+
+__global__ __launch_bounds__(NT) void k_tdgl_f32_batch(const BatchFilm* films, const int2* tiles, int parity) {
+    TileId T;
+    const BatchFilm& F = films[batch_tile(tiles, films, T)];
+    tdgl_f32_body(F.b[parity], F.c, F.Ez, F.nb, T);
 }
 
 // ---------------------------------------------------------------- operator rows
@@ -154,14 +164,13 @@ __device__ __forceinline__ double true_residual(const Fp32Bufs& b, const double*
 // ---------------------------------------------------------------- PCG setup
 // rhs == nullptr: b = -eps0 div P (the forward solve); otherwise b = rhs
 // (0 on identity rows), the adjoint solve of the same operator.
-__global__ __launch_bounds__(NT) void k_pcg_setup_f32(Fp32Bufs b, StepConsts c, int ps, PcgState init,
-                                                      const double* rhs) {
+__device__ __forceinline__ void setup_f32_body(const Fp32Bufs& b, const StepConsts& c, int ps, PcgState init,
+                                               const double* rhs, const TileId& T) {
     __shared__ double sH[16];
     load_lut(sH, b.lut);
     const float* dinv = b.lut32 + LUT_DINV;
     const int nx = c.nx, ny = c.ny, nz = c.nz;
     const long long N = (long long)nx * ny * nz;
-    const TileId T = tile_id();
     const int i = T.bx * BX + threadIdx.x;
     const int j = T.by * BY + threadIdx.y;
     const int k0 = T.bz * c.kz;
@@ -193,9 +202,20 @@ __global__ __launch_bounds__(NT) void k_pcg_setup_f32(Fp32Bufs b, StepConsts c, 
             acc[2] += bval * bval;
         }
     }
-    write_block_partials<3>(acc, b.part[0], ps);
+    write_block_partials<3>(acc, b.part[0], ps, T.blk);
     init.kind = K_SETUP;
-    pass_state(init, b.st);
+    pass_state(init, b.st, T.lead);
+}
+
+__global__ __launch_bounds__(NT) void k_pcg_setup_f32(Fp32Bufs b, StepConsts c, int ps, PcgState init,
+                                                      const double* rhs) {
+    setup_f32_body(b, c, ps, init, rhs, tile_id());
+}
+
+__global__ __launch_bounds__(NT) void k_pcg_setup_f32_batch(const BatchFilm* films, const int2* tiles, int parity) {
+    TileId T;
+    const BatchFilm& F = films[batch_tile(tiles, films, T)];
+    setup_f32_body(F.b[parity], F.c, F.ps, F.init, nullptr, T);
 }
 
 // Scalars wave 0 hands to the rest of the workgroup after the loop test.
@@ -214,9 +234,10 @@ struct Bcast {
 template <int KZ, int PM>
 __global__ __launch_bounds__(NT, 4) void k_pcg_dir_f32(Fp32Bufs b, StepConsts c, int nb, int ps, int t) {
     const int ip = (t - 1) & 1, io = t & 1;
+    const TileId T = tile_id();
     PcgState s = load_state(b.st + ip);
     if (s.done) {
-        pass_state(s, b.st + io);
+        pass_state(s, b.st + io, T.lead);
         return;
     }
     const bool first = s.kind == K_SETUP;
@@ -226,7 +247,7 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_dir_f32(Fp32Bufs b, StepConsts c,
     __shared__ Bcast bc;
     const int nz = c.nz;
     const int tid = linear_tid();
-    const TileId T = tile_id();
+    
     const int k0 = T.bz * KZ;
     const TilePos tp = tile_pos(c, T);
     const TileCols G = tile_cols(tp, c);
@@ -321,10 +342,10 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_dir_f32(Fp32Bufs b, StepConsts c,
             }
         }
         if (s.done) {
-            pass_state(s, b.st + io);
+            pass_state(s, b.st + io, T.lead);
         } else if (fold) {
             s.kind = K_FOLD;
-            pass_state(s, b.st + io);
+            pass_state(s, b.st + io, T.lead);
         } else {
             s.iters += 1;
             s.pbuf ^= 1;
@@ -388,23 +409,24 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_dir_f32(Fp32Bufs b, StepConsts c,
         if (ok) acc[0] += (double)xc * (double)qv;
     }
     RawLDS<1>& sacc = *reinterpret_cast<RawLDS<1>*>(&sraw);
-    write_block_partials<1>(acc, io ? b.part[1] : b.part[0], ps, sacc);
-    pass_state(s, b.st + io);
+    write_block_partials<1>(acc, io ? b.part[1] : b.part[0], ps, T.blk, sacc);
+    pass_state(s, b.st + io, T.lead);
 }
 
 // ---------------------------------------------------------------- update / residual replacement
 template <int KZ, int PM>
 __global__ __launch_bounds__(NT, 4) void k_pcg_update_f32(Fp32Bufs b, StepConsts c, int nb, int ps, int t) {
     const int ip = (t - 1) & 1, io = t & 1;
+    const TileId T = tile_id();
     PcgState s = load_state(b.st + ip);
     if (s.done) {
-        pass_state(s, b.st + io);
+        pass_state(s, b.st + io, T.lead);
         return;
     }
     __shared__ RawLDS<2> sraw;
     const bool resid = s.kind == K_FOLD;
     const int nx = c.nx, ny = c.ny, nz = c.nz;
-    const TileId T = tile_id();
+    
     const int k0 = T.bz * KZ;
     double acc[2] = {0.0, 0.0};
     if (resid) {  // r = b - A x_hi in FP64 (x_lo was folded by the previous launch)
@@ -478,7 +500,7 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_update_f32(Fp32Bufs b, StepConsts
             if (!finite(tot[0]) || !(tot[0] > 0.0)) {
                 s.status = HZO_PCG_BREAKDOWN;
                 s.done = 1;
-                pass_state(s, b.st + io);
+                pass_state(s, b.st + io, T.lead);
             } else {
                 s.alpha = s.rz_old / tot[0];
             }
@@ -524,16 +546,15 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_update_f32(Fp32Bufs b, StepConsts
         }
         s.kind = K_UPDATE;
     }
-    write_block_partials<2>(acc, io ? b.part[1] : b.part[0], ps, sraw);
-    pass_state(s, b.st + io);
+    write_block_partials<2>(acc, io ? b.part[1] : b.part[0], ps, T.blk, sraw);
+    pass_state(s, b.st + io, T.lead);
 }
 
 // ---------------------------------------------------------------- diagnostics
-__global__ __launch_bounds__(NT) void k_diag_f32(const float* __restrict__ P, const uint32_t* __restrict__ cw,
-                                                  Grid g, double* part, int nb) {
+__device__ __forceinline__ void diag_f32_body(const float* __restrict__ P, const uint32_t* __restrict__ cw,
+                                              const Grid& g, double* part, int nb, const TileId& T) {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long N = g.N;
-    const TileId T = tile_id();
     const int i = T.bx * BX + threadIdx.x;
     const int j = T.by * BY + threadIdx.y;
     const int k0 = T.bz * g.kz;
@@ -555,7 +576,18 @@ __global__ __launch_bounds__(NT) void k_diag_f32(const float* __restrict__ P, co
             a[7] += (phase == 0) ? 1.0 : 0.0;
         }
     }
-    write_block_partials<DIAG_Q>(a, part, nb);
+    write_block_partials<DIAG_Q>(a, part, nb, T.blk);
+}
+
+__global__ __launch_bounds__(NT) void k_diag_f32(const float* __restrict__ P, const uint32_t* __restrict__ cw,
+                                                  Grid g, double* part, int nb) {
+    diag_f32_body(P, cw, g, part, nb, tile_id());
+}
+
+__global__ __launch_bounds__(NT) void k_diag_f32_batch(const BatchFilm* films, const int2* tiles, int parity) {
+    TileId T;
+    const BatchFilm& F = films[batch_tile(tiles, films, T)];
+    diag_f32_body(F.b[parity].Pcur, F.b[parity].cw, F.g, F.diag_part, F.nb, T);
 }
 
 void launch_tdgl_f32(hipStream_t s, const Grid& g, const StepConsts& c, double Ez_ext, const Fp32Bufs& b,
@@ -578,6 +610,15 @@ void launch_pcg_update_f32(hipStream_t s, const Grid& g, const StepConsts& c, co
     HZO_PCG_LAUNCH_X(k_pcg_update_f32, g, s, b, c, blocks_for(g), part_stride(g), t);
 }
 #undef HZO_X
+void launch_tdgl_f32_batch(hipStream_t s, int ntiles, const BatchFilm* films, const int2* tiles, int parity) {
+    hipLaunchKernelGGL(k_tdgl_f32_batch, dim3(ntiles), dim3(BX, BY, 1), 0, s, films, tiles, parity);
+}
+void launch_setup_f32_batch(hipStream_t s, int ntiles, const BatchFilm* films, const int2* tiles, int parity) {
+    hipLaunchKernelGGL(k_pcg_setup_f32_batch, dim3(ntiles), dim3(BX, BY, 1), 0, s, films, tiles, parity);
+}
+void launch_diag_f32_batch(hipStream_t s, int ntiles, const BatchFilm* films, const int2* tiles, int parity) {
+    hipLaunchKernelGGL(k_diag_f32_batch, dim3(ntiles), dim3(BX, BY, 1), 0, s, films, tiles, parity);
+}
 void launch_diag_f32(hipStream_t s, const Grid& g, const float* P, const uint32_t* cw, double* part,
                   const LaunchEv& ev) {
     hipExtLaunchKernelGGL(k_diag_f32, grid3(g), dim3(BX, BY, 1), 0, s, ev.start, ev.stop, 0, P, cw, g, part, blocks_for_impl(g));

@@ -147,7 +147,7 @@ __global__ __launch_bounds__(NT) void k_tdgl_f64(Fp64Bufs b, StepConsts c, doubl
             b.Pnew[5 * N + id] = pSz0 * om + A * sum[5] - cL * dfdPsz_b;
         }
     }
-    write_block_partials<TDGL_Q>(loc, b.tdgl_part, nb);
+    write_block_partials<TDGL_Q>(loc, b.tdgl_part, nb, T.blk);
 }
 
 // -(sum of harmonic-mean face fluxes)/h^2, depol_varEPS.cu:76-123
@@ -215,9 +215,9 @@ __global__ __launch_bounds__(NT) void k_pcg_setup_f64(Fp64Bufs b, StepConsts c, 
             acc[2] += bval * bval;
         }
     }
-    write_block_partials<3>(acc, b.part[0], ps);
+    write_block_partials<3>(acc, b.part[0], ps, T.blk);
     init.kind = K_SETUP;
-    pass_state(init, b.st);
+    pass_state(init, b.st, T.lead);
 }
 
 // A x of one non-identity row from the paired face table, in the reference's
@@ -259,9 +259,10 @@ struct Bcast {
 template <int KZ, int PM, bool P2>
 __global__ __launch_bounds__(NT, 4) void k_pcg_dir_f64(Fp64Bufs b, StepConsts c, int nb, int ps, int t) {
     const int ip = (t - 1) & 1, io = t & 1;
+    const TileId T = tile_id();
     PcgState s = load_state(b.st + ip);
     if (s.done) {
-        pass_state(s, b.st + io);
+        pass_state(s, b.st + io, T.lead);
         return;
     }
     const bool first = s.kind == K_SETUP;
@@ -271,7 +272,7 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_dir_f64(Fp64Bufs b, StepConsts c,
     __shared__ Bcast bc;
     const int nz = c.nz;
     const int tid = linear_tid();
-    const TileId T = tile_id();
+    
     const int k0 = T.bz * KZ;
     const TilePos tp = tile_pos(c, T);
     const TileCols G = tile_cols(tp, c);
@@ -354,7 +355,7 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_dir_f64(Fp64Bufs b, StepConsts c,
             }
         }
         if (s.done) {
-            pass_state(s, b.st + io);
+            pass_state(s, b.st + io, T.lead);
         } else {
             s.iters += 1;  // ++kiter (:476)
             s.pbuf ^= 1;
@@ -402,8 +403,8 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_dir_f64(Fp64Bufs b, StepConsts c,
         if (ok) acc[0] += xc * qv;  // p.Ap (:485)
     }
     RawLDS<1>& sacc = *reinterpret_cast<RawLDS<1>*>(&sraw);
-    write_block_partials<1>(acc, io ? b.part[1] : b.part[0], ps, sacc);
-    pass_state(s, b.st + io);
+    write_block_partials<1>(acc, io ? b.part[1] : b.part[0], ps, T.blk, sacc);
+    pass_state(s, b.st + io, T.lead);
 }
 
 // ---------------------------------------------------------------- PCG update
@@ -412,9 +413,10 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_dir_f64(Fp64Bufs b, StepConsts c,
 template <int KZ, int PM, bool P2>
 __global__ __launch_bounds__(NT, 4) void k_pcg_update_f64(Fp64Bufs b, StepConsts c, int nb, int ps, int t) {
     const int ip = (t - 1) & 1, io = t & 1;
+    const TileId T = tile_id();
     PcgState s = load_state(b.st + ip);
     if (s.done) {
-        pass_state(s, b.st + io);
+        pass_state(s, b.st + io, T.lead);
         return;
     }
     __shared__ HPair<double> sHP[64];
@@ -423,7 +425,7 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_update_f64(Fp64Bufs b, StepConsts
     __shared__ Bcast bc;
     const int nz = c.nz;
     const int tid = linear_tid();
-    const TileId T = tile_id();
+    
     const int k0 = T.bz * KZ;
     const TilePos tp = tile_pos(c, T);
     const TileCols G = tile_cols(tp, c);
@@ -472,7 +474,7 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_update_f64(Fp64Bufs b, StepConsts
         if (!finite(tot[0]) || !(tot[0] > 0.0)) {  // :486-490
             s.status = HZO_PCG_BREAKDOWN;
             s.done = 1;
-            pass_state(s, b.st + io);
+            pass_state(s, b.st + io, T.lead);
         } else {
             s.alpha = s.rz_old / tot[0];  // :492
             s.kind = K_UPDATE;
@@ -518,8 +520,8 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_update_f64(Fp64Bufs b, StepConsts
             acc[1] += r * r;
         }
     }
-    write_block_partials<2>(acc, io ? b.part[1] : b.part[0], ps, sraw);
-    pass_state(s, b.st + io);
+    write_block_partials<2>(acc, io ? b.part[1] : b.part[0], ps, T.blk, sraw);
+    pass_state(s, b.st + io, T.lead);
 }
 
 // ---------------------------------------------------------------- diagnostics
@@ -556,7 +558,7 @@ __global__ __launch_bounds__(NT) void k_diag_f64(const double* __restrict__ P,
             a[7] += mafe;
         }
     }
-    write_block_partials<DIAG_Q>(a, part, nb);
+    write_block_partials<DIAG_Q>(a, part, nb, T.blk);
 }
 
 // ---------------------------------------------------------------- launchers

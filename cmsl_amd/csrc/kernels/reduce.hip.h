@@ -72,10 +72,11 @@ __device__ __forceinline__ void wave0_total(const RawLDS<NQ>& R, double (&tot)[N
 }
 
 // Workgroup sum of NQ per-thread values, stored by thread 0 at
-// out[q*stride + block].  Uses R as scratch (one barrier inside); the caller
+// out[q*stride + blk].  Uses R as scratch (one barrier inside); the caller
 // must not reuse R before a later barrier.
 template <int NQ>
-__device__ __forceinline__ void write_block_partials(const double (&v)[NQ], double* out, int stride, RawLDS<NQ>& R) {
+__device__ __forceinline__ void write_block_partials(const double (&v)[NQ], double* out, int stride, int blk,
+                                                     RawLDS<NQ>& R) {
     const int tid = linear_tid();
 #pragma unroll
     for (int q = 0; q < NQ; ++q) R.v[q][tid] = v[q];
@@ -84,17 +85,16 @@ __device__ __forceinline__ void write_block_partials(const double (&v)[NQ], doub
         double tot[NQ];
         wave0_total<NQ>(R, tot);
         if (tid == 0) {
-            const int b = linear_block();
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) out[(long long)q * stride + b] = tot[q];
+            for (int q = 0; q < NQ; ++q) out[(long long)q * stride + blk] = tot[q];
         }
     }
 }
 
 template <int NQ>
-__device__ __forceinline__ void write_block_partials(const double (&v)[NQ], double* out, int stride) {
+__device__ __forceinline__ void write_block_partials(const double (&v)[NQ], double* out, int stride, int blk) {
     __shared__ RawLDS<NQ> R;
-    write_block_partials<NQ>(v, out, stride, R);
+    write_block_partials<NQ>(v, out, stride, blk, R);
 }
 
 // Cross-kernel reduction of the PCG partials.  Partial arrays have a padded

@@ -299,3 +299,62 @@ class Film:
         ms = np.zeros(L.KCLASSES)
         L.call("hzo_kernel_times", self.h, L.ptr(cnt), L.ptr(ms))
         return {n: (int(cnt[i]), float(ms[i])) for i, n in enumerate(L.KCLASS_NAMES)}
+
+
+class Batch:
+    """Films of different sizes and coefficients stepped together on one GPU
+    (FP32-refined, single-reduction CG): one launch per kernel for the whole
+    batch, CG launches replayed from a captured hipGraph.  The films stay
+    owned by the caller and must outlive the batch."""
+
+    def __init__(self, films, tile_kz: int = 2, use_graphs: bool = True, chunk: int = 16):
+        self.films = list(films)
+        arr = (C.c_void_p * len(self.films))(*[f.h.value for f in self.films])
+        self._arr = arr
+        h = C.c_void_p()
+        L.call("hzo_batch_create", arr, len(self.films), int(tile_kz), C.byref(h))
+        self.h = h
+        L.call("hzo_batch_config", h, 1 if use_graphs else 0, int(chunk))
+
+    @property
+    def n(self) -> int:
+        return len(self.films)
+
+    @property
+    def tile_depth(self) -> int:
+        return L.lib().hzo_batch_tile_depth(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.lib().hzo_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, Ez, want_m: bool = False):
+        """Ez: (nsteps, n) applied fields.  Returns the PCG iterations
+        (nsteps, n) and, with want_m, <Pmz> of each film after each step."""
+        Ez = np.ascontiguousarray(np.broadcast_to(np.asarray(Ez, np.float64), (len(Ez), self.n)))
+        S = Ez.shape[0]
+        its = np.zeros((S, self.n), np.int32)
+        m = np.zeros((S, self.n)) if want_m else None
+        L.call("hzo_batch_run", self.h, L.ptr(Ez), S, L.ptr(its), L.ptr(m))
+        return (its, m) if want_m else its
+
+    def autotune(self, Ez0, reps: int = 2):
+        """Times one step at tile depth 2 and 4 (fields Ez0[f]) and keeps the
+        faster; returns (kz, {2: ms, 4: ms})."""
+        Ez0 = np.ascontiguousarray(np.broadcast_to(np.asarray(Ez0, np.float64), (self.n,)))
+        kz = C.c_int(0)
+        ms = np.zeros(2)
+        L.call("hzo_batch_autotune", self.h, L.ptr(Ez0), int(reps), C.byref(kz), L.ptr(ms))
+        return kz.value, {2: float(ms[0]), 4: float(ms[1])}
+
+    def counters(self):
+        out = np.zeros(2, np.int64)
+        L.call("hzo_batch_counters", self.h, L.ptr(out))
+        return {"cg_launches": int(out[0]), "unconverged": int(out[1])}

@@ -208,6 +208,42 @@ int hzo_forward_record(hzo_ctx* ctx, const double* Ez, int nsteps, double* m_out
  * the context at the recorded final state. */
 int hzo_backward(hzo_ctx* ctx, const double* dL_dm, double grad[12], long long* adj_pcg_iters);
 
+/* ---------------------------------------------------------------- batched films
+ * Many films of different sizes and coefficients stepped together: each
+ * kernel of the FP32-refined step (TDGL, PCG setup, single-reduction CG
+ * iteration, diagnostics) runs once for the whole batch over a tile table
+ * that maps every workgroup to (film, tile); each film keeps its own solver
+ * control block, and the CG launches are replayed from a captured hipGraph
+ * in chunks until every film's solve has finished.  No counterpart in the
+ * reference, which steps one film per process (evolve.cu:220-237 in a loop
+ * over decks); it serves parameter sweeps over many microstructures.
+ *
+ * The films must be FP32-refined (HZO_PREC_FP32_REFINED, cg_fused = 1), on
+ * one device, with the same dep_flag, and ready (microstructure, phase table
+ * and state set).  A batch holds the films' device buffers by reference: the
+ * films must outlive it and must not be stepped concurrently with it.  Each
+ * film ends a batched run in exactly the state its own hzo_run would leave
+ * at the same tile depth. */
+typedef struct hzo_batch hzo_batch;
+/* tile_kz: 2 or 4 (planes per workgroup tile; hzo_batch_autotune picks it). */
+int hzo_batch_create(hzo_ctx* const* films, int n, int tile_kz, hzo_batch** out);
+void hzo_batch_destroy(hzo_batch* batch);
+/* use_graphs: replay CG launches from a captured hipGraph (default 1);
+ * chunk: CG launches per replay, even, >= 2 (default 16). */
+int hzo_batch_config(hzo_batch* batch, int use_graphs, int chunk);
+int hzo_batch_tile_depth(hzo_batch* batch);
+/* nsteps steps; Ez[s*n + f] is the applied field of film f at step s.
+ * iters[s*n + f] (may be NULL): PCG iterations of film f at step s.
+ * m_out[s*n + f] (may be NULL): <Pmz> of film f after step s. */
+int hzo_batch_run(hzo_batch* batch, const double* Ez, int nsteps, int* iters, double* m_out);
+/* Times one step at tile depth 2 and at 4 (best of reps after a warm-up step,
+ * applied fields Ez[f]), restores the films' states and keeps the faster
+ * depth.  kz_out and ms[2] (times at depth 2 and 4) may be NULL. */
+int hzo_batch_autotune(hzo_batch* batch, const double* Ez, int reps, int* kz_out, double ms[2]);
+/* out[0]: CG launches issued by all batched runs; out[1]: film solves that
+ * stopped at the launch cap without converging. */
+int hzo_batch_counters(const hzo_batch* batch, long long out[2]);
+
 /* ---------------------------------------------------------------- host side
  * These run on the CPU only (no device needed). */
 

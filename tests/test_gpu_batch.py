@@ -1,0 +1,123 @@
+"""Batched films (hzo_batch_*) against the same films stepped one by one.
+
+A batch runs every film's kernels in shared launches over a (film, tile)
+table, but each film keeps its own tile numbering, partial-sum slots and
+solver control block, so at the same tile depth each film must end in
+exactly the state its own hzo_run leaves: P, x_hi and the PCG iteration
+counts are compared bit for bit.  The films differ in every dimension, grain
+count and Landau coefficients (+-20 % jitter), and run different applied
+fields, so a film whose solve finishes early idles in later CG launches.
+"""
+import numpy as np
+import pytest
+
+import cmsl_amd as H
+from cmsl_amd import _lib as L
+from helpers import film_inputs
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(64, 48, 16, 12), (40, 72, 12, 9), (96, 32, 10, 15), (32, 32, 8, 6)]
+
+
+def _say(*a):
+    print("[gpu]", *a, flush=True)
+
+
+def _inputs():
+    return [film_inputs(nx, ny, nz, ng, fracs=(0.3, 0.5, 0.2), seed=11 + f, jitter=0.2)
+            for f, (nx, ny, nz, ng) in enumerate(SHAPES)]
+
+
+def _film(inp, kz=2):
+    return H.Film(inp, precision=L.PREC_FP32_REFINED, pcg_rule=L.PCG_RULE_RELRES, tol_rel=1e-8, tol_abs=1e-30,
+                  refine_every=20, tile_kz=kz)
+
+
+def _fields(inps, S):
+    amps = [3.0, 2.0, 2.5, 1.5]
+    return np.stack([inp.field_nd(H.triangle_wave(S, amps[f], S)) for f, inp in enumerate(inps)], axis=1)
+
+
+@pytest.mark.parametrize("kz", [2, 4])
+def test_batched_films_match_single_film_runs_bit_for_bit(kz):
+    inps = _inputs()
+    S = 14
+    Ez = _fields(inps, S)
+    _say(f"batch of {len(inps)} films, {S} steps, kz={kz}")
+    films = [_film(inp, kz) for inp in inps]
+    batch = H.Batch(films, tile_kz=kz)
+    assert batch.tile_depth == kz
+    its_b, m_b = batch.run(Ez, want_m=True)
+    cnt = batch.counters()
+    _say("batched iterations per step", its_b.tolist(), cnt)
+    assert cnt["unconverged"] == 0
+    for f, inp in enumerate(inps):
+        single = _film(inp, kz)
+        assert single.tile_depth == kz
+        its_s = single.run(Ez[:, f])
+        Pb, phib, _ = films[f].get_state()
+        Ps, phis, _ = single.get_state()
+        assert np.array_equal(its_b[:, f], its_s), (f, its_b[:, f], its_s)
+        assert np.array_equal(Pb, Ps), f
+        assert np.array_equal(phib, phis), f
+        # <Pmz> after the last step, from the single film's diagnostics
+        d = single.diagnostics()
+        assert abs(m_b[-1, f] - d[0] / inp.N) <= 1e-12 * max(1.0, abs(d[0] / inp.N))
+        single.close()
+    batch.close()
+    for f in films:
+        f.close()
+
+
+def test_graph_replay_and_plain_launches_agree():
+    inps = _inputs()[:3]
+    S = 8
+    Ez = _fields(inps, S)
+    out = []
+    for graphs, chunk in ((True, 16), (False, 6), (True, 2)):
+        _say(f"graphs={graphs} chunk={chunk}")
+        films = [_film(inp) for inp in inps]
+        batch = H.Batch(films, tile_kz=2, use_graphs=graphs, chunk=chunk)
+        its = batch.run(Ez)
+        out.append((its, [f.get_state()[0] for f in films]))
+        batch.close()
+        for f in films:
+            f.close()
+    for its, P in out[1:]:
+        assert np.array_equal(its, out[0][0])
+        for a, b in zip(P, out[0][1]):
+            assert np.array_equal(a, b)
+
+
+def test_autotune_keeps_the_films_state_and_picks_a_depth():
+    inps = _inputs()
+    S = 6
+    Ez = _fields(inps, S)
+    films = [_film(inp) for inp in inps]
+    batch = H.Batch(films, tile_kz=2)
+    batch.run(Ez[:2])
+    before = [f.get_state() for f in films]
+    kz, ms = batch.autotune(Ez[2], reps=1)
+    _say("autotune", kz, ms)
+    assert kz in (2, 4) and batch.tile_depth == kz
+    assert ms[2] > 0.0 and ms[4] > 0.0
+    for f, (P, phi, _) in zip(films, before):
+        P2, phi2, _ = f.get_state()
+        assert np.array_equal(P, P2) and np.array_equal(phi, phi2)
+    its = batch.run(Ez[2:])
+    assert batch.counters()["unconverged"] == 0 and its.min() > 0
+    batch.close()
+    for f in films:
+        f.close()
+
+
+def test_batch_rejects_fp64_and_two_kernel_films():
+    inp = _inputs()[3]
+    f64 = H.Film(inp, precision=L.PREC_FP64)
+    two = H.Film(inp, precision=L.PREC_FP32_REFINED, cg_fused=0)
+    for bad in (f64, two):
+        with pytest.raises(L.HzoError):
+            H.Batch([bad])
+    f64.close()
+    two.close()
