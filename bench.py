@@ -200,6 +200,80 @@ def run_adjoint(args):
             "grad_norm": float(np.linalg.norm(g))}
 
 
+def batch_film_specs(n, seed=2024):
+    """Film shapes of the batched workload: nx = ny from {128, 192, 256, 384},
+    nz from {16, 24, 32, 48}, 50-300 grains, one seed per film."""
+    rng = np.random.default_rng(seed)
+    return [(int(rng.choice([128, 192, 256, 384])), int(rng.choice([16, 24, 32, 48])),
+             int(rng.integers(50, 301)), 100 + f) for f in range(n)]
+
+
+def run_batched(args):
+    """Batched workload: args.batch_films films of uneven sizes, grain counts
+    and Landau coefficients (+-20 %), FP32-refined, sinusoidal +-3 MV/cm over
+    1000 steps; all films advance together (one launch per kernel for the
+    whole batch or per cache-sized group of films, CG launches replayed from
+    a captured hipGraph; tile depth and group budget picked by the
+    autotuner).  Times args.batch_steps steps of the batch, then
+    the same number of following steps with the films stepped one by one."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import cmsl_amd as H
+    from cmsl_amd import _lib as L
+
+    specs = batch_film_specs(args.batch_films)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        inps = list(ex.map(lambda s: H.make_film_inputs(s[0], s[0], s[1], s[2], fracs=FRACS, seed=s[3],
+                                                        jitter=0.2), specs))
+    t_inputs = time.perf_counter() - t0
+    films = [H.Film(inp, precision=L.PREC_FP32_REFINED, pcg_rule=L.PCG_RULE_RELRES, tol_rel=1e-8,
+                    tol_abs=1e-30, refine_every=20) for inp in inps]
+    ntot = sum(inp.N for inp in inps)
+    W, S = args.batch_warmup, args.batch_steps
+    wave = AMP * np.sin(2.0 * np.pi * np.arange(W + 2 * S) / 1000.0)
+    Ez = np.stack([inp.field_nd(wave) for inp in inps], axis=1)  # (steps, films)
+    batch = H.Batch(films, tile_kz=2)
+    shape, tune_ms = batch.autotune(Ez[0], reps=2)
+    batch.run(Ez[:W])
+    cnt0 = batch.counters()
+    t0 = time.perf_counter()
+    its = batch.run(Ez[W:W + S])
+    dt_batch = time.perf_counter() - t0
+    cnt = batch.counters()
+    cnt = {k: cnt[k] - cnt0[k] for k in cnt}
+    batch.close()
+    t0 = time.perf_counter()
+    its_seq = np.stack([f.run(Ez[W + S:W + 2 * S, q]) for q, f in enumerate(films)], axis=1)
+    for f in films:
+        f.synchronize()
+    dt_seq = time.perf_counter() - t0
+    # algorithmic bytes: TDGL + PCG setup per voxel, plus one fused CG
+    # launch per iteration
+    per_vox = algorithmic_bytes_per_cell("tdgl", "fp32") + algorithmic_bytes_per_cell("pcg_setup", "fp32")
+    nvec = np.array([inp.N for inp in inps], np.float64)
+    bytes_batch = float(np.sum(nvec * (S * per_vox + its.sum(axis=0) * algorithmic_bytes_per_cell("pcg_fused", "fp32"))))
+    for f in films:
+        f.close()
+    sizes = {}
+    for nx, nz, _, _ in specs:
+        sizes[f"{nx}x{nx}x{nz}"] = sizes.get(f"{nx}x{nx}x{nz}", 0) + 1
+    return {"workload": "%d films, nx=ny in {128,192,256,384}, nz in {16,24,32,48}, 50-300 grains, +-20%% Landau, "
+                        "FP32-refined, sinusoid +-3 MV/cm / 1000 steps; %d warm-up + %d timed steps of the batch, "
+                        "then the next %d steps film by film" % (args.batch_films, W, S, S),
+            "films": args.batch_films, "cells_total": ntot, "film_shapes": sizes,
+            "value": ntot * S / dt_batch / 1e6, "unit": "Mcell-steps/s",
+            "ms_per_step": dt_batch / S * 1e3,
+            "sequential_value": ntot * S / dt_seq / 1e6,
+            "speedup_vs_sequential": dt_seq / dt_batch,
+            "pcg_iters_mean": float(its.mean()), "pcg_iters_mean_sequential": float(its_seq.mean()),
+            "algorithmic_GBs": bytes_batch / dt_batch / 1e9,
+            "shape": shape,
+            "autotune_ms_per_step": {f"kz{k}_group{m}MiB": v for (k, m), v in tune_ms.items()},
+            "cg_launches_timed": cnt["cg_launches"],
+            "unconverged_solves": cnt["unconverged"], "t_inputs_s": t_inputs}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,6 +285,9 @@ def main():
     ap.add_argument("--tile-kz", type=int, default=0, help="z planes per workgroup (0: automatic)")
     ap.add_argument("--adj-steps", type=int, default=500, help="steps of the reverse-mode workload")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--batch-films", type=int, default=64, help="films of the batched workload (0: skip it)")
+    ap.add_argument("--batch-steps", type=int, default=8)
+    ap.add_argument("--batch-warmup", type=int, default=2)
     ap.add_argument("--no-loop", action="store_true", help="skip the full-loop FP32-vs-FP64 error run")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a device-to-device copy of 1 GiB three times (calibrates the "
@@ -257,6 +334,7 @@ def main():
 
     adj = None if args.no_adjoint else run_adjoint(args)
     loop = None if args.no_loop else run_loop_errors(inp)
+    bat = run_batched(args) if args.batch_films > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -293,6 +371,7 @@ def main():
         "adjoint_fwd_cost": None if adj is None else adj["adjoint_fwd_cost"],
         "adjoint": adj,
         "fp32_vs_fp64_loop": loop,
+        "batched": bat,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_cell": bpc, "launch_ms": ms, "launches_timed": cnt},

@@ -14,6 +14,12 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libhzo.so")
 
 HZO_OK = 0
+HZO_ERR_ARG = -1
+HZO_ERR_HIP = -2
+HZO_ERR_IO = -3
+HZO_ERR_STATE = -4
+HZO_ERR_NOMEM = -5
+HZO_ERR_DATA = -6
 HZO_AFE, HZO_FE, HZO_DE = 0, 1, 2
 ZBC_DIRICHLET, ZBC_NEUMANN = 0, 1
 PREC_FP64, PREC_FP32_REFINED = 0, 1
@@ -100,9 +106,20 @@ SIGNATURES = {
     "hzo_batch_destroy": (None, [_P]),
     "hzo_batch_config": (_I, [_P, _I, _I]),
     "hzo_batch_tile_depth": (_I, [_P]),
+    "hzo_batch_set_shape": (_I, [_P, _I, _D]),
+    "hzo_batch_shape": (_I, [_P, _P]),
     "hzo_batch_run": (_I, [_P, _P, _I, _P, _P]),
     "hzo_batch_autotune": (_I, [_P, _P, _I, _P, _P]),
     "hzo_batch_counters": (_I, [_P, _P]),
+    "hzo_save_snapshot": (_I, [_P, C.c_char_p, C.c_char_p, _D]),
+    "hzo_checkpoint_save": (_I, [_P, C.c_char_p, _I, _I, _D, _D]),
+    "hzo_checkpoint_load": (_I, [_P, C.c_char_p, _P, _P, _P, _P, _P]),
+    "hzo_write_vector_field": (_I, [C.c_char_p, C.c_longlong, _P, _P, _P, _D]),
+    "hzo_write_grain_map": (_I, [C.c_char_p, _I, _I, _I, _P, _D]),
+    "hzo_write_midplane_slice": (_I, [C.c_char_p, _I, _I, _I, _P]),
+    "hzo_write_checkpoint": (_I, [C.c_char_p, _I, _I, _I, _I, _I, _D, _D, _P, _P]),
+    "hzo_read_checkpoint": (_I, [C.c_char_p, _I, _I, _I, _P, _P, _P, _P, _P, _P]),
+    "hzo_write_raw_f64": (_I, [C.c_char_p, C.c_longlong, _P]),
     "hzo_read_deck": (_I, [C.c_char_p, _P]),
     "hzo_read_grain_count": (_I, [C.c_char_p, _P]),
     "hzo_read_gdata": (_I, [C.c_char_p, _I, _I, _I, _P, _P]),

@@ -361,14 +361,9 @@ int copy_stats(const hzo::PcgState& s, hzo_pcg_stats* stats) {
     return HZO_OK;
 }
 
-// FP32-refined solve by the single-reduction CG: the setup (r = b - A x,
-// partials r.z, r.r, b.b) and then one fused launch per iteration, the
-// device deciding from the control block whether a launch iterates,
-// recomputes the true residual in FP64, or re-forms w after it.
-int cg32_solve(hzo_ctx* x, const hzo::Fp32Bufs& b32in, const hzo::StepConsts& c, const hzo::PcgState& init,
-               const double* rhs, int& last_iters, hzo_pcg_stats* stats) {
-    hzo::Fp32Bufs b32 = b32in;
-    b32.r = x->r32;
+// Vectors of the single-reduction CG of a film (the solution, right-hand side,
+// partial sums and control block come from b32).
+hzo::CG32Bufs cg_bufs(hzo_ctx* x, const hzo::Fp32Bufs& b32) {
     hzo::CG32Bufs cb;
     cb.r[0] = x->r32;
     cb.r[1] = x->cg_r1;
@@ -386,6 +381,18 @@ int cg32_solve(hzo_ctx* x, const hzo::Fp32Bufs& b32in, const hzo::StepConsts& c,
     cb.part[0] = b32.part[0];
     cb.part[1] = b32.part[1];
     cb.st = b32.st;
+    return cb;
+}
+
+// FP32-refined solve by the single-reduction CG: the setup (r = b - A x,
+// partials r.z, r.r, b.b) and then one fused launch per iteration, the
+// device deciding from the control block whether a launch iterates,
+// recomputes the true residual in FP64, or re-forms w after it.
+int cg32_solve(hzo_ctx* x, const hzo::Fp32Bufs& b32in, const hzo::StepConsts& c, const hzo::PcgState& init,
+               const double* rhs, int& last_iters, hzo_pcg_stats* stats) {
+    hzo::Fp32Bufs b32 = b32in;
+    b32.r = x->r32;
+    const hzo::CG32Bufs cb = cg_bufs(x, b32);
     int rc = timed(x, 1, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_setup_f32(x->stream, x->g, c, b32, init, rhs, ev); });
     if (rc) return rc;
     // launches: one WCOMP, one per iteration, two per residual replacement
@@ -1160,52 +1167,115 @@ int hzo_backward(hzo_ctx* x, const double* dL_dm, double grad[12], long long* ad
     return HZO_OK;
 }
 
+/* ---------------------------------------------------------------- dumps and restart */
+
+// save_snapshot (generate.cu:266-303): Pm and Ps as interleaved xyz f64
+// dumps scaled by P0.
+int hzo_save_snapshot(hzo_ctx* x, const char* pm_path, const char* ps_path, double P0_scale) {
+    int rc = check_ready(x);
+    if (rc) return rc;
+    if (!pm_path || !ps_path) return set_error(HZO_ERR_ARG, "hzo_save_snapshot: null path");
+    const size_t N = (size_t)x->g.N;
+    std::vector<double> P(6 * N);
+    if ((rc = hzo_get_state(x, P.data(), nullptr, nullptr)) != HZO_OK) return rc;
+    const double* p = P.data();
+    if ((rc = hzo_write_vector_field(pm_path, (long long)N, p, p + N, p + 2 * N, P0_scale)) != HZO_OK) return rc;
+    return hzo_write_vector_field(ps_path, (long long)N, p + 3 * N, p + 4 * N, p + 5 * N, P0_scale);
+}
+
+// write_checkpoint (generate.cu:90-168): the current P and E = -grad phi.
+int hzo_checkpoint_save(hzo_ctx* x, const char* path, int peloop, int count, double Ez_ext, double incEz) {
+    int rc = check_ready(x);
+    if (rc) return rc;
+    const size_t N = (size_t)x->g.N;
+    std::vector<double> P(6 * N), E(3 * N);
+    if ((rc = hzo_get_state(x, P.data(), nullptr, E.data())) != HZO_OK) return rc;
+    return hzo_write_checkpoint(path, x->g.nx, x->g.ny, x->g.nz, peloop, count, Ez_ext, incEz, P.data(), E.data());
+}
+
+// read_checkpoint (generate.cu:170-264) as a restart: P from the file; the
+// file holds E but no potential, so phi is re-solved from P (linear initial
+// guess, then the depolarisation solve when it is on), which reproduces the
+// stored E to the solver tolerance.  E_file (3N, may be NULL) receives the
+// stored field.
+int hzo_checkpoint_load(hzo_ctx* x, const char* path, int* peloop, int* count, double* Ez_ext, double* incEz,
+                        double* E_file) {
+    if (!x) return set_error(HZO_ERR_ARG, "null context");
+    const size_t N = (size_t)x->g.N;
+    std::vector<double> P(6 * N);
+    int rc = hzo_read_checkpoint(path, x->g.nx, x->g.ny, x->g.nz, peloop, count, Ez_ext, incEz, P.data(), E_file);
+    if (rc) return rc;
+    if ((rc = hzo_set_state(x, P.data(), nullptr)) != HZO_OK) return rc;
+    if (x->cfg.dep_flag) return hzo_depol_solve(x, nullptr);
+    return HZO_OK;
+}
+
 /* ---------------------------------------------------------------- batched films */
 
+// Films are packed into groups whose solver working set fits a cache budget;
+// each group's kernels run as one launch over a (film, tile) table, and the
+// groups run one after another, each through all steps of a call, so that a
+// group's CG vectors stay in the 256 MB last-level cache across iterations.
 struct hzo_batch {
+    struct Group {
+        std::vector<int> idx;  // batch indices of the films
+        int single = -1;       // a group of one film runs that film's own kernels and tile depth
+        std::vector<hzo::BatchFilm> hf;
+        std::vector<double*> part, tdgl_part, diag_part;
+        hzo::BatchFilm* dfilms = nullptr;
+        int2* dtiles = nullptr;
+        int ntiles = 0, pm = 1, ps = 0;
+        hipGraphExec_t gexec = nullptr;
+        int g_chunk = -1;
+        int need = 16;  // CG launches the group's last solve needed
+    };
     std::vector<hzo_ctx*> films;
     int n = 0;
     int device = 0;
     hipStream_t stream = nullptr;
-    int kz = 2, pm = 1, ps = 0, ntiles = 0;
-    std::vector<hzo::BatchFilm> hf;
-    std::vector<double*> part, tdgl_part, diag_part;
-    hzo::BatchFilm* dfilms = nullptr;
-    int2* dtiles = nullptr;
+    int kz = 2;
+    double budget = 0.0;  // bytes of CG working set per group (0: one group)
+    std::vector<Group> groups;
     double* dez = nullptr;
     double* hez = nullptr;  // pinned
     int* dinfo = nullptr;   // 4 ints per film: done, iters, status, refinements
     int* hinfo = nullptr;   // pinned
     double* dm = nullptr;
     double* hm = nullptr;   // pinned
-    int last_launches = 16;
-    long long launches_total = 0;  // CG launches of all batched runs so far
-    long long unconverged = 0;     // film solves that hit the launch cap
     int chunk = 16;         // CG launches per graph replay (even)
     bool graphs = true;
-    hipGraphExec_t gexec = nullptr;
-    int g_chunk = -1;
+    long long launches_total = 0;  // CG launches of all batched runs so far
+    long long unconverged = 0;     // film solves that hit the launch cap
 };
 
 namespace {
 
-void batch_free_tables(hzo_batch* B) {
-    if (B->gexec) (void)hipGraphExecDestroy(B->gexec);  // it holds the table addresses
-    B->gexec = nullptr;
-    for (double* p : B->part) (void)hipFree(p);
-    for (double* p : B->tdgl_part) (void)hipFree(p);
-    for (double* p : B->diag_part) (void)hipFree(p);
-    B->part.clear();
-    B->tdgl_part.clear();
-    B->diag_part.clear();
-    (void)hipFree(B->dfilms);
-    (void)hipFree(B->dtiles);
-    B->dfilms = nullptr;
-    B->dtiles = nullptr;
+// Bytes per voxel the single-reduction CG keeps live across iterations:
+// r, w, s (two buffers each), p, x_lo (float) and the stencil code (uint16).
+constexpr double CG_SET_BYTES = 8 * 4 + 2;
+
+void group_free(hzo_batch::Group& G) {
+    if (G.gexec) (void)hipGraphExecDestroy(G.gexec);
+    G.gexec = nullptr;
+    for (double* p : G.part) (void)hipFree(p);
+    for (double* p : G.tdgl_part) (void)hipFree(p);
+    for (double* p : G.diag_part) (void)hipFree(p);
+    G.part.clear();
+    G.tdgl_part.clear();
+    G.diag_part.clear();
+    (void)hipFree(G.dfilms);
+    (void)hipFree(G.dtiles);
+    G.dfilms = nullptr;
+    G.dtiles = nullptr;
+}
+
+void batch_free_groups(hzo_batch* B) {
+    for (hzo_batch::Group& G : B->groups) group_free(G);
+    B->groups.clear();
 }
 
 void batch_free(hzo_batch* B) {
-    batch_free_tables(B);
+    batch_free_groups(B);
     (void)hipFree(B->dez);
     (void)hipFree(B->dinfo);
     (void)hipFree(B->dm);
@@ -1223,43 +1293,47 @@ hzo::Fp32Bufs film_bufs32(hzo_ctx* x, int cur) {
     return b;
 }
 
-// Descriptor table, tile table and batch-owned partial arrays for tile depth
-// kz; P parity of film f at batch step s is (cur_f + s) & 1.
-int batch_tables(hzo_batch* B, int kz) {
-    batch_free_tables(B);
-    B->kz = kz;
-    const int n = B->n;
+// Descriptor table, tile table and group-owned partial arrays of one group
+// at tile depth kz.  P parity of a film at step s of a run is (cur + s) & 1.
+// A group of one film needs none of them: it runs the single-film kernels.
+int group_tables(hzo_batch* B, hzo_batch::Group& G, int kz) {
+    const int n = (int)G.idx.size();
+    if (n == 1) {
+        G.single = G.idx[0];
+        G.ntiles = B->films[G.single]->nb;
+        return HZO_OK;
+    }
     int maxnb = 0;
     std::vector<int> nb(n);
-    for (int f = 0; f < n; ++f) {
-        hzo::Grid g = B->films[f]->g;
+    for (int j = 0; j < n; ++j) {
+        hzo::Grid g = B->films[G.idx[j]]->g;
         g.kz = kz;
-        nb[f] = hzo::blocks_for(g);
-        maxnb = nb[f] > maxnb ? nb[f] : maxnb;
+        nb[j] = hzo::blocks_for(g);
+        maxnb = nb[j] > maxnb ? nb[j] : maxnb;
     }
-    B->pm = maxnb <= 512 ? 1 : maxnb <= 1024 ? 2 : maxnb <= 2048 ? 4 : 0;
-    B->ps = B->pm ? B->pm * 512 : maxnb;
-    B->hf.assign(n, hzo::BatchFilm());
+    G.pm = maxnb <= hzo::NT ? 1 : maxnb <= 2 * hzo::NT ? 2 : maxnb <= 4 * hzo::NT ? 4 : 0;
+    G.ps = G.pm ? G.pm * hzo::NT : maxnb;
+    G.hf.assign(n, hzo::BatchFilm());
     std::vector<int2> tiles;
-    for (int f = 0; f < n; ++f) {
-        hzo_ctx* x = B->films[f];
+    for (int j = 0; j < n; ++j) {
+        hzo_ctx* x = B->films[G.idx[j]];
         double *pa = nullptr, *tp = nullptr, *dp = nullptr;
-        const size_t pbytes = sizeof(double) * 2 * hzo::RED_MAXQ * (size_t)B->ps;
+        const size_t pbytes = sizeof(double) * 2 * hzo::RED_MAXQ * (size_t)G.ps;
         if (hipMalloc((void**)&pa, pbytes) != hipSuccess || hipMemset(pa, 0, pbytes) != hipSuccess ||
-            hipMalloc((void**)&tp, sizeof(double) * hzo::TDGL_Q * nb[f]) != hipSuccess ||
-            hipMalloc((void**)&dp, sizeof(double) * hzo::DIAG_Q * nb[f]) != hipSuccess) {
+            hipMalloc((void**)&tp, sizeof(double) * hzo::TDGL_Q * nb[j]) != hipSuccess ||
+            hipMalloc((void**)&dp, sizeof(double) * hzo::DIAG_Q * nb[j]) != hipSuccess) {
             (void)hipFree(pa);
             (void)hipFree(tp);
             return set_error(HZO_ERR_NOMEM, "hzo_batch: partial-sum allocation failed");
         }
-        B->part.push_back(pa);
-        B->tdgl_part.push_back(tp);
-        B->diag_part.push_back(dp);
-        hzo::BatchFilm& F = B->hf[f];
+        G.part.push_back(pa);
+        G.tdgl_part.push_back(tp);
+        G.diag_part.push_back(dp);
+        hzo::BatchFilm& F = G.hf[j];
         for (int q = 0; q < 2; ++q) {
             F.b[q] = film_bufs32(x, (x->cur + q) & 1);
             F.b[q].part[0] = pa;
-            F.b[q].part[1] = pa + (size_t)hzo::RED_MAXQ * B->ps;
+            F.b[q].part[1] = pa + (size_t)hzo::RED_MAXQ * G.ps;
             F.b[q].tdgl_part = tp;
         }
         F.cg.r[0] = x->r32;
@@ -1276,7 +1350,7 @@ int batch_tables(hzo_batch* B, int kz) {
         F.cg.lut = x->lut;
         F.cg.lut32 = x->lut32;
         F.cg.part[0] = pa;
-        F.cg.part[1] = pa + (size_t)hzo::RED_MAXQ * B->ps;
+        F.cg.part[1] = pa + (size_t)hzo::RED_MAXQ * G.ps;
         F.cg.st = x->st;
         F.c = x->c;
         F.c.kz = kz;
@@ -1286,26 +1360,58 @@ int batch_tables(hzo_batch* B, int kz) {
         F.gx = (x->g.nx + hzo::BX - 1) / hzo::BX;
         F.gy = (x->g.ny + hzo::BY - 1) / hzo::BY;
         F.gz = (x->g.nz + kz - 1) / kz;
-        F.nb = nb[f];
-        F.ps = B->ps;
+        F.nb = nb[j];
+        F.ps = G.ps;
         F.Ez = 0.f;
         F.diag_part = dp;
-        for (int t = 0; t < nb[f]; ++t) tiles.push_back(make_int2(f, t));
+        for (int t = 0; t < nb[j]; ++t) tiles.push_back(make_int2(j, t));
     }
-    B->ntiles = (int)tiles.size();
-    HZO_HIP(hipMalloc((void**)&B->dfilms, sizeof(hzo::BatchFilm) * n));
-    HZO_HIP(hipMalloc((void**)&B->dtiles, sizeof(int2) * tiles.size()));
-    HZO_HIP(hipMemcpy(B->dfilms, B->hf.data(), sizeof(hzo::BatchFilm) * n, hipMemcpyHostToDevice));
-    HZO_HIP(hipMemcpy(B->dtiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice));
+    G.ntiles = (int)tiles.size();
+    HZO_HIP(hipMalloc((void**)&G.dfilms, sizeof(hzo::BatchFilm) * n));
+    HZO_HIP(hipMalloc((void**)&G.dtiles, sizeof(int2) * tiles.size()));
+    HZO_HIP(hipMemcpy(G.dfilms, G.hf.data(), sizeof(hzo::BatchFilm) * n, hipMemcpyHostToDevice));
+    HZO_HIP(hipMemcpy(G.dtiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice));
     return HZO_OK;
 }
 
-// Re-reads each film's P parity into the descriptor table (contexts may have
-// been stepped on their own since the last batched run).
-int batch_sync_parity(hzo_batch* B) {
-    for (int f = 0; f < B->n; ++f) {
-        hzo_ctx* x = B->films[f];
-        hzo::BatchFilm& F = B->hf[f];
+// Packs the films into groups by first-fit decreasing on the CG working set
+// (a film larger than the budget forms a group of its own) and builds each
+// group's tables at tile depth kz.
+int batch_tables(hzo_batch* B, int kz, double budget) {
+    batch_free_groups(B);
+    B->kz = kz;
+    B->budget = budget;
+    std::vector<int> order(B->n);
+    for (int f = 0; f < B->n; ++f) order[f] = f;
+    auto bytes = [&](int f) { return CG_SET_BYTES * (double)B->films[f]->g.N; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return bytes(a) > bytes(b); });
+    std::vector<double> load;
+    for (int f : order) {
+        size_t g = 0;
+        if (budget > 0.0)
+            while (g < load.size() && load[g] + bytes(f) > budget) ++g;
+        if (g == load.size()) {
+            load.push_back(0.0);
+            B->groups.emplace_back();
+        }
+        load[g] += bytes(f);
+        B->groups[g].idx.push_back(f);
+    }
+    for (hzo_batch::Group& G : B->groups) {
+        std::sort(G.idx.begin(), G.idx.end());
+        int rc = group_tables(B, G, kz);
+        if (rc) return rc;
+    }
+    return HZO_OK;
+}
+
+// Re-reads each film's P parity into the group's descriptor table (contexts
+// may have been stepped on their own since the last batched run).
+int group_sync_parity(hzo_batch* B, hzo_batch::Group& G) {
+    if (G.single >= 0) return HZO_OK;
+    for (size_t j = 0; j < G.idx.size(); ++j) {
+        hzo_ctx* x = B->films[G.idx[j]];
+        hzo::BatchFilm& F = G.hf[j];
         for (int q = 0; q < 2; ++q) {
             const hzo::Fp32Bufs b = film_bufs32(x, (x->cur + q) & 1);
             F.b[q].Pold = b.Pold;
@@ -1313,73 +1419,163 @@ int batch_sync_parity(hzo_batch* B) {
             F.b[q].Pcur = b.Pcur;
         }
     }
-    HZO_HIP(hipMemcpyAsync(B->dfilms, B->hf.data(), sizeof(hzo::BatchFilm) * B->n, hipMemcpyHostToDevice,
-                           B->stream));
+    HZO_HIP(hipMemcpy(G.dfilms, G.hf.data(), sizeof(hzo::BatchFilm) * G.idx.size(), hipMemcpyHostToDevice));
     return HZO_OK;
 }
 
-int batch_cg_chunk(hzo_batch* B) {
-    const int K = B->chunk;
-    if (B->graphs) {
-        if (!B->gexec || B->g_chunk != K) {
-            if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
-            B->gexec = nullptr;
-            hipGraph_t g;
-            HZO_HIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
-            for (int t = 1; t <= K; ++t)
-                hzo::launch_cg32_batch(B->stream, B->ntiles, B->kz, B->pm, B->dfilms, B->dtiles, t);
-            HZO_HIP(hipStreamEndCapture(B->stream, &g));
-            const hipError_t e = hipGraphInstantiate(&B->gexec, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            HZO_HIP(e);
-            B->g_chunk = K;
-        }
-        HZO_HIP(hipGraphLaunch(B->gexec, B->stream));
+// K launches of the single-reduction CG for the group (launch index 1..K).
+void group_cg_launches(hzo_batch* B, hzo_batch::Group& G, int K) {
+    if (G.single >= 0) {
+        hzo_ctx* x = B->films[G.single];
+        const hzo::CG32Bufs cb = cg_bufs(x, bufs32(x));
+        for (int t = 1; t <= K; ++t) hzo::launch_cg32(B->stream, x->g, x->c, cb, t);
     } else {
         for (int t = 1; t <= K; ++t)
-            hzo::launch_cg32_batch(B->stream, B->ntiles, B->kz, B->pm, B->dfilms, B->dtiles, t);
+            hzo::launch_cg32_batch(B->stream, G.ntiles, B->kz, G.pm, G.dfilms, G.dtiles, t);
+    }
+}
+
+int group_cg_chunk(hzo_batch* B, hzo_batch::Group& G) {
+    const int K = B->chunk;
+    if (B->graphs) {
+        if (!G.gexec || G.g_chunk != K) {
+            if (G.gexec) (void)hipGraphExecDestroy(G.gexec);
+            G.gexec = nullptr;
+            hipGraph_t g;
+            HZO_HIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+            group_cg_launches(B, G, K);
+            HZO_HIP(hipStreamEndCapture(B->stream, &g));
+            const hipError_t e = hipGraphInstantiate(&G.gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HZO_HIP(e);
+            G.g_chunk = K;
+        }
+        HZO_HIP(hipGraphLaunch(G.gexec, B->stream));
+    } else {
+        group_cg_launches(B, G, K);
         HZO_HIP(hipGetLastError());
     }
     return HZO_OK;
 }
 
-// One batched step: TDGL of every film, PCG setup, then chunks of
-// single-reduction CG launches until every film's solve has finished.
-// Launch index t restarts at 1 in each chunk (chunks are even, so the
-// control-block parity continues).  Returns the CG launches used.
-int batch_step(hzo_batch* B, const double* ez, int s, bool dep, int& launches) {
-    HZO_HIP(hipStreamSynchronize(B->stream));  // the previous step's copy out of hez has run
-    memcpy(B->hez, ez, sizeof(double) * B->n);
-    HZO_HIP(hipMemcpyAsync(B->dez, B->hez, sizeof(double) * B->n, hipMemcpyHostToDevice, B->stream));
-    hzo::launch_batch_set_ez(B->stream, B->dfilms, B->dez, B->n);
-    hzo::launch_tdgl_f32_batch(B->stream, B->ntiles, B->dfilms, B->dtiles, s & 1);
-    HZO_HIP(hipGetLastError());
-    launches = 0;
-    if (!dep) {
-        memset(B->hinfo, 0, sizeof(int) * 4 * B->n);
+// Solver statistics of the group's films after a chunk (launch index K) into
+// hinfo[4j..]: done, iterations, status, residual replacements.
+int group_poll(hzo_batch* B, hzo_batch::Group& G) {
+    const int n = (int)G.idx.size();
+    if (G.single >= 0) {
+        hzo_ctx* x = B->films[G.single];
+        HZO_HIP(hipMemcpyAsync(x->st_host, x->st + (B->chunk & 1), sizeof(hzo::PcgState), hipMemcpyDeviceToHost,
+                               B->stream));
+        HZO_HIP(hipStreamSynchronize(B->stream));
+        B->hinfo[0] = x->st_host->done;
+        B->hinfo[1] = x->st_host->iters;
+        B->hinfo[2] = x->st_host->status;
+        B->hinfo[3] = x->st_host->refinements;
         return HZO_OK;
     }
-    hzo::launch_setup_f32_batch(B->stream, B->ntiles, B->dfilms, B->dtiles, (s + 1) & 1);
+    hzo::launch_batch_done(B->stream, G.dfilms, n, B->chunk, B->dinfo);
+    HZO_HIP(hipMemcpyAsync(B->hinfo, B->dinfo, sizeof(int) * 4 * n, hipMemcpyDeviceToHost, B->stream));
+    HZO_HIP(hipStreamSynchronize(B->stream));
+    return HZO_OK;
+}
+
+// One step of one group: TDGL of every film, PCG setup, then chunks of
+// single-reduction CG launches until every film's solve has finished.  The
+// launch index t restarts at 1 in each chunk (chunks are even, so the
+// control-block parity continues).  hinfo[4j..] receives film j's solver
+// statistics.
+int group_step(hzo_batch* B, hzo_batch::Group& G, const double* Ez, int s, bool dep, int& launches) {
+    const int n = (int)G.idx.size();
+    if (G.single >= 0) {
+        hzo_ctx* x = B->films[G.single];
+        hzo::launch_tdgl_f32(B->stream, x->g, x->c, Ez[(size_t)s * B->n + G.single], bufs32(x));
+        HZO_HIP(hipGetLastError());
+        x->cur ^= 1;
+    } else {
+        HZO_HIP(hipStreamSynchronize(B->stream));  // the previous step's copy out of hez has run
+        for (int j = 0; j < n; ++j) B->hez[j] = Ez[(size_t)s * B->n + G.idx[j]];
+        HZO_HIP(hipMemcpyAsync(B->dez, B->hez, sizeof(double) * n, hipMemcpyHostToDevice, B->stream));
+        hzo::launch_batch_set_ez(B->stream, G.dfilms, B->dez, n);
+        hzo::launch_tdgl_f32_batch(B->stream, G.ntiles, G.dfilms, G.dtiles, s & 1);
+        HZO_HIP(hipGetLastError());
+    }
+    launches = 0;
+    if (!dep) {
+        memset(B->hinfo, 0, sizeof(int) * 4 * n);
+        return HZO_OK;
+    }
+    if (G.single >= 0) {
+        hzo_ctx* x = B->films[G.single];
+        hzo::launch_pcg_setup_f32(B->stream, x->g, x->c, bufs32(x), pcg_init(x), nullptr);
+    } else {
+        hzo::launch_setup_f32_batch(B->stream, G.ntiles, G.dfilms, G.dtiles, (s + 1) & 1);
+    }
     HZO_HIP(hipGetLastError());
     // the most launches any film may need: three per iteration
     int maxl = 0;
-    for (hzo_ctx* x : B->films) maxl = 3 * x->cfg.max_iter + 8 > maxl ? 3 * x->cfg.max_iter + 8 : maxl;
-    int nchunks = (B->last_launches + B->chunk - 1) / B->chunk;
+    for (int f : G.idx) {
+        const int m = 3 * B->films[f]->cfg.max_iter + 8;
+        maxl = m > maxl ? m : maxl;
+    }
+    int nchunks = (G.need + B->chunk - 1) / B->chunk;
     while (true) {
         for (int q = 0; q < nchunks; ++q) {
-            int rc = batch_cg_chunk(B);
+            int rc = group_cg_chunk(B, G);
             if (rc) return rc;
             launches += B->chunk;
         }
-        hzo::launch_batch_done(B->stream, B->dfilms, B->n, B->chunk, B->dinfo);
-        HZO_HIP(hipMemcpyAsync(B->hinfo, B->dinfo, sizeof(int) * 4 * B->n, hipMemcpyDeviceToHost, B->stream));
-        HZO_HIP(hipStreamSynchronize(B->stream));
+        int rc = group_poll(B, G);
+        if (rc) return rc;
         bool all = true;
-        for (int f = 0; f < B->n; ++f) all = all && B->hinfo[4 * f];
+        for (int j = 0; j < n; ++j) all = all && B->hinfo[4 * j];
         if (all || launches >= maxl) break;
         nchunks = 1;
     }
-    B->last_launches = launches;
+    // launches the slowest film needed: setup hand-over, one per iteration,
+    // two per residual replacement
+    int need = 2;
+    for (int j = 0; j < n; ++j) {
+        const int m = B->hinfo[4 * j + 1] + 2 * B->hinfo[4 * j + 3] + 2;
+        need = m > need ? m : need;
+    }
+    G.need = need;
+    return HZO_OK;
+}
+
+// nsteps steps of one group; leaves each film's P parity advanced by nsteps.
+int group_run(hzo_batch* B, hzo_batch::Group& G, const double* Ez, int nsteps, int* iters, double* m_out) {
+    int rc = group_sync_parity(B, G);
+    if (rc) return rc;
+    const int n = (int)G.idx.size();
+    const bool dep = B->films[0]->cfg.dep_flag != 0;
+    for (int s = 0; s < nsteps; ++s) {
+        int l = 0;
+        rc = group_step(B, G, Ez, s, dep, l);
+        if (rc) return rc;
+        B->launches_total += l;
+        for (int j = 0; j < n; ++j) {
+            if (iters) iters[(size_t)s * B->n + G.idx[j]] = B->hinfo[4 * j + 1];
+            B->unconverged += dep && !B->hinfo[4 * j];
+        }
+        if (m_out) {
+            if (G.single >= 0) {
+                hzo_ctx* x = B->films[G.single];
+                hzo::launch_diag_f32(B->stream, x->g, x->P32[x->cur], x->cw, x->diag_part);
+                hzo::launch_sum_column(B->stream, x->g, x->diag_part, B->dm);
+            } else {
+                hzo::launch_diag_f32_batch(B->stream, G.ntiles, G.dfilms, G.dtiles, (s + 1) & 1);
+                hzo::launch_batch_sum_pmz(B->stream, G.dfilms, n, B->dm);
+            }
+            HZO_HIP(hipGetLastError());
+            HZO_HIP(hipMemcpyAsync(B->hm, B->dm, sizeof(double) * n, hipMemcpyDeviceToHost, B->stream));
+            HZO_HIP(hipStreamSynchronize(B->stream));
+            for (int j = 0; j < n; ++j)
+                m_out[(size_t)s * B->n + G.idx[j]] = B->hm[j] / (double)B->films[G.idx[j]]->g.N;
+        }
+    }
+    HZO_HIP(hipStreamSynchronize(B->stream));
+    if (G.single < 0)
+        for (int f : G.idx) B->films[f]->cur = (B->films[f]->cur + nsteps) & 1;
     return HZO_OK;
 }
 
@@ -1396,6 +1592,8 @@ int hzo_batch_create(hzo_ctx* const* films, int n, int tile_kz, hzo_batch** out)
             return set_error(HZO_ERR_ARG, "hzo_batch_create: films must be FP32-refined with the single-reduction CG");
         if (x->device != films[0]->device || x->cfg.dep_flag != films[0]->cfg.dep_flag)
             return set_error(HZO_ERR_ARG, "hzo_batch_create: films must share the device and dep_flag");
+        for (int e = 0; e < f; ++e)
+            if (films[e] == x) return set_error(HZO_ERR_ARG, "hzo_batch_create: film %d appears twice", f);
     }
     hzo_batch* B = new hzo_batch();
     B->films.assign(films, films + n);
@@ -1418,7 +1616,7 @@ int hzo_batch_create(hzo_ctx* const* films, int n, int tile_kz, hzo_batch** out)
     for (hzo_ctx* x : B->films)
         if (hipStreamSynchronize(x->stream) != hipSuccess)
             return fail(set_error(HZO_ERR_HIP, "hzo_batch_create: film stream synchronisation failed"));
-    int rc = batch_tables(B, tile_kz);
+    int rc = batch_tables(B, tile_kz, 0.0);
     if (rc) return fail(rc);
     *out = B;
     return HZO_OK;
@@ -1441,44 +1639,44 @@ int hzo_batch_config(hzo_batch* B, int use_graphs, int chunk) {
 
 int hzo_batch_tile_depth(hzo_batch* B) { return B ? B->kz : 0; }
 
+int hzo_batch_set_shape(hzo_batch* B, int tile_kz, double group_mib) {
+    if (!B || (tile_kz != 2 && tile_kz != 4) || !(group_mib >= 0.0))
+        return set_error(HZO_ERR_ARG, "hzo_batch_set_shape: tile_kz must be 2 or 4, group_mib >= 0");
+    HZO_HIP(hipSetDevice(B->device));
+    HZO_HIP(hipStreamSynchronize(B->stream));
+    return batch_tables(B, tile_kz, group_mib * 1048576.0);
+}
+
+int hzo_batch_shape(const hzo_batch* B, int out[3]) {
+    if (!B || !out) return set_error(HZO_ERR_ARG, "hzo_batch_shape: null argument");
+    out[0] = B->kz;
+    out[1] = (int)(B->budget / 1048576.0 + 0.5);
+    out[2] = (int)B->groups.size();
+    return HZO_OK;
+}
+
 int hzo_batch_run(hzo_batch* B, const double* Ez, int nsteps, int* iters, double* m_out) {
     if (!B || nsteps < 0 || (nsteps > 0 && !Ez)) return set_error(HZO_ERR_ARG, "hzo_batch_run: bad argument");
     HZO_HIP(hipSetDevice(B->device));
     for (hzo_ctx* x : B->films) HZO_HIP(hipStreamSynchronize(x->stream));
-    int rc = batch_sync_parity(B);
-    if (rc) return rc;
-    const bool dep = B->films[0]->cfg.dep_flag != 0;
-    for (int s = 0; s < nsteps; ++s) {
-        int l = 0;
-        rc = batch_step(B, Ez + (size_t)s * B->n, s, dep, l);
+    for (hzo_batch::Group& G : B->groups) {
+        int rc = group_run(B, G, Ez, nsteps, iters, m_out);
         if (rc) return rc;
-        if (iters)
-            for (int f = 0; f < B->n; ++f) iters[(size_t)s * B->n + f] = B->hinfo[4 * f + 1];
-        B->launches_total += l;
-        for (int f = 0; f < B->n; ++f) B->unconverged += dep && !B->hinfo[4 * f];
-        if (m_out) {
-            hzo::launch_diag_f32_batch(B->stream, B->ntiles, B->dfilms, B->dtiles, (s + 1) & 1);
-            hzo::launch_batch_sum_pmz(B->stream, B->dfilms, B->n, B->dm);
-            HZO_HIP(hipGetLastError());
-            HZO_HIP(hipMemcpyAsync(B->hm, B->dm, sizeof(double) * B->n, hipMemcpyDeviceToHost, B->stream));
-            HZO_HIP(hipStreamSynchronize(B->stream));
-            for (int f = 0; f < B->n; ++f) m_out[(size_t)s * B->n + f] = B->hm[f] / (double)B->films[f]->g.N;
-        }
     }
-    HZO_HIP(hipStreamSynchronize(B->stream));
-    for (hzo_ctx* x : B->films) x->cur = (x->cur + nsteps) & 1;
     return HZO_OK;
 }
 
-int hzo_batch_autotune(hzo_batch* B, const double* Ez, int reps, int* kz_out, double ms[2]) {
+int hzo_batch_autotune(hzo_batch* B, const double* Ez, int reps, int* kz_out, double ms[8]) {
     if (!B || !Ez || reps < 1) return set_error(HZO_ERR_ARG, "hzo_batch_autotune: bad argument");
     HZO_HIP(hipSetDevice(B->device));
     for (hzo_ctx* x : B->films) HZO_HIP(hipStreamSynchronize(x->stream));
     // A trial step reads P[cur] and writes only P[cur^1], the potential and
-    // the solver's work vectors, so restoring the potential (x_hi, x_lo)
-    // returns every film to its state before the trial.
+    // the solver's work vectors, so restoring the potential (x_hi, x_lo) and
+    // the parity returns every film to its state before the trial.
     std::vector<double*> hi(B->n, nullptr);
     std::vector<float*> lo(B->n, nullptr);
+    std::vector<int> cur0(B->n);
+    for (int f = 0; f < B->n; ++f) cur0[f] = B->films[f]->cur;
     auto release = [&]() {
         for (int f = 0; f < B->n; ++f) {
             (void)hipFree(hi[f]);
@@ -1497,6 +1695,7 @@ int hzo_batch_autotune(hzo_batch* B, const double* Ez, int reps, int* kz_out, do
         for (int f = 0; f < B->n; ++f) {
             hzo_ctx* x = B->films[f];
             const size_t N = (size_t)x->g.N;
+            x->cur = cur0[f];
             HZO_HIP(hipMemcpyAsync(save ? hi[f] : x->phi, save ? x->phi : hi[f], sizeof(double) * N,
                                    hipMemcpyDeviceToDevice, B->stream));
             HZO_HIP(hipMemcpyAsync(save ? lo[f] : x->xlo, save ? x->xlo : lo[f], sizeof(float) * N,
@@ -1505,36 +1704,35 @@ int hzo_batch_autotune(hzo_batch* B, const double* Ez, int reps, int* kz_out, do
         HZO_HIP(hipStreamSynchronize(B->stream));
         return HZO_OK;
     };
-    const int cand[2] = {2, 4};
-    double best_ms[2] = {1e300, 1e300};
-    const bool dep = B->films[0]->cfg.dep_flag != 0;
+    // candidates: tile depth x group budget (MiB of CG working set; 0 = all
+    // films in one group)
+    const int kzs[2] = {2, 4};
+    const double mib[4] = {0.0, 32.0, 64.0, 128.0};
+    double best_ms[8];
+    for (double& v : best_ms) v = 1e300;
     int rc = copy(true);
-    for (int w = 0; w < 2 && rc == HZO_OK; ++w) {
-        if ((rc = batch_tables(B, cand[w])) != HZO_OK) break;
-        if ((rc = batch_sync_parity(B)) != HZO_OK) break;
-        for (int r = 0; r <= reps && rc == HZO_OK; ++r) {  // r = 0 warms up (code objects, graph)
+    for (int c = 0; c < 8 && rc == HZO_OK; ++c) {
+        if ((rc = batch_tables(B, kzs[c / 4], mib[c % 4] * 1048576.0)) != HZO_OK) break;
+        for (int r = 0; r <= reps && rc == HZO_OK; ++r) {  // r = 0 warms up (code objects, graphs)
             if ((rc = copy(false)) != HZO_OK) break;
             const auto t0 = std::chrono::steady_clock::now();
-            int l = 0;
-            if ((rc = batch_step(B, Ez, 0, dep, l)) != HZO_OK) break;
-            if (hipStreamSynchronize(B->stream) != hipSuccess) {
-                rc = set_error(HZO_ERR_HIP, "hzo_batch_autotune: synchronisation failed");
-                break;
-            }
+            for (hzo_batch::Group& G : B->groups)
+                if ((rc = group_run(B, G, Ez, 1, nullptr, nullptr)) != HZO_OK) break;
+            if (rc) break;
             const double t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-            if (r > 0 && t < best_ms[w]) best_ms[w] = t;
+            if (r > 0 && t < best_ms[c]) best_ms[c] = t;
         }
     }
     if (rc == HZO_OK) rc = copy(false);
     release();
     if (rc) return rc;
-    const int best = best_ms[1] < best_ms[0] ? 1 : 0;
-    if ((rc = batch_tables(B, cand[best])) != HZO_OK) return rc;
-    if (kz_out) *kz_out = cand[best];
-    if (ms) {
-        ms[0] = best_ms[0];
-        ms[1] = best_ms[1];
-    }
+    int best = 0;
+    for (int c = 1; c < 8; ++c)
+        if (best_ms[c] < best_ms[best]) best = c;
+    if ((rc = batch_tables(B, kzs[best / 4], mib[best % 4] * 1048576.0)) != HZO_OK) return rc;
+    if (kz_out) *kz_out = kzs[best / 4];
+    if (ms)
+        for (int c = 0; c < 8; ++c) ms[c] = best_ms[c];
     return HZO_OK;
 }
 

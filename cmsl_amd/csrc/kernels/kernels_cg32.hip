@@ -367,11 +367,14 @@ __global__ __launch_bounds__(NT, KZ >= 16 ? 2 : 4) void k_cg32(CG32Bufs b, StepC
 // (film, logical tile); every film runs its own control block, and the tiles
 // of a film whose solve has finished return at once.
 template <int KZ, int PM>
-__global__ __launch_bounds__(NT, KZ >= 16 ? 2 : 4) void k_cg32_batch(const BatchFilm* films, const int2* tiles,
+__global__ __launch_bounds__(NT, KZ >= 16 ? 2 : 4) void k_cg32_batch(const BatchFilm* __restrict__ films, const int2* __restrict__ tiles,
                                                                       int t) {
     TileId T;
     const BatchFilm& F = films[batch_tile(tiles, films, T)];
-    cg32_body<KZ, PM>(F.cg, F.c, F.nb, F.ps, t, T);
+    // descriptor fields read once, before any store (scalar loads)
+    const CG32Bufs b = F.cg;
+    const StepConsts c = F.c;
+    cg32_body<KZ, PM>(b, c, F.nb, F.ps, t, T);
 }
 
 void launch_cg32_batch(hipStream_t s, int ntiles, int kz, int pm, const BatchFilm* films, const int2* tiles,

@@ -118,7 +118,8 @@ __global__ __launch_bounds__(NT) void k_tdgl_f32(Fp32Bufs b, StepConsts c, float
     tdgl_f32_body(b, c, Ez_ext, nb, tile_id());
 }
 
-__global__ __launch_bounds__(NT) void k_tdgl_f32_batch(const BatchFilm* films, const int2* tiles, int parity) {
+__global__ __launch_bounds__(NT) void k_tdgl_f32_batch(const BatchFilm* __restrict__ films, const int2* __restrict__ tiles,
+                                                       int parity) {
     TileId T;
     const BatchFilm& F = films[batch_tile(tiles, films, T)];
     tdgl_f32_body(F.b[parity], F.c, F.Ez, F.nb, T);
@@ -212,7 +213,8 @@ __global__ __launch_bounds__(NT) void k_pcg_setup_f32(Fp32Bufs b, StepConsts c, 
     setup_f32_body(b, c, ps, init, rhs, tile_id());
 }
 
-__global__ __launch_bounds__(NT) void k_pcg_setup_f32_batch(const BatchFilm* films, const int2* tiles, int parity) {
+__global__ __launch_bounds__(NT) void k_pcg_setup_f32_batch(const BatchFilm* __restrict__ films, const int2* __restrict__ tiles,
+                                                            int parity) {
     TileId T;
     const BatchFilm& F = films[batch_tile(tiles, films, T)];
     setup_f32_body(F.b[parity], F.c, F.ps, F.init, nullptr, T);
@@ -584,7 +586,8 @@ __global__ __launch_bounds__(NT) void k_diag_f32(const float* __restrict__ P, co
     diag_f32_body(P, cw, g, part, nb, tile_id());
 }
 
-__global__ __launch_bounds__(NT) void k_diag_f32_batch(const BatchFilm* films, const int2* tiles, int parity) {
+__global__ __launch_bounds__(NT) void k_diag_f32_batch(const BatchFilm* __restrict__ films, const int2* __restrict__ tiles,
+                                                       int parity) {
     TileId T;
     const BatchFilm& F = films[batch_tile(tiles, films, T)];
     diag_f32_body(F.b[parity].Pcur, F.b[parity].cw, F.g, F.diag_part, F.nb, T);

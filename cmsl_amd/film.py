@@ -178,6 +178,50 @@ def sweep_schedule(p: L.SweepParams, max_rows=100000):
     return E[: n.value], cnt[: n.value]
 
 
+def write_vector_field(path, vx, vy, vz, scale=1.0):
+    """write_vector_field_binary (generate.cu:11-30)."""
+    vx, vy, vz = (np.ascontiguousarray(v, np.float64) for v in (vx, vy, vz))
+    L.call("hzo_write_vector_field", str(path).encode(), vx.size, L.ptr(vx), L.ptr(vy), L.ptr(vz), float(scale))
+
+
+def write_grain_map(path, nx, ny, nz, grain_map, scale=1.0):
+    """write_field (generate.cu:33-65): text grain map."""
+    g = np.ascontiguousarray(grain_map, np.int32)
+    L.call("hzo_write_grain_map", str(path).encode(), nx, ny, nz, L.ptr(g), float(scale))
+
+
+def write_midplane_slice(path, nx, ny, nz, v):
+    """write_midplane_slice (generate.cu:67-88)."""
+    v = np.ascontiguousarray(v, np.float64)
+    L.call("hzo_write_midplane_slice", str(path).encode(), nx, ny, nz, L.ptr(v))
+
+
+def write_checkpoint(path, nx, ny, nz, P, E, peloop=0, count=0, Ez_ext=0.0, incEz=0.0):
+    """Reference checkpoint file (generate.cu:90-168); P[6N], E[3N] SoA."""
+    P = np.ascontiguousarray(P, np.float64)
+    E = np.ascontiguousarray(E, np.float64)
+    L.call("hzo_write_checkpoint", str(path).encode(), nx, ny, nz, int(peloop), int(count), float(Ez_ext),
+           float(incEz), L.ptr(P), L.ptr(E))
+
+
+def read_checkpoint(path, nx, ny, nz):
+    """Reads a reference checkpoint (generate.cu:170-264): header dict, P[6N], E[3N]."""
+    N = nx * ny * nz
+    P = np.zeros(6 * N)
+    E = np.zeros(3 * N)
+    pe, cnt = C.c_int(0), C.c_int(0)
+    ez, inc = C.c_double(0.0), C.c_double(0.0)
+    L.call("hzo_read_checkpoint", str(path).encode(), nx, ny, nz, C.byref(pe), C.byref(cnt), C.byref(ez),
+           C.byref(inc), L.ptr(P), L.ptr(E))
+    return {"peloop": pe.value, "count": cnt.value, "Ez_ext": ez.value, "incEz": inc.value}, P, E
+
+
+def write_raw_f64(path, v):
+    """Raw f64 dump in voxel order (used for the potential)."""
+    v = np.ascontiguousarray(v, np.float64)
+    L.call("hzo_write_raw_f64", str(path).encode(), v.size, L.ptr(v))
+
+
 class Film:
     """One film on one GPU: the device context of libhzo.so."""
 
@@ -288,6 +332,26 @@ class Film:
                L.ptr(rows), C.byref(n))
         return rows[: min(n.value, max_rows)]
 
+    # ---------------------------------------------------------- dumps and restart
+    def save_snapshot(self, pm_path, ps_path, P0_scale: float):
+        """Pm and Ps as interleaved xyz f64 dumps scaled by P0 (generate.cu:266-303)."""
+        L.call("hzo_save_snapshot", self.h, str(pm_path).encode(), str(ps_path).encode(), float(P0_scale))
+
+    def checkpoint_save(self, path, peloop=0, count=0, Ez_ext=0.0, incEz=0.0):
+        """Reference checkpoint file of the current P and E (generate.cu:90-168)."""
+        L.call("hzo_checkpoint_save", self.h, str(path).encode(), int(peloop), int(count), float(Ez_ext),
+               float(incEz))
+
+    def checkpoint_load(self, path):
+        """Restart from a reference checkpoint: P from the file, phi re-solved.
+        Returns the header (peloop, count, Ez_ext, incEz) and the stored E."""
+        pe, cnt = C.c_int(0), C.c_int(0)
+        ez, inc = C.c_double(0.0), C.c_double(0.0)
+        E = np.zeros(3 * self.inp.N)
+        L.call("hzo_checkpoint_load", self.h, str(path).encode(), C.byref(pe), C.byref(cnt), C.byref(ez),
+               C.byref(inc), L.ptr(E))
+        return {"peloop": pe.value, "count": cnt.value, "Ez_ext": ez.value, "incEz": inc.value}, E
+
     def synchronize(self):
         L.call("hzo_synchronize", self.h)
 
@@ -345,14 +409,25 @@ class Batch:
         L.call("hzo_batch_run", self.h, L.ptr(Ez), S, L.ptr(its), L.ptr(m))
         return (its, m) if want_m else its
 
+    def set_shape(self, tile_kz: int, group_mib: float = 0.0):
+        """Tile depth (2 or 4) and group budget in MiB of CG working set (0: one group)."""
+        L.call("hzo_batch_set_shape", self.h, int(tile_kz), float(group_mib))
+
+    @property
+    def shape(self):
+        out = np.zeros(3, np.int32)
+        L.call("hzo_batch_shape", self.h, L.ptr(out))
+        return {"tile_kz": int(out[0]), "group_mib": int(out[1]), "groups": int(out[2])}
+
     def autotune(self, Ez0, reps: int = 2):
-        """Times one step at tile depth 2 and 4 (fields Ez0[f]) and keeps the
-        faster; returns (kz, {2: ms, 4: ms})."""
+        """Times one step per shape (tile depth 2/4 x group budget 0/32/64/128
+        MiB, fields Ez0[f]) and keeps the fastest; returns (shape, {(kz, mib): ms})."""
         Ez0 = np.ascontiguousarray(np.broadcast_to(np.asarray(Ez0, np.float64), (self.n,)))
         kz = C.c_int(0)
-        ms = np.zeros(2)
+        ms = np.zeros(8)
         L.call("hzo_batch_autotune", self.h, L.ptr(Ez0), int(reps), C.byref(kz), L.ptr(ms))
-        return kz.value, {2: float(ms[0]), 4: float(ms[1])}
+        times = {((2, 4)[c // 4], (0, 32, 64, 128)[c % 4]): float(ms[c]) for c in range(8)}
+        return self.shape, times
 
     def counters(self):
         out = np.zeros(2, np.int64)

@@ -208,13 +208,30 @@ int hzo_forward_record(hzo_ctx* ctx, const double* Ez, int nsteps, double* m_out
  * the context at the recorded final state. */
 int hzo_backward(hzo_ctx* ctx, const double* dL_dm, double grad[12], long long* adj_pcg_iters);
 
+/* ---------------------------------------------------------------- dumps and restart */
+/* save_snapshot (generate.cu:266-303): Pm and Ps of the current state as
+ * interleaved (x, y, z) f64 per voxel, multiplied by P0_scale. */
+int hzo_save_snapshot(hzo_ctx* ctx, const char* pm_path, const char* ps_path, double P0_scale);
+/* write_checkpoint (generate.cu:90-168): header nx, ny, nz, peloop, count
+ * (int32), Ez_ext, incEz (f64), then Pmx..Psz and Ex, Ey, Ez of the current
+ * state (N f64 each, non-dimensional). */
+int hzo_checkpoint_save(hzo_ctx* ctx, const char* path, int peloop, int count, double Ez_ext, double incEz);
+/* read_checkpoint (generate.cu:170-264) as a restart: sets P from the file
+ * and re-solves phi from it (the file holds E, not phi).  Header values and
+ * E_file[3N] (the stored field) may be NULL. */
+int hzo_checkpoint_load(hzo_ctx* ctx, const char* path, int* peloop, int* count, double* Ez_ext, double* incEz,
+                        double* E_file);
+
 /* ---------------------------------------------------------------- batched films
  * Many films of different sizes and coefficients stepped together: each
  * kernel of the FP32-refined step (TDGL, PCG setup, single-reduction CG
- * iteration, diagnostics) runs once for the whole batch over a tile table
+ * iteration, diagnostics) runs once per group of films over a tile table
  * that maps every workgroup to (film, tile); each film keeps its own solver
  * control block, and the CG launches are replayed from a captured hipGraph
- * in chunks until every film's solve has finished.  No counterpart in the
+ * in chunks until every film of the group has converged.  Films are packed
+ * into groups whose CG working set fits a cache budget, and each group runs
+ * all steps of a call before the next group starts; a group of one film runs
+ * that film's own single-film kernels at its own tile depth.  No counterpart in the
  * reference, which steps one film per process (evolve.cu:220-237 in a loop
  * over decks); it serves parameter sweeps over many microstructures.
  *
@@ -222,24 +239,30 @@ int hzo_backward(hzo_ctx* ctx, const double* dL_dm, double grad[12], long long* 
  * one device, with the same dep_flag, and ready (microstructure, phase table
  * and state set).  A batch holds the films' device buffers by reference: the
  * films must outlive it and must not be stepped concurrently with it.  Each
- * film ends a batched run in exactly the state its own hzo_run would leave
- * at the same tile depth. */
+ * film ends a batched run in exactly the state its own hzo_run leaves at the
+ * same tile depth. */
 typedef struct hzo_batch hzo_batch;
-/* tile_kz: 2 or 4 (planes per workgroup tile; hzo_batch_autotune picks it). */
+/* tile_kz: 2 or 4 (planes per workgroup tile); all films in one group. */
 int hzo_batch_create(hzo_ctx* const* films, int n, int tile_kz, hzo_batch** out);
 void hzo_batch_destroy(hzo_batch* batch);
 /* use_graphs: replay CG launches from a captured hipGraph (default 1);
  * chunk: CG launches per replay, even, >= 2 (default 16). */
 int hzo_batch_config(hzo_batch* batch, int use_graphs, int chunk);
+/* Tile depth (2 or 4) of the groups of several films, and group budget in
+ * MiB of CG working set (34 bytes per voxel; 0: one group of all films). */
+int hzo_batch_set_shape(hzo_batch* batch, int tile_kz, double group_mib);
+/* out[3] = tile depth, group budget (MiB), number of groups. */
+int hzo_batch_shape(const hzo_batch* batch, int out[3]);
 int hzo_batch_tile_depth(hzo_batch* batch);
 /* nsteps steps; Ez[s*n + f] is the applied field of film f at step s.
  * iters[s*n + f] (may be NULL): PCG iterations of film f at step s.
  * m_out[s*n + f] (may be NULL): <Pmz> of film f after step s. */
 int hzo_batch_run(hzo_batch* batch, const double* Ez, int nsteps, int* iters, double* m_out);
-/* Times one step at tile depth 2 and at 4 (best of reps after a warm-up step,
- * applied fields Ez[f]), restores the films' states and keeps the faster
- * depth.  kz_out and ms[2] (times at depth 2 and 4) may be NULL. */
-int hzo_batch_autotune(hzo_batch* batch, const double* Ez, int reps, int* kz_out, double ms[2]);
+/* Times one step (applied fields Ez[f], best of reps after a warm-up step)
+ * for each shape c = 4*k + g: tile depth {2, 4}[k] x group budget
+ * {0, 32, 64, 128}[g] MiB, restores the films' states after every trial and
+ * keeps the fastest shape.  kz_out and ms[8] (per-shape times) may be NULL. */
+int hzo_batch_autotune(hzo_batch* batch, const double* Ez, int reps, int* kz_out, double ms[8]);
 /* out[0]: CG launches issued by all batched runs; out[1]: film solves that
  * stopped at the launch cap without converging. */
 int hzo_batch_counters(const hzo_batch* batch, long long out[2]);
@@ -268,6 +291,25 @@ int hzo_read_gdata(const char* path, int nx, int ny, int nz, int32_t* omega, dou
 /* "gchar_<tag>.txt": gid cx cy cz phi1 Phi phi2 (radians) per line
  * (grain_io.cu:85-99).  centroids[3*ng], euler[3*ng]; missing ids keep -1/0. */
 int hzo_read_gchar(const char* path, int ng, double* centroids, double* euler);
+/* Dump formats (generate.cu:11-88), host arrays in voxel order. */
+/* write_vector_field_binary: (vx, vy, vz) * scale interleaved, f64. */
+int hzo_write_vector_field(const char* path, long long n, const double* vx, const double* vy, const double* vz,
+                           double scale);
+/* write_field: text rows "i\tj\tk\tv" (v = (int)(map * scale)), i outer, k
+ * inner, a blank line after each j and each i. */
+int hzo_write_grain_map(const char* path, int nx, int ny, int nz, const int32_t* map, double scale);
+/* write_midplane_slice: text rows "i j %.16e" of plane nz/2, j outer, a blank
+ * line after each row. */
+int hzo_write_midplane_slice(const char* path, int nx, int ny, int nz, const double* v);
+/* Checkpoint file (generate.cu:90-264): P[6N] SoA (Pmx..Psz), E[3N] SoA. */
+int hzo_write_checkpoint(const char* path, int nx, int ny, int nz, int peloop, int count, double Ez_ext,
+                         double incEz, const double* P, const double* E);
+/* Reads a checkpoint of an nx x ny x nz grid (HZO_ERR_DATA on a grid
+ * mismatch); every output may be NULL. */
+int hzo_read_checkpoint(const char* path, int nx, int ny, int nz, int* peloop, int* count, double* Ez_ext,
+                        double* incEz, double* P, double* E);
+/* Raw f64 dump of n values (the potential, which the reference never writes). */
+int hzo_write_raw_f64(const char* path, long long n, const double* v);
 /* Rotation matrices from Bunge Euler angles (grain_io.cu:102-126). */
 int hzo_euler_to_TR(int ng, const double* euler, double* TR);
 /* Writers of the generator formats (voronoi.c:162-223). */

@@ -39,15 +39,16 @@ def _fields(inps, S):
     return np.stack([inp.field_nd(H.triangle_wave(S, amps[f], S)) for f, inp in enumerate(inps)], axis=1)
 
 
-@pytest.mark.parametrize("kz", [2, 4])
-def test_batched_films_match_single_film_runs_bit_for_bit(kz):
+@pytest.mark.parametrize("kz,mib,groups", [(2, 0.0, 1), (4, 0.0, 1), (2, 1.5, 3)])
+def test_batched_films_match_single_film_runs_bit_for_bit(kz, mib, groups):
     inps = _inputs()
     S = 14
     Ez = _fields(inps, S)
-    _say(f"batch of {len(inps)} films, {S} steps, kz={kz}")
+    _say(f"batch of {len(inps)} films, {S} steps, kz={kz}, group budget {mib} MiB")
     films = [_film(inp, kz) for inp in inps]
     batch = H.Batch(films, tile_kz=kz)
-    assert batch.tile_depth == kz
+    batch.set_shape(kz, mib)
+    assert batch.tile_depth == kz and batch.shape["groups"] == groups
     its_b, m_b = batch.run(Ez, want_m=True)
     cnt = batch.counters()
     _say("batched iterations per step", its_b.tolist(), cnt)
@@ -98,10 +99,11 @@ def test_autotune_keeps_the_films_state_and_picks_a_depth():
     batch = H.Batch(films, tile_kz=2)
     batch.run(Ez[:2])
     before = [f.get_state() for f in films]
-    kz, ms = batch.autotune(Ez[2], reps=1)
-    _say("autotune", kz, ms)
-    assert kz in (2, 4) and batch.tile_depth == kz
-    assert ms[2] > 0.0 and ms[4] > 0.0
+    shape, ms = batch.autotune(Ez[2], reps=1)
+    _say("autotune", shape, ms)
+    assert shape["tile_kz"] in (2, 4) and batch.tile_depth == shape["tile_kz"]
+    assert len(ms) == 8 and min(ms.values()) > 0.0
+    assert ms[(shape["tile_kz"], shape["group_mib"])] == min(ms.values())
     for f, (P, phi, _) in zip(films, before):
         P2, phi2, _ = f.get_state()
         assert np.array_equal(P, P2) and np.array_equal(phi, phi2)
