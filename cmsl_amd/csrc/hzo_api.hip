@@ -40,6 +40,8 @@ struct hzo_ctx {
     float* cg_r1 = nullptr;  // single-reduction CG: second r buffer, w and s pairs
     float* cg_w[2] = {nullptr, nullptr};
     float* cg_s[2] = {nullptr, nullptr};
+    // single-reduction CG arenas: r32 cg_r1 cg_w[0] cg_w[1] | cg_s[0] cg_s[1] p32[0] xlo
+    float* cg_arena[2] = {nullptr, nullptr};
     float* TR32 = nullptr;
     hzo::PhaseC32* ptab32 = nullptr;
     float* lut32 = nullptr;
@@ -365,13 +367,8 @@ int copy_stats(const hzo::PcgState& s, hzo_pcg_stats* stats) {
 // partial sums and control block come from b32).
 hzo::CG32Bufs cg_bufs(hzo_ctx* x, const hzo::Fp32Bufs& b32) {
     hzo::CG32Bufs cb;
-    cb.r[0] = x->r32;
-    cb.r[1] = x->cg_r1;
-    cb.w[0] = x->cg_w[0];
-    cb.w[1] = x->cg_w[1];
-    cb.s[0] = x->cg_s[0];
-    cb.s[1] = x->cg_s[1];
-    cb.p = x->p32[0];
+    cb.va = x->cg_arena[0];
+    cb.vb = x->cg_arena[1];
     cb.xlo = b32.xlo;
     cb.xhi = b32.xhi;
     cb.b = b32.b;
@@ -500,16 +497,16 @@ void free_all(hzo_ctx* x) {
     (void)hipFree(x->st);
     (void)hipFree(x->P32[0]);
     (void)hipFree(x->P32[1]);
-    (void)hipFree(x->xlo);
-    (void)hipFree(x->r32);
-    (void)hipFree(x->p32[0]);
+    if (x->cg_arena[0] || x->cg_arena[1]) {  // the vectors are views into the arenas
+        (void)hipFree(x->cg_arena[0]);
+        (void)hipFree(x->cg_arena[1]);
+    } else {
+        (void)hipFree(x->xlo);
+        (void)hipFree(x->r32);
+        (void)hipFree(x->p32[0]);
+    }
     (void)hipFree(x->p32[1]);
     (void)hipFree(x->b64);
-    (void)hipFree(x->cg_r1);
-    (void)hipFree(x->cg_w[0]);
-    (void)hipFree(x->cg_w[1]);
-    (void)hipFree(x->cg_s[0]);
-    (void)hipFree(x->cg_s[1]);
     (void)hipFree(x->TR32);
     (void)hipFree(x->ptab32);
     (void)hipFree(x->lut32);
@@ -575,28 +572,45 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
     if (cfg->max_iter < 0) return set_error(HZO_ERR_ARG, "hzo_create: max_iter < 0");
     const long long N = (long long)cfg->nx * cfg->ny * cfg->nz;
     if (N > (1LL << 28)) return set_error(HZO_ERR_ARG, "hzo_create: grid too large (at most 2^28 voxels)");
+    // the single-reduction CG addresses four of its vectors through one
+    // buffer range of 16 N bytes, which must stay below 2^31: larger films
+    // run the two-kernel FP32 PCG
+    const bool cg_fused = cfg->cg_fused && 16 * N < (1LL << 31);
+    if (cfg->tile_kz == 16 && cfg->precision == HZO_PREC_FP32_REFINED && !cg_fused)
+        return set_error(HZO_ERR_ARG, "hzo_create: tile_kz 16 needs the single-reduction CG (at most 2^27 - 1 voxels)");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return set_error(HZO_ERR_HIP, "hzo_create: no HIP device available");
 
     hzo_ctx* x = new hzo_ctx();
     x->cfg = *cfg;
+    x->cfg.cg_fused = cg_fused ? 1 : 0;
     x->g.nx = cfg->nx;
     x->g.ny = cfg->ny;
     x->g.nz = cfg->nz;
     x->g.N = N;
     // Tile depth (z planes per 64x8 workgroup) unless the caller fixes it;
     // the inner-product grouping, hence the last bits of the dot products,
-    // follows the tiling.  Two-kernel FP32 PCG: the smallest depth giving at
+    // follows the tiling.  Single-reduction FP32 CG (z-streaming, no register
+    // growth with depth): the deepest of {8, 4} that still gives at least 512
+    // workgroups, else 2 -- deeper tiles mean fewer partial sums and fewer
+    // z-ghost re-reads.  Two-kernel FP32 PCG: the smallest depth giving at
     // most 512 workgroups (one partial per consuming thread, two workgroups
-    // per CU), else 8.  FP64 and the single-reduction FP32 CG: the smallest of
-    // {2, 4} giving at most 1024 workgroups, else 4 (their depth-8 tiles
-    // exceed the 128-VGPR budget of 4 waves per SIMD).
+    // per CU), else 8.  FP64: the smallest of {2, 4} giving at most 1024
+    // workgroups, else 4 (its depth-8 tiles exceed 128 VGPRs).
     {
         const long long txy = (long long)((cfg->nx + 63) / 64) * ((cfg->ny + 7) / 8);
-        const bool f32 = cfg->precision == HZO_PREC_FP32_REFINED && !cfg->cg_fused;
+        const bool f32 = cfg->precision == HZO_PREC_FP32_REFINED && !x->cfg.cg_fused;
+        const bool fused = cfg->precision == HZO_PREC_FP32_REFINED && x->cfg.cg_fused;
         int kz = cfg->tile_kz;
-        if (kz == 0) {
+        if (kz == 0 && fused) {
+            kz = 2;
+            for (int cand : {8, 4})
+                if (txy * ((cfg->nz + cand - 1) / cand) >= 512) {
+                    kz = cand;
+                    break;
+                }
+        } else if (kz == 0) {
             kz = f32 ? 8 : 4;
             const long long cap = f32 ? 512 : 1024;
             for (int cand : {2, 4, 8}) {
@@ -651,18 +665,26 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
     if (x->fp32) {
         HZO_ALLOC(x->P32[0], sizeof(float) * 6 * N);
         HZO_ALLOC(x->P32[1], sizeof(float) * 6 * N);
-        HZO_ALLOC(x->xlo, sizeof(float) * N);
-        HZO_ALLOC(x->r32, sizeof(float) * N);
-        HZO_ALLOC(x->p32[0], sizeof(float) * N);
+        if (x->cfg.cg_fused) {
+            // one buffer resource addresses four arrays: 16 N bytes must
+            // stay below 2^31
+            HZO_ALLOC(x->cg_arena[0], sizeof(float) * 4 * N);
+            HZO_ALLOC(x->cg_arena[1], sizeof(float) * 4 * N);
+            x->r32 = x->cg_arena[0];
+            x->cg_r1 = x->cg_arena[0] + N;
+            x->cg_w[0] = x->cg_arena[0] + 2 * N;
+            x->cg_w[1] = x->cg_arena[0] + 3 * N;
+            x->cg_s[0] = x->cg_arena[1];
+            x->cg_s[1] = x->cg_arena[1] + N;
+            x->p32[0] = x->cg_arena[1] + 2 * N;
+            x->xlo = x->cg_arena[1] + 3 * N;
+        } else {
+            HZO_ALLOC(x->xlo, sizeof(float) * N);
+            HZO_ALLOC(x->r32, sizeof(float) * N);
+            HZO_ALLOC(x->p32[0], sizeof(float) * N);
+        }
         HZO_ALLOC(x->p32[1], sizeof(float) * N);
         HZO_ALLOC(x->b64, sizeof(double) * N);
-        if (cfg->cg_fused) {
-            HZO_ALLOC(x->cg_r1, sizeof(float) * N);
-            for (int q = 0; q < 2; ++q) {
-                HZO_ALLOC(x->cg_w[q], sizeof(float) * N);
-                HZO_ALLOC(x->cg_s[q], sizeof(float) * N);
-            }
-        }
         HZO_ALLOC(x->ptab32, sizeof(hzo::PhaseC32) * hzo::PTAB_ENTRIES);
         HZO_ALLOC(x->lut32, sizeof(float) * hzo::LUT_DOUBLES);
     } else {
@@ -1336,13 +1358,8 @@ int group_tables(hzo_batch* B, hzo_batch::Group& G, int kz) {
             F.b[q].part[1] = pa + (size_t)hzo::RED_MAXQ * G.ps;
             F.b[q].tdgl_part = tp;
         }
-        F.cg.r[0] = x->r32;
-        F.cg.r[1] = x->cg_r1;
-        F.cg.w[0] = x->cg_w[0];
-        F.cg.w[1] = x->cg_w[1];
-        F.cg.s[0] = x->cg_s[0];
-        F.cg.s[1] = x->cg_s[1];
-        F.cg.p = x->p32[0];
+        F.cg.va = x->cg_arena[0];
+        F.cg.vb = x->cg_arena[1];
         F.cg.xlo = x->xlo;
         F.cg.xhi = x->phi;
         F.cg.b = x->b64;

@@ -144,10 +144,11 @@ struct Fp32Bufs {
 // Single-reduction CG of the FP32-refined form (kernels_cg32.hip): r, w = A M r
 // and s = A p double-buffered by the control block's parity.
 struct CG32Bufs {
-    float* r[2];
-    float* w[2];
-    float* s[2];
-    float* p;
+    // FP32 vectors in arenas of arrays of N (one buffer resource each):
+    // va = r[0] r[1] w[0] w[1], vb = s[0] s[1] p (a film keeps its x_lo as
+    // the fourth array of vb; a solve may bring another)
+    float* va;
+    float* vb;
     float* xlo;
     double* xhi;
     const double* b;

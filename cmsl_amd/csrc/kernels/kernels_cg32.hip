@@ -48,22 +48,18 @@ __device__ __forceinline__ float row_Ax32(unsigned code, const HPair<float>* sHP
     return -(num * ih2);
 }
 
-// r = b - A (x_hi + x_lo) in FP64 for one interior voxel (direct loads).
-__device__ __forceinline__ double resid64(const CG32Bufs& b, const double* sH, const StepConsts& c, const Nbr& v,
-                                          unsigned code, int k) {
-    auto X = [&](long long q) { return b.xhi[q] + (double)b.xlo[q]; };
-    double Ax;
-    if (code & SC_IDENTITY) {
-        Ax = op_identity(X(v.c), k, c);
-    } else {
-        const double* Hr = sH + (code & 3u) * 4u;
-        const double xc = X(v.c);
-        const double num = Hr[(code >> 2) & 3u] * (X(v.e) - xc) - Hr[(code >> 4) & 3u] * (xc - X(v.w)) +
-                           Hr[(code >> 6) & 3u] * (X(v.n) - xc) - Hr[(code >> 8) & 3u] * (xc - X(v.s)) +
-                           Hr[(code >> 10) & 3u] * (X(v.u) - xc) - Hr[(code >> 12) & 3u] * (xc - X(v.d));
-        Ax = -(num * c.inv_h2);
-    }
-    return b.b[v.c] - Ax;
+// An empty asm statement that takes v as an operand: the value must exist
+// at this point, which keeps the load that produces it ahead of a branch.
+template <class V>
+__device__ __forceinline__ void keep_live(V v) {
+    asm volatile("" ::"v"(v));
+}
+template <int NQ, int PM>
+__device__ __forceinline__ void keep_live_partials(const Partials<NQ, PM>& P) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int m = 0; m < (PM > 0 ? PM : 1); ++m) keep_live(P.v[q][m]);
 }
 
 // The loop logic after the previous launch's totals (wave 0 only; every lane
@@ -126,73 +122,97 @@ __device__ __forceinline__ int cg_decide(PcgState& s, const double (&tot)[3], do
     return 0;
 }
 
+// One plane of a tile as the z-streaming loop holds it: the centre column's
+// r, w, s (w and s only for an iteration), D^-1 and, on interior planes, p
+// and x_lo; the halo column's r, w, s and D^-1.
+struct Plane32 {
+    float r, w, s, d, p, x;
+    float rh, wh, sh, dh;
+};
+
 template <int KZ, int PM>
 __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c, int nb, int ps, int t,
                                           const TileId& T) {
     const int ip = (t - 1) & 1, io = t & 1;
-    PcgState s = load_state(b.st + ip);
-    if (s.done) {
-        pass_state(s, b.st + io, T.lead);
-        return;
-    }
     __shared__ HPair<float> sHP[64];
-    __shared__ float pl[KZ][HY][HX];
+    __shared__ float pl[3][HY][HX];  // u of planes q % 3 (centre + x-y halo)
     __shared__ RawLDS<3> sraw;
     __shared__ Bcast32 bc;
     const int nz = c.nz;
     const int tid = linear_tid();
-    
+
+    // Loads that do not depend on the control block go out first, so that
+    // their latency overlaps the control-block load: the face-pair table,
+    // the previous launch's partial sums and the stencil codes.
     const int k0 = T.bz * KZ;
     const TilePos tp = tile_pos(c, T);
     const TileCols G = tile_cols(tp, c);
     const int tx = threadIdx.x, ty = threadIdx.y;
     const unsigned pb4 = c.plane * 4u, pb2 = c.plane * 2u;
-    const int cur = s.pbuf;
-    const rsrc_t Rr = make_rsrc(cur ? b.r[1] : b.r[0], c.nvox * 4u);
-    const rsrc_t Rw = make_rsrc(cur ? b.w[1] : b.w[0], c.nvox * 4u);
-    const rsrc_t Rs = make_rsrc(cur ? b.s[1] : b.s[0], c.nvox * 4u);
     const rsrc_t Rsc = make_rsrc(b.sc, c.nvox * 2u);
     const rsrc_t Rdi = make_rsrc(b.lut32 + LUT_DINV, SC_LUT_SIZE * 4u);
-    const bool after_true = s.kind == K_SETUP || s.kind == K_RESID;  // this launch re-forms w (WCOMP)
     const float ih2 = (float)c.inv_h2;
-    const unsigned oob = c.nvox * 4u;
-
     const HPair<float> hpv = load_hpair(b.lut32);
     Partials<3, PM> P;
     load_partials<3, PM>(ip ? b.part[1] : b.part[0], nb, ps, P);
+    // plane q of the tile's window is z = so[q]: q = 0 and KZ + 1 are the
+    // ghost planes below and above (clamped at the film's faces)
     unsigned so[KZ + 2];
 #pragma unroll
     for (int q = 0; q < KZ + 2; ++q) so[q] = (unsigned)min(max(k0 - 1 + q, 0), nz - 1);
-    unsigned cc[KZ + 2], ch[KZ];
+    unsigned cc[KZ + 2], ch[KZ + 2];
 #pragma unroll
-    for (int q = 0; q < KZ + 2; ++q) cc[q] = bload_code(Rsc, G.cofs * 2u, so[q] * pb2);
-#pragma unroll
-    for (int q = 0; q < KZ; ++q) ch[q] = bload_code(Rsc, G.hofs * 2u, so[q + 1] * pb2);
-    float rc[KZ + 2], rh[KZ];
-#pragma unroll
-    for (int q = 0; q < KZ + 2; ++q) rc[q] = bload<float>(Rr, G.cofs * 4u, so[q] * pb4);
-#pragma unroll
-    for (int q = 0; q < KZ; ++q) rh[q] = bload<float>(Rr, G.hofs * 4u, so[q + 1] * pb4);
-    float wc[KZ + 2], sc_[KZ + 2], wh[KZ], sh[KZ];
-    const rsrc_t Rp = make_rsrc(b.p, c.nvox * 4u);
-    const rsrc_t Rx = make_rsrc(b.xlo, c.nvox * 4u);
-    if (!after_true) {
-#pragma unroll
-        for (int q = 0; q < KZ + 2; ++q) {
-            wc[q] = bload<float>(Rw, G.cofs * 4u, so[q] * pb4);
-            sc_[q] = bload<float>(Rs, G.cofs * 4u, so[q] * pb4);
-        }
-#pragma unroll
-        for (int q = 0; q < KZ; ++q) {
-            wh[q] = bload<float>(Rw, G.hofs * 4u, so[q + 1] * pb4);
-            sh[q] = bload<float>(Rs, G.hofs * 4u, so[q + 1] * pb4);
-        }
+    for (int q = 0; q < KZ + 2; ++q) {
+        cc[q] = bload_code(Rsc, G.cofs * 2u, so[q] * pb2);
+        ch[q] = (q >= 1 && q <= KZ) ? bload_code(Rsc, G.hofs * 2u, so[q] * pb2) : 0u;
     }
-    float dc[KZ + 2], dh[KZ];
+
+    PcgState s = load_state(b.st + ip);
+    if (s.done) {
+        // the values are consumed here too, so the loads above stay ahead of
+        // the branch instead of being sunk behind the control-block wait
+        keep_live(hpv.p);
 #pragma unroll
-    for (int q = 0; q < KZ + 2; ++q) dc[q] = bload<float>(Rdi, cc[q] * 4u, 0u);
-#pragma unroll
-    for (int q = 0; q < KZ; ++q) dh[q] = bload<float>(Rdi, ch[q] * 4u, 0u);
+        for (int q = 0; q < KZ + 2; ++q) keep_live(cc[q]);
+        keep_live_partials(P);
+        pass_state(s, b.st + io, T.lead);
+        return;
+    }
+    // two resources cover seven vectors: RA = r[0] r[1] w[0] w[1], RB =
+    // s[0] s[1] p; an array is chosen by a workgroup-uniform byte offset.
+    // x_lo has its own (an adjoint solve brings its own solution vector).
+    // `aend` (one past an arena) is out of range of every resource here.
+    const unsigned cur = (unsigned)s.pbuf;
+    const unsigned seg = c.nvox * 4u, aend = 4u * seg;
+    const rsrc_t RA = make_rsrc(b.va, aend);
+    const rsrc_t RB = make_rsrc(b.vb, aend);
+    const rsrc_t RX = make_rsrc(b.xlo, seg);
+    const unsigned o_r = cur * seg, o_w = (2u + cur) * seg, o_s = cur * seg;
+    const unsigned o_p = 2u * seg;
+    const unsigned cv = G.cofs * 4u, hv = tp.has_h ? G.hofs * 4u : aend;
+    // an iteration follows unless the last launch set up or replaced the
+    // residual (then this launch re-forms w and needs neither w nor s)
+    const bool after_true = s.kind == K_SETUP || s.kind == K_RESID;
+
+    // Plane loads, two planes ahead of their use.
+    Plane32 L[KZ + 2];
+    auto issue = [&](int q) {
+        const unsigned sv = so[q] * pb4;
+        L[q].r = bload<float>(RA, cv, o_r + sv);
+        L[q].d = bload<float>(Rdi, cc[q] * 4u, 0u);
+        L[q].w = after_true ? 0.0f : bload<float>(RA, cv, o_w + sv);
+        L[q].s = after_true ? 0.0f : bload<float>(RB, cv, o_s + sv);
+        if (q >= 1 && q <= KZ) {
+            L[q].p = after_true ? 0.0f : bload<float>(RB, cv, o_p + sv);
+            L[q].x = after_true ? 0.0f : bload<float>(RX, cv, sv);
+            L[q].rh = bload<float>(RA, hv, o_r + sv);
+            L[q].dh = bload<float>(Rdi, ch[q] * 4u, 0u);
+            L[q].wh = after_true ? 0.0f : bload<float>(RA, hv, o_w + sv);
+            L[q].sh = after_true ? 0.0f : bload<float>(RB, hv, o_s + sv);
+        }
+    };
+    issue(0);
+    issue(1);
 
     stage_partials<3, PM>(P, sraw);
     __syncthreads();
@@ -202,17 +222,18 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
         wave0_total<3>(sraw, tot);
         double alpha = 0.0, beta = 0.0;
         const int mode = cg_decide(s, tot, alpha, beta);
-        if (mode == 1) {
-            pass_state(s, b.st + io, T.lead);
-        } else if (mode == 2) {
+        if (mode == 2) {
             s.kind = K_RESID;
         } else if (mode == 3) {
             s.kind = K_WCOMP;
-        } else {
+        } else if (mode == 0) {
             s.iters += 1;
             s.pbuf ^= 1;
             s.kind = K_ITER;
         }
+        // handed on at once: nothing later in the launch reads the block, so
+        // it need not stay live in registers through the tile work
+        pass_state(s, b.st + io, T.lead);
         if (tid == 0) {
             bc.alpha = alpha;
             bc.beta = beta;
@@ -224,142 +245,141 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
     if (mode == 1) return;
     double acc[3] = {0.0, 0.0, 0.0};
     if (mode == 2) {
-        // RESID: r = b - A (x_hi + x_lo) in FP64 over the tile (direct loads)
+        // RESID: r = b - A (x_hi + x_lo) in FP64, one plane at a time, with
+        // buffer loads at 32-bit column offsets (neighbour planes clamped
+        // into the film; identity rows, where the clamp applies under
+        // Dirichlet electrodes, use only the centre)
         __shared__ double sH64[16];
         load_lut(sH64, b.lut);
-        float* rout = cur ? b.r[1] : b.r[0];
-        if (tp.live) {
-            for (int k = k0; k < min(k0 + KZ, nz); ++k) {
-                const Nbr v = op_nbrs(tp.i, tp.j, k, c);
-                const unsigned code = b.sc[v.c];
-                const float r = (float)resid64(b, sH64, c, v, code, k);
-                rout[v.c] = r;
-                const float z = r * b.lut32[LUT_DINV + code];
+        const rsrc_t Rxh = make_rsrc(b.xhi, c.nvox * 8u);
+        const rsrc_t Rxl = make_rsrc(b.xlo, c.nvox * 4u);
+        const rsrc_t Rb = make_rsrc(b.b, c.nvox * 8u);
+        const int nx = c.nx, i = tp.ci, j = tp.cj;
+        const unsigned ie = (i + 1 == nx) ? 0 : i + 1, iw = (i == 0) ? nx - 1 : i - 1;
+        const unsigned jn = (j + 1 == c.ny) ? 0 : j + 1, js = (j == 0) ? c.ny - 1 : j - 1;
+        // dead lanes use element offset 4 N: out of range of every array here
+        const unsigned dead = 4u * c.nvox;
+        const unsigned oc = tp.live ? G.cofs : dead;
+        const unsigned oe = tp.live ? ie + (unsigned)nx * j : dead, ow = tp.live ? iw + (unsigned)nx * j : dead;
+        const unsigned on = tp.live ? i + (unsigned)nx * jn : dead, os = tp.live ? i + (unsigned)nx * js : dead;
+        auto X = [&](unsigned col, unsigned pofs) {
+            return bload<double>(Rxh, col * 8u, pofs * 8u) + (double)bload<float>(Rxl, col * 4u, pofs * 4u);
+        };
+        const int k1 = min(k0 + KZ, nz);
+#pragma unroll 1
+        for (int k = k0; k < k1; ++k) {
+            int up = k + 1, dn = k - 1;
+            if (c.zbc != 0) {
+                if (k == 0) up = dn = 1;
+                else if (k == nz - 1) up = dn = nz - 2;
+            }
+            const unsigned pk = (unsigned)k * c.plane;
+            const unsigned pu = (unsigned)min(up, nz - 1) * c.plane, pd = (unsigned)max(dn, 0) * c.plane;
+            const unsigned code = bload_code(Rsc, oc * 2u, pk * 2u);
+            const double xc = X(oc, pk);
+            double Ax;
+            if (code & SC_IDENTITY) {
+                Ax = op_identity(xc, k, c);
+            } else {
+                const double* Hr = sH64 + (code & 3u) * 4u;
+                const double num = Hr[(code >> 2) & 3u] * (X(oe, pk) - xc) - Hr[(code >> 4) & 3u] * (xc - X(ow, pk)) +
+                                   Hr[(code >> 6) & 3u] * (X(on, pk) - xc) - Hr[(code >> 8) & 3u] * (xc - X(os, pk)) +
+                                   Hr[(code >> 10) & 3u] * (X(oc, pu) - xc) - Hr[(code >> 12) & 3u] * (xc - X(oc, pd));
+                Ax = -(num * c.inv_h2);
+            }
+            const float r = (float)(bload<double>(Rb, oc * 8u, pk * 8u) - Ax);
+            bstore(RA, oc * 4u, o_r + pk * 4u, r);
+            if (tp.live) {
+                const float z = r * bload<float>(Rdi, code * 4u, 0u);
                 acc[0] += (double)r * (double)z;
                 acc[1] += (double)r * (double)r;
             }
         }
-    } else if (mode == 3) {
-        // WCOMP: fold x_lo into x_hi, u = M r, w = A u, (gamma, delta, r.r)
-        if (tp.live) {
-#pragma unroll
-            for (int q = 0; q < KZ; ++q) {
-                const int k = k0 + q;
-                if (k < nz) {
-                    const long long id = (long long)G.cofs + (long long)c.plane * k;
-                    b.xhi[id] += (double)b.xlo[id];
-                    b.xlo[id] = 0.0f;
-                }
-            }
-        }
-        float uc[KZ + 2];
-#pragma unroll
-        for (int q = 0; q < KZ + 2; ++q) {
-            uc[q] = rc[q] * dc[q];
-            if (q >= 1 && q <= KZ) pl[q - 1][ty + 1][tx + 1] = uc[q];
-        }
-#pragma unroll
-        for (int q = 0; q < KZ; ++q)
-            if (tp.has_h) pl[q][tp.hy][tp.hx] = rh[q] * dh[q];
-        __syncthreads();
-#pragma unroll
-        for (int q = 1; q <= KZ; ++q) {
-            const int k = k0 - 1 + q;
-            const float xc = uc[q];
-            const unsigned code = cc[q];
-            float xu = uc[q + 1], xd = uc[q - 1];
-            if (c.zbc != 0) {
-                if (k == 0) xd = xu;
-                else if (k == nz - 1) xu = xd;
-            }
-            const float qs = row_Ax32(code, sHP, ih2, xc, pl[q - 1][ty + 1][tx + 2], pl[q - 1][ty + 1][tx],
-                                      pl[q - 1][ty + 2][tx + 1], pl[q - 1][ty][tx + 1], xu, xd);
-            const float wv = (code & SC_IDENTITY) ? op_identity(xc, k, c) : qs;
-            const bool ok = tp.live && k < nz;
-            bstore(Rw, ok ? G.cofs * 4u : oob, so[q] * pb4, wv);
-            if (ok) {
-                acc[0] += (double)rc[q] * (double)xc;
-                acc[1] += (double)wv * (double)xc;
-                acc[2] += (double)rc[q] * (double)rc[q];
-            }
-        }
     } else {
-        // ITER: s = w + beta s, r -= alpha s on tile + halo; p = M r_old + beta p, x += alpha p;
-        // u = M r, w = A u; (gamma, delta, r.r)
+        // ITER (mode 0): s = w + beta s and r -= alpha s on the tile and its
+        // halo, p = M r_old + beta p, x += alpha p, then u = M r and w = A u;
+        // partials (gamma, delta, r.r).  WCOMP (mode 3): x_lo folded into
+        // x_hi, u = M r, w = A u; partials (r.u, w.u, r.r).  The planes are
+        // streamed bottom to top: plane q's update and u, then the stencil of
+        // plane q - 1, whose z neighbours are u of planes q - 2 and q.
+        const bool iter = mode == 0;
         const float alpha = (float)bc.alpha, beta = (float)bc.beta;
         const bool fresh = bc.beta == 0.0;  // first iteration of a solve: no previous p, s
-        const rsrc_t Rr2 = make_rsrc(cur ? b.r[0] : b.r[1], c.nvox * 4u);
-        const rsrc_t Rw2 = make_rsrc(cur ? b.w[0] : b.w[1], c.nvox * 4u);
-        const rsrc_t Rs2 = make_rsrc(cur ? b.s[0] : b.s[1], c.nvox * 4u);
-        float uc[KZ + 2], uo[KZ];
+        const unsigned o_r2 = (cur ^ 1u) * seg, o_s2 = (cur ^ 1u) * seg;
+        // w goes to the other buffer after an iteration, in place after WCOMP
+        const unsigned o_wo = iter ? (3u - cur) * seg : o_w;
+        float u[KZ + 2];
 #pragma unroll
         for (int q = 0; q < KZ + 2; ++q) {
+            if (q + 2 < KZ + 2) issue(q + 2);
             const int k = k0 - 1 + q;
-            const float sn = fresh ? wc[q] : wc[q] + beta * sc_[q];
-            const float rn = rc[q] - alpha * sn;
-            uc[q] = rn * dc[q];
-            if (q >= 1 && q <= KZ) {
-                uo[q - 1] = rc[q] * dc[q];
-                pl[q - 1][ty + 1][tx + 1] = uc[q];
-                const bool ok = tp.live && k < nz;
-                const unsigned vo = ok ? G.cofs * 4u : oob;
-                bstore(Rs2, vo, so[q] * pb4, sn);
-                bstore(Rr2, vo, so[q] * pb4, rn);
-                if (ok) {
-                    acc[0] += (double)rn * (double)uc[q];
-                    acc[2] += (double)rn * (double)rn;
+            const bool interior = q >= 1 && q <= KZ;
+            const bool ok = interior && tp.live && k < nz;
+            const unsigned vo = ok ? cv : aend;
+            const unsigned sv = so[q] * pb4;
+            // A(q): update and u of plane q
+            float rn = L[q].r;
+            if (iter) {
+                const float sn = fresh ? L[q].w : L[q].w + beta * L[q].s;
+                rn = L[q].r - alpha * sn;
+                if (interior) {
+                    bstore(RB, vo, o_s2 + sv, sn);
+                    bstore(RA, vo, o_r2 + sv, rn);
+                    const float uo = L[q].r * L[q].d;
+                    const float pn = fresh ? uo : uo + beta * L[q].p;
+                    bstore(RB, vo, o_p + sv, pn);
+                    bstore(RX, vo, sv, L[q].x + alpha * pn);
                 }
+            } else if (interior && ok) {
+                const long long id = (long long)G.cofs + (long long)c.plane * k;
+                b.xhi[id] += (double)b.xlo[id];
+                b.xlo[id] = 0.0f;
             }
-        }
-#pragma unroll
-        for (int q = 0; q < KZ; ++q) {
-            const float shn = fresh ? wh[q] : wh[q] + beta * sh[q];
-            const float rhn = rh[q] - alpha * shn;
-            if (tp.has_h) pl[q][tp.hy][tp.hx] = rhn * dh[q];
-        }
-        __syncthreads();
-        // p and x are needed only by the interior update after the stencil:
-        // their loads are issued here and land while w = A u is formed
-        float pv[KZ], xv[KZ];
-#pragma unroll
-        for (int q = 0; q < KZ; ++q) {
-            pv[q] = bload<float>(Rp, G.cofs * 4u, so[q + 1] * pb4);
-            xv[q] = bload<float>(Rx, G.cofs * 4u, so[q + 1] * pb4);
-        }
-#pragma unroll
-        for (int q = 1; q <= KZ; ++q) {
-            const int k = k0 - 1 + q;
-            const float xc = uc[q];
-            const unsigned code = cc[q];
-            float xu = uc[q + 1], xd = uc[q - 1];
-            if (c.zbc != 0) {
-                if (k == 0) xd = xu;
-                else if (k == nz - 1) xu = xd;
+            u[q] = rn * L[q].d;
+            if (ok) {
+                acc[0] += (double)rn * (double)u[q];
+                acc[2] += (double)rn * (double)rn;
             }
-            const float qs = row_Ax32(code, sHP, ih2, xc, pl[q - 1][ty + 1][tx + 2], pl[q - 1][ty + 1][tx],
-                                      pl[q - 1][ty + 2][tx + 1], pl[q - 1][ty][tx + 1], xu, xd);
-            const float wv = (code & SC_IDENTITY) ? op_identity(xc, k, c) : qs;
-            const bool ok = tp.live && k < nz;
-            bstore(Rw2, ok ? G.cofs * 4u : oob, so[q] * pb4, wv);
-            if (ok) acc[1] += (double)wv * (double)xc;
-        }
-#pragma unroll
-        for (int q = 1; q <= KZ; ++q) {
-            const int k = k0 - 1 + q;
-            const float pn = fresh ? uo[q - 1] : uo[q - 1] + beta * pv[q - 1];
-            const float xn = xv[q - 1] + alpha * pn;
-            const bool ok = tp.live && k < nz;
-            const unsigned vo = ok ? G.cofs * 4u : oob;
-            bstore(Rp, vo, so[q] * pb4, pn);
-            bstore(Rx, vo, so[q] * pb4, xn);
+            if (interior) {
+                pl[q % 3][ty + 1][tx + 1] = u[q];
+                float rhn = L[q].rh;
+                if (iter) {
+                    const float shn = fresh ? L[q].wh : L[q].wh + beta * L[q].sh;
+                    rhn = L[q].rh - alpha * shn;
+                }
+                if (tp.has_h) pl[q % 3][tp.hy][tp.hx] = rhn * L[q].dh;
+            }
+            // B(q - 1): stencil of plane q - 1
+            if (q >= 2) {
+                __syncthreads();
+                const int m = q - 1, km = k - 1;
+                const float xc = u[m];
+                const unsigned code = cc[m];
+                float xu = u[m + 1], xd = u[m - 1];
+                if (c.zbc != 0) {
+                    if (km == 0) xd = xu;
+                    else if (km == nz - 1) xu = xd;
+                }
+                const float(*pm)[HX] = pl[m % 3];
+                const float qs = row_Ax32(code, sHP, ih2, xc, pm[ty + 1][tx + 2], pm[ty + 1][tx], pm[ty + 2][tx + 1],
+                                          pm[ty][tx + 1], xu, xd);
+                const float wv = (code & SC_IDENTITY) ? op_identity(xc, km, c) : qs;
+                const bool okm = tp.live && km < nz;
+                bstore(RA, okm ? cv : aend, o_wo + so[m] * pb4, wv);
+                if (okm) acc[1] += (double)wv * (double)xc;
+            }
         }
     }
     write_block_partials<3>(acc, io ? b.part[1] : b.part[0], ps, T.blk, sraw);
-    pass_state(s, b.st + io, T.lead);
 }
 
+// Waves per SIMD the register budget is set for: 8 (4 workgroups per CU,
+// <= 64 VGPRs) at two planes per tile, 6 at four and eight, 2 at sixteen.
+// The batch kernel, which also holds its film's descriptor, is set for 5.
+constexpr int cg32_waves(int kz) { return kz >= 16 ? 2 : (kz <= 2 ? 8 : 6); }
+
 template <int KZ, int PM>
-__global__ __launch_bounds__(NT, KZ >= 16 ? 2 : 4) void k_cg32(CG32Bufs b, StepConsts c, int nb, int ps, int t) {
+__global__ __launch_bounds__(NT, cg32_waves(KZ)) void k_cg32(CG32Bufs b, StepConsts c, int nb, int ps, int t) {
     cg32_body<KZ, PM>(b, c, nb, ps, t, tile_id());
 }
 
@@ -367,7 +387,7 @@ __global__ __launch_bounds__(NT, KZ >= 16 ? 2 : 4) void k_cg32(CG32Bufs b, StepC
 // (film, logical tile); every film runs its own control block, and the tiles
 // of a film whose solve has finished return at once.
 template <int KZ, int PM>
-__global__ __launch_bounds__(NT, KZ >= 16 ? 2 : 4) void k_cg32_batch(const BatchFilm* __restrict__ films, const int2* __restrict__ tiles,
+__global__ __launch_bounds__(NT, 5) void k_cg32_batch(const BatchFilm* __restrict__ films, const int2* __restrict__ tiles,
                                                                       int t) {
     TileId T;
     const BatchFilm& F = films[batch_tile(tiles, films, T)];
