@@ -289,6 +289,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=8)
     ap.add_argument("--batch-warmup", type=int, default=2)
     ap.add_argument("--no-loop", action="store_true", help="skip the full-loop FP32-vs-FP64 error run")
+    ap.add_argument("--stream-probe", action="store_true",
+                    help="only time a plain kernel with the single-reduction CG's per-voxel traffic "
+                         "(42 B) at 2^21 and 2^24 voxels: the attainable rate beside the CG kernel")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a device-to-device copy of 1 GiB three times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -297,6 +300,17 @@ def main():
         print(f"bench.py measures one GPU only (the film fits one MI355X); --gpus {args.gpus} "
               "is not supported", file=sys.stderr)
         sys.exit(2)
+    if args.stream_probe:
+        import ctypes as C
+        from cmsl_amd import _lib as L
+        res = {}
+        for n in (1 << 21, 1 << 24):
+            for grid in (512, 1024, 2048, 4096):
+                us = C.c_double(0.0)
+                L.call("hzo_probe_stream", n, grid, 200, C.byref(us))
+                res[f"n{n}_grid{grid}"] = {"us": us.value, "TBs": 42.0 * n / (us.value * 1e-6) / 1e12}
+        print(json.dumps({"stream_probe_bytes_per_element": 42, "results": res}))
+        return
     if args.copy_probe:
         import torch
         n = 1 << 30  # larger than the 256 MiB last-level cache: every byte crosses HBM

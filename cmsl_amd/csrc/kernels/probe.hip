@@ -1,0 +1,64 @@
+// probe.hip -- streaming-traffic probe: the attainable rate of a kernel that
+// moves exactly the per-voxel traffic of the single-reduction CG (five f32
+// arrays and one u16 array read, five f32 arrays written: 42 B per element)
+// with no stencil, halo or reduction.  Used to place the CG kernel's achieved
+// bandwidth between this floor and the HBM peak.
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "../hzo_internal.h"
+#include "../host/errors.h"
+
+namespace hzo {
+
+__global__ __launch_bounds__(512) void k_probe_stream(const float* __restrict__ a, const float* __restrict__ b,
+                                                      const float* __restrict__ c, const float* __restrict__ d,
+                                                      const float* __restrict__ e, const uint16_t* __restrict__ code,
+                                                      float* __restrict__ oa, float* __restrict__ ob,
+                                                      float* __restrict__ oc, float* __restrict__ od,
+                                                      float* __restrict__ oe, long long n) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float s = a[i] + b[i] * c[i] + d[i] - e[i] + (float)code[i];
+        oa[i] = s;
+        ob[i] = 2.0f * s;
+        oc[i] = s + 1.0f;
+        od[i] = s - 1.0f;
+        oe[i] = 0.5f * s;
+    }
+}
+
+}  // namespace hzo
+
+extern "C" int hzo_probe_stream(long long n, int grid, int reps, double* us_per_launch) {
+    if (n < 1 || grid < 1 || reps < 1 || !us_per_launch) return hzo::set_error(HZO_ERR_ARG, "hzo_probe_stream: bad argument");
+    float* v[10] = {};
+    uint16_t* code = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    int rc = HZO_OK;
+    auto fail = [&](const char* what) { rc = hzo::set_error(HZO_ERR_HIP, "hzo_probe_stream: %s", what); };
+    for (int q = 0; q < 10 && rc == HZO_OK; ++q)
+        if (hipMalloc((void**)&v[q], sizeof(float) * n) != hipSuccess || hipMemset(v[q], 0, sizeof(float) * n) != hipSuccess)
+            fail("allocation");
+    if (rc == HZO_OK && (hipMalloc((void**)&code, 2 * n) != hipSuccess || hipMemset(code, 0, 2 * n) != hipSuccess))
+        fail("allocation");
+    if (rc == HZO_OK && (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess)) fail("events");
+    if (rc == HZO_OK) {
+        auto launch = [&]() {
+            hipLaunchKernelGGL(hzo::k_probe_stream, dim3(grid), dim3(512), 0, 0, v[0], v[1], v[2], v[3], v[4], code, v[5],
+                               v[6], v[7], v[8], v[9], n);
+        };
+        for (int w = 0; w < 10; ++w) launch();
+        (void)hipEventRecord(t0, 0);
+        for (int r = 0; r < reps; ++r) launch();
+        (void)hipEventRecord(t1, 0);
+        float ms = 0.f;
+        if (hipEventSynchronize(t1) != hipSuccess || hipEventElapsedTime(&ms, t0, t1) != hipSuccess) fail("timing");
+        else *us_per_launch = 1e3 * ms / reps;
+    }
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    for (float* p : v) (void)hipFree(p);
+    (void)hipFree(code);
+    return rc;
+}
