@@ -1600,7 +1600,8 @@ int group_run(hzo_batch* B, hzo_batch::Group& G, const double* Ez, int nsteps, i
 
 int hzo_batch_create(hzo_ctx* const* films, int n, int tile_kz, hzo_batch** out) {
     if (!films || n < 1 || !out) return set_error(HZO_ERR_ARG, "hzo_batch_create: bad argument");
-    if (tile_kz != 2 && tile_kz != 4) return set_error(HZO_ERR_ARG, "hzo_batch_create: tile_kz must be 2 or 4");
+    if (tile_kz != 2 && tile_kz != 4 && tile_kz != 8)
+        return set_error(HZO_ERR_ARG, "hzo_batch_create: tile_kz must be 2, 4 or 8");
     for (int f = 0; f < n; ++f) {
         hzo_ctx* x = films[f];
         int rc = check_ready(x);
@@ -1657,8 +1658,8 @@ int hzo_batch_config(hzo_batch* B, int use_graphs, int chunk) {
 int hzo_batch_tile_depth(hzo_batch* B) { return B ? B->kz : 0; }
 
 int hzo_batch_set_shape(hzo_batch* B, int tile_kz, double group_mib) {
-    if (!B || (tile_kz != 2 && tile_kz != 4) || !(group_mib >= 0.0))
-        return set_error(HZO_ERR_ARG, "hzo_batch_set_shape: tile_kz must be 2 or 4, group_mib >= 0");
+    if (!B || (tile_kz != 2 && tile_kz != 4 && tile_kz != 8) || !(group_mib >= 0.0))
+        return set_error(HZO_ERR_ARG, "hzo_batch_set_shape: tile_kz must be 2, 4 or 8, group_mib >= 0");
     HZO_HIP(hipSetDevice(B->device));
     HZO_HIP(hipStreamSynchronize(B->stream));
     return batch_tables(B, tile_kz, group_mib * 1048576.0);
@@ -1683,7 +1684,7 @@ int hzo_batch_run(hzo_batch* B, const double* Ez, int nsteps, int* iters, double
     return HZO_OK;
 }
 
-int hzo_batch_autotune(hzo_batch* B, const double* Ez, int reps, int* kz_out, double ms[8]) {
+int hzo_batch_autotune(hzo_batch* B, const double* Ez, int reps, int* kz_out, double ms[12]) {
     if (!B || !Ez || reps < 1) return set_error(HZO_ERR_ARG, "hzo_batch_autotune: bad argument");
     HZO_HIP(hipSetDevice(B->device));
     for (hzo_ctx* x : B->films) HZO_HIP(hipStreamSynchronize(x->stream));
@@ -1723,12 +1724,12 @@ int hzo_batch_autotune(hzo_batch* B, const double* Ez, int reps, int* kz_out, do
     };
     // candidates: tile depth x group budget (MiB of CG working set; 0 = all
     // films in one group)
-    const int kzs[2] = {2, 4};
+    const int kzs[3] = {2, 4, 8};
     const double mib[4] = {0.0, 32.0, 64.0, 128.0};
-    double best_ms[8];
+    double best_ms[12];
     for (double& v : best_ms) v = 1e300;
     int rc = copy(true);
-    for (int c = 0; c < 8 && rc == HZO_OK; ++c) {
+    for (int c = 0; c < 12 && rc == HZO_OK; ++c) {
         if ((rc = batch_tables(B, kzs[c / 4], mib[c % 4] * 1048576.0)) != HZO_OK) break;
         for (int r = 0; r <= reps && rc == HZO_OK; ++r) {  // r = 0 warms up (code objects, graphs)
             if ((rc = copy(false)) != HZO_OK) break;
@@ -1744,12 +1745,12 @@ int hzo_batch_autotune(hzo_batch* B, const double* Ez, int reps, int* kz_out, do
     release();
     if (rc) return rc;
     int best = 0;
-    for (int c = 1; c < 8; ++c)
+    for (int c = 1; c < 12; ++c)
         if (best_ms[c] < best_ms[best]) best = c;
     if ((rc = batch_tables(B, kzs[best / 4], mib[best % 4] * 1048576.0)) != HZO_OK) return rc;
     if (kz_out) *kz_out = kzs[best / 4];
     if (ms)
-        for (int c = 0; c < 8; ++c) ms[c] = best_ms[c];
+        for (int c = 0; c < 12; ++c) ms[c] = best_ms[c];
     return HZO_OK;
 }
 

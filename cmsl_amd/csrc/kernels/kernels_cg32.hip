@@ -375,7 +375,8 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
 
 // Waves per SIMD the register budget is set for: 8 (4 workgroups per CU,
 // <= 64 VGPRs) at two planes per tile, 6 at four and eight, 2 at sixteen.
-// The batch kernel, which also holds its film's descriptor, is set for 5.
+// The batch kernel, which also holds its film's descriptor, is set for 5
+// (4 at depth 8).
 constexpr int cg32_waves(int kz) { return kz >= 16 ? 2 : (kz <= 2 ? 8 : 6); }
 
 template <int KZ, int PM>
@@ -387,7 +388,7 @@ __global__ __launch_bounds__(NT, cg32_waves(KZ)) void k_cg32(CG32Bufs b, StepCon
 // (film, logical tile); every film runs its own control block, and the tiles
 // of a film whose solve has finished return at once.
 template <int KZ, int PM>
-__global__ __launch_bounds__(NT, 5) void k_cg32_batch(const BatchFilm* __restrict__ films, const int2* __restrict__ tiles,
+__global__ __launch_bounds__(NT, KZ >= 8 ? 4 : 5) void k_cg32_batch(const BatchFilm* __restrict__ films, const int2* __restrict__ tiles,
                                                                       int t) {
     TileId T;
     const BatchFilm& F = films[batch_tile(tiles, films, T)];
@@ -409,8 +410,10 @@ void launch_cg32_batch(hipStream_t s, int ntiles, int kz, int pm, const BatchFil
     }
     if (kz == 2) {
         HZO_CGB(2)
-    } else {
+    } else if (kz == 4) {
         HZO_CGB(4)
+    } else {
+        HZO_CGB(8)
     }
 #undef HZO_CGB
 }

@@ -410,7 +410,7 @@ class Batch:
         return (its, m) if want_m else its
 
     def set_shape(self, tile_kz: int, group_mib: float = 0.0):
-        """Tile depth (2 or 4) and group budget in MiB of CG working set (0: one group)."""
+        """Tile depth (2, 4 or 8) and group budget in MiB of CG working set (0: one group)."""
         L.call("hzo_batch_set_shape", self.h, int(tile_kz), float(group_mib))
 
     @property
@@ -420,13 +420,13 @@ class Batch:
         return {"tile_kz": int(out[0]), "group_mib": int(out[1]), "groups": int(out[2])}
 
     def autotune(self, Ez0, reps: int = 2):
-        """Times one step per shape (tile depth 2/4 x group budget 0/32/64/128
+        """Times one step per shape (tile depth 2/4/8 x group budget 0/32/64/128
         MiB, fields Ez0[f]) and keeps the fastest; returns (shape, {(kz, mib): ms})."""
         Ez0 = np.ascontiguousarray(np.broadcast_to(np.asarray(Ez0, np.float64), (self.n,)))
         kz = C.c_int(0)
-        ms = np.zeros(8)
+        ms = np.zeros(12)
         L.call("hzo_batch_autotune", self.h, L.ptr(Ez0), int(reps), C.byref(kz), L.ptr(ms))
-        times = {((2, 4)[c // 4], (0, 32, 64, 128)[c % 4]): float(ms[c]) for c in range(8)}
+        times = {((2, 4, 8)[c // 4], (0, 32, 64, 128)[c % 4]): float(ms[c]) for c in range(12)}
         return self.shape, times
 
     def counters(self):

@@ -251,13 +251,13 @@ int hzo_checkpoint_load(hzo_ctx* ctx, const char* path, int* peloop, int* count,
  * film ends a batched run in exactly the state its own hzo_run leaves at the
  * same tile depth. */
 typedef struct hzo_batch hzo_batch;
-/* tile_kz: 2 or 4 (planes per workgroup tile); all films in one group. */
+/* tile_kz: 2, 4 or 8 (planes per workgroup tile); all films in one group. */
 int hzo_batch_create(hzo_ctx* const* films, int n, int tile_kz, hzo_batch** out);
 void hzo_batch_destroy(hzo_batch* batch);
 /* use_graphs: replay CG launches from a captured hipGraph (default 1);
  * chunk: CG launches per replay, even, >= 2 (default 16). */
 int hzo_batch_config(hzo_batch* batch, int use_graphs, int chunk);
-/* Tile depth (2 or 4) of the groups of several films, and group budget in
+/* Tile depth (2, 4 or 8) of the groups of several films, and group budget in
  * MiB of CG working set (34 bytes per voxel; 0: one group of all films). */
 int hzo_batch_set_shape(hzo_batch* batch, int tile_kz, double group_mib);
 /* out[3] = tile depth, group budget (MiB), number of groups. */
@@ -268,10 +268,10 @@ int hzo_batch_tile_depth(hzo_batch* batch);
  * m_out[s*n + f] (may be NULL): <Pmz> of film f after step s. */
 int hzo_batch_run(hzo_batch* batch, const double* Ez, int nsteps, int* iters, double* m_out);
 /* Times one step (applied fields Ez[f], best of reps after a warm-up step)
- * for each shape c = 4*k + g: tile depth {2, 4}[k] x group budget
+ * for each shape c = 4*k + g: tile depth {2, 4, 8}[k] x group budget
  * {0, 32, 64, 128}[g] MiB, restores the films' states after every trial and
- * keeps the fastest shape.  kz_out and ms[8] (per-shape times) may be NULL. */
-int hzo_batch_autotune(hzo_batch* batch, const double* Ez, int reps, int* kz_out, double ms[8]);
+ * keeps the fastest shape.  kz_out and ms[12] (per-shape times) may be NULL. */
+int hzo_batch_autotune(hzo_batch* batch, const double* Ez, int reps, int* kz_out, double ms[12]);
 /* out[0]: CG launches issued by all batched runs; out[1]: film solves that
  * stopped at the launch cap without converging. */
 int hzo_batch_counters(const hzo_batch* batch, long long out[2]);
