@@ -127,8 +127,11 @@ SIGNATURES = {
     "hzo_read_gchar": (_I, [C.c_char_p, _I, _P, _P]),
     "hzo_euler_to_TR": (_I, [_I, _P, _P]),
     "hzo_write_gdata": (_I, [C.c_char_p, _I, _I, _I, _P]),
+    "hzo_count_structures": (_I, [C.c_char_p, _P]),
+    "hzo_grain_boundary_cells": (_I, [_I, _I, _I, _P, _I, _D, C.c_uint64, _P, _P]),
     "hzo_write_gchar": (_I, [C.c_char_p, _I, _P, _P]),
     "hzo_voronoi": (_I, [_I, _I, _I, _I, _P, _I, C.c_uint64, _P, _P, _P]),
+    "hzo_voronoi_gpu": (_I, [_I, _I, _I, _I, _I, _P, _I, C.c_uint64, _P, _P, _P]),
     "hzo_nondim": (_I, [_P, _I, _P, _P]),
     "hzo_assign_phases": (_I, [_I, _I, _I, _P, _I, _P, _I, _P, _P, C.c_uint, _P]),
     "hzo_interface_flag": (_I, [_I, _I, _I, _P, _I, _P, _P]),

@@ -6,6 +6,7 @@
 //   hzo_read_gchar         grain_io.cu:71-99
 //   hzo_euler_to_TR        grain_io.cu:101-126 (Bunge ZXZ, passive)
 //   hzo_write_gdata/gchar  voronoi.c:197-223 (generator output formats)
+//   hzo_count_structures   main.cu:430-440 (inputs/struc1, struc2, ... until one is missing)
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -13,6 +14,8 @@
 #include <cstring>
 #include <string>
 #include <vector>
+
+#include <sys/stat.h>
 
 #include "../../../include/hzo.h"
 #include "errors.h"
@@ -223,6 +226,20 @@ int hzo_write_gchar(const char* path, int ng, const double* cent, const double* 
         fprintf(f, "%d\t%lf\t%lf\t%lf\t%lf\t%lf\t%lf\n", g, cent[3 * g], cent[3 * g + 1], cent[3 * g + 2],
                 euler[3 * g], euler[3 * g + 1], euler[3 * g + 2]);
     fclose(f);
+    return HZO_OK;
+}
+
+int hzo_count_structures(const char* inputs_dir, int* n) {
+    if (!inputs_dir || !n) return set_error(HZO_ERR_ARG, "hzo_count_structures: null argument");
+    *n = 0;
+    for (int s = 1;; ++s) {
+        char path[4096];
+        if (snprintf(path, sizeof(path), "%s/struc%d", inputs_dir, s) >= (int)sizeof(path))
+            return set_error(HZO_ERR_ARG, "hzo_count_structures: path too long");
+        struct stat st;
+        if (stat(path, &st) != 0 || !S_ISDIR(st.st_mode)) break;
+        *n = s;
+    }
     return HZO_OK;
 }
 

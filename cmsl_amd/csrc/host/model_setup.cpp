@@ -6,7 +6,11 @@
 //   hzo_interface_flag      main.cu:13-42
 //   hzo_init_polarization   init_conf.cu:14-66
 //   hzo_voronoi             voronoi.c:55-153 (layered periodic Voronoi)
+//   hzo_grain_boundary_cells  a seeded share of grain-boundary cells moved to
+//                           new grains (the dielectric boundary cells of the
+//                           512x512x64 workload; no counterpart in the reference)
 //   hzo_splitmix64          the seeded stream that replaces libc rand()/time() and cuRAND
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -47,6 +51,119 @@ double compute_p0(double alpha, double beta, double g, double gamma1, int transi
     const double Num = 2.0 * fabs(beta) + 2.0 * sqrt(disc);
     if (gamma1 == 0.0) return 1.0;
     return sqrt(Num / gamma1);
+}
+
+}  // namespace
+
+namespace hzo {
+// GPU labelling of every layer (kernels/voronoi.hip).
+int voronoi_label_gpu(int device, int nx, int ny, int nz, int n_layers, const int* goffs, const int* nls,
+                      const int* z0s, const int* spans, const double* cent, int ng, int32_t* omega_host);
+}  // namespace hzo
+
+namespace {
+
+// Layered periodic Voronoi (voronoi.c:55-153): per-layer grain counts, then
+// per layer the orientations and integer centroids (one seeded SplitMix64
+// stream), then the nearest-centroid labels -- on CPU threads (device < 0)
+// or on the GPU (the same labels bit for bit).
+int voronoi_impl(int device, int nx, int ny, int nz, int n_layers, const int* layer_counts, int ng,
+                 uint64_t seed, int32_t* omega, double* cent, double* euler) {
+    if (nx <= 0 || ny <= 0 || nz <= 0 || n_layers <= 0 || n_layers > 16 || !layer_counts || ng <= 0 ||
+        !omega || !cent || !euler)
+        return set_error(HZO_ERR_ARG, "hzo_voronoi: bad argument");
+    int sumz = 0;
+    for (int L = 0; L < n_layers; ++L) sumz += layer_counts[L];
+    if (sumz != nz) return set_error(HZO_ERR_DATA, "hzo_voronoi: layer_counts (%d) != nz (%d)", sumz, nz);
+    std::vector<int> ngl(n_layers);  // voronoi.c:59-66
+    int acc = 0;
+    for (int L = 0; L < n_layers; ++L) {
+        ngl[L] = (int)llround((double)layer_counts[L] / (double)sumz * (double)ng);
+        acc += ngl[L];
+    }
+    while (acc < ng) {
+        ngl[0]++;
+        acc++;
+    }
+    while (acc > ng) {
+        ngl[0]--;
+        acc--;
+    }
+    SplitMix rng(seed);
+    const double two_pi = 2.0 * acos(-1.0);
+    std::vector<int> goffs(n_layers), z0s(n_layers);
+    int goff = 0, z0 = 0;
+    for (int L = 0; L < n_layers; ++L) {
+        const int span = layer_counts[L], nL = ngl[L];
+        goffs[L] = goff;
+        z0s[L] = z0;
+        if (nL > 0) {
+            for (int k = 0; k < nL; ++k) {  // orientations uniform on SO(3) (Bunge angles)
+                const double u1 = rng.uniform(), u2 = rng.uniform(), u3 = rng.uniform();
+                euler[3 * (goff + k)] = two_pi * u1;
+                euler[3 * (goff + k) + 1] = acos(2.0 * u2 - 1.0);
+                euler[3 * (goff + k) + 2] = two_pi * u3;
+            }
+            for (int k = 0; k < nL; ++k) {  // integer centroids, voronoi.c:108-112
+                cent[3 * (goff + k)] = (double)(rng.next() % (uint64_t)nx);
+                cent[3 * (goff + k) + 1] = (double)(rng.next() % (uint64_t)ny);
+                cent[3 * (goff + k) + 2] = (double)(z0 + (int)(rng.next() % (uint64_t)span));
+            }
+        }
+        goff += nL;
+        z0 += span;
+    }
+    if (device >= 0)
+        return hzo::voronoi_label_gpu(device, nx, ny, nz, n_layers, goffs.data(), ngl.data(), z0s.data(), layer_counts,
+                                      cent, ng, omega);
+    for (int L = 0; L < n_layers; ++L) {
+        const int span = layer_counts[L], nL = ngl[L];
+        if (nL <= 0) continue;
+        const int zb = z0s[L], z1 = zb + span;
+        // nearest centroid with minimal image in x, y and within the slab in z
+        // (voronoi.c:118-153; strict '<' keeps the lowest index on ties)
+        const double* C = cent + 3 * goffs[L];
+        const int gl = goffs[L];
+        auto work = [&](int i_begin, int i_end) {
+            for (int i = i_begin; i < i_end; ++i)
+                for (int j = 0; j < ny; ++j)
+                    for (int h = zb; h < z1; ++h) {
+                        double rmin = 1e300;
+                        int nmin = 0;
+                        for (int k = 0; k < nL; ++k) {
+                            double dx = C[3 * k] - (double)i;
+                            if (dx > 0.5 * nx) dx -= nx;
+                            else if (dx < -0.5 * nx) dx += nx;
+                            double dy = C[3 * k + 1] - (double)j;
+                            if (dy > 0.5 * ny) dy -= ny;
+                            else if (dy < -0.5 * ny) dy += ny;
+                            const double hz = (double)h - (double)zb;
+                            const double czl = C[3 * k + 2] - (double)zb;
+                            double dz = czl - hz;
+                            if (dz > 0.5 * span) dz -= span;
+                            else if (dz < -0.5 * span) dz += span;
+                            const double r2 = dx * dx + dy * dy + dz * dz;
+                            if (r2 < rmin) {
+                                rmin = r2;
+                                nmin = k;
+                            }
+                        }
+                        omega[(long long)i + (long long)nx * ((long long)j + (long long)ny * h)] = gl + nmin;
+                    }
+        };
+        unsigned nt = std::thread::hardware_concurrency();
+        if (nt == 0) nt = 1;
+        if (nt > 16) nt = 16;
+        if ((long long)nx * ny * span * nL < 4000000LL) nt = 1;
+        std::vector<std::thread> pool;
+        const int chunk = (nx + (int)nt - 1) / (int)nt;
+        for (unsigned t = 0; t < nt; ++t) {
+            const int b = (int)t * chunk, e = std::min(nx, b + chunk);
+            if (b < e) pool.emplace_back(work, b, e);
+        }
+        for (auto& th : pool) th.join();
+    }
+    return HZO_OK;
 }
 
 }  // namespace
@@ -222,91 +339,59 @@ int hzo_init_polarization(int nx, int ny, int nz, const int32_t* omega, int ng,
 
 int hzo_voronoi(int nx, int ny, int nz, int n_layers, const int* layer_counts, int ng,
                 uint64_t seed, int32_t* omega, double* cent, double* euler) {
-    if (nx <= 0 || ny <= 0 || nz <= 0 || n_layers <= 0 || n_layers > 16 || !layer_counts || ng <= 0 ||
-        !omega || !cent || !euler)
-        return set_error(HZO_ERR_ARG, "hzo_voronoi: bad argument");
-    int sumz = 0;
-    for (int L = 0; L < n_layers; ++L) sumz += layer_counts[L];
-    if (sumz != nz) return set_error(HZO_ERR_DATA, "hzo_voronoi: layer_counts (%d) != nz (%d)", sumz, nz);
-    std::vector<int> ngl(n_layers);  // voronoi.c:59-66
-    int acc = 0;
-    for (int L = 0; L < n_layers; ++L) {
-        ngl[L] = (int)llround((double)layer_counts[L] / (double)sumz * (double)ng);
-        acc += ngl[L];
+    return voronoi_impl(-1, nx, ny, nz, n_layers, layer_counts, ng, seed, omega, cent, euler);
+}
+
+int hzo_voronoi_gpu(int device, int nx, int ny, int nz, int n_layers, const int* layer_counts, int ng,
+                    uint64_t seed, int32_t* omega, double* cent, double* euler) {
+    if (device < 0) return set_error(HZO_ERR_ARG, "hzo_voronoi_gpu: device < 0");
+    return voronoi_impl(device, nx, ny, nz, n_layers, layer_counts, ng, seed, omega, cent, euler);
+}
+
+int hzo_grain_boundary_cells(int nx, int ny, int nz, int32_t* omega, int ng, double fraction, uint64_t seed,
+                             int32_t* parent, int* ng_new) {
+    if (nx <= 0 || ny <= 0 || nz <= 0 || !omega || ng <= 0 || !parent || !ng_new || !(fraction >= 0.0) ||
+        fraction > 1.0)
+        return set_error(HZO_ERR_ARG, "hzo_grain_boundary_cells: bad argument");
+    const long long plane = (long long)nx * ny, N = plane * nz;
+    // boundary cells: a face neighbour (x, y periodic; z inside the film) of
+    // another grain
+    std::vector<long long> gb;
+    for (long long id = 0; id < N; ++id) {
+        const int i = (int)(id % nx), j = (int)((id / nx) % ny), k = (int)(id / plane);
+        const int g = omega[id];
+        const long long e = id - i + (i + 1 == nx ? 0 : i + 1), w = id - i + (i == 0 ? nx - 1 : i - 1);
+        const long long n = id + (long long)nx * ((j + 1 == ny ? 0 : j + 1) - j);
+        const long long s = id + (long long)nx * ((j == 0 ? ny - 1 : j - 1) - j);
+        bool b = omega[e] != g || omega[w] != g || omega[n] != g || omega[s] != g;
+        if (k + 1 < nz) b = b || omega[id + plane] != g;
+        if (k > 0) b = b || omega[id - plane] != g;
+        if (b) gb.push_back(id);
     }
-    while (acc < ng) {
-        ngl[0]++;
-        acc++;
-    }
-    while (acc > ng) {
-        ngl[0]--;
-        acc--;
-    }
+    // a seeded choice of round(fraction N) of them (all if fewer): the first
+    // entries of a partial Fisher-Yates shuffle by SplitMix64
+    const long long want = std::min((long long)gb.size(), (long long)llround(fraction * (double)N));
     SplitMix rng(seed);
-    const double two_pi = 2.0 * acos(-1.0);
-    int goff = 0, z0 = 0;
-    for (int L = 0; L < n_layers; ++L) {
-        const int span = layer_counts[L], nL = ngl[L];
-        if (nL <= 0) {
-            z0 += span;
-            continue;
-        }
-        for (int k = 0; k < nL; ++k) {  // orientations uniform on SO(3) (Bunge angles)
-            const double u1 = rng.uniform(), u2 = rng.uniform(), u3 = rng.uniform();
-            euler[3 * (goff + k)] = two_pi * u1;
-            euler[3 * (goff + k) + 1] = acos(2.0 * u2 - 1.0);
-            euler[3 * (goff + k) + 2] = two_pi * u3;
-        }
-        for (int k = 0; k < nL; ++k) {  // integer centroids, voronoi.c:108-112
-            cent[3 * (goff + k)] = (double)(rng.next() % (uint64_t)nx);
-            cent[3 * (goff + k) + 1] = (double)(rng.next() % (uint64_t)ny);
-            cent[3 * (goff + k) + 2] = (double)(z0 + (int)(rng.next() % (uint64_t)span));
-        }
-        // nearest centroid with minimal image in x, y and within the slab in z
-        // (voronoi.c:118-153; strict '<' keeps the lowest index on ties)
-        const int z1 = z0 + span;
-        const double* C = cent + 3 * goff;
-        auto work = [&](int i_begin, int i_end) {
-            for (int i = i_begin; i < i_end; ++i)
-                for (int j = 0; j < ny; ++j)
-                    for (int h = z0; h < z1; ++h) {
-                        double rmin = 1e300;
-                        int nmin = 0;
-                        for (int k = 0; k < nL; ++k) {
-                            double dx = C[3 * k] - (double)i;
-                            if (dx > 0.5 * nx) dx -= nx;
-                            else if (dx < -0.5 * nx) dx += nx;
-                            double dy = C[3 * k + 1] - (double)j;
-                            if (dy > 0.5 * ny) dy -= ny;
-                            else if (dy < -0.5 * ny) dy += ny;
-                            const double hz = (double)h - (double)z0;
-                            const double czl = C[3 * k + 2] - (double)z0;
-                            double dz = czl - hz;
-                            if (dz > 0.5 * span) dz -= span;
-                            else if (dz < -0.5 * span) dz += span;
-                            const double r2 = dx * dx + dy * dy + dz * dz;
-                            if (r2 < rmin) {
-                                rmin = r2;
-                                nmin = k;
-                            }
-                        }
-                        omega[(long long)i + (long long)nx * ((long long)j + (long long)ny * h)] = goff + nmin;
-                    }
-        };
-        unsigned nt = std::thread::hardware_concurrency();
-        if (nt == 0) nt = 1;
-        if (nt > 16) nt = 16;
-        if ((long long)nx * ny * span * nL < 4000000LL) nt = 1;
-        std::vector<std::thread> pool;
-        const int chunk = (nx + (int)nt - 1) / (int)nt;
-        for (unsigned t = 0; t < nt; ++t) {
-            const int b = (int)t * chunk, e = std::min(nx, b + chunk);
-            if (b < e) pool.emplace_back(work, b, e);
-        }
-        for (auto& th : pool) th.join();
-        goff += nL;
-        z0 = z1;
+    for (long long t = 0; t < want; ++t) {
+        const long long r = t + (long long)(rng.next() % (uint64_t)(gb.size() - t));
+        std::swap(gb[t], gb[r]);
     }
+    std::sort(gb.begin(), gb.begin() + want);
+    // the chosen cells of grain g move to a new grain (ids from ng on, in
+    // order of first appearance); parent[new - ng] = g
+    std::vector<int> slot(ng, -1);
+    int nn = 0;
+    for (long long t = 0; t < want; ++t) {
+        const int g = omega[gb[t]];
+        if (g < 0 || g >= ng) continue;
+        if (slot[g] < 0) {
+            slot[g] = nn;
+            parent[nn] = g;
+            ++nn;
+        }
+        omega[gb[t]] = ng + slot[g];
+    }
+    *ng_new = ng + nn;
     return HZO_OK;
 }
 

@@ -408,7 +408,7 @@ int cg32_solve(hzo_ctx* x, const hzo::Fp32Bufs& b32in, const hzo::StepConsts& c,
                                x->stream));
         HZO_HIP(hipStreamSynchronize(x->stream));
         if (x->st_host->done || launches >= maxl) break;
-        batch = last_iters / 8 > 2 ? last_iters / 8 : 2;
+        batch = last_iters / 8 > 8 ? last_iters / 8 : 8;  // follow-up rounds: at least 8 launches per poll
     }
     hzo::PcgState& s = *x->st_host;
     if (!s.done) s.status = HZO_PCG_MAXITER;
@@ -449,7 +449,7 @@ int pcg_solve(hzo_ctx* x, const hzo::Fp64Bufs& b, const hzo::Fp32Bufs& b32, cons
                                x->stream));
         HZO_HIP(hipStreamSynchronize(x->stream));
         if (x->st_host->done || pairs >= maxpairs) break;
-        batch = last_iters / 8 > 2 ? last_iters / 8 : 2;
+        batch = last_iters / 8 > 8 ? last_iters / 8 : 8;  // follow-up rounds: at least 8 launches per poll
     }
     hzo::PcgState& s = *x->st_host;
     if (!s.done) s.status = HZO_PCG_MAXITER;

@@ -56,14 +56,20 @@ def nondim(raw: dict, ref_type: int = L.HZO_FE):
     return rows, {"P0": sc[0], "E0": sc[1], "eps0_nd": sc[2], "alpha_ref": sc[3]}
 
 
-def voronoi(nx, ny, nz, ng, seed, layer_counts=None):
+def voronoi(nx, ny, nz, ng, seed, layer_counts=None, device=None):
+    """Layered periodic Voronoi structure; device = a GPU index labels the
+    voxels there (the same labels as the CPU labeller)."""
     layer_counts = list(layer_counts or [nz])
     lc = np.array(layer_counts, np.int32)
     omega = np.zeros(nx * ny * nz, np.int32)
     cent = np.zeros(3 * ng)
     eul = np.zeros(3 * ng)
-    L.call("hzo_voronoi", nx, ny, nz, len(layer_counts), L.ptr(lc), ng, C.c_uint64(seed),
-           L.ptr(omega), L.ptr(cent), L.ptr(eul))
+    if device is None:
+        L.call("hzo_voronoi", nx, ny, nz, len(layer_counts), L.ptr(lc), ng, C.c_uint64(seed),
+               L.ptr(omega), L.ptr(cent), L.ptr(eul))
+    else:
+        L.call("hzo_voronoi_gpu", int(device), nx, ny, nz, len(layer_counts), L.ptr(lc), ng, C.c_uint64(seed),
+               L.ptr(omega), L.ptr(cent), L.ptr(eul))
     return omega, cent.reshape(ng, 3), eul.reshape(ng, 3)
 
 
@@ -135,11 +141,14 @@ class FilmInputs:
 
 def make_film_inputs(nx, ny, nz, ng, fracs=(0.0, 1.0, 0.0), seed=1, del_h=0.5, del_t=0.01,
                      jitter=0.0, amp_fe_SI=0.01, raw=None, ref_type=L.HZO_FE, layer_counts=None,
-                     phase_seed=123456) -> FilmInputs:
+                     phase_seed=123456, label_device=None, gb_dielectric=0.0) -> FilmInputs:
     """Seeded film: Voronoi grains, phase assignment, Landau jitter, initial P.
 
     jitter: alpha, beta, gamma of each phase are multiplied by U(1-jitter, 1+jitter)
     (one draw per phase and coefficient).  amp_fe_SI: FE initial |P| bound in C/m^2.
+    label_device: a GPU index labels the Voronoi voxels there (same labels).
+    gb_dielectric: share of all cells, taken from grain-boundary cells, that
+    carry the dielectric phase (new grains with their parents' orientation).
     """
     raw = {q: dict(v) for q, v in (raw or BASE_RAW).items()}
     if jitter:
@@ -148,8 +157,17 @@ def make_film_inputs(nx, ny, nz, ng, fracs=(0.0, 1.0, 0.0), seed=1, del_h=0.5, d
             for t, k in enumerate(("alpha", "beta", "gamma")):
                 raw[q][k] = raw[q][k] * (1.0 - jitter + 2.0 * jitter * u[q, t])
     rows, scales = nondim(raw, ref_type)
-    omega, cent, eul = voronoi(nx, ny, nz, ng, seed, layer_counts)
+    omega, cent, eul = voronoi(nx, ny, nz, ng, seed, layer_counts, device=label_device)
     gphase = assign_phases(nx, ny, nz, omega, cent[:, 2].copy(), fracs, layer_counts, phase_seed)
+    if gb_dielectric > 0.0:
+        parent = np.zeros(ng, np.int32)
+        ng_new = C.c_int(0)
+        L.call("hzo_grain_boundary_cells", nx, ny, nz, L.ptr(omega), ng, float(gb_dielectric),
+               C.c_uint64(seed ^ 0x6B6B6B6B), L.ptr(parent), C.byref(ng_new))
+        par = parent[: ng_new.value - ng]
+        gphase = np.concatenate([gphase, np.full(par.size, L.HZO_DE, np.int32)])
+        cent = np.concatenate([cent, cent[par]])
+        eul = np.concatenate([eul, eul[par]])
     TR = euler_to_TR(eul)
     iface = interface_flag(nx, ny, nz, omega, gphase)
     P0 = init_polarization(nx, ny, nz, omega, gphase, amp_fe_SI / scales["P0"], seed + 7)
@@ -220,6 +238,44 @@ def write_raw_f64(path, v):
     """Raw f64 dump in voxel order (used for the potential)."""
     v = np.ascontiguousarray(v, np.float64)
     L.call("hzo_write_raw_f64", str(path).encode(), v.size, L.ptr(v))
+
+
+def count_structures(inputs_dir) -> int:
+    """Number of consecutive inputs_dir/strucN directories (main.cu:430-440)."""
+    n = C.c_int(0)
+    L.call("hzo_count_structures", str(inputs_dir).encode(), C.byref(n))
+    return n.value
+
+
+def write_structure(inputs_dir, s, nx, ny, nz, omega, cent, euler):
+    """Writes structure s of the multi-structure layout: inputs_dir/strucS/
+    gdata_poly.bin and gchar_poly.txt, and inputs_dir/gnumb_poly.txt."""
+    import os
+    d = os.path.join(str(inputs_dir), f"struc{s}")
+    os.makedirs(d, exist_ok=True)
+    omega = np.ascontiguousarray(omega, np.int32)
+    cent = np.ascontiguousarray(cent, np.float64).reshape(-1)
+    euler = np.ascontiguousarray(euler, np.float64).reshape(-1)
+    ng = cent.size // 3
+    L.call("hzo_write_gdata", os.path.join(d, "gdata_poly.bin").encode(), nx, ny, nz, L.ptr(omega))
+    L.call("hzo_write_gchar", os.path.join(d, "gchar_poly.txt").encode(), ng, L.ptr(cent), L.ptr(euler))
+    with open(os.path.join(str(inputs_dir), "gnumb_poly.txt"), "w") as fh:
+        fh.write(f"{ng}\n")
+
+
+def load_structure(inputs_dir, s, nx, ny, nz):
+    """Structure s of the multi-structure layout (grain_io.cu:10-99): omega
+    (C order), centroids (ng, 3), Euler angles (ng, 3)."""
+    import os
+    ng = C.c_int(0)
+    L.call("hzo_read_grain_count", os.path.join(str(inputs_dir), "gnumb_poly.txt").encode(), C.byref(ng))
+    d = os.path.join(str(inputs_dir), f"struc{s}")
+    omega = np.zeros(nx * ny * nz, np.int32)
+    L.call("hzo_read_gdata", os.path.join(d, "gdata_poly.bin").encode(), nx, ny, nz, L.ptr(omega), None)
+    cent = np.zeros(3 * ng.value)
+    eul = np.zeros(3 * ng.value)
+    L.call("hzo_read_gchar", os.path.join(d, "gchar_poly.txt").encode(), ng.value, L.ptr(cent), L.ptr(eul))
+    return omega, cent.reshape(-1, 3), eul.reshape(-1, 3)
 
 
 class Film:

@@ -321,6 +321,21 @@ int hzo_read_checkpoint(const char* path, int nx, int ny, int nz, int* peloop, i
 int hzo_write_raw_f64(const char* path, long long n, const double* v);
 /* Rotation matrices from Bunge Euler angles (grain_io.cu:102-126). */
 int hzo_euler_to_TR(int ng, const double* euler, double* TR);
+/* Dielectric grain-boundary cells (no counterpart in the reference; the
+ * large-film workload puts a dielectric Landau term on 1 % of the cells,
+ * in grain boundaries).  Boundary cells are those with a face neighbour of
+ * another grain (x, y periodic, z inside the film); round(fraction * N) of
+ * them (all if fewer), chosen by a seeded SplitMix64 shuffle, move in place
+ * to new grains ng, ng+1, ... (one per parent grain, in order of first
+ * appearance), parent[new - ng] = parent grain (capacity ng).  The caller
+ * gives the new grains the dielectric phase and the parents' orientations;
+ * the per-grain model then carries the per-cell phase exactly. */
+int hzo_grain_boundary_cells(int nx, int ny, int nz, int32_t* omega, int ng, double fraction, uint64_t seed,
+                             int32_t* parent, int* ng_new);
+/* Multi-structure layout (main.cu:430-440, README_changes:9-26): n = the
+ * number of consecutive directories inputs_dir/struc1, struc2, ...; each
+ * holds gchar_poly.txt and gdata_poly.bin, gnumb_poly.txt is in inputs_dir. */
+int hzo_count_structures(const char* inputs_dir, int* n);
 /* Writers of the generator formats (voronoi.c:162-223). */
 int hzo_write_gdata(const char* path, int nx, int ny, int nz, const int32_t* omega_c);
 int hzo_write_gchar(const char* path, int ng, const double* centroids, const double* euler);
@@ -331,6 +346,10 @@ int hzo_write_gchar(const char* path, int ng, const double* centroids, const dou
  * centroids[3*ng], euler[3*ng]. */
 int hzo_voronoi(int nx, int ny, int nz, int n_layers, const int* layer_counts,
                 int ng, uint64_t seed, int32_t* omega, double* centroids, double* euler);
+/* The same structure with the nearest-centroid labelling on GPU `device`
+ * (integer distances; labels equal to hzo_voronoi's bit for bit). */
+int hzo_voronoi_gpu(int device, int nx, int ny, int nz, int n_layers, const int* layer_counts, int ng,
+                    uint64_t seed, int32_t* omega, double* cent, double* euler);
 
 /* Non-dimensionalisation (main.cu:200-256, 268-274, 343-349).
  * ref_type selects the reference phase.  out_nd[3] indexed by phase;

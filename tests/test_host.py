@@ -282,3 +282,70 @@ def test_sweep_schedule_matches_shipped_sweep_file():
     p2 = F.sweep_params({"E0": 2.0e8, "P0": 1.0}, 3.0e8, 1.5e7, 800)
     E2, cnt2 = F.sweep_schedule(p2)
     assert cnt2[0] == 17600 and abs(E2[0] - 3.15) < 1e-9
+
+
+def test_multi_structure_layout_round_trip(tmp_path):
+    nx, ny, nz = 9, 7, 4
+    assert F.count_structures(tmp_path) == 0
+    made = []
+    for s in (1, 2, 3):
+        om, cent, eul = F.voronoi(nx, ny, nz, 5, 40 + s)
+        F.write_structure(tmp_path, s, nx, ny, nz, om, cent, eul)
+        made.append((om, cent, eul))
+    (tmp_path / "struc5").mkdir()  # a gap ends the count, as in the reference scan
+    assert F.count_structures(tmp_path) == 3
+    assert (tmp_path / "gnumb_poly.txt").read_text().split() == ["5"]
+    for s, (om, cent, eul) in enumerate(made, start=1):
+        om2, cent2, eul2 = F.load_structure(tmp_path, s, nx, ny, nz)
+        assert np.array_equal(om, om2)
+        # gchar holds "%lf" (6 decimals)
+        assert np.allclose(cent, cent2, atol=5e-7) and np.allclose(eul, eul2, atol=5e-7)
+
+
+def _boundary_mask(om, nx, ny, nz):
+    g = om.reshape(nz, ny, nx)
+    b = np.zeros_like(g, dtype=bool)
+    for ax, per in ((2, True), (1, True)):
+        for sh in (1, -1):
+            b |= np.roll(g, sh, axis=ax) != g
+    b[1:] |= g[1:] != g[:-1]
+    b[:-1] |= g[:-1] != g[1:]
+    return b.reshape(-1)
+
+
+def test_grain_boundary_cells_pick_a_seeded_share_of_boundary_cells():
+    nx, ny, nz, ng = 24, 20, 6, 9
+    om0, _, _ = F.voronoi(nx, ny, nz, ng, 5)
+    gb = _boundary_mask(om0, nx, ny, nz)
+    N = om0.size
+
+    def run(frac, seed):
+        om = om0.copy()
+        parent = np.zeros(ng, np.int32)
+        nn = ctypes.c_int(0)
+        L.call("hzo_grain_boundary_cells", nx, ny, nz, L.ptr(om), ng, frac, ctypes.c_uint64(seed), L.ptr(parent),
+               ctypes.byref(nn))
+        return om, parent[: nn.value - ng]
+
+    om, par = run(0.05, 1)
+    moved = om != om0
+    assert moved.sum() == round(0.05 * N) and np.all(gb[moved])           # only boundary cells, the asked share
+    assert np.array_equal(par[om[moved] - ng], om0[moved])                  # each new grain maps to its parent
+    assert len(set(par.tolist())) == par.size and om[moved].min() == ng     # one new grain per parent
+    om_again, _ = run(0.05, 1)
+    assert np.array_equal(om, om_again)
+    om_other, _ = run(0.05, 2)
+    assert not np.array_equal(om, om_other)
+    om_all, _ = run(0.99, 1)                                                # more than there are: all of them
+    assert np.array_equal(om_all != om0, gb)
+
+
+def test_film_inputs_with_dielectric_boundary_cells():
+    inp = H.make_film_inputs(24, 20, 6, 9, fracs=(0.3, 0.5, 0.2), seed=4, gb_dielectric=0.02)
+    base = H.make_film_inputs(24, 20, 6, 9, fracs=(0.3, 0.5, 0.2), seed=4)
+    moved = inp.omega != base.omega
+    assert moved.sum() == round(0.02 * inp.N)
+    new = np.unique(inp.omega[moved])
+    assert np.all(inp.grain_phase[new] == L.HZO_DE)
+    par = base.omega[moved]
+    assert np.array_equal(inp.TR.reshape(-1, 9)[inp.omega[moved]], base.TR.reshape(-1, 9)[par])
