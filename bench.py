@@ -274,6 +274,47 @@ def run_batched(args):
             "unconverged_solves": cnt["unconverged"], "t_inputs_s": t_inputs}
 
 
+def run_large(args):
+    """Large-film workload, the pure-HBM case: 512x512x64 film (16.8 M
+    cells), 1000 grains FE/AFE/DE 0.5/0.3/0.2, 1 % of the cells
+    (grain-boundary cells) dielectric, +-20 % Landau coefficients, triangle
+    +-4 MV/cm over 5000 steps; args.large_steps timed steps after 2 warm-up
+    steps in each precision, then one step with kernel timing."""
+    import cmsl_amd as H
+    from cmsl_amd import _lib as L
+
+    inp = H.make_film_inputs(512, 512, 64, 1000, fracs=(0.3, 0.5, 0.2), seed=41, jitter=0.2, label_device=0,
+                             gb_dielectric=0.01)
+    S, W = args.large_steps, 2
+    Ez = inp.field_nd(H.triangle_wave(5000, 4.0, W + S + 1))
+    out = {"workload": "512x512x64 film, 1000 grains FE/AFE/DE 0.5/0.3/0.2, 1%% dielectric grain-boundary cells, "
+                       "+-20%% Landau, triangle +-4 MV/cm / 5000 steps, PCG ||r||<=1e-8||b||; "
+                       "%d warm-up + %d timed steps" % (W, S)}
+    for name, prec in (("fp32", L.PREC_FP32_REFINED), ("fp64", L.PREC_FP64)):
+        film = H.Film(inp, precision=prec, pcg_rule=L.PCG_RULE_RELRES, tol_rel=1e-8, tol_abs=1e-30,
+                      refine_every=20)
+        film.run(Ez[:W])
+        film.synchronize()
+        t0 = time.perf_counter()
+        its = film.run(Ez[W:W + S])
+        film.synchronize()
+        dt = time.perf_counter() - t0
+        film.set_timing(True)
+        film.run(Ez[W + S:])
+        kt = film.kernel_times()
+        film.close()
+        entry = {"value": inp.N * S / dt / 1e6, "unit": "Mcell-steps/s", "ms_per_step": dt / S * 1e3,
+                 "pcg_iters_mean": float(np.mean(its)), "kernel_ms": {k: v[1] for k, v in kt.items()}}
+        if name == "fp32":
+            ms = kt["pcg_fused"][1]
+            achieved = algorithmic_bytes_per_cell("pcg_fused", "fp32") * inp.N / (ms * 1e-3) / 1e9
+            entry["roofline_pcg_fused"] = {"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                           "frac": achieved / HBM_PEAK_GBS}
+        out[name] = entry
+    out["speedup_fp32_refined_vs_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -286,6 +327,8 @@ def main():
     ap.add_argument("--adj-steps", type=int, default=500, help="steps of the reverse-mode workload")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--batch-films", type=int, default=64, help="films of the batched workload (0: skip it)")
+    ap.add_argument("--large-steps", type=int, default=4,
+                    help="timed steps of the 512x512x64 large-film workload (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=8)
     ap.add_argument("--batch-warmup", type=int, default=2)
     ap.add_argument("--no-loop", action="store_true", help="skip the full-loop FP32-vs-FP64 error run")
@@ -349,6 +392,7 @@ def main():
     adj = None if args.no_adjoint else run_adjoint(args)
     loop = None if args.no_loop else run_loop_errors(inp)
     bat = run_batched(args) if args.batch_films > 0 else None
+    large = run_large(args) if args.large_steps > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -386,6 +430,7 @@ def main():
         "adjoint": adj,
         "fp32_vs_fp64_loop": loop,
         "batched": bat,
+        "large_film": large,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_cell": bpc, "launch_ms": ms, "launches_timed": cnt},

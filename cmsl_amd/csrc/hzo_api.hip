@@ -591,10 +591,10 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
     x->g.N = N;
     // Tile depth (z planes per 64x8 workgroup) unless the caller fixes it;
     // the inner-product grouping, hence the last bits of the dot products,
-    // follows the tiling.  Single-reduction FP32 CG (z-streaming, no register
-    // growth with depth): the deepest of {8, 4} that still gives at least 512
-    // workgroups, else 2 -- deeper tiles mean fewer partial sums and fewer
-    // z-ghost re-reads.  Two-kernel FP32 PCG: the smallest depth giving at
+    // follows the tiling.  Single-reduction FP32 CG (z-streaming): the deepest
+    // of {8, 4} that still gives at least 512 workgroups, else 2 -- deeper
+    // tiles mean fewer partial sums and fewer z-ghost re-reads -- and 16 when
+    // depth 8 would leave more than 2048 partial sums.  Two-kernel FP32 PCG: the smallest depth giving at
     // most 512 workgroups (one partial per consuming thread, two workgroups
     // per CU), else 8.  FP64: the smallest of {2, 4} giving at most 1024
     // workgroups, else 4 (its depth-8 tiles exceed 128 VGPRs).
@@ -610,6 +610,9 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
                     kz = cand;
                     break;
                 }
+            // very large films: depth 16 keeps the partial sums at most
+            // 2048 (four per consuming thread) instead of a reduction loop
+            if (kz == 8 && txy * ((cfg->nz + 7) / 8) > 2048 && txy * ((cfg->nz + 15) / 16) >= 512) kz = 16;
         } else if (kz == 0) {
             kz = f32 ? 8 : 4;
             const long long cap = f32 ? 512 : 1024;

@@ -374,10 +374,10 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
 }
 
 // Waves per SIMD the register budget is set for: 8 (4 workgroups per CU,
-// <= 64 VGPRs) at two planes per tile, 6 at four and eight, 2 at sixteen.
+// <= 64 VGPRs) at two planes per tile, 6 at four and eight, 4 at sixteen.
 // The batch kernel, which also holds its film's descriptor, is set for 5
 // (4 at depth 8).
-constexpr int cg32_waves(int kz) { return kz >= 16 ? 2 : (kz <= 2 ? 8 : 6); }
+constexpr int cg32_waves(int kz) { return kz >= 16 ? 4 : (kz <= 2 ? 8 : 6); }
 
 template <int KZ, int PM>
 __global__ __launch_bounds__(NT, cg32_waves(KZ)) void k_cg32(CG32Bufs b, StepConsts c, int nb, int ps, int t) {
