@@ -32,10 +32,6 @@
 
 namespace hzo {
 
-struct Bcast32 {
-    double alpha, beta;
-    int flag;  // 0 work, 1 done, 2 residual replacement
-};
 
 __device__ __forceinline__ float row_Ax32(unsigned code, const HPair<float>* sHP, float ih2, float xc, float xe,
                                           float xw, float xn, float xs, float xu, float xd) {
@@ -136,8 +132,7 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
     const int ip = (t - 1) & 1, io = t & 1;
     __shared__ HPair<float> sHP[64];
     __shared__ float pl[3][HY][HX];  // u of planes q % 3 (centre + x-y halo)
-    __shared__ RawLDS<3> sraw;
-    __shared__ Bcast32 bc;
+    __shared__ WaveLDS<3> sred, spart;  // previous launch's totals; this launch's partials
     const int nz = c.nz;
     const int tid = linear_tid();
 
@@ -214,34 +209,28 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
     issue(0);
     issue(1);
 
-    stage_partials<3, PM>(P, sraw);
+    // The previous launch's totals: wave sums through LDS, then every wave
+    // adds the 8 slots and runs the (deterministic) loop logic itself, so no
+    // broadcast barrier follows.
+    if (tid < 64) sHP[tid] = hpv;
+    wave_partials<3, PM>(P, sred);
     __syncthreads();
-    if (tid < 64) {
-        sHP[tid] = hpv;
-        double tot[3];
-        wave0_total<3>(sraw, tot);
-        double alpha = 0.0, beta = 0.0;
-        const int mode = cg_decide(s, tot, alpha, beta);
-        if (mode == 2) {
-            s.kind = K_RESID;
-        } else if (mode == 3) {
-            s.kind = K_WCOMP;
-        } else if (mode == 0) {
-            s.iters += 1;
-            s.pbuf ^= 1;
-            s.kind = K_ITER;
-        }
-        // handed on at once: nothing later in the launch reads the block, so
-        // it need not stay live in registers through the tile work
-        pass_state(s, b.st + io, T.lead);
-        if (tid == 0) {
-            bc.alpha = alpha;
-            bc.beta = beta;
-            bc.flag = mode;
-        }
+    double tot[3];
+    wave_lds_total<3>(sred, tot);
+    double alpha_d = 0.0, beta_d = 0.0;
+    const int mode = cg_decide(s, tot, alpha_d, beta_d);
+    if (mode == 2) {
+        s.kind = K_RESID;
+    } else if (mode == 3) {
+        s.kind = K_WCOMP;
+    } else if (mode == 0) {
+        s.iters += 1;
+        s.pbuf ^= 1;
+        s.kind = K_ITER;
     }
-    __syncthreads();
-    const int mode = bc.flag;
+    // handed on at once: nothing later in the launch reads the block, so it
+    // need not stay live in registers through the tile work
+    pass_state(s, b.st + io, T.lead);
     if (mode == 1) return;
     double acc[3] = {0.0, 0.0, 0.0};
     if (mode == 2) {
@@ -303,8 +292,8 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
         // streamed bottom to top: plane q's update and u, then the stencil of
         // plane q - 1, whose z neighbours are u of planes q - 2 and q.
         const bool iter = mode == 0;
-        const float alpha = (float)bc.alpha, beta = (float)bc.beta;
-        const bool fresh = bc.beta == 0.0;  // first iteration of a solve: no previous p, s
+        const float alpha = (float)alpha_d, beta = (float)beta_d;
+        const bool fresh = beta_d == 0.0;  // first iteration of a solve: no previous p, s
         const unsigned o_r2 = (cur ^ 1u) * seg, o_s2 = (cur ^ 1u) * seg;
         // w goes to the other buffer after an iteration, in place after WCOMP
         const unsigned o_wo = iter ? (3u - cur) * seg : o_w;
@@ -370,7 +359,7 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
             }
         }
     }
-    write_block_partials<3>(acc, io ? b.part[1] : b.part[0], ps, T.blk, sraw);
+    write_block_partials_w<3>(acc, io ? b.part[1] : b.part[0], ps, T.blk, spart);
 }
 
 // Waves per SIMD the register budget is set for: 8 (4 workgroups per CU,

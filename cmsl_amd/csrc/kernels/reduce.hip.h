@@ -140,4 +140,62 @@ __device__ __forceinline__ void stage_partials(const Partials<NQ, PM>& P, RawLDS
     }
 }
 
+
+// Workgroup totals by waves (512 threads = 8 waves): each wave's DPP sum of
+// one value per lane goes to slot [q][wave]; after a barrier the 8 slots
+// are added in wave order.  No 512-entry staging and no single-wave
+// bottleneck; every thread that adds the slots gets the same bits.
+template <int NQ>
+struct WaveLDS {
+    double v[NQ][8];
+};
+
+// Consuming side: thread sums of the PM partials (in order), then wave sums
+// into W.  Call with the whole workgroup active, then barrier, then
+// wave_lds_total.
+template <int NQ, int PM>
+__device__ __forceinline__ void wave_partials(const Partials<NQ, PM>& P, WaveLDS<NQ>& W) {
+    const int tid = linear_tid();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double a = P.v[q][0];
+#pragma unroll
+        for (int m = 1; m < (PM > 0 ? PM : 1); ++m) a += P.v[q][m];
+        a = wave_sum(a);
+        if ((tid & 63) == 0) W.v[q][tid >> 6] = a;
+    }
+}
+
+template <int NQ>
+__device__ __forceinline__ void wave_lds_total(const WaveLDS<NQ>& W, double (&tot)[NQ]) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double t = W.v[q][0];
+#pragma unroll
+        for (int w = 1; w < 8; ++w) t += W.v[q][w];
+        tot[q] = t;
+    }
+}
+
+// Producing side: workgroup sum of NQ per-thread values, stored by thread 0
+// at out[q*stride + blk] (one barrier inside; W must not be read by anyone
+// else in this launch).
+template <int NQ>
+__device__ __forceinline__ void write_block_partials_w(const double (&v)[NQ], double* out, int stride, int blk,
+                                                       WaveLDS<NQ>& W) {
+    const int tid = linear_tid();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const double a = wave_sum(v[q]);
+        if ((tid & 63) == 0) W.v[q][tid >> 6] = a;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double tot[NQ];
+        wave_lds_total<NQ>(W, tot);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) out[(long long)q * stride + blk] = tot[q];
+    }
+}
+
 }  // namespace hzo
