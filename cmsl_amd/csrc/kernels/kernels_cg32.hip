@@ -129,9 +129,13 @@ struct Plane32 {
 template <int KZ, int PM>
 __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c, int nb, int ps, int t,
                                           const TileId& T) {
+    static_assert(KZ % 2 == 0, "the plane loop closes its stencil pairs on an odd last plane");
     const int ip = (t - 1) & 1, io = t & 1;
     __shared__ HPair<float> sHP[64];
-    __shared__ float pl[3][HY][HX];  // u of planes q % 3 (centre + x-y halo)
+    // u of plane q (centre + x-y halo) in buffer q % 6: the stencils of two
+    // planes follow each barrier, and a buffer is rewritten only after the
+    // barrier that ends its last reads
+    __shared__ float pl[6][HY][HX];
     __shared__ WaveLDS<3> sred, spart;  // previous launch's totals; this launch's partials
     const int nz = c.nz;
     const int tid = linear_tid();
@@ -330,32 +334,37 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
                 acc[2] += (double)rn * (double)rn;
             }
             if (interior) {
-                pl[q % 3][ty + 1][tx + 1] = u[q];
+                pl[q % 6][ty + 1][tx + 1] = u[q];
                 float rhn = L[q].rh;
                 if (iter) {
                     const float shn = fresh ? L[q].wh : L[q].wh + beta * L[q].sh;
                     rhn = L[q].rh - alpha * shn;
                 }
-                if (tp.has_h) pl[q % 3][tp.hy][tp.hx] = rhn * L[q].dh;
+                if (tp.has_h) pl[q % 6][tp.hy][tp.hx] = rhn * L[q].dh;
             }
-            // B(q - 1): stencil of plane q - 1
-            if (q >= 2) {
+            // B(q - 2), B(q - 1): stencils of planes q - 2 and q - 1 after
+            // every second plane (KZ is even, so the last plane q = KZ + 1 is
+            // odd and closes the loop)
+            if (q >= 3 && (q & 1)) {
                 __syncthreads();
-                const int m = q - 1, km = k - 1;
-                const float xc = u[m];
-                const unsigned code = cc[m];
-                float xu = u[m + 1], xd = u[m - 1];
-                if (c.zbc != 0) {
-                    if (km == 0) xd = xu;
-                    else if (km == nz - 1) xu = xd;
+#pragma unroll
+                for (int m = q - 2; m <= q - 1; ++m) {
+                    const int km = k0 - 1 + m;
+                    const float xc = u[m];
+                    const unsigned code = cc[m];
+                    float xu = u[m + 1], xd = u[m - 1];
+                    if (c.zbc != 0) {
+                        if (km == 0) xd = xu;
+                        else if (km == nz - 1) xu = xd;
+                    }
+                    const float(*pm)[HX] = pl[m % 6];
+                    const float qs = row_Ax32(code, sHP, ih2, xc, pm[ty + 1][tx + 2], pm[ty + 1][tx],
+                                              pm[ty + 2][tx + 1], pm[ty][tx + 1], xu, xd);
+                    const float wv = (code & SC_IDENTITY) ? op_identity(xc, km, c) : qs;
+                    const bool okm = tp.live && km < nz;
+                    bstore(RA, okm ? cv : aend, o_wo + so[m] * pb4, wv);
+                    if (okm) acc[1] += (double)wv * (double)xc;
                 }
-                const float(*pm)[HX] = pl[m % 3];
-                const float qs = row_Ax32(code, sHP, ih2, xc, pm[ty + 1][tx + 2], pm[ty + 1][tx], pm[ty + 2][tx + 1],
-                                          pm[ty][tx + 1], xu, xd);
-                const float wv = (code & SC_IDENTITY) ? op_identity(xc, km, c) : qs;
-                const bool okm = tp.live && km < nz;
-                bstore(RA, okm ? cv : aend, o_wo + so[m] * pb4, wv);
-                if (okm) acc[1] += (double)wv * (double)xc;
             }
         }
     }
