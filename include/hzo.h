@@ -82,7 +82,8 @@ typedef struct hzo_config {
     double tol_abs, tol_rel;
     int max_iter;
     int refine_every;    /* FP32-refined: FP64 residual recompute period (iterations) */
-    int tile_kz;         /* z planes per workgroup of the stencil kernels: 2, 4, 8; 0 = automatic */
+    int tile_kz;         /* z planes per workgroup of the stencil kernels: 2, 4, 8 (16: single-
+                            reduction CG only); 0 = automatic */
     int cg_fused;        /* FP32-refined: 1 = single-reduction CG, one fused kernel per iteration
                             (Chronopoulos-Gear); 0 = the two-kernel PCG of the FP64 form.
                             Films of 2^27 voxels or more always run the two-kernel PCG. */
