@@ -56,6 +56,7 @@ struct hzo_ctx {
     double* lut = nullptr;
     double* part = nullptr;       // 2 parity slots x RED_MAXQ x ps (zero tail beyond nb)
     int ps = 0;                   // stride of one partial array
+    double* fold = nullptr;       // more than 4 x 512 partials: 2 x RED_MAXQ x 512 folded totals (zero tail)
     double* tdgl_part = nullptr;  // TDGL_Q x nb
     double* diag_part = nullptr;  // DIAG_Q x nb
     hzo::PcgState* st = nullptr;  // [2]
@@ -114,6 +115,16 @@ double harm(double a, double b) {
     return (2.0 * a * b) / denom;
 }
 
+// The partials the next launch reads: the arrays themselves, or (films with
+// more than 4 x 512 workgroups) their totals folded into 512-entry arrays.
+template <class B>
+void set_pin(hzo_ctx* x, B& b) {
+    for (int q = 0; q < 2; ++q)
+        b.pin[q] = x->fold ? x->fold + (size_t)q * hzo::RED_MAXQ * hzo::NT : b.part[q];
+    b.nb_in = x->fold ? 1 : x->nb;
+    b.ps_in = x->fold ? hzo::NT : x->ps;
+}
+
 hzo::Fp64Bufs bufs(hzo_ctx* x) {
     hzo::Fp64Bufs b;
     b.Pold = x->P[x->cur];
@@ -130,6 +141,7 @@ hzo::Fp64Bufs bufs(hzo_ctx* x) {
     b.lut = x->lut;
     b.part[0] = x->part;
     b.part[1] = x->part + (size_t)hzo::RED_MAXQ * x->ps;
+    set_pin(x, b);
     b.st = x->st;
     b.tdgl_part = x->tdgl_part;
     return b;
@@ -154,9 +166,19 @@ hzo::Fp32Bufs bufs32(hzo_ctx* x) {
     b.lut32 = x->lut32;
     b.part[0] = x->part;
     b.part[1] = x->part + (size_t)hzo::RED_MAXQ * x->ps;
+    set_pin(x, b);
     b.st = x->st;
     b.tdgl_part = x->tdgl_part;
     return b;
+}
+
+// After a launch that wrote the partials of parity t & 1 (the setup is t =
+// 0): fold them when the film has more than a consuming thread loads.
+void fold_after(hzo_ctx* x, hipStream_t s, int t) {
+    if (!x->fold) return;
+    const size_t q = (size_t)(t & 1);
+    hzo::launch_fold_partials(s, x->part + q * hzo::RED_MAXQ * x->ps, x->nb, x->ps,
+                              x->fold + q * hzo::RED_MAXQ * hzo::NT);
 }
 
 template <class F>
@@ -377,6 +399,10 @@ hzo::CG32Bufs cg_bufs(hzo_ctx* x, const hzo::Fp32Bufs& b32) {
     cb.lut32 = x->lut32;
     cb.part[0] = b32.part[0];
     cb.part[1] = b32.part[1];
+    cb.pin[0] = b32.pin[0];
+    cb.pin[1] = b32.pin[1];
+    cb.nb_in = b32.nb_in;
+    cb.ps_in = b32.ps_in;
     cb.st = b32.st;
     return cb;
 }
@@ -392,6 +418,7 @@ int cg32_solve(hzo_ctx* x, const hzo::Fp32Bufs& b32in, const hzo::StepConsts& c,
     const hzo::CG32Bufs cb = cg_bufs(x, b32);
     int rc = timed(x, 1, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_setup_f32(x->stream, x->g, c, b32, init, rhs, ev); });
     if (rc) return rc;
+    fold_after(x, x->stream, 0);
     // launches: one WCOMP, one per iteration, two per residual replacement
     // at most three launches per iteration (replacement, re-form, iterate)
     const long long maxl = 3LL * init.max_iter + 8;
@@ -403,6 +430,7 @@ int cg32_solve(hzo_ctx* x, const hzo::Fp32Bufs& b32in, const hzo::StepConsts& c,
             const int tt = ++t;
             rc = timed(x, 4, [&](const hzo::LaunchEv& ev) { hzo::launch_cg32(x->stream, x->g, c, cb, tt, ev); });
             if (rc) return rc;
+            fold_after(x, x->stream, tt);
         }
         HZO_HIP(hipMemcpyAsync(x->st_host, x->st + (t & 1), sizeof(hzo::PcgState), hipMemcpyDeviceToHost,
                                x->stream));
@@ -422,6 +450,7 @@ int pcg_solve(hzo_ctx* x, const hzo::Fp64Bufs& b, const hzo::Fp32Bufs& b32, cons
     int rc = x->fp32 ? timed(x, 1, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_setup_f32(x->stream, x->g, c, b32, init, rhs, ev); })
                      : timed(x, 1, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_setup_f64(x->stream, x->g, c, b, init, rhs, ev); });
     if (rc) return rc;
+    fold_after(x, x->stream, 0);
     // One (dir, update) launch pair per CG iteration; the pair that follows
     // the last iteration finishes its inner products and raises `done`.  The
     // FP32-refined form also spends pairs on residual replacements, so up to
@@ -437,13 +466,16 @@ int pcg_solve(hzo_ctx* x, const hzo::Fp64Bufs& b, const hzo::Fp32Bufs& b32, cons
             if (x->fp32) {
                 rc = timed(x, 2, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_dir_f32(x->stream, x->g, c, b32, ta, ev); });
                 if (rc) return rc;
+                fold_after(x, x->stream, ta);
                 rc = timed(x, 3, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_update_f32(x->stream, x->g, c, b32, tb, ev); });
             } else {
                 rc = timed(x, 2, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_dir_f64(x->stream, x->g, c, b, ta, ev); });
                 if (rc) return rc;
+                fold_after(x, x->stream, ta);
                 rc = timed(x, 3, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_update_f64(x->stream, x->g, c, b, tb, ev); });
             }
             if (rc) return rc;
+            fold_after(x, x->stream, tb);
         }
         HZO_HIP(hipMemcpyAsync(x->st_host, x->st + (t & 1), sizeof(hzo::PcgState), hipMemcpyDeviceToHost,
                                x->stream));
@@ -492,6 +524,7 @@ void free_all(hzo_ctx* x) {
     (void)hipFree(x->ptab);
     (void)hipFree(x->lut);
     (void)hipFree(x->part);
+    (void)hipFree(x->fold);
     (void)hipFree(x->tdgl_part);
     (void)hipFree(x->diag_part);
     (void)hipFree(x->st);
@@ -702,6 +735,7 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
     HZO_ALLOC(x->ptab, sizeof(hzo::PhaseC) * hzo::PTAB_ENTRIES);
     HZO_ALLOC(x->lut, sizeof(double) * hzo::LUT_DOUBLES);
     HZO_ALLOC(x->part, sizeof(double) * 2 * hzo::RED_MAXQ * (size_t)x->ps);
+    if (x->nb > 4 * hzo::NT) HZO_ALLOC(x->fold, sizeof(double) * 2 * hzo::RED_MAXQ * hzo::NT);
     HZO_ALLOC(x->tdgl_part, sizeof(double) * hzo::TDGL_Q * (size_t)x->nb);
     HZO_ALLOC(x->diag_part, sizeof(double) * hzo::DIAG_Q * (size_t)x->nb);
     HZO_ALLOC(x->st, sizeof(hzo::PcgState) * 2);
@@ -709,6 +743,8 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
     if (hipHostMalloc((void**)&x->st_host, sizeof(hzo::PcgState), hipHostMallocDefault) != hipSuccess)
         return fail(set_error(HZO_ERR_NOMEM, "hzo_create: pinned host allocation failed"));
     if (hipMemset(x->part, 0, sizeof(double) * 2 * hzo::RED_MAXQ * (size_t)x->ps) != hipSuccess)
+        return fail(set_error(HZO_ERR_HIP, "hzo_create: memset failed"));
+    if (x->fold && hipMemset(x->fold, 0, sizeof(double) * 2 * hzo::RED_MAXQ * hzo::NT) != hipSuccess)
         return fail(set_error(HZO_ERR_HIP, "hzo_create: memset failed"));
     if (hipMemset(x->tdgl_part, 0, sizeof(double) * hzo::TDGL_Q * (size_t)x->nb) != hipSuccess)
         return fail(set_error(HZO_ERR_HIP, "hzo_create: memset failed"));
@@ -1359,6 +1395,10 @@ int group_tables(hzo_batch* B, hzo_batch::Group& G, int kz) {
             F.b[q] = film_bufs32(x, (x->cur + q) & 1);
             F.b[q].part[0] = pa;
             F.b[q].part[1] = pa + (size_t)hzo::RED_MAXQ * G.ps;
+            F.b[q].pin[0] = F.b[q].part[0];
+            F.b[q].pin[1] = F.b[q].part[1];
+            F.b[q].nb_in = nb[j];
+            F.b[q].ps_in = G.ps;
             F.b[q].tdgl_part = tp;
         }
         F.cg.va = x->cg_arena[0];
@@ -1371,6 +1411,10 @@ int group_tables(hzo_batch* B, hzo_batch::Group& G, int kz) {
         F.cg.lut32 = x->lut32;
         F.cg.part[0] = pa;
         F.cg.part[1] = pa + (size_t)hzo::RED_MAXQ * G.ps;
+        F.cg.pin[0] = F.cg.part[0];
+        F.cg.pin[1] = F.cg.part[1];
+        F.cg.nb_in = nb[j];
+        F.cg.ps_in = G.ps;
         F.cg.st = x->st;
         F.c = x->c;
         F.c.kz = kz;
@@ -1448,7 +1492,10 @@ void group_cg_launches(hzo_batch* B, hzo_batch::Group& G, int K) {
     if (G.single >= 0) {
         hzo_ctx* x = B->films[G.single];
         const hzo::CG32Bufs cb = cg_bufs(x, bufs32(x));
-        for (int t = 1; t <= K; ++t) hzo::launch_cg32(B->stream, x->g, x->c, cb, t);
+        for (int t = 1; t <= K; ++t) {
+            hzo::launch_cg32(B->stream, x->g, x->c, cb, t);
+            fold_after(x, B->stream, t);
+        }
     } else {
         for (int t = 1; t <= K; ++t)
             hzo::launch_cg32_batch(B->stream, G.ntiles, B->kz, G.pm, G.dfilms, G.dtiles, t);
@@ -1527,6 +1574,7 @@ int group_step(hzo_batch* B, hzo_batch::Group& G, const double* Ez, int s, bool 
     if (G.single >= 0) {
         hzo_ctx* x = B->films[G.single];
         hzo::launch_pcg_setup_f32(B->stream, x->g, x->c, bufs32(x), pcg_init(x), nullptr);
+        fold_after(x, B->stream, 0);
     } else {
         hzo::launch_setup_f32_batch(B->stream, G.ntiles, G.dfilms, G.dtiles, (s + 1) & 1);
     }

@@ -112,6 +112,8 @@ struct Fp64Bufs {
     const PhaseC* ptab;
     const double* lut;
     double* part[2];    // PCG partials by launch parity, RED_MAXQ * nb each
+    const double* pin[2];  // partials a launch reads, by parity: part[] itself, or their folded totals
+    int nb_in, ps_in;      // count and stride of pin (1 and 512 when folded)
     PcgState* st;       // [2] control blocks by launch parity
     double* tdgl_part;  // TDGL_Q * nb: grain-frame z sums of the last TDGL call
 };
@@ -137,6 +139,8 @@ struct Fp32Bufs {
     const double* lut;    // FP64 harm + dinv (residual replacement)
     const float* lut32;   // FP32 harm + dinv (CG iteration)
     double* part[2];
+    const double* pin[2];  // partials a launch reads, by parity: part[] itself, or their folded totals
+    int nb_in, ps_in;      // count and stride of pin (1 and 512 when folded)
     PcgState* st;
     double* tdgl_part;
 };
@@ -156,6 +160,8 @@ struct CG32Bufs {
     const double* lut;
     const float* lut32;
     double* part[2];
+    const double* pin[2];  // partials a launch reads, by parity: part[] itself, or their folded totals
+    int nb_in, ps_in;      // count and stride of pin (1 and 512 when folded)
     PcgState* st;
 };
 
@@ -217,6 +223,10 @@ void launch_setup_f32_batch(hipStream_t s, int ntiles, const BatchFilm* films, c
 void launch_diag_f32_batch(hipStream_t s, int ntiles, const BatchFilm* films, const int2* tiles, int parity);
 void launch_cg32_batch(hipStream_t s, int ntiles, int kz, int pm, const BatchFilm* films, const int2* tiles, int t);
 void launch_batch_set_ez(hipStream_t s, BatchFilm* films, const double* ez, int n);
+// Sum of each of the RED_MAXQ partial arrays (nb entries, stride ps) into
+// out[q*NT] (the rest of out stays zero), one workgroup: for films whose
+// partial sums outnumber what a consuming thread loads directly.
+void launch_fold_partials(hipStream_t s, const double* in, int nb, int ps, double* out);
 // info: done, iters, status, refinements per film
 void launch_batch_done(hipStream_t s, const BatchFilm* films, int n, int t, int* info);
 void launch_batch_sum_pmz(hipStream_t s, const BatchFilm* films, int n, double* m);

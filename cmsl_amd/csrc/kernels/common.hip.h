@@ -19,7 +19,8 @@ inline int part_stride_impl(int nb) {
 }
 
 // Launch of a PCG kernel template <int KZ, int PM, EXTRA...> with the
-// instance that matches the grid's tile depth and partial layout.  `X` is the
+// instance that matches the grid's tile depth and the layout of the partials
+// it reads (nbin of them).  `X` is the
 // kernel's trailing template argument list (or nothing).
 #define HZO_PCG_LAUNCH_PM(kern, KZ, pm, gr, bl, s, ...)                                   \
     switch (pm) {                                                                         \
@@ -28,9 +29,9 @@ inline int part_stride_impl(int nb) {
         case 4: hipExtLaunchKernelGGL((kern<KZ, 4 HZO_X>), gr, bl, 0, s, ev.start, ev.stop, 0, __VA_ARGS__); break;  \
         default: hipExtLaunchKernelGGL((kern<KZ, 0 HZO_X>), gr, bl, 0, s, ev.start, ev.stop, 0, __VA_ARGS__); break; \
     }
-#define HZO_PCG_LAUNCH_X(kern, g, s, ...)                                                 \
+#define HZO_PCG_LAUNCH_X(kern, g, nbin, s, ...)                                           \
     do {                                                                                  \
-        const int pm_ = part_pm(blocks_for_impl(g));                                      \
+        const int pm_ = part_pm(nbin);                                                    \
         const dim3 gr_ = grid3(g), bl_(BX, BY, 1);                                        \
         switch ((g).kz) {                                                                 \
             case 2: HZO_PCG_LAUNCH_PM(kern, 2, pm_, gr_, bl_, s, __VA_ARGS__); break;     \

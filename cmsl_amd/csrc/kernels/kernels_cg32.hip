@@ -153,7 +153,7 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
     const float ih2 = (float)c.inv_h2;
     const HPair<float> hpv = load_hpair(b.lut32);
     Partials<3, PM> P;
-    load_partials<3, PM>(ip ? b.part[1] : b.part[0], nb, ps, P);
+    load_partials<3, PM>(ip ? b.pin[1] : b.pin[0], b.nb_in, b.ps_in, P);
     // plane q of the tile's window is z = so[q]: q = 0 and KZ + 1 are the
     // ghost planes below and above (clamped at the film's faces)
     unsigned so[KZ + 2];
@@ -420,12 +420,12 @@ void launch_cg32_batch(hipStream_t s, int ntiles, int kz, int pm, const BatchFil
 void launch_cg32(hipStream_t s, const Grid& g, const StepConsts& c, const CG32Bufs& b, int t,
                  const LaunchEv& ev) {
     if (g.kz == 16) {  // one workgroup per CU, up to 256 VGPRs: the whole film in one round
-        const int nb = blocks_for_impl(g), pm = part_pm(nb);
+        const int nb = blocks_for_impl(g), pm = part_pm(b.nb_in);
         const dim3 gr = grid3(g), bl(BX, BY, 1);
         HZO_PCG_LAUNCH_PM(k_cg32, 16, pm, gr, bl, s, b, c, nb, part_stride_impl(nb), t);
         return;
     }
-    HZO_PCG_LAUNCH_X(k_cg32, g, s, b, c, blocks_for_impl(g), part_stride_impl(blocks_for_impl(g)), t);
+    HZO_PCG_LAUNCH_X(k_cg32, g, b.nb_in, s, b, c, blocks_for_impl(g), part_stride_impl(blocks_for_impl(g)), t);
 }
 #undef HZO_X
 

@@ -260,15 +260,15 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_dir_f32(Fp32Bufs b, StepConsts c,
     const rsrc_t Rpn = make_rsrc(s.pbuf ? b.p[0] : b.p[1], c.nvox * 4u);
     const rsrc_t Rsc = make_rsrc(b.sc, c.nvox * 2u);
     const rsrc_t Rdi = make_rsrc(b.lut32 + LUT_DINV, SC_LUT_SIZE * 4u);
-    const double* pin = ip ? b.part[1] : b.part[0];
+    const double* pin = ip ? b.pin[1] : b.pin[0];
 
     const HPair<float> hpv = load_hpair(b.lut32);
     Partials<3, PM> P;
     if (first) {
-        load_partials<3, PM>(pin, nb, ps, P);
+        load_partials<3, PM>(pin, b.nb_in, b.ps_in, P);
     } else {
         Partials<2, PM> Q2;
-        load_partials<2, PM>(pin, nb, ps, Q2);
+        load_partials<2, PM>(pin, b.nb_in, b.ps_in, Q2);
 #pragma unroll
         for (int m = 0; m < (PM > 0 ? PM : 1); ++m) {
             P.v[0][m] = Q2.v[0][m];
@@ -467,7 +467,7 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_update_f32(Fp32Bufs b, StepConsts
 
         const HPair<float> hpv = load_hpair(b.lut32);
         Partials<1, PM> P;
-        load_partials<1, PM>(ip ? b.part[1] : b.part[0], nb, ps, P);
+        load_partials<1, PM>(ip ? b.pin[1] : b.pin[0], b.nb_in, b.ps_in, P);
         unsigned so[KZ + 2];
 #pragma unroll
         for (int q = 0; q < KZ + 2; ++q) so[q] = (unsigned)min(max(k0 - 1 + q, 0), nz - 1);
@@ -606,11 +606,11 @@ void launch_pcg_setup_f32(hipStream_t s, const Grid& g, const StepConsts& c, con
 #define HZO_X
 void launch_pcg_dir_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b, int t,
                   const LaunchEv& ev) {
-    HZO_PCG_LAUNCH_X(k_pcg_dir_f32, g, s, b, c, blocks_for(g), part_stride(g), t);
+    HZO_PCG_LAUNCH_X(k_pcg_dir_f32, g, b.nb_in, s, b, c, blocks_for(g), part_stride(g), t);
 }
 void launch_pcg_update_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b, int t,
                   const LaunchEv& ev) {
-    HZO_PCG_LAUNCH_X(k_pcg_update_f32, g, s, b, c, blocks_for(g), part_stride(g), t);
+    HZO_PCG_LAUNCH_X(k_pcg_update_f32, g, b.nb_in, s, b, c, blocks_for(g), part_stride(g), t);
 }
 #undef HZO_X
 void launch_tdgl_f32_batch(hipStream_t s, int ntiles, const BatchFilm* films, const int2* tiles, int parity) {

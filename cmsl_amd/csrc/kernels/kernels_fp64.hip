@@ -283,15 +283,15 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_dir_f64(Fp64Bufs b, StepConsts c,
     const rsrc_t Rpn = make_rsrc(s.pbuf ? b.p[0] : b.p[1], c.nvox * 8u);
     const rsrc_t Rsc = make_rsrc(b.sc, c.nvox * 2u);
     const rsrc_t Rdi = make_rsrc(b.lut + LUT_DINV, SC_LUT_SIZE * 8u);
-    const double* pin = ip ? b.part[1] : b.part[0];
+    const double* pin = ip ? b.pin[1] : b.pin[0];
 
     const HPair<double> hpv = load_hpair(b.lut);
     Partials<3, PM> P;
     if (first) {
-        load_partials<3, PM>(pin, nb, ps, P);
+        load_partials<3, PM>(pin, b.nb_in, b.ps_in, P);
     } else {
         Partials<2, PM> Q2;
-        load_partials<2, PM>(pin, nb, ps, Q2);
+        load_partials<2, PM>(pin, b.nb_in, b.ps_in, Q2);
 #pragma unroll
         for (int m = 0; m < (PM > 0 ? PM : 1); ++m) {
             P.v[0][m] = Q2.v[0][m];
@@ -439,7 +439,7 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_update_f64(Fp64Bufs b, StepConsts
 
     const HPair<double> hpv = load_hpair(b.lut);
     Partials<1, PM> P;
-    load_partials<1, PM>(ip ? b.part[1] : b.part[0], nb, ps, P);
+    load_partials<1, PM>(ip ? b.pin[1] : b.pin[0], b.nb_in, b.ps_in, P);
     unsigned so[KZ + 2];
 #pragma unroll
     for (int q = 0; q < KZ + 2; ++q) so[q] = (unsigned)min(max(k0 - 1 + q, 0), nz - 1);
@@ -577,11 +577,11 @@ void launch_pcg_setup_f64(hipStream_t s, const Grid& g, const StepConsts& c, con
 #define HZO_X , P2
 template <bool P2>
 static void launch_dir64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t, const LaunchEv& ev) {
-    HZO_PCG_LAUNCH_X(k_pcg_dir_f64, g, s, b, c, blocks_for(g), part_stride(g), t);
+    HZO_PCG_LAUNCH_X(k_pcg_dir_f64, g, b.nb_in, s, b, c, blocks_for(g), part_stride(g), t);
 }
 template <bool P2>
 static void launch_update64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t, const LaunchEv& ev) {
-    HZO_PCG_LAUNCH_X(k_pcg_update_f64, g, s, b, c, blocks_for(g), part_stride(g), t);
+    HZO_PCG_LAUNCH_X(k_pcg_update_f64, g, b.nb_in, s, b, c, blocks_for(g), part_stride(g), t);
 }
 
 #undef HZO_X
@@ -601,6 +601,30 @@ void launch_pcg_update_f64(hipStream_t s, const Grid& g, const StepConsts& c, co
 void launch_diag_f64(hipStream_t s, const Grid& g, const double* P, const uint32_t* cw, double* part,
                   const LaunchEv& ev) {
     hipExtLaunchKernelGGL(k_diag_f64, grid3(g), dim3(BX, BY, 1), 0, s, ev.start, ev.stop, 0, P, cw, g, part, blocks_for(g));
+}
+
+__global__ __launch_bounds__(NT) void k_fold_partials(const double* __restrict__ in, int nb, int ps,
+                                                      double* __restrict__ out) {
+    __shared__ WaveLDS<RED_MAXQ> W;
+    const int tid = linear_tid();
+#pragma unroll
+    for (int q = 0; q < RED_MAXQ; ++q) {
+        double a = 0.0;
+        for (int t = tid; t < nb; t += NT) a += in[(long long)q * ps + t];
+        a = wave_sum(a);
+        if ((tid & 63) == 0) W.v[q][tid >> 6] = a;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double tot[RED_MAXQ];
+        wave_lds_total<RED_MAXQ>(W, tot);
+#pragma unroll
+        for (int q = 0; q < RED_MAXQ; ++q) out[q * NT] = tot[q];
+    }
+}
+
+void launch_fold_partials(hipStream_t s, const double* in, int nb, int ps, double* out) {
+    hipLaunchKernelGGL(k_fold_partials, dim3(1), dim3(BX, BY, 1), 0, s, in, nb, ps, out);
 }
 
 }  // namespace hzo

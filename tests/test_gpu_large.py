@@ -56,3 +56,31 @@ def test_fp32_step_at_512x512x64_with_dielectric_boundary_cells():
     rel = np.linalg.norm(r) / np.linalg.norm(b)
     print(f"[gpu] true relative residual {rel:.3e}", flush=True)
     assert rel <= 1.0e-8 * 1.05
+
+
+@pytest.mark.parametrize("prec", [L.PREC_FP64, L.PREC_FP32_REFINED])
+def test_folded_partials_beyond_2048_workgroups(prec):
+    # 320x256x32 at tile depth 2: 5 x 32 x 16 = 2560 workgroups, so every
+    # launch's partial sums are folded by a separate one-workgroup kernel
+    inp = H.make_film_inputs(320, 256, 32, 40, fracs=(0.2, 0.6, 0.2), seed=51, jitter=0.2)
+    print(f"[gpu] folded partials, 320x256x32 kz 2, precision {prec}", flush=True)
+    Ez = inp.field_nd(H.triangle_wave(40, 3.0, 3))
+    g = H.Film(inp, precision=prec, pcg_rule=L.PCG_RULE_RELRES, tol_rel=1e-10 if prec == L.PREC_FP64 else 1e-8,
+               tol_abs=1e-30, tile_kz=2)
+    its = g.run(Ez)
+    P, phi, _ = g.get_state()
+    g.close()
+    o = oracle_film(inp, pcg_rule=1, tol_rel=1e-10, tol_abs=1e-30)
+    if prec == L.PREC_FP64:
+        Po = inp.P0.copy()
+        phio = np.zeros(inp.N)
+        its_o, _ = o.run(Po, phio, np.zeros(3 * inp.N), Ez)
+        print(f"[gpu] iterations {list(its)} oracle {list(its_o)}", flush=True)
+        assert list(its) == list(its_o)
+        assert rel_err(P, Po) < 1e-10 and rel_err(phi, phio) < 1e-10
+    else:
+        b = -O.divergence(o.c, P)
+        b.reshape(inp.nz, -1)[[0, -1]] = 0.0
+        rel = np.linalg.norm(b - O.apply_A(o.c, o.eps, phi)) / np.linalg.norm(b)
+        print(f"[gpu] iterations {list(its)}, true relative residual {rel:.3e}", flush=True)
+        assert rel <= 1.0e-8 * 1.05
