@@ -1289,6 +1289,7 @@ struct hzo_batch {
         hipGraphExec_t gexec = nullptr;
         int g_chunk = -1;
         int need = 16;  // CG launches the group's last solve needed
+        bool mirrored = false;  // some film has Neumann electrodes
     };
     std::vector<hzo_ctx*> films;
     int n = 0;
@@ -1366,7 +1367,9 @@ int group_tables(hzo_batch* B, hzo_batch::Group& G, int kz) {
     }
     int maxnb = 0;
     std::vector<int> nb(n);
+    G.mirrored = false;
     for (int j = 0; j < n; ++j) {
+        G.mirrored = G.mirrored || B->films[G.idx[j]]->c.zbc != 0;
         hzo::Grid g = B->films[G.idx[j]]->g;
         g.kz = kz;
         nb[j] = hzo::blocks_for(g);
@@ -1498,7 +1501,7 @@ void group_cg_launches(hzo_batch* B, hzo_batch::Group& G, int K) {
         }
     } else {
         for (int t = 1; t <= K; ++t)
-            hzo::launch_cg32_batch(B->stream, G.ntiles, B->kz, G.pm, G.dfilms, G.dtiles, t);
+            hzo::launch_cg32_batch(B->stream, G.ntiles, B->kz, G.pm, G.mirrored, G.dfilms, G.dtiles, t);
     }
 }
 

@@ -221,7 +221,9 @@ void launch_cg32(hipStream_t s, const Grid& g, const StepConsts& c, const CG32Bu
 void launch_tdgl_f32_batch(hipStream_t s, int ntiles, const BatchFilm* films, const int2* tiles, int parity);
 void launch_setup_f32_batch(hipStream_t s, int ntiles, const BatchFilm* films, const int2* tiles, int parity);
 void launch_diag_f32_batch(hipStream_t s, int ntiles, const BatchFilm* films, const int2* tiles, int parity);
-void launch_cg32_batch(hipStream_t s, int ntiles, int kz, int pm, const BatchFilm* films, const int2* tiles, int t);
+// mirrored: some film of the launch has Neumann electrodes.
+void launch_cg32_batch(hipStream_t s, int ntiles, int kz, int pm, bool mirrored, const BatchFilm* films,
+                       const int2* tiles, int t);
 void launch_batch_set_ez(hipStream_t s, BatchFilm* films, const double* ez, int n);
 // Sum of each of the RED_MAXQ partial arrays (nb entries, stride ps) into
 // out[q*NT] (the rest of out stays zero), one workgroup: for films whose

@@ -142,6 +142,30 @@ __device__ __forceinline__ T op_identity(T xc, int k, const StepConsts& c) {
     return xc;
 }
 
+// Inner-product term of plane k in the FP32 solves' CG sums (r.z, p.Ap,
+// w.u): acc += w_k x y.  Under Neumann electrodes a mirrored plane (k = 0 or
+// nz - 1) couples to its one neighbour plane with twice the coefficient that
+// plane couples back with: A = S A_s, S = 2 on those planes and 1 elsewhere,
+// A_s symmetric.  Single-reduction CG needs a symmetric operator (its p.Ap
+// recurrence assumes u.Ap = p.Au) and stalls on A itself, so the FP32 solves
+// run CG on A_s x = S^-1 b, which has the same solution: with Jacobi
+// D = S D_s the iterates stay those of A and only these sums weight the
+// mirrored planes by w = 1/2.  The half is taken as (2 acc + x y) / 2, two
+// power-of-two scalings of the running sum, so the result is exactly the
+// weighted sum and no product is held beside it.  r.r and b.b, which the
+// relative-residual rule reads, stay unweighted.  MIR = false compiles the
+// plain sum, for kernels launched only on films with Dirichlet electrodes.
+template <bool MIR>
+__device__ __forceinline__ void cg_dot_add(double& acc, float x, float y, const StepConsts& c, int k) {
+    if constexpr (MIR) {
+        const bool mirrored = c.zbc != 0 && (k == 0 || k == c.nz - 1);
+        const double up = mirrored ? 2.0 : 1.0, down = mirrored ? 0.5 : 1.0;  // uniform
+        acc = __builtin_fma((double)x, (double)y, acc * up) * down;
+    } else {
+        acc = __builtin_fma((double)x, (double)y, acc);
+    }
+}
+
 __device__ __forceinline__ bool keep_going(const PcgState& st, double rz, double rr) {
     if (st.iters >= st.max_iter) return false;
     if (st.rule == 0) return rz > st.tol_abs2 && (st.rz0 <= 0.0 || (rz / st.rz0) > st.tol_rel2);

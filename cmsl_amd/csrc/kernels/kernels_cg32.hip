@@ -126,7 +126,7 @@ struct Plane32 {
     float rh, wh, sh, dh;
 };
 
-template <int KZ, int PM>
+template <int KZ, int PM, bool MIR>
 __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c, int nb, int ps, int t,
                                           const TileId& T) {
     static_assert(KZ % 2 == 0, "the plane loop closes its stencil pairs on an odd last plane");
@@ -284,7 +284,7 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
             bstore(RA, oc * 4u, o_r + pk * 4u, r);
             if (tp.live) {
                 const float z = r * bload<float>(Rdi, code * 4u, 0u);
-                acc[0] += (double)r * (double)z;
+                cg_dot_add<MIR>(acc[0], r, z, c, k);
                 acc[1] += (double)r * (double)r;
             }
         }
@@ -330,7 +330,7 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
             }
             u[q] = rn * L[q].d;
             if (ok) {
-                acc[0] += (double)rn * (double)u[q];
+                cg_dot_add<MIR>(acc[0], rn, u[q], c, k);
                 acc[2] += (double)rn * (double)rn;
             }
             if (interior) {
@@ -363,7 +363,7 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
                     const float wv = (code & SC_IDENTITY) ? op_identity(xc, km, c) : qs;
                     const bool okm = tp.live && km < nz;
                     bstore(RA, okm ? cv : aend, o_wo + so[m] * pb4, wv);
-                    if (okm) acc[1] += (double)wv * (double)xc;
+                    if (okm) cg_dot_add<MIR>(acc[1], wv, xc, c, km);
                 }
             }
         }
@@ -377,15 +377,17 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
 // (4 at depth 8).
 constexpr int cg32_waves(int kz) { return kz >= 16 ? 4 : (kz <= 2 ? 8 : 6); }
 
-template <int KZ, int PM>
+// MIR: the film has Neumann electrodes (mirrored boundary planes weighted in
+// the CG sums); Dirichlet films run the MIR = false build.
+template <int KZ, int PM, bool MIR>
 __global__ __launch_bounds__(NT, cg32_waves(KZ)) void k_cg32(CG32Bufs b, StepConsts c, int nb, int ps, int t) {
-    cg32_body<KZ, PM>(b, c, nb, ps, t, tile_id());
+    cg32_body<KZ, PM, MIR>(b, c, nb, ps, t, tile_id());
 }
 
 // All films of a batch in one launch: the tile table maps each workgroup to
 // (film, logical tile); every film runs its own control block, and the tiles
 // of a film whose solve has finished return at once.
-template <int KZ, int PM>
+template <int KZ, int PM, bool MIR>
 __global__ __launch_bounds__(NT, KZ >= 8 ? 4 : 5) void k_cg32_batch(const BatchFilm* __restrict__ films, const int2* __restrict__ tiles,
                                                                       int t) {
     TileId T;
@@ -393,18 +395,24 @@ __global__ __launch_bounds__(NT, KZ >= 8 ? 4 : 5) void k_cg32_batch(const BatchF
     // descriptor fields read once, before any store (scalar loads)
     const CG32Bufs b = F.cg;
     const StepConsts c = F.c;
-    cg32_body<KZ, PM>(b, c, F.nb, F.ps, t, T);
+    cg32_body<KZ, PM, MIR>(b, c, F.nb, F.ps, t, T);
 }
 
-void launch_cg32_batch(hipStream_t s, int ntiles, int kz, int pm, const BatchFilm* films, const int2* tiles,
-                       int t) {
+void launch_cg32_batch(hipStream_t s, int ntiles, int kz, int pm, bool mirrored, const BatchFilm* films,
+                       const int2* tiles, int t) {
     const dim3 gr(ntiles), bl(BX, BY, 1);
-#define HZO_CGB(KZ)                                                                                   \
-    switch (pm) {                                                                                     \
-        case 1: hipLaunchKernelGGL((k_cg32_batch<KZ, 1>), gr, bl, 0, s, films, tiles, t); break;       \
-        case 2: hipLaunchKernelGGL((k_cg32_batch<KZ, 2>), gr, bl, 0, s, films, tiles, t); break;       \
-        case 4: hipLaunchKernelGGL((k_cg32_batch<KZ, 4>), gr, bl, 0, s, films, tiles, t); break;       \
-        default: hipLaunchKernelGGL((k_cg32_batch<KZ, 0>), gr, bl, 0, s, films, tiles, t); break;      \
+#define HZO_CGB_PM(KZ, M)                                                                                \
+    switch (pm) {                                                                                        \
+        case 1: hipLaunchKernelGGL((k_cg32_batch<KZ, 1, M>), gr, bl, 0, s, films, tiles, t); break;       \
+        case 2: hipLaunchKernelGGL((k_cg32_batch<KZ, 2, M>), gr, bl, 0, s, films, tiles, t); break;       \
+        case 4: hipLaunchKernelGGL((k_cg32_batch<KZ, 4, M>), gr, bl, 0, s, films, tiles, t); break;       \
+        default: hipLaunchKernelGGL((k_cg32_batch<KZ, 0, M>), gr, bl, 0, s, films, tiles, t); break;      \
+    }
+#define HZO_CGB(KZ)             \
+    if (mirrored) {             \
+        HZO_CGB_PM(KZ, true)    \
+    } else {                    \
+        HZO_CGB_PM(KZ, false)   \
     }
     if (kz == 2) {
         HZO_CGB(2)
@@ -414,11 +422,13 @@ void launch_cg32_batch(hipStream_t s, int ntiles, int kz, int pm, const BatchFil
         HZO_CGB(8)
     }
 #undef HZO_CGB
+#undef HZO_CGB_PM
 }
 
-#define HZO_X
-void launch_cg32(hipStream_t s, const Grid& g, const StepConsts& c, const CG32Bufs& b, int t,
-                 const LaunchEv& ev) {
+template <bool MIR>
+static void launch_cg32_as(hipStream_t s, const Grid& g, const StepConsts& c, const CG32Bufs& b, int t,
+                           const LaunchEv& ev) {
+#define HZO_X , MIR
     if (g.kz == 16) {  // one workgroup per CU, up to 256 VGPRs: the whole film in one round
         const int nb = blocks_for_impl(g), pm = part_pm(b.nb_in);
         const dim3 gr = grid3(g), bl(BX, BY, 1);
@@ -426,8 +436,16 @@ void launch_cg32(hipStream_t s, const Grid& g, const StepConsts& c, const CG32Bu
         return;
     }
     HZO_PCG_LAUNCH_X(k_cg32, g, b.nb_in, s, b, c, blocks_for_impl(g), part_stride_impl(blocks_for_impl(g)), t);
-}
 #undef HZO_X
+}
+
+void launch_cg32(hipStream_t s, const Grid& g, const StepConsts& c, const CG32Bufs& b, int t,
+                 const LaunchEv& ev) {
+    if (c.zbc != 0)
+        launch_cg32_as<true>(s, g, c, b, t, ev);
+    else
+        launch_cg32_as<false>(s, g, c, b, t, ev);
+}
 
 }  // namespace hzo
 

@@ -198,7 +198,7 @@ __device__ __forceinline__ void setup_f32_body(const Fp32Bufs& b, const StepCons
             const float r = (float)true_residual(b, sH, c, v, code, k, bval, true);
             const float z = r * dinv[code];
             b.r[v.c] = r;
-            acc[0] += (double)r * (double)z;
+            cg_dot_add<true>(acc[0], r, z, c, k);
             acc[1] += (double)r * (double)r;
             acc[2] += bval * bval;
         }
@@ -408,7 +408,7 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_dir_f32(Fp32Bufs b, StepConsts c,
         const float qv = (code & SC_IDENTITY) ? op_identity(xc, k, c) : qs;
         const bool ok = tp.live && k < nz;
         bstore(Rpn, ok ? G.cofs * 4u : oob, so[q] * pb4, xc);
-        if (ok) acc[0] += (double)xc * (double)qv;
+        if (ok) cg_dot_add<true>(acc[0], xc, qv, c, k);
     }
     RawLDS<1>& sacc = *reinterpret_cast<RawLDS<1>*>(&sraw);
     write_block_partials<1>(acc, io ? b.part[1] : b.part[0], ps, T.blk, sacc);
@@ -445,7 +445,7 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_update_f32(Fp32Bufs b, StepConsts
                 const float r = (float)true_residual(b, sH64, c, v, code, k, b.b[v.c], false);
                 b.r[v.c] = r;
                 const float z = r * dinv[code];
-                acc[0] += (double)r * (double)z;
+                cg_dot_add<true>(acc[0], r, z, c, k);
                 acc[1] += (double)r * (double)r;
             }
         }
@@ -542,7 +542,7 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_update_f32(Fp32Bufs b, StepConsts
             bstore(Rr, vo, so[q] * pb4, r);
             const float z = r * dv[q - 1];
             if (ok) {
-                acc[0] += (double)r * (double)z;
+                cg_dot_add<true>(acc[0], r, z, c, k);
                 acc[1] += (double)r * (double)r;
             }
         }

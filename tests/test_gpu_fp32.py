@@ -72,3 +72,42 @@ def test_fp32_refined_tracks_fp64_oracle_over_a_sweep(fused):
     assert eP < 1e-3 and ephi < 1e-3
     assert abs(pr_g - pr_o) <= 1e-4 * abs(pr_o)
     g.close()
+
+
+@pytest.mark.parametrize("fused", [1, 0])
+def test_fp32_refined_neumann_electrodes(fused):
+    """Neumann electrodes: the mirrored planes k = 0, nz - 1 make A = S A_s
+    non-symmetric, and the FP32 solves weight those planes by 1/2 in their CG
+    sums.  Without that weight the single-reduction CG stalls and then
+    diverges on this film (phi ~ 1e18, then NaN).  Each solve must converge
+    with a true FP64 residual within the tolerance, and the sweep must track
+    the FP64 oracle (plain Jacobi PCG, solved 100x tighter)."""
+    inp = film_inputs(48, 40, 12, 8, fracs=(0.2, 0.6, 0.2), seed=12, jitter=0.2)
+    nsteps = 12
+    Ez = inp.field_nd(H.triangle_wave(nsteps, 3.0, nsteps))
+    _say(f"fp32 neumann 48x40x12 {nsteps} steps fused={fused}")
+    g = H.Film(inp, precision=L.PREC_FP32_REFINED, pcg_rule=L.PCG_RULE_RELRES, tol_rel=1e-8, tol_abs=1e-30,
+               max_iter=3000, zbc=L.ZBC_NEUMANN, cg_fused=fused)
+    o = oracle_film(inp, zbc=L.ZBC_NEUMANN, pcg_rule=1, tol_rel=1e-10, tol_abs=1e-30, max_iter=20000)
+    P = inp.P0.copy()
+    phi = np.zeros(inp.N)
+    E3 = np.zeros(3 * inp.N)
+    its = []
+    for s in range(nsteps):
+        st = g.step(Ez[s])
+        its.append((st.iters, st.status, st.refinements))
+        assert st.status == 0, its
+        o.run(P, phi, E3, Ez[s:s + 1])
+    Pg, phig, _ = g.get_state()
+    ref = inp.nx // 2 + inp.nx * (inp.ny // 2 + inp.ny * (inp.nz // 2))  # gauge node, phi = 0
+    b = -O.divergence(o.c, Pg)
+    b[ref] = 0.0
+    r = b - O.apply_A(o.c, o.eps, phig)
+    rel = np.linalg.norm(r) / np.linalg.norm(b)
+    eP, ephi = rel_err(Pg, P), rel_err(phig, phi)
+    _say(f"(iters, status, refinements) {its}; true rel residual {rel:.3e}; rel err P {eP:.2e} phi {ephi:.2e}")
+    assert phig[ref] == 0.0
+    assert rel <= 1.0e-8 * 1.05
+    assert max(i for i, _, _ in its) < 1500
+    assert eP < 1e-5 and ephi < 1e-5
+    g.close()

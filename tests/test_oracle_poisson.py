@@ -175,14 +175,98 @@ def test_pcg_converges_to_dense_solution(zbc):
 
 def test_neumann_gauge_row_breaks_symmetry_as_in_reference():
     # The gauge pin (depol_varEPS.cu:53-57) replaces row ref by the identity but
-    # keeps column ref in the neighbouring rows, so A is not symmetric and CG
-    # stagnates.  The oracle reproduces that behaviour rather than fixing it.
+    # keeps column ref in the neighbouring rows, and the mirrored planes see
+    # their one z neighbour twice, so A is not symmetric and the reference's
+    # PCG stagnates at tight tolerances.  The oracle reproduces that behaviour
+    # rather than fixing it.
     c, f, P, phi, E3, st, divP, hist = _solve_case(1, rule=1, tol=1e-12, nx=4, ny=4, nz=5)
     A = _assemble(c, f.eps)
     ref = c.nx // 2 + c.nx * (c.ny // 2 + c.ny * (c.nz // 2))
     assert np.all(A[ref] == np.eye(A.shape[0])[ref])
     assert np.count_nonzero(A[:, ref]) > 1
     assert st.status == 1 and st.rr < st.rr0
+
+
+def _neumann_film(nx, ny, nz, seed):
+    rng = np.random.default_rng(seed)
+    c = _cfg(nx, ny, nz, zbc=1, pcg_rule=1, tol_abs=1e-30, tol_rel=1e-8)
+    eps = rng.choice([15.0, 30.0, 40.0], nx * ny * nz) * rng.uniform(0.8, 1.2, nx * ny * nz)
+    S = np.ones(nx * ny * nz)
+    S[: nx * ny] = 2.0
+    S[-nx * ny:] = 2.0
+    return c, eps, S, nx // 2 + nx * (ny // 2 + ny * (nz // 2))
+
+
+def test_neumann_operator_is_symmetric_after_halving_the_mirrored_rows():
+    # A mirrored plane (k = 0 or nz - 1) takes its ghost neighbour as the image
+    # of plane 1 (nz - 2), so its row couples to that plane with twice the face
+    # coefficient the neighbour's row couples back with: A = S A_s, S = 2 on
+    # those planes.  With the pinned gauge unknown dropped, A_s = S^-1 A is
+    # symmetric positive definite: the operator that CG with the mirrored
+    # planes' inner-product terms halved runs on.
+    nx, ny, nz = 4, 3, 6
+    c, eps, S, ref = _neumann_film(nx, ny, nz, 11)
+    A = _assemble(c, eps)
+    keep = np.arange(A.shape[0]) != ref
+    Ak = A[np.ix_(keep, keep)]
+    As = (A / S[:, None])[np.ix_(keep, keep)]
+    assert not np.allclose(Ak, Ak.T)
+    assert np.allclose(As, As.T, rtol=1e-14, atol=1e-12 * np.abs(As).max())
+    assert np.linalg.eigvalsh(As).min() > 0
+    # the mirrored rows are exactly twice the symmetric ones
+    k0 = np.arange(nx * ny)
+    assert np.allclose(A[k0, nx * ny + k0], 2 * A[nx * ny + k0, k0])
+
+
+def _single_reduction_cg(A, dinv, b, wt, iters):
+    """Chronopoulos-Gear CG (one inner-product round per iteration) with the
+    inner products r.u and w.u weighted by wt; returns the relative residual
+    history of the recursively updated r."""
+    x = np.zeros_like(b)
+    r = b.copy()
+    u = dinv * r
+    w = A(u)
+    g, d = (wt * r) @ u, (wt * w) @ u
+    p = s = None
+    g_old = a_old = None
+    hist = []
+    for i in range(iters):
+        if i == 0:
+            beta, alpha = 0.0, g / d
+            p, s = u.copy(), w.copy()
+        else:
+            beta = g / g_old
+            alpha = g / (d - beta * g / a_old)
+            p, s = u + beta * p, w + beta * s
+        x += alpha * p
+        r -= alpha * s
+        u = dinv * r
+        w = A(u)
+        g_old, a_old = g, alpha
+        g, d = (wt * r) @ u, (wt * w) @ u
+        hist.append(np.sqrt((r @ r) / (b @ b)))
+        if hist[-1] <= 1e-8 or not np.isfinite(hist[-1]):
+            break
+    return x, np.array(hist)
+
+
+def test_single_reduction_cg_needs_the_mirrored_plane_weights():
+    # The single-reduction recurrence for p.Ap assumes u.Ap = p.Au.  On the
+    # Neumann operator with plain inner products it stalls; with the mirrored
+    # planes' terms halved it is CG on the symmetric A_s and converges to the
+    # solution of A x = b (same iterates, same solution).
+    nx, ny, nz = 16, 12, 8
+    c, eps, S, ref = _neumann_film(nx, ny, nz, 12)
+    A = lambda v: O.apply_A(c, eps, np.ascontiguousarray(v))
+    dinv = O.diag_inv(c, eps)
+    b = np.random.default_rng(3).standard_normal(nx * ny * nz)
+    b[ref] = 0.0
+    x_w, h_w = _single_reduction_cg(A, dinv, b, 1.0 / S, 2000)
+    x_p, h_p = _single_reduction_cg(A, dinv, b, np.ones_like(b), 2000)
+    assert h_w[-1] <= 1e-8 and len(h_w) < 500
+    assert np.linalg.norm(b - A(x_w)) <= 1.05e-8 * np.linalg.norm(b)
+    assert not (np.all(np.isfinite(h_p)) and h_p[-1] <= 1e-8 and len(h_p) <= len(h_w))
+    assert np.nanmin(h_p[len(h_w):]) > 1e-7  # plain sums: no convergence in 4x the weighted count
 
 
 def test_reference_stopping_rule_and_warm_start():
