@@ -33,7 +33,7 @@ def build_inputs():
     return H.make_film_inputs(NX, NY, NZ, NG, fracs=FRACS, seed=1, jitter=0.2)
 
 
-def algorithmic_bytes_per_cell(kclass: str, precision: str) -> float:
+def algorithmic_bytes_per_cell(kclass: str, precision: str, cg_form: int = 1) -> float:
     """Compulsory HBM bytes per voxel of one launch (each array touched once).
 
     fp64: pcg_dir_spmv reads r, p_old, code (8+8+2) and writes p_new (8) = 26;
@@ -44,9 +44,13 @@ def algorithmic_bytes_per_cell(kclass: str, precision: str) -> float:
           pcg_update reads x_lo, p, r (12), code (2), writes x_lo, r (8) = 22;
           pcg_setup reads Pmx,Pmy,Pmz (12), x_hi, x_lo (12), code (2), writes b (8), r (4) = 38;
           tdgl reads P (24), x_hi + x_lo (12), cell word (4), writes P (24) = 64;
-          pcg_fused (single-reduction CG, one launch per iteration) reads r, w, s,
-          p, x_lo (20), code (2), writes s, r, w, p, x_lo (20) = 42.
+          pcg_fused (single-reduction CG, one launch per iteration) carrying w and
+          s reads r, w, s, p, x_lo (20), code (2), writes s, r, w, p, x_lo (20) = 42;
+          forming s = A p and w = A u by stencils (cg_form 2) it reads r, p, x_lo
+          (12), code (2), writes r, p, x_lo (12) = 26.
     """
+    if kclass == "pcg_fused" and precision == "fp32" and cg_form == 2:
+        return 26.0
     table = {
         "fp64": {"tdgl": 108.0, "pcg_setup": 42.0, "pcg_dir_spmv": 26.0, "pcg_update": 42.0, "pcg_fused": 0.0},
         "fp32": {"tdgl": 64.0, "pcg_setup": 38.0, "pcg_dir_spmv": 14.0, "pcg_update": 22.0, "pcg_fused": 42.0},
@@ -76,8 +80,9 @@ def run_gpu(args, inp, precision):
     film.run(Ez2)
     kt = film.kernel_times()
     film.set_timing(False)
+    form = film.cg_form
     film.close()
-    return t1 - t0, its, kt
+    return t1 - t0, its, kt, form
 
 
 def run_cpu_baseline(inp, nsteps):
@@ -302,14 +307,17 @@ def run_large(args):
         film.set_timing(True)
         film.run(Ez[W + S:])
         kt = film.kernel_times()
+        form, kz = film.cg_form, film.tile_depth
         film.close()
         entry = {"value": inp.N * S / dt / 1e6, "unit": "Mcell-steps/s", "ms_per_step": dt / S * 1e3,
-                 "pcg_iters_mean": float(np.mean(its)), "kernel_ms": {k: v[1] for k, v in kt.items()}}
+                 "pcg_iters_mean": float(np.mean(its)), "kernel_ms": {k: v[1] for k, v in kt.items()},
+                 "tile_kz": kz, "cg_form": form}
         if name == "fp32":
             ms = kt["pcg_fused"][1]
-            achieved = algorithmic_bytes_per_cell("pcg_fused", "fp32") * inp.N / (ms * 1e-3) / 1e9
+            bpc = algorithmic_bytes_per_cell("pcg_fused", "fp32", form)
+            achieved = bpc * inp.N / (ms * 1e-3) / 1e9
             entry["roofline_pcg_fused"] = {"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                           "frac": achieved / HBM_PEAK_GBS}
+                                           "frac": achieved / HBM_PEAK_GBS, "bytes_per_cell": bpc}
         out[name] = entry
     out["speedup_fp32_refined_vs_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
     return out
@@ -369,8 +377,8 @@ def main():
     N = inp.N
     res = {}
     for prec in ("fp64", "fp32"):
-        dt, its, kt = run_gpu(args, inp, prec)
-        res[prec] = dict(dt=dt, its=its, kt=kt, value=N * args.steps / dt / 1e6)
+        dt, its, kt, form = run_gpu(args, inp, prec)
+        res[prec] = dict(dt=dt, its=its, kt=kt, form=form, value=N * args.steps / dt / 1e6)
     head = res["fp32"]
     value = head["value"]
     kt = head["kt"]
@@ -379,7 +387,7 @@ def main():
     tot = {k: c * ms for k, (c, ms) in kt.items() if c > 0}
     dom = max(tot, key=tot.get)
     cnt, ms = kt[dom]
-    bpc = algorithmic_bytes_per_cell(dom, "fp32")
+    bpc = algorithmic_bytes_per_cell(dom, "fp32", head["form"])
     achieved = bpc * N / (ms * 1e-3) / 1e9
 
     traffic = None

@@ -95,6 +95,7 @@ SIGNATURES = {
     "hzo_synchronize": (_I, [_P]),
     "hzo_grid_dims": (_I, [_P, _P]),
     "hzo_tile_depth": (_I, [_P]),
+    "hzo_cg_form": (_I, [_P]),
     "hzo_sweep_schedule": (_I, [_P, _I, _P, _P, _P]),
     "hzo_sweep_run": (_I, [_P, _P, C.c_char_p, _I, _P, _P]),
     "hzo_set_timing": (_I, [_P, _I]),

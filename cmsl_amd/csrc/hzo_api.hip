@@ -31,6 +31,7 @@ struct hzo_ctx {
     int nb = 0;
 
     bool fp32 = false;  // HZO_PREC_FP32_REFINED
+    bool cg_two = false;  // single-reduction CG forming s = A p and w = A u in the launch
     double* P[2] = {nullptr, nullptr};
     float* P32[2] = {nullptr, nullptr};
     float* xlo = nullptr;
@@ -428,7 +429,7 @@ int cg32_solve(hzo_ctx* x, const hzo::Fp32Bufs& b32in, const hzo::StepConsts& c,
     while (true) {
         for (int m = 0; m < batch && launches < maxl; ++m, ++launches) {
             const int tt = ++t;
-            rc = timed(x, 4, [&](const hzo::LaunchEv& ev) { hzo::launch_cg32(x->stream, x->g, c, cb, tt, ev); });
+            rc = timed(x, 4, [&](const hzo::LaunchEv& ev) { hzo::launch_cg32(x->stream, x->g, c, cb, tt, x->cg_two, ev); });
             if (rc) return rc;
             fold_after(x, x->stream, tt);
         }
@@ -608,6 +609,7 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
     // the single-reduction CG addresses four of its vectors through one
     // buffer range of 16 N bytes, which must stay below 2^31: larger films
     // run the two-kernel FP32 PCG
+    if (cfg->cg_fused < 0 || cfg->cg_fused > 3) return set_error(HZO_ERR_ARG, "hzo_create: cg_fused %d", cfg->cg_fused);
     const bool cg_fused = cfg->cg_fused && 16 * N < (1LL << 31);
     if (cfg->tile_kz == 16 && cfg->precision == HZO_PREC_FP32_REFINED && !cg_fused)
         return set_error(HZO_ERR_ARG, "hzo_create: tile_kz 16 needs the single-reduction CG (at most 2^27 - 1 voxels)");
@@ -617,7 +619,7 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
 
     hzo_ctx* x = new hzo_ctx();
     x->cfg = *cfg;
-    x->cfg.cg_fused = cg_fused ? 1 : 0;
+    x->cfg.cg_fused = cg_fused ? cfg->cg_fused : 0;
     x->g.nx = cfg->nx;
     x->g.ny = cfg->ny;
     x->g.nz = cfg->nz;
@@ -658,6 +660,10 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
             }
         }
         x->g.kz = kz;
+        // the two-stencil form moves 26 instead of 42 bytes per voxel and
+        // iteration but does more work per byte: it wins on films whose
+        // tiles are 16 deep (HBM-bound), the carried form on smaller ones
+        x->cg_two = fused && (cfg->cg_fused == 3 || (cfg->cg_fused == 1 && kz == 16));
     }
     hzo::StepConsts& c = x->c;
     c.nx = cfg->nx;
@@ -982,6 +988,7 @@ int hzo_grid_dims(hzo_ctx* x, int dims[3]) {
 }
 
 int hzo_tile_depth(hzo_ctx* x) { return x ? x->g.kz : 0; }
+int hzo_cg_form(hzo_ctx* x) { return !x ? -1 : (!x->fp32 || !x->cfg.cg_fused) ? 0 : (x->cg_two ? 2 : 1); }
 
 int hzo_set_timing(hzo_ctx* x, int enable) {
     if (!x) return set_error(HZO_ERR_ARG, "null context");
@@ -1496,7 +1503,7 @@ void group_cg_launches(hzo_batch* B, hzo_batch::Group& G, int K) {
         hzo_ctx* x = B->films[G.single];
         const hzo::CG32Bufs cb = cg_bufs(x, bufs32(x));
         for (int t = 1; t <= K; ++t) {
-            hzo::launch_cg32(B->stream, x->g, x->c, cb, t);
+            hzo::launch_cg32(B->stream, x->g, x->c, cb, t, x->cg_two);
             fold_after(x, B->stream, t);
         }
     } else {

@@ -214,7 +214,9 @@ void launch_pcg_update_f32(hipStream_t s, const Grid& g, const StepConsts& c, co
                  const LaunchEv& ev = {});
 void launch_diag_f32(hipStream_t s, const Grid& g, const float* P, const uint32_t* cw, double* part,
                  const LaunchEv& ev = {});
-void launch_cg32(hipStream_t s, const Grid& g, const StepConsts& c, const CG32Bufs& b, int t,
+// two_stencil: the form that recomputes s = A p and w = A u in the launch
+// (reads r, p; writes r, p, x_lo) instead of carrying w and s.
+void launch_cg32(hipStream_t s, const Grid& g, const StepConsts& c, const CG32Bufs& b, int t, bool two_stencil,
                  const LaunchEv& ev = {});
 // Batched FP32-refined launches over every tile of every film (1-D grid of
 // ntiles workgroups; tiles[] maps each to (film, logical tile)).

@@ -126,6 +126,64 @@ struct Plane32 {
     float rh, wh, sh, dh;
 };
 
+// RESID: r = b - A (x_hi + x_lo) in FP64 over the tile's planes, written as
+// FP32 to the r array at byte offset o_r of RA; partials (r.z, r.r).  One
+// plane at a time, with buffer loads at 32-bit column offsets (neighbour
+// planes clamped into the film; identity rows, where the clamp applies under
+// Dirichlet electrodes, use only the centre).
+template <int KZ, bool MIR>
+__device__ __forceinline__ void cg32_resid(const CG32Bufs& b, const StepConsts& c, const TilePos& tp,
+                                           const TileCols& G, int k0, rsrc_t RA, unsigned o_r, rsrc_t Rsc,
+                                           rsrc_t Rdi, double (&acc)[3]) {
+    const int nz = c.nz;
+    __shared__ double sH64[16];
+    load_lut(sH64, b.lut);
+    const rsrc_t Rxh = make_rsrc(b.xhi, c.nvox * 8u);
+    const rsrc_t Rxl = make_rsrc(b.xlo, c.nvox * 4u);
+    const rsrc_t Rb = make_rsrc(b.b, c.nvox * 8u);
+    const int nx = c.nx, i = tp.ci, j = tp.cj;
+    const unsigned ie = (i + 1 == nx) ? 0 : i + 1, iw = (i == 0) ? nx - 1 : i - 1;
+    const unsigned jn = (j + 1 == c.ny) ? 0 : j + 1, js = (j == 0) ? c.ny - 1 : j - 1;
+    // dead lanes use element offset 4 N: out of range of every array here
+    const unsigned dead = 4u * c.nvox;
+    const unsigned oc = tp.live ? G.cofs : dead;
+    const unsigned oe = tp.live ? ie + (unsigned)nx * j : dead, ow = tp.live ? iw + (unsigned)nx * j : dead;
+    const unsigned on = tp.live ? i + (unsigned)nx * jn : dead, os = tp.live ? i + (unsigned)nx * js : dead;
+    auto X = [&](unsigned col, unsigned pofs) {
+        return bload<double>(Rxh, col * 8u, pofs * 8u) + (double)bload<float>(Rxl, col * 4u, pofs * 4u);
+    };
+    const int k1 = min(k0 + KZ, nz);
+#pragma unroll 1
+    for (int k = k0; k < k1; ++k) {
+        int up = k + 1, dn = k - 1;
+        if (c.zbc != 0) {
+            if (k == 0) up = dn = 1;
+            else if (k == nz - 1) up = dn = nz - 2;
+        }
+        const unsigned pk = (unsigned)k * c.plane;
+        const unsigned pu = (unsigned)min(up, nz - 1) * c.plane, pd = (unsigned)max(dn, 0) * c.plane;
+        const unsigned code = bload_code(Rsc, oc * 2u, pk * 2u);
+        const double xc = X(oc, pk);
+        double Ax;
+        if (code & SC_IDENTITY) {
+            Ax = op_identity(xc, k, c);
+        } else {
+            const double* Hr = sH64 + (code & 3u) * 4u;
+            const double num = Hr[(code >> 2) & 3u] * (X(oe, pk) - xc) - Hr[(code >> 4) & 3u] * (xc - X(ow, pk)) +
+                               Hr[(code >> 6) & 3u] * (X(on, pk) - xc) - Hr[(code >> 8) & 3u] * (xc - X(os, pk)) +
+                               Hr[(code >> 10) & 3u] * (X(oc, pu) - xc) - Hr[(code >> 12) & 3u] * (xc - X(oc, pd));
+            Ax = -(num * c.inv_h2);
+        }
+        const float r = (float)(bload<double>(Rb, oc * 8u, pk * 8u) - Ax);
+        bstore(RA, oc * 4u, o_r + pk * 4u, r);
+        if (tp.live) {
+            const float z = r * bload<float>(Rdi, code * 4u, 0u);
+            cg_dot_add<MIR>(acc[0], r, z, c, k);
+            acc[1] += (double)r * (double)r;
+        }
+    }
+}
+
 template <int KZ, int PM, bool MIR>
 __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c, int nb, int ps, int t,
                                           const TileId& T) {
@@ -238,56 +296,7 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
     if (mode == 1) return;
     double acc[3] = {0.0, 0.0, 0.0};
     if (mode == 2) {
-        // RESID: r = b - A (x_hi + x_lo) in FP64, one plane at a time, with
-        // buffer loads at 32-bit column offsets (neighbour planes clamped
-        // into the film; identity rows, where the clamp applies under
-        // Dirichlet electrodes, use only the centre)
-        __shared__ double sH64[16];
-        load_lut(sH64, b.lut);
-        const rsrc_t Rxh = make_rsrc(b.xhi, c.nvox * 8u);
-        const rsrc_t Rxl = make_rsrc(b.xlo, c.nvox * 4u);
-        const rsrc_t Rb = make_rsrc(b.b, c.nvox * 8u);
-        const int nx = c.nx, i = tp.ci, j = tp.cj;
-        const unsigned ie = (i + 1 == nx) ? 0 : i + 1, iw = (i == 0) ? nx - 1 : i - 1;
-        const unsigned jn = (j + 1 == c.ny) ? 0 : j + 1, js = (j == 0) ? c.ny - 1 : j - 1;
-        // dead lanes use element offset 4 N: out of range of every array here
-        const unsigned dead = 4u * c.nvox;
-        const unsigned oc = tp.live ? G.cofs : dead;
-        const unsigned oe = tp.live ? ie + (unsigned)nx * j : dead, ow = tp.live ? iw + (unsigned)nx * j : dead;
-        const unsigned on = tp.live ? i + (unsigned)nx * jn : dead, os = tp.live ? i + (unsigned)nx * js : dead;
-        auto X = [&](unsigned col, unsigned pofs) {
-            return bload<double>(Rxh, col * 8u, pofs * 8u) + (double)bload<float>(Rxl, col * 4u, pofs * 4u);
-        };
-        const int k1 = min(k0 + KZ, nz);
-#pragma unroll 1
-        for (int k = k0; k < k1; ++k) {
-            int up = k + 1, dn = k - 1;
-            if (c.zbc != 0) {
-                if (k == 0) up = dn = 1;
-                else if (k == nz - 1) up = dn = nz - 2;
-            }
-            const unsigned pk = (unsigned)k * c.plane;
-            const unsigned pu = (unsigned)min(up, nz - 1) * c.plane, pd = (unsigned)max(dn, 0) * c.plane;
-            const unsigned code = bload_code(Rsc, oc * 2u, pk * 2u);
-            const double xc = X(oc, pk);
-            double Ax;
-            if (code & SC_IDENTITY) {
-                Ax = op_identity(xc, k, c);
-            } else {
-                const double* Hr = sH64 + (code & 3u) * 4u;
-                const double num = Hr[(code >> 2) & 3u] * (X(oe, pk) - xc) - Hr[(code >> 4) & 3u] * (xc - X(ow, pk)) +
-                                   Hr[(code >> 6) & 3u] * (X(on, pk) - xc) - Hr[(code >> 8) & 3u] * (xc - X(os, pk)) +
-                                   Hr[(code >> 10) & 3u] * (X(oc, pu) - xc) - Hr[(code >> 12) & 3u] * (xc - X(oc, pd));
-                Ax = -(num * c.inv_h2);
-            }
-            const float r = (float)(bload<double>(Rb, oc * 8u, pk * 8u) - Ax);
-            bstore(RA, oc * 4u, o_r + pk * 4u, r);
-            if (tp.live) {
-                const float z = r * bload<float>(Rdi, code * 4u, 0u);
-                cg_dot_add<MIR>(acc[0], r, z, c, k);
-                acc[1] += (double)r * (double)r;
-            }
-        }
+        cg32_resid<KZ, MIR>(b, c, tp, G, k0, RA, o_r, Rsc, Rdi, acc);
     } else {
         // ITER (mode 0): s = w + beta s and r -= alpha s on the tile and its
         // halo, p = M r_old + beta p, x += alpha p, then u = M r and w = A u;
@@ -371,6 +380,260 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
     write_block_partials_w<3>(acc, io ? b.part[1] : b.part[0], ps, T.blk, spart);
 }
 
+// ---------------------------------------------------------------- two-stencil iteration
+// The same single-reduction iteration with s = A p and w = A u formed in the
+// launch instead of carried in memory: a launch reads r_i and p_(i-1) and
+// writes r_(i+1) and p_i (both double-buffered by the control block's
+// parity) and x_lo, 26 bytes per voxel instead of 42.  In exact arithmetic
+// s_i = A p_i is the vector the recurrence s = w + beta s carries, so the
+// iterates are those of the same Krylov method; the stencil is applied
+// twice per voxel instead of once, on a wider halo.
+//   p_i = M r_i + beta p_(i-1)   tile and a two-voxel x-y ring
+//   s_i = A p_i                   tile and its one-voxel ring
+//   r_(i+1) = r_i - alpha s_i, u = M r_(i+1)   same cells
+//   w = A u                       tile; x_lo += alpha p_i on the tile
+// Partials (gamma, delta, r.r) as in the carried form; RESID and WCOMP are
+// the same launch modes (WCOMP forms u and w from the true residual).
+//
+// Ring cells: thread h < R1 owns a cell of the one-voxel ring without its
+// corners (rows above and below the tile, then the left and right columns),
+// R1 <= h < R2 one of the four corners or of the outer ring's rows and
+// columns (the outer ring's corners are never read).  A thread keeps its own
+// column and ring cell on every plane, so z neighbours come from registers
+// and only in-plane neighbours go through LDS.
+constexpr int R1 = 2 * TX + 2 * TY;           // 144
+constexpr int R2 = R1 + 4 + 2 * TX + 2 * TY;  // 292
+constexpr int PX = TX + 4, PY = TY + 4;
+
+struct Ring2 {
+    bool has, inner;
+    int lx, ly;  // position relative to the tile's first voxel: -2 .. TX + 1, -2 .. TY + 1
+};
+
+__device__ __forceinline__ Ring2 ring2_cell(int h) {
+    Ring2 r;
+    r.has = h < R2;
+    r.inner = h < R1;
+    int x = 0, y = 0;
+    if (h < TX) {
+        x = h, y = -1;
+    } else if (h < 2 * TX) {
+        x = h - TX, y = TY;
+    } else if (h < 2 * TX + TY) {
+        x = -1, y = h - 2 * TX;
+    } else if (h < R1) {
+        x = TX, y = h - 2 * TX - TY;
+    } else if (h < R1 + 4) {
+        x = ((h - R1) & 1) ? TX : -1, y = ((h - R1) & 2) ? TY : -1;
+    } else if (h < R1 + 4 + TX) {
+        x = h - R1 - 4, y = -2;
+    } else if (h < R1 + 4 + 2 * TX) {
+        x = h - R1 - 4 - TX, y = TY + 1;
+    } else if (h < R1 + 4 + 2 * TX + TY) {
+        x = -2, y = h - R1 - 4 - 2 * TX;
+    } else if (h < R2) {
+        x = TX + 1, y = h - R1 - 4 - 2 * TX - TY;
+    }
+    r.lx = x;
+    r.ly = y;
+    return r;
+}
+
+// One window plane: the centre column's r, p_(i-1), x_lo and D^-1, and the
+// ring cell's r, p_(i-1) and D^-1.
+struct Plane2 {
+    float r, p, x, d;
+    float rh, ph, dh;
+};
+
+template <int KZ, int PM, bool MIR>
+__device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& c, int nb, int ps, int t,
+                                           const TileId& T) {
+    constexpr int NW = KZ + 4;  // window plane m is z = k0 - 2 + m; the tile's planes are m = 2 .. KZ + 1
+    const int ip = (t - 1) & 1, io = t & 1;
+    __shared__ HPair<float> sHP[64];
+    // p of plane m (tile + two-voxel ring) in sp[m % 2], u (tile + one-voxel
+    // ring) in su[m % 2]: each is written in one phase of the plane loop and
+    // read only in the next
+    __shared__ float sp[2][PY][PX];
+    __shared__ float su[2][HY][HX];
+    __shared__ WaveLDS<3> sred, spart;
+    const int nz = c.nz;
+    const int tid = linear_tid();
+    const int k0 = T.bz * KZ;
+    const TilePos tp = tile_pos(c, T);
+    const TileCols G = tile_cols(tp, c);
+    const Ring2 R = ring2_cell(tid);
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const unsigned rofs = R.has ? (unsigned)wrapi(T.bx * TX + R.lx, c.nx) +
+                                      (unsigned)c.nx * (unsigned)wrapi(T.by * TY + R.ly, c.ny)
+                                : c.nvox;
+    // the ring stencil of a lane without a one-voxel-ring cell runs on its
+    // own column (in range, result unused)
+    const int bx = R.inner ? R.lx : tx, by = R.inner ? R.ly : ty;
+    const unsigned pb4 = c.plane * 4u, pb2 = c.plane * 2u;
+    const rsrc_t Rsc = make_rsrc(b.sc, c.nvox * 2u);
+    const rsrc_t Rdi = make_rsrc(b.lut32 + LUT_DINV, SC_LUT_SIZE * 4u);
+    const float ih2 = (float)c.inv_h2;
+    const HPair<float> hpv = load_hpair(b.lut32);
+    Partials<3, PM> P;
+    load_partials<3, PM>(ip ? b.pin[1] : b.pin[0], b.nb_in, b.ps_in, P);
+    unsigned so[NW], cd[NW];  // plane offsets; stencil codes (centre | ring << 16)
+#pragma unroll
+    for (int m = 0; m < NW; ++m) {
+        so[m] = (unsigned)min(max(k0 - 2 + m, 0), nz - 1);
+        cd[m] = bload_code(Rsc, G.cofs * 2u, so[m] * pb2) | (bload_code(Rsc, rofs * 2u, so[m] * pb2) << 16);
+    }
+
+    PcgState s = load_state(b.st + ip);
+    if (s.done) {
+        keep_live(hpv.p);
+#pragma unroll
+        for (int m = 0; m < NW; ++m) keep_live(cd[m]);
+        keep_live_partials(P);
+        pass_state(s, b.st + io, T.lead);
+        return;
+    }
+    // RA = r[0] r[1] (w slots unused), RB = p[0] p[1] (in the s slots)
+    const unsigned cur = (unsigned)s.pbuf;
+    const unsigned seg = c.nvox * 4u, aend = 4u * seg;
+    const rsrc_t RA = make_rsrc(b.va, aend);
+    const rsrc_t RB = make_rsrc(b.vb, aend);
+    const rsrc_t RX = make_rsrc(b.xlo, seg);
+    const unsigned o_r = cur * seg, o_rn = (cur ^ 1u) * seg;
+    const unsigned o_p = cur * seg, o_pn = (cur ^ 1u) * seg;
+    const unsigned cv = G.cofs * 4u, rv = R.has ? rofs * 4u : aend;
+    const bool after_true = s.kind == K_SETUP || s.kind == K_RESID;  // no p needed: WCOMP or done
+
+    Plane2 L[NW];
+    auto issue = [&](int m) {
+        const unsigned sv = so[m] * pb4;
+        L[m].r = bload<float>(RA, cv, o_r + sv);
+        L[m].d = bload<float>(Rdi, (cd[m] & 0xFFFFu) * 4u, 0u);
+        L[m].rh = bload<float>(RA, rv, o_r + sv);
+        L[m].dh = bload<float>(Rdi, (cd[m] >> 16) * 4u, 0u);
+        L[m].p = after_true ? 0.0f : bload<float>(RB, cv, o_p + sv);
+        L[m].ph = after_true ? 0.0f : bload<float>(RB, rv, o_p + sv);
+        L[m].x = (after_true || m < 2 || m > KZ + 1) ? 0.0f : bload<float>(RX, cv, sv);
+    };
+    issue(0);
+    issue(1);
+
+    if (tid < 64) sHP[tid] = hpv;
+    wave_partials<3, PM>(P, sred);
+    __syncthreads();
+    double tot[3];
+    wave_lds_total<3>(sred, tot);
+    double alpha_d = 0.0, beta_d = 0.0;
+    const int mode = cg_decide(s, tot, alpha_d, beta_d);
+    if (mode == 2) {
+        s.kind = K_RESID;
+    } else if (mode == 3) {
+        s.kind = K_WCOMP;
+    } else if (mode == 0) {
+        s.iters += 1;
+        s.pbuf ^= 1;
+        s.kind = K_ITER;
+    }
+    pass_state(s, b.st + io, T.lead);
+    if (mode == 1) return;
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (mode == 2) {
+        cg32_resid<KZ, MIR>(b, c, tp, G, k0, RA, o_r, Rsc, Rdi, acc);
+    } else {
+        const bool iter = mode == 0;
+        const float alpha = (float)alpha_d, beta = (float)beta_d;
+        const bool fresh = beta_d == 0.0;
+        float pc[NW], ph[NW], uc[NW];
+        // A(m): p of plane m (tile and two-voxel ring) into LDS; on the tile
+        // p_i and x_lo += alpha p_i are stored (WCOMP: x_lo folded instead)
+        auto stageA = [&](int m) {
+            const int k = k0 - 2 + m;
+            const bool tile_plane = m >= 2 && m <= KZ + 1;
+            const bool ok = tile_plane && tp.live && k < nz;
+            if (iter) {
+                const float uo = L[m].r * L[m].d, uoh = L[m].rh * L[m].dh;
+                pc[m] = fresh ? uo : uo + beta * L[m].p;
+                ph[m] = fresh ? uoh : uoh + beta * L[m].ph;
+                sp[m & 1][ty + 2][tx + 2] = pc[m];
+                if (R.has) sp[m & 1][R.ly + 2][R.lx + 2] = ph[m];
+                if (tile_plane) {
+                    const unsigned vo = ok ? cv : aend, sv = so[m] * pb4;
+                    bstore(RB, vo, o_pn + sv, pc[m]);
+                    bstore(RX, vo, sv, L[m].x + alpha * pc[m]);
+                }
+            } else if (ok) {
+                const long long id = (long long)G.cofs + (long long)c.plane * k;
+                b.xhi[id] += (double)b.xlo[id];
+                b.xlo[id] = 0.0f;
+            }
+        };
+        // B(m): s = A p, r -= alpha s, u = M r on the tile and the one-voxel
+        // ring of plane m (in-plane p from LDS, z neighbours from registers);
+        // u into LDS, the new r stored on the tile
+        auto stageB = [&](int m) {
+            const int k = k0 - 2 + m;
+            float sc = 0.0f, sh = 0.0f;
+            if (iter) {
+                const unsigned cc = cd[m] & 0xFFFFu, ch = cd[m] >> 16;
+                float xu = pc[m + 1], xd = pc[m - 1], hu = ph[m + 1], hd = ph[m - 1];
+                if (c.zbc != 0) {
+                    if (k == 0) xd = xu, hd = hu;
+                    else if (k == nz - 1) xu = xd, hu = hd;
+                }
+                const float(*Pm)[PX] = sp[m & 1];
+                const float qc = row_Ax32(cc, sHP, ih2, pc[m], Pm[ty + 2][tx + 3], Pm[ty + 2][tx + 1],
+                                          Pm[ty + 3][tx + 2], Pm[ty + 1][tx + 2], xu, xd);
+                sc = (cc & SC_IDENTITY) ? op_identity(pc[m], k, c) : qc;
+                const float qh = row_Ax32(ch, sHP, ih2, ph[m], Pm[by + 2][bx + 3], Pm[by + 2][bx + 1],
+                                          Pm[by + 3][bx + 2], Pm[by + 1][bx + 2], hu, hd);
+                sh = (ch & SC_IDENTITY) ? op_identity(ph[m], k, c) : qh;
+            }
+            const float rc = iter ? L[m].r - alpha * sc : L[m].r;
+            const float rh = iter ? L[m].rh - alpha * sh : L[m].rh;
+            uc[m] = rc * L[m].d;
+            su[m & 1][ty + 1][tx + 1] = uc[m];
+            if (R.inner) su[m & 1][R.ly + 1][R.lx + 1] = rh * L[m].dh;
+            if (m >= 2 && m <= KZ + 1) {
+                const bool ok = tp.live && k < nz;
+                if (iter) bstore(RA, ok ? cv : aend, o_rn + so[m] * pb4, rc);
+                if (ok) {
+                    cg_dot_add<MIR>(acc[0], rc, uc[m], c, k);
+                    acc[2] += (double)rc * (double)rc;
+                }
+            }
+        };
+        // C(m): w = A u on the tile of plane m and its term of w.u
+        auto stageC = [&](int m) {
+            const int k = k0 - 2 + m;
+            const unsigned cc = cd[m] & 0xFFFFu;
+            float xu = uc[m + 1], xd = uc[m - 1];
+            if (c.zbc != 0) {
+                if (k == 0) xd = xu;
+                else if (k == nz - 1) xu = xd;
+            }
+            const float(*Um)[HX] = su[m & 1];
+            const float qs = row_Ax32(cc, sHP, ih2, uc[m], Um[ty + 1][tx + 2], Um[ty + 1][tx], Um[ty + 2][tx + 1],
+                                      Um[ty][tx + 1], xu, xd);
+            const float wv = (cc & SC_IDENTITY) ? op_identity(uc[m], k, c) : qs;
+            if (tp.live && k < nz) cg_dot_add<MIR>(acc[1], wv, uc[m], c, k);
+        };
+        // Phase n: A(n); B(n - 1), whose in-plane p was written in phase
+        // n - 1 and whose upper z neighbour A(n) has just formed; C(n - 2),
+        // whose u was written in phase n - 1.  One barrier per phase; a
+        // plane's LDS buffer is rewritten two phases after it was filled.
+#pragma unroll
+        for (int n = 0; n < NW; ++n) {
+            if (n + 2 < NW) issue(n + 2);
+            stageA(n);
+            if (n >= 2) stageB(n - 1);
+            if (n >= 4) stageC(n - 2);
+            if (n + 1 < NW) __syncthreads();
+        }
+    }
+    write_block_partials_w<3>(acc, io ? b.part[1] : b.part[0], ps, T.blk, spart);
+}
+
 // Waves per SIMD the register budget is set for: 8 (4 workgroups per CU,
 // <= 64 VGPRs) at two planes per tile, 6 at four and eight, 4 at sixteen.
 // The batch kernel, which also holds its film's descriptor, is set for 5
@@ -382,6 +645,16 @@ constexpr int cg32_waves(int kz) { return kz >= 16 ? 4 : (kz <= 2 ? 8 : 6); }
 template <int KZ, int PM, bool MIR>
 __global__ __launch_bounds__(NT, cg32_waves(KZ)) void k_cg32(CG32Bufs b, StepConsts c, int nb, int ps, int t) {
     cg32_body<KZ, PM, MIR>(b, c, nb, ps, t, tile_id());
+}
+
+// The two-stencil form holds p, u and r of three planes besides the loads in
+// flight: 6 waves per SIMD at two planes per tile, 5 at four, 4 at eight and
+// sixteen.
+constexpr int cg32r_waves(int kz) { return kz <= 2 ? 6 : (kz == 4 ? 5 : 4); }
+
+template <int KZ, int PM, bool MIR>
+__global__ __launch_bounds__(NT, cg32r_waves(KZ)) void k_cg32r(CG32Bufs b, StepConsts c, int nb, int ps, int t) {
+    cg32r_body<KZ, PM, MIR>(b, c, nb, ps, t, tile_id());
 }
 
 // All films of a batch in one launch: the tile table maps each workgroup to
@@ -425,28 +698,42 @@ void launch_cg32_batch(hipStream_t s, int ntiles, int kz, int pm, bool mirrored,
 #undef HZO_CGB_PM
 }
 
-template <bool MIR>
+template <bool MIR, bool TWO>
 static void launch_cg32_as(hipStream_t s, const Grid& g, const StepConsts& c, const CG32Bufs& b, int t,
                            const LaunchEv& ev) {
 #define HZO_X , MIR
+    const int nb = blocks_for_impl(g), ps = part_stride_impl(nb);
     if (g.kz == 16) {  // one workgroup per CU, up to 256 VGPRs: the whole film in one round
-        const int nb = blocks_for_impl(g), pm = part_pm(b.nb_in);
+        const int pm = part_pm(b.nb_in);
         const dim3 gr = grid3(g), bl(BX, BY, 1);
-        HZO_PCG_LAUNCH_PM(k_cg32, 16, pm, gr, bl, s, b, c, nb, part_stride_impl(nb), t);
+        if constexpr (TWO) {
+            HZO_PCG_LAUNCH_PM(k_cg32r, 16, pm, gr, bl, s, b, c, nb, ps, t);
+        } else {
+            HZO_PCG_LAUNCH_PM(k_cg32, 16, pm, gr, bl, s, b, c, nb, ps, t);
+        }
         return;
     }
-    HZO_PCG_LAUNCH_X(k_cg32, g, b.nb_in, s, b, c, blocks_for_impl(g), part_stride_impl(blocks_for_impl(g)), t);
+    if constexpr (TWO) {
+        HZO_PCG_LAUNCH_X(k_cg32r, g, b.nb_in, s, b, c, nb, ps, t);
+    } else {
+        HZO_PCG_LAUNCH_X(k_cg32, g, b.nb_in, s, b, c, nb, ps, t);
+    }
 #undef HZO_X
 }
 
-void launch_cg32(hipStream_t s, const Grid& g, const StepConsts& c, const CG32Bufs& b, int t,
+void launch_cg32(hipStream_t s, const Grid& g, const StepConsts& c, const CG32Bufs& b, int t, bool two_stencil,
                  const LaunchEv& ev) {
-    if (c.zbc != 0)
-        launch_cg32_as<true>(s, g, c, b, t, ev);
-    else
-        launch_cg32_as<false>(s, g, c, b, t, ev);
+    if (two_stencil) {
+        if (c.zbc != 0)
+            launch_cg32_as<true, true>(s, g, c, b, t, ev);
+        else
+            launch_cg32_as<false, true>(s, g, c, b, t, ev);
+    } else if (c.zbc != 0) {
+        launch_cg32_as<true, false>(s, g, c, b, t, ev);
+    } else {
+        launch_cg32_as<false, false>(s, g, c, b, t, ev);
+    }
 }
-
 }  // namespace hzo
 
 namespace hzo {

@@ -304,6 +304,12 @@ class Film:
     def tile_depth(self) -> int:
         return L.lib().hzo_tile_depth(self.h)
 
+    @property
+    def cg_form(self) -> int:
+        """0 two-kernel PCG, 1 single-reduction CG carrying w and s, 2 the
+        single-reduction CG forming them by stencils."""
+        return L.lib().hzo_cg_form(self.h)
+
     def close(self):
         if getattr(self, "h", None):
             L.lib().hzo_destroy(self.h)

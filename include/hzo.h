@@ -84,8 +84,12 @@ typedef struct hzo_config {
     int refine_every;    /* FP32-refined: FP64 residual recompute period (iterations) */
     int tile_kz;         /* z planes per workgroup of the stencil kernels: 2, 4, 8 (16: single-
                             reduction CG only); 0 = automatic */
-    int cg_fused;        /* FP32-refined: 1 = single-reduction CG, one fused kernel per iteration
-                            (Chronopoulos-Gear); 0 = the two-kernel PCG of the FP64 form.
+    int cg_fused;        /* FP32-refined: 0 = the two-kernel PCG of the FP64 form; otherwise the
+                            single-reduction CG (Chronopoulos-Gear), one fused kernel per
+                            iteration, in one of two forms: 2 = carrying w = A u and s = A p
+                            in memory (42 bytes per voxel and iteration), 3 = forming both by
+                            stencils in the launch (26 bytes, more arithmetic), 1 = automatic
+                            (form 3 when the tiles are 16 planes deep, else form 2).
                             Films of 2^27 voxels or more always run the two-kernel PCG. */
 } hzo_config;
 
@@ -149,6 +153,11 @@ int hzo_synchronize(hzo_ctx* ctx);
 int hzo_grid_dims(hzo_ctx* ctx, int dims[3]);
 /* Tile depth (z planes per workgroup) the context launches with. */
 int hzo_tile_depth(hzo_ctx* ctx);
+/* Poisson solver the context runs: 0 two-kernel PCG (FP64 form or FP32
+ * cg_fused = 0), 1 single-reduction CG carrying w and s, 2 single-reduction
+ * CG forming them by stencils; -1 for a null context.  No reference
+ * counterpart (the reference has one solver). */
+int hzo_cg_form(hzo_ctx* ctx);
 
 /* ---------------------------------------------------------------- sweep driver
  * The staircase P-E loop of Evolve() (evolve.cu:213-410): the field starts at

@@ -29,9 +29,9 @@ def _inputs():
             for f, (nx, ny, nz, ng) in enumerate(SHAPES)]
 
 
-def _film(inp, kz=2):
+def _film(inp, kz=2, zbc=L.ZBC_DIRICHLET):
     return H.Film(inp, precision=L.PREC_FP32_REFINED, pcg_rule=L.PCG_RULE_RELRES, tol_rel=1e-8, tol_abs=1e-30,
-                  refine_every=20, tile_kz=kz)
+                  refine_every=20, tile_kz=kz, zbc=zbc)
 
 
 def _fields(inps, S):
@@ -65,6 +65,34 @@ def test_batched_films_match_single_film_runs_bit_for_bit(kz, mib, groups):
         # <Pmz> after the last step, from the single film's diagnostics
         d = single.diagnostics()
         assert abs(m_b[-1, f] - d[0] / inp.N) <= 1e-12 * max(1.0, abs(d[0] / inp.N))
+        single.close()
+    batch.close()
+    for f in films:
+        f.close()
+
+
+@pytest.mark.parametrize("kz", [2, 8])
+def test_batch_with_a_neumann_film_matches_single_film_runs(kz):
+    # film 1 has Neumann electrodes, so the group runs the kernel build that
+    # weights mirrored planes in the CG sums; the Dirichlet films of the same
+    # launches take weight 1 there and must still match their own runs
+    inps = _inputs()
+    S = 10
+    Ez = _fields(inps, S)
+    zbc = [L.ZBC_DIRICHLET, L.ZBC_NEUMANN, L.ZBC_DIRICHLET, L.ZBC_DIRICHLET]
+    _say(f"batch with a Neumann film, kz={kz}")
+    films = [_film(inp, kz, z) for inp, z in zip(inps, zbc)]
+    batch = H.Batch(films, tile_kz=kz)
+    batch.set_shape(kz, 0.0)
+    its_b = batch.run(Ez)
+    assert batch.counters()["unconverged"] == 0
+    for f, inp in enumerate(inps):
+        single = _film(inp, kz, zbc[f])
+        its_s = single.run(Ez[:, f])
+        Pb, phib, _ = films[f].get_state()
+        Ps, phis, _ = single.get_state()
+        assert np.array_equal(its_b[:, f], its_s), (f, its_b[:, f], its_s)
+        assert np.array_equal(Pb, Ps) and np.array_equal(phib, phis), f
         single.close()
     batch.close()
     for f in films:

@@ -1,6 +1,8 @@
 """FP32-refined path against the FP64 oracle.
 
-The FP32 form is not bit-comparable; it is held to:
+The FP32 form is not bit-comparable; it is held to (for each solver:
+cg_fused 1 automatic, 0 two-kernel PCG, 3 single-reduction CG forming
+s = A p and w = A u by stencils in the launch):
 * the Poisson solve meets ||b - A phi|| <= tol ||b|| in FP64 arithmetic
   (the residual is recomputed by the oracle from the returned phi and P);
 * after a P-E sweep, P and phi stay within stated relative errors of the FP64
@@ -21,7 +23,7 @@ def _say(*a):
     print("[gpu]", *a, flush=True)
 
 
-@pytest.mark.parametrize("fused", [1, 0])
+@pytest.mark.parametrize("fused", [1, 0, 3])
 def test_fp32_refined_solve_meets_fp64_tolerance(fused):
     inp = film_inputs(64, 48, 16, 12, fracs=(0.2, 0.6, 0.2), seed=6, jitter=0.2)
     _say(f"fp32 solve 64x48x16 fused={fused}")
@@ -42,7 +44,7 @@ def test_fp32_refined_solve_meets_fp64_tolerance(fused):
     g.close()
 
 
-@pytest.mark.parametrize("fused", [1, 0])
+@pytest.mark.parametrize("fused", [1, 0, 3])
 def test_fp32_refined_tracks_fp64_oracle_over_a_sweep(fused):
     inp = film_inputs(32, 32, 10, 8, fracs=(0.2, 0.6, 0.2), seed=9, jitter=0.2)
     nsteps = 160
@@ -74,7 +76,7 @@ def test_fp32_refined_tracks_fp64_oracle_over_a_sweep(fused):
     g.close()
 
 
-@pytest.mark.parametrize("fused", [1, 0])
+@pytest.mark.parametrize("fused", [1, 0, 3])
 def test_fp32_refined_neumann_electrodes(fused):
     """Neumann electrodes: the mirrored planes k = 0, nz - 1 make A = S A_s
     non-symmetric, and the FP32 solves weight those planes by 1/2 in their CG
@@ -111,3 +113,26 @@ def test_fp32_refined_neumann_electrodes(fused):
     assert max(i for i, _, _ in its) < 1500
     assert eP < 1e-5 and ephi < 1e-5
     g.close()
+
+
+@pytest.mark.parametrize("kz", [2, 4, 8, 16])
+def test_two_stencil_form_matches_the_carried_form(kz):
+    """The single-reduction CG forming s = A p and w = A u in the launch is
+    the same Krylov iteration as the form carrying w and s: over a few steps
+    the iteration counts agree to one iteration per solve and P, phi to
+    rounding.  The automatic choice takes the two-stencil form exactly at
+    tile depth 16."""
+    inp = film_inputs(72, 40, 36, 10, fracs=(0.2, 0.6, 0.2), seed=14, jitter=0.2)
+    Ez = inp.field_nd(H.triangle_wave(40, 3.0, 6))
+    out = {}
+    for fused in (2, 3, 1):
+        g = H.Film(inp, precision=L.PREC_FP32_REFINED, pcg_rule=L.PCG_RULE_RELRES, tol_rel=1e-8, tol_abs=1e-30,
+                   tile_kz=kz, cg_fused=fused)
+        assert g.tile_depth == kz
+        its = g.run(Ez)
+        out[fused] = (its, *g.get_state()[:2], g.cg_form)
+        g.close()
+    _say(f"kz={kz} iterations carried {out[2][0].tolist()} two-stencil {out[3][0].tolist()}")
+    assert out[2][3] == 1 and out[3][3] == 2 and out[1][3] == (2 if kz == 16 else 1)
+    assert np.max(np.abs(out[2][0] - out[3][0])) <= 1
+    assert rel_err(out[3][1], out[2][1]) < 1e-6 and rel_err(out[3][2], out[2][2]) < 1e-6
