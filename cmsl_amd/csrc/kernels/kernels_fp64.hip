@@ -603,16 +603,36 @@ void launch_diag_f64(hipStream_t s, const Grid& g, const double* P, const uint32
     hipExtLaunchKernelGGL(k_diag_f64, grid3(g), dim3(BX, BY, 1), 0, s, ev.start, ev.stop, 0, P, cw, g, part, blocks_for(g));
 }
 
+// The partials were written by the previous launch, mostly on other XCDs, so
+// each load is a round trip to memory: every thread issues FOLD_U loads per
+// column before it adds any, instead of one dependent load per step.
+constexpr int FOLD_U = 8;
+
 __global__ __launch_bounds__(NT) void k_fold_partials(const double* __restrict__ in, int nb, int ps,
                                                       double* __restrict__ out) {
     __shared__ WaveLDS<RED_MAXQ> W;
     const int tid = linear_tid();
+    double a[RED_MAXQ];
+#pragma unroll
+    for (int q = 0; q < RED_MAXQ; ++q) a[q] = 0.0;
+    for (int t0 = tid; t0 < nb; t0 += FOLD_U * NT) {
+        double v[RED_MAXQ][FOLD_U];
+#pragma unroll
+        for (int q = 0; q < RED_MAXQ; ++q)
+#pragma unroll
+            for (int u = 0; u < FOLD_U; ++u) {
+                const int t = t0 + u * NT;
+                v[q][u] = t < nb ? in[(long long)q * ps + t] : 0.0;
+            }
+#pragma unroll
+        for (int q = 0; q < RED_MAXQ; ++q)
+#pragma unroll
+            for (int u = 0; u < FOLD_U; ++u) a[q] += v[q][u];
+    }
 #pragma unroll
     for (int q = 0; q < RED_MAXQ; ++q) {
-        double a = 0.0;
-        for (int t = tid; t < nb; t += NT) a += in[(long long)q * ps + t];
-        a = wave_sum(a);
-        if ((tid & 63) == 0) W.v[q][tid >> 6] = a;
+        const double w = wave_sum(a[q]);
+        if ((tid & 63) == 0) W.v[q][tid >> 6] = w;
     }
     __syncthreads();
     if (tid == 0) {

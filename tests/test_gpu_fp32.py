@@ -115,24 +115,37 @@ def test_fp32_refined_neumann_electrodes(fused):
     g.close()
 
 
-@pytest.mark.parametrize("kz", [2, 4, 8, 16])
-def test_two_stencil_form_matches_the_carried_form(kz):
+@pytest.mark.parametrize("kz,zbc", [(2, 0), (4, 0), (8, 0), (16, 0), (4, 1), (16, 1)])
+def test_two_stencil_form_matches_the_carried_form(kz, zbc):
     """The single-reduction CG forming s = A p and w = A u in the launch is
-    the same Krylov iteration as the form carrying w and s: over a few steps
-    the iteration counts agree to one iteration per solve and P, phi to
-    rounding.  The automatic choice takes the two-stencil form exactly at
-    tile depth 16."""
+    the same Krylov iteration as the form carrying w and s.  With Dirichlet
+    electrodes the iteration counts agree to one iteration per solve and P,
+    phi to rounding.  With Neumann electrodes (mirrored planes, gauge node,
+    weighted inner products) the system is near-singular and the carried
+    recurrence s = w + beta s drifts from A p, so the counts differ: every
+    solve must converge, the two-stencil form (s = A p exactly) may not need
+    more iterations than 2 % above the carried one, and P, phi agree to the
+    solver tolerance.  The automatic choice takes the two-stencil form
+    exactly at tile depth 16."""
     inp = film_inputs(72, 40, 36, 10, fracs=(0.2, 0.6, 0.2), seed=14, jitter=0.2)
     Ez = inp.field_nd(H.triangle_wave(40, 3.0, 6))
     out = {}
     for fused in (2, 3, 1):
         g = H.Film(inp, precision=L.PREC_FP32_REFINED, pcg_rule=L.PCG_RULE_RELRES, tol_rel=1e-8, tol_abs=1e-30,
-                   tile_kz=kz, cg_fused=fused)
+                   tile_kz=kz, cg_fused=fused, zbc=zbc)
         assert g.tile_depth == kz
-        its = g.run(Ez)
-        out[fused] = (its, *g.get_state()[:2], g.cg_form)
+        its = []
+        for e in Ez:
+            st = g.step(float(e))
+            assert st.status == 0, (fused, st.status)
+            its.append(st.iters)
+        out[fused] = (np.array(its), *g.get_state()[:2], g.cg_form)
         g.close()
-    _say(f"kz={kz} iterations carried {out[2][0].tolist()} two-stencil {out[3][0].tolist()}")
+    _say(f"kz={kz} zbc={zbc} iterations carried {out[2][0].tolist()} two-stencil {out[3][0].tolist()}")
     assert out[2][3] == 1 and out[3][3] == 2 and out[1][3] == (2 if kz == 16 else 1)
-    assert np.max(np.abs(out[2][0] - out[3][0])) <= 1
-    assert rel_err(out[3][1], out[2][1]) < 1e-6 and rel_err(out[3][2], out[2][2]) < 1e-6
+    if zbc == 0:
+        assert np.max(np.abs(out[2][0] - out[3][0])) <= 1
+        assert rel_err(out[3][1], out[2][1]) < 1e-6 and rel_err(out[3][2], out[2][2]) < 1e-6
+    else:
+        assert out[3][0].sum() <= 1.02 * out[2][0].sum()
+        assert rel_err(out[3][1], out[2][1]) < 1e-5 and rel_err(out[3][2], out[2][2]) < 1e-5
