@@ -600,8 +600,10 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
         return set_error(HZO_ERR_ARG, "hzo_create: precision %d", cfg->precision);
     if (cfg->refine_every < 0) return set_error(HZO_ERR_ARG, "hzo_create: refine_every < 0");
     if (cfg->tile_kz != 0 && cfg->tile_kz != 2 && cfg->tile_kz != 4 && cfg->tile_kz != 8 &&
-        !(cfg->tile_kz == 16 && cfg->precision == HZO_PREC_FP32_REFINED && cfg->cg_fused))
-        return set_error(HZO_ERR_ARG, "hzo_create: tile_kz must be 0, 2, 4 or 8 (16: fused FP32 CG only)");
+        !(cfg->tile_kz == 16 && cfg->precision == HZO_PREC_FP32_REFINED && cfg->cg_fused) &&
+        !(cfg->tile_kz == 32 && cfg->precision == HZO_PREC_FP32_REFINED && (cfg->cg_fused == 1 || cfg->cg_fused == 3)))
+        return set_error(HZO_ERR_ARG, "hzo_create: tile_kz must be 0, 2, 4 or 8 (16: single-reduction FP32 CG only; "
+                                      "32: its two-stencil form only)");
     if (cfg->pcg_rule != 0 && cfg->pcg_rule != 1) return set_error(HZO_ERR_ARG, "hzo_create: pcg_rule %d", cfg->pcg_rule);
     if (cfg->max_iter < 0) return set_error(HZO_ERR_ARG, "hzo_create: max_iter < 0");
     const long long N = (long long)cfg->nx * cfg->ny * cfg->nz;
@@ -611,8 +613,9 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
     // run the two-kernel FP32 PCG
     if (cfg->cg_fused < 0 || cfg->cg_fused > 3) return set_error(HZO_ERR_ARG, "hzo_create: cg_fused %d", cfg->cg_fused);
     const bool cg_fused = cfg->cg_fused && 16 * N < (1LL << 31);
-    if (cfg->tile_kz == 16 && cfg->precision == HZO_PREC_FP32_REFINED && !cg_fused)
-        return set_error(HZO_ERR_ARG, "hzo_create: tile_kz 16 needs the single-reduction CG (at most 2^27 - 1 voxels)");
+    if (cfg->tile_kz >= 16 && cfg->precision == HZO_PREC_FP32_REFINED && !cg_fused)
+        return set_error(HZO_ERR_ARG, "hzo_create: tile_kz %d needs the single-reduction CG (at most 2^27 - 1 voxels)",
+                         cfg->tile_kz);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return set_error(HZO_ERR_HIP, "hzo_create: no HIP device available");
@@ -646,8 +649,13 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
                     break;
                 }
             // very large films: depth 16 keeps the partial sums at most
-            // 2048 (four per consuming thread) instead of a reduction loop
-            if (kz == 8 && txy * ((cfg->nz + 7) / 8) > 2048 && txy * ((cfg->nz + 15) / 16) >= 512) kz = 16;
+            // 2048 (four per consuming thread) instead of a fold launch, and
+            // the two-stencil form (automatic choice) goes to depth 32 when
+            // that still gives 512 workgroups, halving its z-ghost planes
+            if (kz == 8 && txy * ((cfg->nz + 7) / 8) > 2048 && txy * ((cfg->nz + 15) / 16) >= 512) {
+                kz = 16;
+                if (cfg->cg_fused != 2 && txy * ((cfg->nz + 31) / 32) >= 512) kz = 32;
+            }
         } else if (kz == 0) {
             kz = f32 ? 8 : 4;
             const long long cap = f32 ? 512 : 1024;
@@ -663,7 +671,7 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
         // the two-stencil form moves 26 instead of 42 bytes per voxel and
         // iteration but does more work per byte: it wins on films whose
         // tiles are 16 deep (HBM-bound), the carried form on smaller ones
-        x->cg_two = fused && (cfg->cg_fused == 3 || (cfg->cg_fused == 1 && kz == 16));
+        x->cg_two = fused && (cfg->cg_fused == 3 || (cfg->cg_fused == 1 && kz >= 16));
     }
     hzo::StepConsts& c = x->c;
     c.nx = cfg->nx;

@@ -446,7 +446,10 @@ struct Plane2 {
     float rh, ph, dh;
 };
 
-template <int KZ, int PM, bool MIR>
+// The mirrored-plane weights of Neumann films are applied at run time (a
+// weight of exactly 1 elsewhere leaves every sum unchanged); a build without
+// them allocates more registers at deep tiles and runs slower.
+template <int KZ, int PM>
 __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& c, int nb, int ps, int t,
                                            const TileId& T) {
     constexpr int NW = KZ + 4;  // window plane m is z = k0 - 2 + m; the tile's planes are m = 2 .. KZ + 1
@@ -471,6 +474,10 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
     // the ring stencil of a lane without a one-voxel-ring cell runs on its
     // own column (in range, result unused)
     const int bx = R.inner ? R.lx : tx, by = R.inner ? R.ly : ty;
+    // A wave is one tile row (64 x 8 threads); the one-voxel ring's cells
+    // sit on waves 0-2, so the ring stencil is skipped by the other waves
+    // (a scalar branch with no loads inside).
+    const bool winner = __builtin_amdgcn_readfirstlane(ty) * TX < R1;
     const unsigned pb4 = c.plane * 4u, pb2 = c.plane * 2u;
     const rsrc_t Rsc = make_rsrc(b.sc, c.nvox * 2u);
     const rsrc_t Rdi = make_rsrc(b.lut32 + LUT_DINV, SC_LUT_SIZE * 4u);
@@ -478,18 +485,22 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
     const HPair<float> hpv = load_hpair(b.lut32);
     Partials<3, PM> P;
     load_partials<3, PM>(ip ? b.pin[1] : b.pin[0], b.nb_in, b.ps_in, P);
-    unsigned so[NW], cd[NW];  // plane offsets; stencil codes (centre | ring << 16)
+    // plane offsets; stencil codes (centre | ring << 16), each loaded one
+    // phase before its plane's data, whose D^-1 gathers need it
+    unsigned so[NW], cd[NW];
 #pragma unroll
-    for (int m = 0; m < NW; ++m) {
-        so[m] = (unsigned)min(max(k0 - 2 + m, 0), nz - 1);
+    for (int m = 0; m < NW; ++m) so[m] = (unsigned)min(max(k0 - 2 + m, 0), nz - 1);
+    auto load_code = [&](int m) {
         cd[m] = bload_code(Rsc, G.cofs * 2u, so[m] * pb2) | (bload_code(Rsc, rofs * 2u, so[m] * pb2) << 16);
-    }
+    };
+#pragma unroll
+    for (int m = 0; m < 4; ++m) load_code(m);
 
     PcgState s = load_state(b.st + ip);
     if (s.done) {
         keep_live(hpv.p);
 #pragma unroll
-        for (int m = 0; m < NW; ++m) keep_live(cd[m]);
+        for (int m = 0; m < 4; ++m) keep_live(cd[m]);
         keep_live_partials(P);
         pass_state(s, b.st + io, T.lead);
         return;
@@ -539,7 +550,7 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
     if (mode == 1) return;
     double acc[3] = {0.0, 0.0, 0.0};
     if (mode == 2) {
-        cg32_resid<KZ, MIR>(b, c, tp, G, k0, RA, o_r, Rsc, Rdi, acc);
+        cg32_resid<KZ, true>(b, c, tp, G, k0, RA, o_r, Rsc, Rdi, acc);
     } else {
         const bool iter = mode == 0;
         const float alpha = (float)alpha_d, beta = (float)beta_d;
@@ -585,20 +596,24 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
                 const float qc = row_Ax32(cc, sHP, ih2, pc[m], Pm[ty + 2][tx + 3], Pm[ty + 2][tx + 1],
                                           Pm[ty + 3][tx + 2], Pm[ty + 1][tx + 2], xu, xd);
                 sc = (cc & SC_IDENTITY) ? op_identity(pc[m], k, c) : qc;
-                const float qh = row_Ax32(ch, sHP, ih2, ph[m], Pm[by + 2][bx + 3], Pm[by + 2][bx + 1],
-                                          Pm[by + 3][bx + 2], Pm[by + 1][bx + 2], hu, hd);
-                sh = (ch & SC_IDENTITY) ? op_identity(ph[m], k, c) : qh;
+                if (winner) {
+                    const float qh = row_Ax32(ch, sHP, ih2, ph[m], Pm[by + 2][bx + 3], Pm[by + 2][bx + 1],
+                                              Pm[by + 3][bx + 2], Pm[by + 1][bx + 2], hu, hd);
+                    sh = (ch & SC_IDENTITY) ? op_identity(ph[m], k, c) : qh;
+                }
             }
             const float rc = iter ? L[m].r - alpha * sc : L[m].r;
-            const float rh = iter ? L[m].rh - alpha * sh : L[m].rh;
             uc[m] = rc * L[m].d;
             su[m & 1][ty + 1][tx + 1] = uc[m];
-            if (R.inner) su[m & 1][R.ly + 1][R.lx + 1] = rh * L[m].dh;
+            if (R.inner) {
+                const float rh = iter ? L[m].rh - alpha * sh : L[m].rh;
+                su[m & 1][R.ly + 1][R.lx + 1] = rh * L[m].dh;
+            }
             if (m >= 2 && m <= KZ + 1) {
                 const bool ok = tp.live && k < nz;
                 if (iter) bstore(RA, ok ? cv : aend, o_rn + so[m] * pb4, rc);
                 if (ok) {
-                    cg_dot_add<MIR>(acc[0], rc, uc[m], c, k);
+                    cg_dot_add<true>(acc[0], rc, uc[m], c, k);
                     acc[2] += (double)rc * (double)rc;
                 }
             }
@@ -616,7 +631,7 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
             const float qs = row_Ax32(cc, sHP, ih2, uc[m], Um[ty + 1][tx + 2], Um[ty + 1][tx], Um[ty + 2][tx + 1],
                                       Um[ty][tx + 1], xu, xd);
             const float wv = (cc & SC_IDENTITY) ? op_identity(uc[m], k, c) : qs;
-            if (tp.live && k < nz) cg_dot_add<MIR>(acc[1], wv, uc[m], c, k);
+            if (tp.live && k < nz) cg_dot_add<true>(acc[1], wv, uc[m], c, k);
         };
         // Phase n: A(n); B(n - 1), whose in-plane p was written in phase
         // n - 1 and whose upper z neighbour A(n) has just formed; C(n - 2),
@@ -624,6 +639,7 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
         // plane's LDS buffer is rewritten two phases after it was filled.
 #pragma unroll
         for (int n = 0; n < NW; ++n) {
+            if (n + 4 < NW) load_code(n + 4);
             if (n + 2 < NW) issue(n + 2);
             stageA(n);
             if (n >= 2) stageB(n - 1);
@@ -648,13 +664,13 @@ __global__ __launch_bounds__(NT, cg32_waves(KZ)) void k_cg32(CG32Bufs b, StepCon
 }
 
 // The two-stencil form holds p, u and r of three planes besides the loads in
-// flight: 6 waves per SIMD at two planes per tile, 5 at four, 4 at eight and
-// sixteen.
-constexpr int cg32r_waves(int kz) { return kz <= 2 ? 6 : (kz == 4 ? 5 : 4); }
+// flight: 6 waves per SIMD at two planes per tile, 5 at four, 4 at eight,
+// sixteen and thirty-two.
+constexpr int cg32r_waves(int kz) { return kz <= 2 ? 6 : (kz == 4 ? 5 : 4); }  // 4 at 16 and 32
 
-template <int KZ, int PM, bool MIR>
+template <int KZ, int PM>
 __global__ __launch_bounds__(NT, cg32r_waves(KZ)) void k_cg32r(CG32Bufs b, StepConsts c, int nb, int ps, int t) {
-    cg32r_body<KZ, PM, MIR>(b, c, nb, ps, t, tile_id());
+    cg32r_body<KZ, PM>(b, c, nb, ps, t, tile_id());
 }
 
 // All films of a batch in one launch: the tile table maps each workgroup to
@@ -698,41 +714,47 @@ void launch_cg32_batch(hipStream_t s, int ntiles, int kz, int pm, bool mirrored,
 #undef HZO_CGB_PM
 }
 
-template <bool MIR, bool TWO>
+template <bool MIR>
 static void launch_cg32_as(hipStream_t s, const Grid& g, const StepConsts& c, const CG32Bufs& b, int t,
                            const LaunchEv& ev) {
 #define HZO_X , MIR
     const int nb = blocks_for_impl(g), ps = part_stride_impl(nb);
-    if (g.kz == 16) {  // one workgroup per CU, up to 256 VGPRs: the whole film in one round
+    if (g.kz == 16) {  // deep tiles for large films
         const int pm = part_pm(b.nb_in);
         const dim3 gr = grid3(g), bl(BX, BY, 1);
-        if constexpr (TWO) {
-            HZO_PCG_LAUNCH_PM(k_cg32r, 16, pm, gr, bl, s, b, c, nb, ps, t);
+        HZO_PCG_LAUNCH_PM(k_cg32, 16, pm, gr, bl, s, b, c, nb, ps, t);
+        return;
+    }
+    HZO_PCG_LAUNCH_X(k_cg32, g, b.nb_in, s, b, c, nb, ps, t);
+#undef HZO_X
+}
+
+static void launch_cg32r(hipStream_t s, const Grid& g, const StepConsts& c, const CG32Bufs& b, int t,
+                         const LaunchEv& ev) {
+#define HZO_X
+    const int nb = blocks_for_impl(g), ps = part_stride_impl(nb);
+    if (g.kz >= 16) {  // deep tiles for large films
+        const int pm = part_pm(b.nb_in);
+        const dim3 gr = grid3(g), bl(BX, BY, 1);
+        if (g.kz == 32) {
+            HZO_PCG_LAUNCH_PM(k_cg32r, 32, pm, gr, bl, s, b, c, nb, ps, t);
         } else {
-            HZO_PCG_LAUNCH_PM(k_cg32, 16, pm, gr, bl, s, b, c, nb, ps, t);
+            HZO_PCG_LAUNCH_PM(k_cg32r, 16, pm, gr, bl, s, b, c, nb, ps, t);
         }
         return;
     }
-    if constexpr (TWO) {
-        HZO_PCG_LAUNCH_X(k_cg32r, g, b.nb_in, s, b, c, nb, ps, t);
-    } else {
-        HZO_PCG_LAUNCH_X(k_cg32, g, b.nb_in, s, b, c, nb, ps, t);
-    }
+    HZO_PCG_LAUNCH_X(k_cg32r, g, b.nb_in, s, b, c, nb, ps, t);
 #undef HZO_X
 }
 
 void launch_cg32(hipStream_t s, const Grid& g, const StepConsts& c, const CG32Bufs& b, int t, bool two_stencil,
                  const LaunchEv& ev) {
-    if (two_stencil) {
-        if (c.zbc != 0)
-            launch_cg32_as<true, true>(s, g, c, b, t, ev);
-        else
-            launch_cg32_as<false, true>(s, g, c, b, t, ev);
-    } else if (c.zbc != 0) {
-        launch_cg32_as<true, false>(s, g, c, b, t, ev);
-    } else {
-        launch_cg32_as<false, false>(s, g, c, b, t, ev);
-    }
+    if (two_stencil)
+        launch_cg32r(s, g, c, b, t, ev);
+    else if (c.zbc != 0)
+        launch_cg32_as<true>(s, g, c, b, t, ev);
+    else
+        launch_cg32_as<false>(s, g, c, b, t, ev);
 }
 }  // namespace hzo
 

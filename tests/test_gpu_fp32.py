@@ -115,7 +115,7 @@ def test_fp32_refined_neumann_electrodes(fused):
     g.close()
 
 
-@pytest.mark.parametrize("kz,zbc", [(2, 0), (4, 0), (8, 0), (16, 0), (4, 1), (16, 1)])
+@pytest.mark.parametrize("kz,zbc", [(2, 0), (4, 0), (8, 0), (16, 0), (32, 0), (4, 1), (16, 1), (32, 1)])
 def test_two_stencil_form_matches_the_carried_form(kz, zbc):
     """The single-reduction CG forming s = A p and w = A u in the launch is
     the same Krylov iteration as the form carrying w and s.  With Dirichlet
@@ -126,14 +126,16 @@ def test_two_stencil_form_matches_the_carried_form(kz, zbc):
     solve must converge, the two-stencil form (s = A p exactly) may not need
     more iterations than 2 % above the carried one, and P, phi agree to the
     solver tolerance.  The automatic choice takes the two-stencil form
-    exactly at tile depth 16."""
+    exactly at tile depths 16 and 32; depth 32 exists only in that form, so
+    the carried reference runs at depth 16 there."""
     inp = film_inputs(72, 40, 36, 10, fracs=(0.2, 0.6, 0.2), seed=14, jitter=0.2)
     Ez = inp.field_nd(H.triangle_wave(40, 3.0, 6))
     out = {}
     for fused in (2, 3, 1):
+        kf = min(kz, 16) if fused == 2 else kz
         g = H.Film(inp, precision=L.PREC_FP32_REFINED, pcg_rule=L.PCG_RULE_RELRES, tol_rel=1e-8, tol_abs=1e-30,
-                   tile_kz=kz, cg_fused=fused, zbc=zbc)
-        assert g.tile_depth == kz
+                   tile_kz=kf, cg_fused=fused, zbc=zbc)
+        assert g.tile_depth == kf
         its = []
         for e in Ez:
             st = g.step(float(e))
@@ -142,7 +144,7 @@ def test_two_stencil_form_matches_the_carried_form(kz, zbc):
         out[fused] = (np.array(its), *g.get_state()[:2], g.cg_form)
         g.close()
     _say(f"kz={kz} zbc={zbc} iterations carried {out[2][0].tolist()} two-stencil {out[3][0].tolist()}")
-    assert out[2][3] == 1 and out[3][3] == 2 and out[1][3] == (2 if kz == 16 else 1)
+    assert out[2][3] == 1 and out[3][3] == 2 and out[1][3] == (2 if kz >= 16 else 1)
     if zbc == 0:
         assert np.max(np.abs(out[2][0] - out[3][0])) <= 1
         assert rel_err(out[3][1], out[2][1]) < 1e-6 and rel_err(out[3][2], out[2][2]) < 1e-6

@@ -42,6 +42,28 @@ def test_device_count_and_create_without_gpu_fail_cleanly():
         assert b"no HIP device" in L.lib().hzo_last_error()
 
 
+@pytest.mark.parametrize("kw,msg", [
+    (dict(cg_fused=4), b"cg_fused"),
+    (dict(cg_fused=-1), b"cg_fused"),
+    (dict(tile_kz=32, cg_fused=2), b"tile_kz"),
+    (dict(tile_kz=32, cg_fused=0), b"tile_kz"),
+    (dict(tile_kz=16, cg_fused=0), b"tile_kz"),
+    (dict(tile_kz=32, precision=L.PREC_FP64), b"tile_kz"),
+    (dict(tile_kz=12), b"tile_kz"),
+])
+def test_create_rejects_solver_and_tile_combinations(kw, msg):
+    # argument checks run before any device call, so they hold without a GPU:
+    # depth 16 needs the single-reduction CG, depth 32 its two-stencil form
+    import ctypes as C
+    base = dict(nx=8, ny=8, nz=4, precision=L.PREC_FP32_REFINED)
+    base.update(kw)
+    cfg = L.default_config(**base)
+    h = C.c_void_p()
+    rc = L.lib().hzo_create(C.byref(cfg), C.byref(h))
+    assert rc == -1 and not h.value
+    assert msg in L.lib().hzo_last_error()
+
+
 def test_splitmix64_published_sequence():
     # SplitMix64 seeded with 0 (Steele, Lea, Flood 2014; Vigna's reference code)
     out = F.splitmix64(0, 3)
