@@ -629,33 +629,39 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
     x->g.N = N;
     // Tile depth (z planes per 64x8 workgroup) unless the caller fixes it;
     // the inner-product grouping, hence the last bits of the dot products,
-    // follows the tiling.  Single-reduction FP32 CG (z-streaming): the deepest
-    // of {8, 4} that still gives at least 512 workgroups, else 2 -- deeper
-    // tiles mean fewer partial sums and fewer z-ghost re-reads -- and 16 when
-    // depth 8 would leave more than 2048 partial sums.  Two-kernel FP32 PCG: the smallest depth giving at
-    // most 512 workgroups (one partial per consuming thread, two workgroups
-    // per CU), else 8.  FP64: the smallest of {2, 4} giving at most 1024
-    // workgroups, else 4 (its depth-8 tiles exceed 128 VGPRs).
+    // follows the tiling.  Single-reduction FP32 CG (z-streaming), measured
+    // on MI355X: once depth 8 would need more than one round of workgroups
+    // (512: two per CU), the two-stencil form at the deepest of {32, 16}
+    // that still gives 512 workgroups and pads at most an eighth of the
+    // planes (automatic choice or form 3); otherwise the carried form at the
+    // deepest of {8, 4} giving at least 512 workgroups, else 2, and 16 when
+    // depth 8 would leave more than 2048 partial sums.  Two-kernel FP32 PCG:
+    // the smallest depth giving at most 512 workgroups (one partial per
+    // consuming thread, two workgroups per CU), else 8.  FP64: the smallest
+    // of {2, 4} giving at most 1024 workgroups, else 4 (its depth-8 tiles
+    // exceed 128 VGPRs).
     {
         const long long txy = (long long)((cfg->nx + 63) / 64) * ((cfg->ny + 7) / 8);
         const bool f32 = cfg->precision == HZO_PREC_FP32_REFINED && !x->cfg.cg_fused;
         const bool fused = cfg->precision == HZO_PREC_FP32_REFINED && x->cfg.cg_fused;
+        auto tiles_at = [&](int k) { return txy * ((cfg->nz + k - 1) / k); };
         int kz = cfg->tile_kz;
+        if (kz == 0 && fused && cfg->cg_fused != 2 && tiles_at(8) > 512)
+            for (int cand : {32, 16})
+                if (tiles_at(cand) >= 512 && 8LL * ((cfg->nz + cand - 1) / cand) * cand <= 9LL * cfg->nz) {
+                    kz = cand;
+                    break;
+                }
         if (kz == 0 && fused) {
             kz = 2;
             for (int cand : {8, 4})
-                if (txy * ((cfg->nz + cand - 1) / cand) >= 512) {
+                if (tiles_at(cand) >= 512) {
                     kz = cand;
                     break;
                 }
             // very large films: depth 16 keeps the partial sums at most
-            // 2048 (four per consuming thread) instead of a fold launch, and
-            // the two-stencil form (automatic choice) goes to depth 32 when
-            // that still gives 512 workgroups, halving its z-ghost planes
-            if (kz == 8 && txy * ((cfg->nz + 7) / 8) > 2048 && txy * ((cfg->nz + 15) / 16) >= 512) {
-                kz = 16;
-                if (cfg->cg_fused != 2 && txy * ((cfg->nz + 31) / 32) >= 512) kz = 32;
-            }
+            // 2048 (four per consuming thread) instead of a fold launch
+            if (kz == 8 && tiles_at(8) > 2048 && tiles_at(16) >= 512) kz = 16;
         } else if (kz == 0) {
             kz = f32 ? 8 : 4;
             const long long cap = f32 ? 512 : 1024;
@@ -669,8 +675,9 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
         }
         x->g.kz = kz;
         // the two-stencil form moves 26 instead of 42 bytes per voxel and
-        // iteration but does more work per byte: it wins on films whose
-        // tiles are 16 deep (HBM-bound), the carried form on smaller ones
+        // iteration but does more work per byte: it runs the deep tiles
+        // (16, 32) of films beyond one round of depth-8 workgroups, the
+        // carried form the shallow tiles of smaller ones
         x->cg_two = fused && (cfg->cg_fused == 3 || (cfg->cg_fused == 1 && kz >= 16));
     }
     hzo::StepConsts& c = x->c;
