@@ -22,103 +22,164 @@
 namespace hzo {
 
 // ---------------------------------------------------------------- TDGL
+// z-streaming: a 64x8 workgroup walks its kz planes bottom to top.  Each
+// plane's P (6 components) and potential phi = x_hi + x_lo are staged in LDS
+// with a one-voxel x-y halo (the in-plane neighbours of the Laplacian and
+// of E = -grad phi), double-buffered so one barrier per plane suffices; a
+// thread keeps its own column's values of the planes below and above in
+// registers (the z neighbours, clamped at the film's faces as the reference
+// clamps them), and the next plane's loads, including its voxels' grain
+// rotations, are issued before the current plane is evaluated.  The phase
+// table sits in LDS.  Per voxel the arithmetic and its order are those of
+// kernels_evol.cu:47-242.
+struct Col32 {
+    float P[6];
+    double phi;
+};
+
 __device__ __forceinline__ void tdgl_f32_body(const Fp32Bufs& b, const StepConsts& c, float Ez_ext, int nb,
                                               const TileId& T) {
-    const int nx = c.nx, ny = c.ny, nz = c.nz;
-    const long long N = (long long)nx * ny * nz;
-    const int i = T.bx * BX + threadIdx.x;
-    const int j = T.by * BY + threadIdx.y;
-    const int k0 = T.bz * c.kz;
+    __shared__ float sP[2][6][HY][HX];
+    __shared__ double sphi[2][HY][HX];
+    __shared__ PhaseC32 sph[PTAB_ENTRIES];
+    const int nz = c.nz;
+    const long long N = (long long)c.nx * c.ny * nz, plane = (long long)c.nx * c.ny;
+    const TilePos tp = tile_pos(c, T);
+    const TileCols G = tile_cols(tp, c);
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = linear_tid();
+    if (tid < PTAB_ENTRIES) sph[tid] = b.ptab32[tid];
+    const long long cc = G.cofs, hc = tp.has_h ? G.hofs : G.cofs;
+    // buffer loads: one resource per array (each below 2^32 bytes), the
+    // column as a 32-bit lane offset, the plane as a uniform offset
+    rsrc_t RP[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) RP[q] = make_rsrc(b.Pold + q * N, c.nvox * 4u);
+    const rsrc_t Rxh = make_rsrc(b.xhi, c.nvox * 8u), Rxl = make_rsrc(b.xlo, c.nvox * 4u);
+    const unsigned pb = c.plane;
+    auto ld = [&](long long col, int k, Col32& v) {
+        const unsigned o = (unsigned)col, po = (unsigned)k * pb;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) v.P[q] = bload<float>(RP[q], o * 4u, po * 4u);
+        v.phi = c.dep_flag ? bload<double>(Rxh, o * 8u, po * 8u) + (double)bload<float>(Rxl, o * 4u, po * 4u) : 0.0;
+    };
+    const int k0 = T.bz * c.kz, k1 = min(k0 + c.kz, nz);
+    Col32 below, cur, above, ring;
+    ld(cc, max(k0 - 1, 0), below);
+    ld(cc, k0, cur);
+    ld(cc, min(k0 + 1, nz - 1), above);
+    ld(hc, k0, ring);
+    uint32_t cw = b.cw[cc + plane * k0];
+    float Tn[9];
+    {
+        const float* Tr = b.TR32 + (long long)(cw & CW_GID_MASK) * 9;
+#pragma unroll
+        for (int e = 0; e < 9; ++e) Tn[e] = Tr[e];
+    }
     double loc[2] = {0.0, 0.0};
-    if (i < nx && j < ny) {
-        const int e = (i + 1 == nx) ? 0 : i + 1;
-        const int w = (i == 0) ? nx - 1 : i - 1;
-        const int n = (j + 1 == ny) ? 0 : j + 1;
-        const int s = (j == 0) ? ny - 1 : j - 1;
-        const int k1 = min(k0 + c.kz, nz);
-        for (int k = k0; k < k1; ++k) {
-            const long long id = vid(i, j, k, nx, ny);
-            const uint32_t cw = b.cw[id];
+    for (int k = k0; k < k1; ++k) {
+        const int sb = (k - k0) & 1;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) sP[sb][q][ty + 1][tx + 1] = cur.P[q];
+        sphi[sb][ty + 1][tx + 1] = cur.phi;
+        if (tp.has_h) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) sP[sb][q][tp.hy][tp.hx] = ring.P[q];
+            sphi[sb][tp.hy][tp.hx] = ring.phi;
+        }
+        // the next plane: its halo cell, the column two planes up, its cell word
+        Col32 ring_n, above_n;
+        const int kn = min(k + 1, k1 - 1);
+        ld(hc, kn, ring_n);
+        ld(cc, min(k + 2, nz - 1), above_n);
+        const uint32_t cw_n = b.cw[cc + plane * kn];
+        float T9[9];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) T9[e] = Tn[e];
+        __syncthreads();
+        if (tp.live) {
+            const long long id = cc + plane * k;
             if (cw & CW_VOID) {
 #pragma unroll
                 for (int q = 0; q < 6; ++q) b.Pnew[q * N + id] = 0.0f;
-                continue;
-            }
-            const int u = (k == nz - 1) ? k : k + 1;
-            const int d = (k == 0) ? k : k - 1;
-            const long long ie = vid(e, j, k, nx, ny), iw = vid(w, j, k, nx, ny);
-            const long long in = vid(i, n, k, nx, ny), is = vid(i, s, k, nx, ny);
-            const long long iu = vid(i, j, u, nx, ny), idn = vid(i, j, d, nx, ny);
-            float sum[6], ctr[6];
+            } else {
+                float sum[6];
 #pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                const float* P = b.Pold + q * N;
-                sum[q] = P[ie] + P[iw] + P[in] + P[is] + P[iu] + P[idn];
-                ctr[q] = P[id];
-            }
-            const PhaseC32 pc = b.ptab32[((cw >> CW_PHASE_SHIFT) & 3u) * 2u + ((cw & CW_IFACE) ? 1u : 0u)];
-            const float* T = b.TR32 + (long long)(cw & CW_GID_MASK) * 9;
-            const float t11 = T[0], t12 = T[1], t13 = T[2], t21 = T[3], t22 = T[4], t23 = T[5];
-            const float t31 = T[6], t32 = T[7], t33 = T[8];
-            const float mx = ctr[0], my = ctr[1], mz = ctr[2], sx = ctr[3], sy = ctr[4], sz = ctr[5];
-            const float mXl = t11 * mx + t12 * my + t13 * mz;
-            const float mYl = t21 * mx + t22 * my + t23 * mz;
-            const float mZl = t31 * mx + t32 * my + t33 * mz;
-            const float sXl = t11 * sx + t12 * sy + t13 * sz;
-            const float sYl = t21 * sx + t22 * sy + t23 * sz;
-            const float sZl = t31 * sx + t32 * sy + t33 * sz;
-            loc[0] += (double)mZl;
-            loc[1] += (double)sZl;
-            const float m2 = mZl * mZl, m3 = m2 * mZl, m4 = m3 * mZl, m5 = m4 * mZl;
-            const float s2 = sZl * sZl, s3 = s2 * sZl, s4 = s3 * sZl, s5 = s4 * sZl;
-            const float fmz = pc.ag * mZl + pc.beta_4 * m3 + pc.beta_12 * mZl * s2 +
-                              pc.gamma_6 * (m5 + 10.0f * m3 * s2 + 5.0f * mZl * s4);
-            const float fsz = pc.ags * sZl + pc.beta_4 * s3 + pc.beta_12 * sZl * m2 +
-                              pc.gamma_6 * (s5 + 10.0f * m2 * s3 + 5.0f * sZl * m4);
-            const float fmx = pc.a0_2 * mXl, fmy = pc.a0_2 * mYl, fsx = pc.a0_2 * sXl, fsy = pc.a0_2 * sYl;
-            float dmx = t11 * fmx + t21 * fmy + t31 * fmz;
-            float dmy = t12 * fmx + t22 * fmy + t32 * fmz;
-            float dmz = t13 * fmx + t23 * fmy + t33 * fmz;
-            const float dsx = t11 * fsx + t21 * fsy + t31 * fsz;
-            const float dsy = t12 * fsx + t22 * fsy + t32 * fsz;
-            const float dsz = t13 * fsx + t23 * fsy + t33 * fsz;
-            if (c.dep_flag) {  // E = -grad(x_hi + x_lo)
-                auto phi = [&](long long v) { return b.xhi[v] + (double)b.xlo[v]; };
-                double up_v, dn_v;
-                if (c.zbc == 0) {
-                    up_v = (k == nz - 1) ? c.g_top : phi(vid(i, j, k + 1, nx, ny));
-                    dn_v = (k == 0) ? c.g_bot : phi(vid(i, j, k - 1, nx, ny));
-                } else {
-                    const int uu = (k == 0) ? 1 : ((k == nz - 1) ? nz - 2 : k + 1);
-                    const int dd = (k == 0) ? 1 : ((k == nz - 1) ? nz - 2 : k - 1);
-                    up_v = phi(vid(i, j, uu, nx, ny));
-                    dn_v = phi(vid(i, j, dd, nx, ny));
+                for (int q = 0; q < 6; ++q) {
+                    const float(*S)[HX] = sP[sb][q];
+                    sum[q] = S[ty + 1][tx + 2] + S[ty + 1][tx] + S[ty + 2][tx + 1] + S[ty][tx + 1] + above.P[q] +
+                             below.P[q];
                 }
-                const float Ex = (float)(-(phi(ie) - phi(iw)) * c.inv_two_h);
-                const float Ey = (float)(-(phi(in) - phi(is)) * c.inv_two_h);
-                const float Ez = (float)(-(up_v - dn_v) * c.inv_two_h);
-                dmx -= Ex;
-                dmy -= Ey;
-                dmz -= Ez;
+                const PhaseC32 pc = sph[((cw >> CW_PHASE_SHIFT) & 3u) * 2u + ((cw & CW_IFACE) ? 1u : 0u)];
+                const float t11 = T9[0], t12 = T9[1], t13 = T9[2], t21 = T9[3], t22 = T9[4], t23 = T9[5];
+                const float t31 = T9[6], t32 = T9[7], t33 = T9[8];
+                const float mx = cur.P[0], my = cur.P[1], mz = cur.P[2], sx = cur.P[3], sy = cur.P[4], sz = cur.P[5];
+                const float mXl = t11 * mx + t12 * my + t13 * mz;
+                const float mYl = t21 * mx + t22 * my + t23 * mz;
+                const float mZl = t31 * mx + t32 * my + t33 * mz;
+                const float sXl = t11 * sx + t12 * sy + t13 * sz;
+                const float sYl = t21 * sx + t22 * sy + t23 * sz;
+                const float sZl = t31 * sx + t32 * sy + t33 * sz;
+                loc[0] += (double)mZl;
+                loc[1] += (double)sZl;
+                const float m2 = mZl * mZl, m3 = m2 * mZl, m4 = m3 * mZl, m5 = m4 * mZl;
+                const float s2 = sZl * sZl, s3 = s2 * sZl, s4 = s3 * sZl, s5 = s4 * sZl;
+                const float fmz = pc.ag * mZl + pc.beta_4 * m3 + pc.beta_12 * mZl * s2 +
+                                  pc.gamma_6 * (m5 + 10.0f * m3 * s2 + 5.0f * mZl * s4);
+                const float fsz = pc.ags * sZl + pc.beta_4 * s3 + pc.beta_12 * sZl * m2 +
+                                  pc.gamma_6 * (s5 + 10.0f * m2 * s3 + 5.0f * sZl * m4);
+                const float fmx = pc.a0_2 * mXl, fmy = pc.a0_2 * mYl, fsx = pc.a0_2 * sXl, fsy = pc.a0_2 * sYl;
+                float dmx = t11 * fmx + t21 * fmy + t31 * fmz;
+                float dmy = t12 * fmx + t22 * fmy + t32 * fmz;
+                float dmz = t13 * fmx + t23 * fmy + t33 * fmz;
+                const float dsx = t11 * fsx + t21 * fsy + t31 * fsz;
+                const float dsy = t12 * fsx + t22 * fsy + t32 * fsz;
+                const float dsz = t13 * fsx + t23 * fsy + t33 * fsz;
+                if (c.dep_flag) {  // E = -grad(x_hi + x_lo)
+                    const double(*F)[HX] = sphi[sb];
+                    double up_v, dn_v;
+                    if (c.zbc == 0) {
+                        up_v = (k == nz - 1) ? c.g_top : above.phi;
+                        dn_v = (k == 0) ? c.g_bot : below.phi;
+                    } else {  // mirror: planes 1 and nz - 2 stand on both sides of the faces
+                        up_v = (k == nz - 1) ? below.phi : above.phi;
+                        dn_v = (k == 0) ? above.phi : below.phi;
+                    }
+                    const float Ex = (float)(-(F[ty + 1][tx + 2] - F[ty + 1][tx]) * c.inv_two_h);
+                    const float Ey = (float)(-(F[ty + 2][tx + 1] - F[ty][tx + 1]) * c.inv_two_h);
+                    const float Ez = (float)(-(up_v - dn_v) * c.inv_two_h);
+                    dmx -= Ex;
+                    dmy -= Ey;
+                    dmz -= Ez;
+                }
+                dmz -= Ez_ext;
+                const float A = pc.A, om = pc.one_m6A, cL = pc.cL;
+                b.Pnew[0 * N + id] = mx * om + A * sum[0] - cL * dmx;
+                b.Pnew[1 * N + id] = my * om + A * sum[1] - cL * dmy;
+                b.Pnew[2 * N + id] = mz * om + A * sum[2] - cL * dmz;
+                b.Pnew[3 * N + id] = sx * om + A * sum[3] - cL * dsx;
+                b.Pnew[4 * N + id] = sy * om + A * sum[4] - cL * dsy;
+                b.Pnew[5 * N + id] = sz * om + A * sum[5] - cL * dsz;
             }
-            dmz -= Ez_ext;
-            const float A = pc.A, om = pc.one_m6A, cL = pc.cL;
-            b.Pnew[0 * N + id] = mx * om + A * sum[0] - cL * dmx;
-            b.Pnew[1 * N + id] = my * om + A * sum[1] - cL * dmy;
-            b.Pnew[2 * N + id] = mz * om + A * sum[2] - cL * dmz;
-            b.Pnew[3 * N + id] = sx * om + A * sum[3] - cL * dsx;
-            b.Pnew[4 * N + id] = sy * om + A * sum[4] - cL * dsy;
-            b.Pnew[5 * N + id] = sz * om + A * sum[5] - cL * dsz;
+        }
+        below = cur;
+        cur = above;
+        above = above_n;
+        ring = ring_n;
+        cw = cw_n;
+        {
+            const float* Tr = b.TR32 + (long long)(cw & CW_GID_MASK) * 9;
+#pragma unroll
+            for (int e = 0; e < 9; ++e) Tn[e] = Tr[e];
         }
     }
     write_block_partials<TDGL_Q>(loc, b.tdgl_part, nb, T.blk);
 }
 
-__global__ __launch_bounds__(NT) void k_tdgl_f32(Fp32Bufs b, StepConsts c, float Ez_ext, int nb) {
+__global__ __launch_bounds__(NT, 4) void k_tdgl_f32(Fp32Bufs b, StepConsts c, float Ez_ext, int nb) {
     tdgl_f32_body(b, c, Ez_ext, nb, tile_id());
 }
 
-__global__ __launch_bounds__(NT) void k_tdgl_f32_batch(const BatchFilm* __restrict__ films, const int2* __restrict__ tiles,
+__global__ __launch_bounds__(NT, 4) void k_tdgl_f32_batch(const BatchFilm* __restrict__ films, const int2* __restrict__ tiles,
                                                        int parity) {
     TileId T;
     const BatchFilm& F = films[batch_tile(tiles, films, T)];
