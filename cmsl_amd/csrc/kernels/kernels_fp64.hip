@@ -29,123 +29,166 @@ int blocks_for(const Grid& g) { return blocks_for_impl(g); }
 int part_stride(const Grid& g) { return part_stride_impl(blocks_for_impl(g)); }
 
 // ---------------------------------------------------------------- TDGL
+// z-streaming, as the FP32 form: each plane's P (6 components) and phi are
+// staged in LDS with a one-voxel x-y halo (one buffer of 37 KB, two
+// barriers per plane), the column's planes below and above stay in
+// registers (z neighbours, clamped at the faces).  The phase table sits in
+// LDS.  The FP64 arithmetic needs about 176 VGPRs, so one workgroup runs
+// per CU and the plane loads are not issued a plane ahead (a 128-VGPR build
+// for two workgroups per CU spilled and ran 2.4x slower; measured 69 us
+// against 104 for the per-voxel form at 256x256x32).  The operands reach
+// each expression in the reference's order, so every voxel value is
+// bit-identical to the per-voxel evaluation.
+struct Col64 {
+    double P[6];
+    double phi;
+};
+
 __global__ __launch_bounds__(NT) void k_tdgl_f64(Fp64Bufs b, StepConsts c, double Ez_ext, int nb) {
-    const int nx = c.nx, ny = c.ny, nz = c.nz;
-    const long long N = (long long)nx * ny * nz;
+    __shared__ double sP[1][6][HY][HX];
+    __shared__ double sphi[1][HY][HX];
+    __shared__ PhaseC sph[PTAB_ENTRIES];
+    const int nz = c.nz;
+    const long long N = (long long)c.nx * c.ny * nz;
     const TileId T = tile_id();
-    const int i = T.bx * BX + threadIdx.x;
-    const int j = T.by * BY + threadIdx.y;
-    const int k0 = T.bz * c.kz;
+    const TilePos tp = tile_pos(c, T);
+    const TileCols G = tile_cols(tp, c);
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = linear_tid();
+    if (tid < PTAB_ENTRIES) sph[tid] = b.ptab[tid];
+    const unsigned cc = G.cofs, hc = tp.has_h ? G.hofs : G.cofs, pb = c.plane;
+    // buffer loads: one resource per array (each below 2^31 bytes for the
+    // 2^28-voxel limit), the column as a lane offset, the plane uniform
+    rsrc_t RP[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) RP[q] = make_rsrc(b.Pold + q * N, c.nvox * 8u);
+    const rsrc_t Rph = make_rsrc(b.phi, c.nvox * 8u);
+    auto ld = [&](unsigned col, int k, Col64& v) {
+        const unsigned po = (unsigned)k * pb;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) v.P[q] = bload<double>(RP[q], col * 8u, po * 8u);
+        v.phi = c.dep_flag ? bload<double>(Rph, col * 8u, po * 8u) : 0.0;
+    };
+    const int k0 = T.bz * c.kz, k1 = min(k0 + c.kz, nz);
+    Col64 below, cur;
+    ld(cc, max(k0 - 1, 0), below);
+    ld(cc, k0, cur);
     double loc[2] = {0.0, 0.0};  // sum Pmz_local, sum Psz_local (evolve.cu:269-276)
-    if (i < nx && j < ny) {
-        const int e = (i + 1 == nx) ? 0 : i + 1;
-        const int w = (i == 0) ? nx - 1 : i - 1;
-        const int n = (j + 1 == ny) ? 0 : j + 1;
-        const int s = (j == 0) ? ny - 1 : j - 1;
-        const int k1 = min(k0 + c.kz, nz);
-        for (int k = k0; k < k1; ++k) {
-            const long long id = vid(i, j, k, nx, ny);
-            const uint32_t cw = b.cw[id];
+    for (int k = k0; k < k1; ++k) {
+        const int sb = 0;
+        // this plane's halo cell and the column's plane above: loaded here,
+        // not a plane ahead (FP64 registers); other workgroups on the CU
+        // cover the latency
+        Col64 ring, above;
+        ld(hc, k, ring);
+        ld(cc, min(k + 1, nz - 1), above);
+        const uint32_t cw = b.cw[cc + (long long)pb * k];
+        if (k > k0) __syncthreads();  // the previous plane's reads are done
+#pragma unroll
+        for (int q = 0; q < 6; ++q) sP[sb][q][ty + 1][tx + 1] = cur.P[q];
+        sphi[sb][ty + 1][tx + 1] = cur.phi;
+        if (tp.has_h) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) sP[sb][q][tp.hy][tp.hx] = ring.P[q];
+            sphi[sb][tp.hy][tp.hx] = ring.phi;
+        }
+        __syncthreads();
+        if (tp.live) {
+            const long long id = (long long)cc + (long long)pb * k;
             if (cw & CW_VOID) {  // never written by the reference: zero buffer
 #pragma unroll
                 for (int q = 0; q < 6; ++q) b.Pnew[q * N + id] = 0.0;
-                continue;
-            }
-            const int u = (k == nz - 1) ? k : k + 1;
-            const int d = (k == 0) ? k : k - 1;
-            const long long ie = vid(e, j, k, nx, ny), iw = vid(w, j, k, nx, ny);
-            const long long in = vid(i, n, k, nx, ny), is = vid(i, s, k, nx, ny);
-            const long long iu = vid(i, j, u, nx, ny), idn = vid(i, j, d, nx, ny);
-
-            double sum[6], ctr[6];
+            } else {
+                // neighbour sum in the reference order: e, w, n, s, u, d
+                double sum[6];
 #pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                const double* P = b.Pold + q * N;
-                sum[q] = P[ie] + P[iw] + P[in] + P[is] + P[iu] + P[idn];
-                ctr[q] = P[id];
-            }
-            const PhaseC pc = b.ptab[((cw >> CW_PHASE_SHIFT) & 3u) * 2u + ((cw & CW_IFACE) ? 1u : 0u)];
-            const double* T = b.TR + (long long)(cw & CW_GID_MASK) * 9;
-            const double tr11 = T[0], tr12 = T[1], tr13 = T[2];
-            const double tr21 = T[3], tr22 = T[4], tr23 = T[5];
-            const double tr31 = T[6], tr32 = T[7], tr33 = T[8];
-            const double pMx0 = ctr[0], pMy0 = ctr[1], pMz0 = ctr[2];
-            const double pSx0 = ctr[3], pSy0 = ctr[4], pSz0 = ctr[5];
-
-            const double pMXl = tr11 * pMx0 + tr12 * pMy0 + tr13 * pMz0;
-            const double pMYl = tr21 * pMx0 + tr22 * pMy0 + tr23 * pMz0;
-            const double pMZl = tr31 * pMx0 + tr32 * pMy0 + tr33 * pMz0;
-            const double pSXl = tr11 * pSx0 + tr12 * pSy0 + tr13 * pSz0;
-            const double pSYl = tr21 * pSx0 + tr22 * pSy0 + tr23 * pSz0;
-            const double pSZl = tr31 * pSx0 + tr32 * pSy0 + tr33 * pSz0;
-            loc[0] += pMZl;
-            loc[1] += pSZl;
-
-            const double dfdPmxl = pc.a0_2 * pMXl;
-            const double dfdPmyl = pc.a0_2 * pMYl;
-            const double dfdPsxl = pc.a0_2 * pSXl;
-            const double dfdPsyl = pc.a0_2 * pSYl;
-            const double phiMz2 = pMZl * pMZl;
-            const double phiMz3 = phiMz2 * pMZl;
-            const double phiMz4 = phiMz3 * pMZl;
-            const double phiMz5 = phiMz4 * pMZl;
-            const double phiSz2 = pSZl * pSZl;
-            const double phiSz3 = phiSz2 * pSZl;
-            const double phiSz4 = phiSz3 * pSZl;
-            const double phiSz5 = phiSz4 * pSZl;
-            const double dfdPmzl = pc.ag * pMZl + pc.beta_4 * phiMz3 + pc.beta_12 * pMZl * phiSz2 +
-                                   pc.gamma_6 * (phiMz5 + 10.0 * phiMz3 * phiSz2 + 5.0 * pMZl * phiSz4);
-            const double dfdPszl = pc.ags * pSZl + pc.beta_4 * phiSz3 + pc.beta_12 * pSZl * phiMz2 +
-                                   pc.gamma_6 * (phiSz5 + 10.0 * phiMz2 * phiSz3 + 5.0 * pSZl * phiMz4);
-            double dfdPmx_b = tr11 * dfdPmxl + tr21 * dfdPmyl + tr31 * dfdPmzl;
-            double dfdPmy_b = tr12 * dfdPmxl + tr22 * dfdPmyl + tr32 * dfdPmzl;
-            double dfdPmz_b = tr13 * dfdPmxl + tr23 * dfdPmyl + tr33 * dfdPmzl;
-            const double dfdPsx_b = tr11 * dfdPsxl + tr21 * dfdPsyl + tr31 * dfdPszl;
-            const double dfdPsy_b = tr12 * dfdPsxl + tr22 * dfdPsyl + tr32 * dfdPszl;
-            const double dfdPsz_b = tr13 * dfdPsxl + tr23 * dfdPsyl + tr33 * dfdPszl;
-
-            double Ex = 0.0, Ey = 0.0, Ez = 0.0;
-            if (c.dep_flag) {  // E = -grad phi, depol_varEPS.cu:246-273
-                const double* ph = b.phi;
-                const double dx = -(ph[ie] - ph[iw]);
-                const double dy = -(ph[in] - ph[is]);
-                double up_v, dn_v;
-                if (c.zbc == 0) {
-                    up_v = (k == nz - 1) ? c.g_top : ph[vid(i, j, k + 1, nx, ny)];
-                    dn_v = (k == 0) ? c.g_bot : ph[vid(i, j, k - 1, nx, ny)];
-                } else {
-                    const int uu = (k == 0) ? 1 : ((k == nz - 1) ? nz - 2 : k + 1);
-                    const int dd = (k == 0) ? 1 : ((k == nz - 1) ? nz - 2 : k - 1);
-                    up_v = ph[vid(i, j, uu, nx, ny)];
-                    dn_v = ph[vid(i, j, dd, nx, ny)];
+                for (int q = 0; q < 6; ++q) {
+                    const double(*S)[HX] = sP[sb][q];
+                    sum[q] = S[ty + 1][tx + 2] + S[ty + 1][tx] + S[ty + 2][tx + 1] + S[ty][tx + 1] + above.P[q] +
+                             below.P[q];
                 }
-                const double dz = -(up_v - dn_v);
-                if (c.two_h_pow2) {
-                    Ex = dx * c.inv_two_h;
-                    Ey = dy * c.inv_two_h;
-                    Ez = dz * c.inv_two_h;
-                } else {
-                    Ex = dx / c.two_h;
-                    Ey = dy / c.two_h;
-                    Ez = dz / c.two_h;
-                }
-            }
-            const double dfdPmz_app = -Ez_ext;
-            const double dfdPmx_e = c.dep_flag ? -1.0 * Ex : 0.0;
-            const double dfdPmy_e = c.dep_flag ? -1.0 * Ey : 0.0;
-            const double dfdPmz_e = c.dep_flag ? -1.0 * Ez : 0.0;
-            dfdPmx_b += dfdPmx_e;
-            dfdPmy_b += dfdPmy_e;
-            dfdPmz_b += dfdPmz_app + dfdPmz_e;
+                const PhaseC pc = sph[((cw >> CW_PHASE_SHIFT) & 3u) * 2u + ((cw & CW_IFACE) ? 1u : 0u)];
+                const double* Tm = b.TR + (long long)(cw & CW_GID_MASK) * 9;
+                const double tr11 = Tm[0], tr12 = Tm[1], tr13 = Tm[2];
+                const double tr21 = Tm[3], tr22 = Tm[4], tr23 = Tm[5];
+                const double tr31 = Tm[6], tr32 = Tm[7], tr33 = Tm[8];
+                const double pMx0 = cur.P[0], pMy0 = cur.P[1], pMz0 = cur.P[2];
+                const double pSx0 = cur.P[3], pSy0 = cur.P[4], pSz0 = cur.P[5];
 
-            const double A = pc.A, om = pc.one_m6A, cL = pc.cL;
-            b.Pnew[0 * N + id] = pMx0 * om + A * sum[0] - cL * dfdPmx_b;
-            b.Pnew[1 * N + id] = pMy0 * om + A * sum[1] - cL * dfdPmy_b;
-            b.Pnew[2 * N + id] = pMz0 * om + A * sum[2] - cL * dfdPmz_b;
-            b.Pnew[3 * N + id] = pSx0 * om + A * sum[3] - cL * dfdPsx_b;
-            b.Pnew[4 * N + id] = pSy0 * om + A * sum[4] - cL * dfdPsy_b;
-            b.Pnew[5 * N + id] = pSz0 * om + A * sum[5] - cL * dfdPsz_b;
+                const double pMXl = tr11 * pMx0 + tr12 * pMy0 + tr13 * pMz0;
+                const double pMYl = tr21 * pMx0 + tr22 * pMy0 + tr23 * pMz0;
+                const double pMZl = tr31 * pMx0 + tr32 * pMy0 + tr33 * pMz0;
+                const double pSXl = tr11 * pSx0 + tr12 * pSy0 + tr13 * pSz0;
+                const double pSYl = tr21 * pSx0 + tr22 * pSy0 + tr23 * pSz0;
+                const double pSZl = tr31 * pSx0 + tr32 * pSy0 + tr33 * pSz0;
+                loc[0] += pMZl;
+                loc[1] += pSZl;
+
+                const double dfdPmxl = pc.a0_2 * pMXl;
+                const double dfdPmyl = pc.a0_2 * pMYl;
+                const double dfdPsxl = pc.a0_2 * pSXl;
+                const double dfdPsyl = pc.a0_2 * pSYl;
+                const double phiMz2 = pMZl * pMZl;
+                const double phiMz3 = phiMz2 * pMZl;
+                const double phiMz4 = phiMz3 * pMZl;
+                const double phiMz5 = phiMz4 * pMZl;
+                const double phiSz2 = pSZl * pSZl;
+                const double phiSz3 = phiSz2 * pSZl;
+                const double phiSz4 = phiSz3 * pSZl;
+                const double phiSz5 = phiSz4 * pSZl;
+                const double dfdPmzl = pc.ag * pMZl + pc.beta_4 * phiMz3 + pc.beta_12 * pMZl * phiSz2 +
+                                       pc.gamma_6 * (phiMz5 + 10.0 * phiMz3 * phiSz2 + 5.0 * pMZl * phiSz4);
+                const double dfdPszl = pc.ags * pSZl + pc.beta_4 * phiSz3 + pc.beta_12 * pSZl * phiMz2 +
+                                       pc.gamma_6 * (phiSz5 + 10.0 * phiMz2 * phiSz3 + 5.0 * pSZl * phiMz4);
+                double dfdPmx_b = tr11 * dfdPmxl + tr21 * dfdPmyl + tr31 * dfdPmzl;
+                double dfdPmy_b = tr12 * dfdPmxl + tr22 * dfdPmyl + tr32 * dfdPmzl;
+                double dfdPmz_b = tr13 * dfdPmxl + tr23 * dfdPmyl + tr33 * dfdPmzl;
+                const double dfdPsx_b = tr11 * dfdPsxl + tr21 * dfdPsyl + tr31 * dfdPszl;
+                const double dfdPsy_b = tr12 * dfdPsxl + tr22 * dfdPsyl + tr32 * dfdPszl;
+                const double dfdPsz_b = tr13 * dfdPsxl + tr23 * dfdPsyl + tr33 * dfdPszl;
+
+                double Ex = 0.0, Ey = 0.0, Ez = 0.0;
+                if (c.dep_flag) {  // E = -grad phi, depol_varEPS.cu:246-273
+                    const double(*F)[HX] = sphi[sb];
+                    const double dx = -(F[ty + 1][tx + 2] - F[ty + 1][tx]);
+                    const double dy = -(F[ty + 2][tx + 1] - F[ty][tx + 1]);
+                    double up_v, dn_v;
+                    if (c.zbc == 0) {
+                        up_v = (k == nz - 1) ? c.g_top : above.phi;
+                        dn_v = (k == 0) ? c.g_bot : below.phi;
+                    } else {  // mirror: planes 1 and nz - 2 stand on both sides of the faces
+                        up_v = (k == nz - 1) ? below.phi : above.phi;
+                        dn_v = (k == 0) ? above.phi : below.phi;
+                    }
+                    const double dz = -(up_v - dn_v);
+                    if (c.two_h_pow2) {
+                        Ex = dx * c.inv_two_h;
+                        Ey = dy * c.inv_two_h;
+                        Ez = dz * c.inv_two_h;
+                    } else {
+                        Ex = dx / c.two_h;
+                        Ey = dy / c.two_h;
+                        Ez = dz / c.two_h;
+                    }
+                }
+                const double dfdPmz_app = -Ez_ext;
+                const double dfdPmx_e = c.dep_flag ? -1.0 * Ex : 0.0;
+                const double dfdPmy_e = c.dep_flag ? -1.0 * Ey : 0.0;
+                const double dfdPmz_e = c.dep_flag ? -1.0 * Ez : 0.0;
+                dfdPmx_b += dfdPmx_e;
+                dfdPmy_b += dfdPmy_e;
+                dfdPmz_b += dfdPmz_app + dfdPmz_e;
+
+                const double A = pc.A, om = pc.one_m6A, cL = pc.cL;
+                b.Pnew[0 * N + id] = pMx0 * om + A * sum[0] - cL * dfdPmx_b;
+                b.Pnew[1 * N + id] = pMy0 * om + A * sum[1] - cL * dfdPmy_b;
+                b.Pnew[2 * N + id] = pMz0 * om + A * sum[2] - cL * dfdPmz_b;
+                b.Pnew[3 * N + id] = pSx0 * om + A * sum[3] - cL * dfdPsx_b;
+                b.Pnew[4 * N + id] = pSy0 * om + A * sum[4] - cL * dfdPsy_b;
+                b.Pnew[5 * N + id] = pSz0 * om + A * sum[5] - cL * dfdPsz_b;
+            }
         }
+        below = cur;
+        cur = above;
     }
     write_block_partials<TDGL_Q>(loc, b.tdgl_part, nb, T.blk);
 }
