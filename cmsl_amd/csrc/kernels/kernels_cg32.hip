@@ -127,10 +127,12 @@ struct Plane32 {
 };
 
 // RESID: r = b - A (x_hi + x_lo) in FP64 over the tile's planes, written as
-// FP32 to the r array at byte offset o_r of RA; partials (r.z, r.r).  One
-// plane at a time, with buffer loads at 32-bit column offsets (neighbour
-// planes clamped into the film; identity rows, where the clamp applies under
-// Dirichlet electrodes, use only the centre).
+// FP32 to the r array at byte offset o_r of RA; partials (r.z, r.r).  Buffer
+// loads at 32-bit column offsets; the column's potential of the planes below
+// and above stays in registers (z neighbours, clamped into the film;
+// identity rows, where the clamp applies under Dirichlet electrodes, use
+// only the centre), and the next plane's in-plane neighbours, b and code are
+// loaded while the current plane is evaluated.
 template <int KZ, bool MIR>
 __device__ __forceinline__ void cg32_resid(const CG32Bufs& b, const StepConsts& c, const TilePos& tp,
                                            const TileCols& G, int k0, rsrc_t RA, unsigned o_r, rsrc_t Rsc,
@@ -149,38 +151,59 @@ __device__ __forceinline__ void cg32_resid(const CG32Bufs& b, const StepConsts& 
     const unsigned oc = tp.live ? G.cofs : dead;
     const unsigned oe = tp.live ? ie + (unsigned)nx * j : dead, ow = tp.live ? iw + (unsigned)nx * j : dead;
     const unsigned on = tp.live ? i + (unsigned)nx * jn : dead, os = tp.live ? i + (unsigned)nx * js : dead;
-    auto X = [&](unsigned col, unsigned pofs) {
+    auto X = [&](unsigned col, int k) {
+        const unsigned pofs = (unsigned)k * c.plane;
         return bload<double>(Rxh, col * 8u, pofs * 8u) + (double)bload<float>(Rxl, col * 4u, pofs * 4u);
     };
+    struct InPlane {
+        double e, w, n, s, b;
+        unsigned code;
+    };
+    auto ld_plane = [&](int k, InPlane& v) {
+        const unsigned pk = (unsigned)k * c.plane;
+        v.code = bload_code(Rsc, oc * 2u, pk * 2u);
+        v.e = X(oe, k);
+        v.w = X(ow, k);
+        v.n = X(on, k);
+        v.s = X(os, k);
+        v.b = bload<double>(Rb, oc * 8u, pk * 8u);
+    };
     const int k1 = min(k0 + KZ, nz);
+    double xm = X(oc, max(k0 - 1, 0)), x0 = X(oc, k0), xp = X(oc, min(k0 + 1, nz - 1));
+    InPlane cur;
+    ld_plane(k0, cur);
 #pragma unroll 1
     for (int k = k0; k < k1; ++k) {
-        int up = k + 1, dn = k - 1;
-        if (c.zbc != 0) {
-            if (k == 0) up = dn = 1;
-            else if (k == nz - 1) up = dn = nz - 2;
+        InPlane nxt;
+        ld_plane(min(k + 1, k1 - 1), nxt);
+        const double xpp = X(oc, min(k + 2, nz - 1));
+        double xu = xp, xd = xm;
+        if (c.zbc != 0) {  // mirror: planes 1 and nz - 2 stand on both sides of the faces
+            if (k == 0) xd = xu;
+            else if (k == nz - 1) xu = xd;
         }
-        const unsigned pk = (unsigned)k * c.plane;
-        const unsigned pu = (unsigned)min(up, nz - 1) * c.plane, pd = (unsigned)max(dn, 0) * c.plane;
-        const unsigned code = bload_code(Rsc, oc * 2u, pk * 2u);
-        const double xc = X(oc, pk);
+        const unsigned code = cur.code;
         double Ax;
         if (code & SC_IDENTITY) {
-            Ax = op_identity(xc, k, c);
+            Ax = op_identity(x0, k, c);
         } else {
             const double* Hr = sH64 + (code & 3u) * 4u;
-            const double num = Hr[(code >> 2) & 3u] * (X(oe, pk) - xc) - Hr[(code >> 4) & 3u] * (xc - X(ow, pk)) +
-                               Hr[(code >> 6) & 3u] * (X(on, pk) - xc) - Hr[(code >> 8) & 3u] * (xc - X(os, pk)) +
-                               Hr[(code >> 10) & 3u] * (X(oc, pu) - xc) - Hr[(code >> 12) & 3u] * (xc - X(oc, pd));
+            const double num = Hr[(code >> 2) & 3u] * (cur.e - x0) - Hr[(code >> 4) & 3u] * (x0 - cur.w) +
+                               Hr[(code >> 6) & 3u] * (cur.n - x0) - Hr[(code >> 8) & 3u] * (x0 - cur.s) +
+                               Hr[(code >> 10) & 3u] * (xu - x0) - Hr[(code >> 12) & 3u] * (x0 - xd);
             Ax = -(num * c.inv_h2);
         }
-        const float r = (float)(bload<double>(Rb, oc * 8u, pk * 8u) - Ax);
-        bstore(RA, oc * 4u, o_r + pk * 4u, r);
+        const float r = (float)(cur.b - Ax);
+        bstore(RA, oc * 4u, o_r + (unsigned)k * c.plane * 4u, r);
         if (tp.live) {
             const float z = r * bload<float>(Rdi, code * 4u, 0u);
             cg_dot_add<MIR>(acc[0], r, z, c, k);
             acc[1] += (double)r * (double)r;
         }
+        xm = x0;
+        x0 = xp;
+        xp = xpp;
+        cur = nxt;
     }
 }
 
