@@ -408,6 +408,19 @@ hzo::CG32Bufs cg_bufs(hzo_ctx* x, const hzo::Fp32Bufs& b32) {
     return cb;
 }
 
+// Size of the next follow-up round of launches after a solve outlasts the
+// round sized from the previous solve.  Iteration counts drift slowly from
+// step to step, so such a solve usually needs only a few more launches:
+// the rounds start at 4 and double up to an eighth of the previous solve
+// (at least 8), which bounds both the surplus launches (each returns at
+// once but costs a dispatch) and the number of host polls.
+int follow_up_round(int& next, int last_iters) {
+    const int cap = last_iters / 8 > 8 ? last_iters / 8 : 8;
+    const int r = next < cap ? next : cap;
+    next = 2 * r;
+    return r;
+}
+
 // FP32-refined solve by the single-reduction CG: the setup (r = b - A x,
 // partials r.z, r.r, b.b) and then one fused launch per iteration, the
 // device deciding from the control block whether a launch iterates,
@@ -425,7 +438,7 @@ int cg32_solve(hzo_ctx* x, const hzo::Fp32Bufs& b32in, const hzo::StepConsts& c,
     const long long maxl = 3LL * init.max_iter + 8;
     long long launches = 0;
     int t = 0;
-    int batch = last_iters + 2;
+    int batch = last_iters + 2, next = 4;
     while (true) {
         for (int m = 0; m < batch && launches < maxl; ++m, ++launches) {
             const int tt = ++t;
@@ -437,7 +450,7 @@ int cg32_solve(hzo_ctx* x, const hzo::Fp32Bufs& b32in, const hzo::StepConsts& c,
                                x->stream));
         HZO_HIP(hipStreamSynchronize(x->stream));
         if (x->st_host->done || launches >= maxl) break;
-        batch = last_iters / 8 > 8 ? last_iters / 8 : 8;  // follow-up rounds: at least 8 launches per poll
+        batch = follow_up_round(next, last_iters);
     }
     hzo::PcgState& s = *x->st_host;
     if (!s.done) s.status = HZO_PCG_MAXITER;
@@ -460,7 +473,7 @@ int pcg_solve(hzo_ctx* x, const hzo::Fp64Bufs& b, const hzo::Fp32Bufs& b32, cons
     const long long maxpairs = (long long)init.max_iter * (x->fp32 ? 2 : 1) + 4;
     long long pairs = 0;
     int t = 0;  // launch index since the setup
-    int batch = last_iters + 1;
+    int batch = last_iters + 1, next = 4;
     while (true) {
         for (int m = 0; m < batch && pairs < maxpairs; ++m, ++pairs) {
             const int ta = ++t, tb = ++t;
@@ -482,7 +495,7 @@ int pcg_solve(hzo_ctx* x, const hzo::Fp64Bufs& b, const hzo::Fp32Bufs& b32, cons
                                x->stream));
         HZO_HIP(hipStreamSynchronize(x->stream));
         if (x->st_host->done || pairs >= maxpairs) break;
-        batch = last_iters / 8 > 8 ? last_iters / 8 : 8;  // follow-up rounds: at least 8 launches per poll
+        batch = follow_up_round(next, last_iters);
     }
     hzo::PcgState& s = *x->st_host;
     if (!s.done) s.status = HZO_PCG_MAXITER;
