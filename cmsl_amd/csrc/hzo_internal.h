@@ -145,8 +145,11 @@ struct Fp32Bufs {
     double* tdgl_part;
 };
 
-// Single-reduction CG of the FP32-refined form (kernels_cg32.hip): r, w = A M r
-// and s = A p double-buffered by the control block's parity.
+// Single-reduction CG of the FP32-refined form (kernels_cg32.hip).  The form
+// carrying w = A M r and s = A p double-buffers r, w, s by the control
+// block's parity; the two-stencil form (k_cg32r) carries only r and p, both
+// double-buffered (p in the s slots), and leaves the w slots and the p slot
+// unused.
 struct CG32Bufs {
     // FP32 vectors in arenas of arrays of N (one buffer resource each):
     // va = r[0] r[1] w[0] w[1], vb = s[0] s[1] p (a film keeps its x_lo as

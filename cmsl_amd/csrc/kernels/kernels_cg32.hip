@@ -25,6 +25,11 @@
 //
 // Every launch writes three partial sums: (gamma, delta, r.r) after ITER and
 // WCOMP, (gamma, r.r, 0) after RESID, (r.z, r.r, b.b) after the setup.
+//
+// Two kernels run this iteration: k_cg32 carries w and s in memory (42
+// bytes per voxel and iteration), k_cg32r forms s = A p and w = A u by two
+// stencils in the launch and carries only r and p (26 bytes).  The host
+// picks by film size; both share the control logic and the RESID mode.
 #include "../hzo_internal.h"
 #include "common.hip.h"
 #include "tile.hip.h"

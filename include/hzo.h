@@ -83,7 +83,7 @@ typedef struct hzo_config {
     int max_iter;
     int refine_every;    /* FP32-refined: FP64 residual recompute period (iterations) */
     int tile_kz;         /* z planes per workgroup of the stencil kernels: 2, 4, 8 (16: single-
-                            reduction CG only); 0 = automatic */
+                            reduction CG only; 32: its two-stencil form only); 0 = automatic */
     int cg_fused;        /* FP32-refined: 0 = the two-kernel PCG of the FP64 form; otherwise the
                             single-reduction CG (Chronopoulos-Gear), one fused kernel per
                             iteration, in one of two forms: 2 = carrying w = A u and s = A p
