@@ -513,8 +513,8 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
     const HPair<float> hpv = load_hpair(b.lut32);
     Partials<3, PM> P;
     load_partials<3, PM>(ip ? b.pin[1] : b.pin[0], b.nb_in, b.ps_in, P);
-    // plane offsets; stencil codes (centre | ring << 16), each loaded one
-    // phase before its plane's data, whose D^-1 gathers need it
+    // plane offsets; stencil codes (centre | ring << 16), each loaded two
+    // phases before its plane's data, whose D^-1 gathers need it
     unsigned so[NW], cd[NW];
 #pragma unroll
     for (int m = 0; m < NW; ++m) so[m] = (unsigned)min(max(k0 - 2 + m, 0), nz - 1);
