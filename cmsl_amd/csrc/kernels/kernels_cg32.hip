@@ -132,74 +132,67 @@ struct Plane32 {
 };
 
 // RESID: r = b - A (x_hi + x_lo) in FP64 over the tile's planes, written as
-// FP32 to the r array at byte offset o_r of RA; partials (r.z, r.r).  Buffer
-// loads at 32-bit column offsets; the column's potential of the planes below
-// and above stays in registers (z neighbours, clamped into the film;
-// identity rows, where the clamp applies under Dirichlet electrodes, use
-// only the centre), and the next plane's in-plane neighbours, b and code are
-// loaded while the current plane is evaluated.
+// FP32 to the r array at byte offset o_r of RA; partials (r.z, r.r).  Each
+// plane's potential is staged in LDS with the x-y halo (double-buffered, one
+// barrier per plane), the column's planes below and above stay in registers
+// (z neighbours, clamped into the film; identity rows, where the clamp
+// applies under Dirichlet electrodes, use only the centre), and the next
+// plane's values are loaded while the current one is evaluated.  Lanes of
+// an edge tile beyond the film stage the periodic image and store nothing.
 template <int KZ, bool MIR>
 __device__ __forceinline__ void cg32_resid(const CG32Bufs& b, const StepConsts& c, const TilePos& tp,
                                            const TileCols& G, int k0, rsrc_t RA, unsigned o_r, rsrc_t Rsc,
                                            rsrc_t Rdi, double (&acc)[3]) {
     const int nz = c.nz;
+    const int tx = threadIdx.x, ty = threadIdx.y;
     __shared__ double sH64[16];
+    __shared__ double sx[2][HY][HX];
     load_lut(sH64, b.lut);
     const rsrc_t Rxh = make_rsrc(b.xhi, c.nvox * 8u);
     const rsrc_t Rxl = make_rsrc(b.xlo, c.nvox * 4u);
     const rsrc_t Rb = make_rsrc(b.b, c.nvox * 8u);
-    const int nx = c.nx, i = tp.ci, j = tp.cj;
-    const unsigned ie = (i + 1 == nx) ? 0 : i + 1, iw = (i == 0) ? nx - 1 : i - 1;
-    const unsigned jn = (j + 1 == c.ny) ? 0 : j + 1, js = (j == 0) ? c.ny - 1 : j - 1;
-    // dead lanes use element offset 4 N: out of range of every array here
-    const unsigned dead = 4u * c.nvox;
-    const unsigned oc = tp.live ? G.cofs : dead;
-    const unsigned oe = tp.live ? ie + (unsigned)nx * j : dead, ow = tp.live ? iw + (unsigned)nx * j : dead;
-    const unsigned on = tp.live ? i + (unsigned)nx * jn : dead, os = tp.live ? i + (unsigned)nx * js : dead;
+    // halo lanes stage their cell; the others load at offset N (answered
+    // with 0, never staged)
+    const unsigned oc = G.cofs, oh = G.hofs, ost = tp.live ? G.cofs : 4u * c.nvox;
     auto X = [&](unsigned col, int k) {
         const unsigned pofs = (unsigned)k * c.plane;
         return bload<double>(Rxh, col * 8u, pofs * 8u) + (double)bload<float>(Rxl, col * 4u, pofs * 4u);
     };
-    struct InPlane {
-        double e, w, n, s, b;
-        unsigned code;
-    };
-    auto ld_plane = [&](int k, InPlane& v) {
-        const unsigned pk = (unsigned)k * c.plane;
-        v.code = bload_code(Rsc, oc * 2u, pk * 2u);
-        v.e = X(oe, k);
-        v.w = X(ow, k);
-        v.n = X(on, k);
-        v.s = X(os, k);
-        v.b = bload<double>(Rb, oc * 8u, pk * 8u);
-    };
+    auto B = [&](int k) { return bload<double>(Rb, oc * 8u, (unsigned)k * c.plane * 8u); };
+    auto CODE = [&](int k) { return bload_code(Rsc, oc * 2u, (unsigned)k * c.plane * 2u); };
     const int k1 = min(k0 + KZ, nz);
-    double xm = X(oc, max(k0 - 1, 0)), x0 = X(oc, k0), xp = X(oc, min(k0 + 1, nz - 1));
-    InPlane cur;
-    ld_plane(k0, cur);
+    double xm = X(oc, max(k0 - 1, 0)), x0 = X(oc, k0), xp = X(oc, min(k0 + 1, nz - 1)), xh = X(oh, k0);
+    double bv = B(k0);
+    unsigned code = CODE(k0);
 #pragma unroll 1
     for (int k = k0; k < k1; ++k) {
-        InPlane nxt;
-        ld_plane(min(k + 1, k1 - 1), nxt);
-        const double xpp = X(oc, min(k + 2, nz - 1));
+        const int sb = (k - k0) & 1;
+        sx[sb][ty + 1][tx + 1] = x0;
+        if (tp.has_h) sx[sb][tp.hy][tp.hx] = xh;
+        const int kn = min(k + 1, k1 - 1);
+        const double xpp = X(oc, min(k + 2, nz - 1)), xhn = X(oh, kn), bn = B(kn);
+        const unsigned coden = CODE(kn);
+        __syncthreads();
         double xu = xp, xd = xm;
         if (c.zbc != 0) {  // mirror: planes 1 and nz - 2 stand on both sides of the faces
             if (k == 0) xd = xu;
             else if (k == nz - 1) xu = xd;
         }
-        const unsigned code = cur.code;
         double Ax;
         if (code & SC_IDENTITY) {
             Ax = op_identity(x0, k, c);
         } else {
+            const double(*S)[HX] = sx[sb];
             const double* Hr = sH64 + (code & 3u) * 4u;
-            const double num = Hr[(code >> 2) & 3u] * (cur.e - x0) - Hr[(code >> 4) & 3u] * (x0 - cur.w) +
-                               Hr[(code >> 6) & 3u] * (cur.n - x0) - Hr[(code >> 8) & 3u] * (x0 - cur.s) +
-                               Hr[(code >> 10) & 3u] * (xu - x0) - Hr[(code >> 12) & 3u] * (x0 - xd);
+            const double num = Hr[(code >> 2) & 3u] * (S[ty + 1][tx + 2] - x0) -
+                               Hr[(code >> 4) & 3u] * (x0 - S[ty + 1][tx]) +
+                               Hr[(code >> 6) & 3u] * (S[ty + 2][tx + 1] - x0) -
+                               Hr[(code >> 8) & 3u] * (x0 - S[ty][tx + 1]) + Hr[(code >> 10) & 3u] * (xu - x0) -
+                               Hr[(code >> 12) & 3u] * (x0 - xd);
             Ax = -(num * c.inv_h2);
         }
-        const float r = (float)(cur.b - Ax);
-        bstore(RA, oc * 4u, o_r + (unsigned)k * c.plane * 4u, r);
+        const float r = (float)(bv - Ax);
+        bstore(RA, ost * 4u, o_r + (unsigned)k * c.plane * 4u, r);
         if (tp.live) {
             const float z = r * bload<float>(Rdi, code * 4u, 0u);
             cg_dot_add<MIR>(acc[0], r, z, c, k);
@@ -208,7 +201,9 @@ __device__ __forceinline__ void cg32_resid(const CG32Bufs& b, const StepConsts& 
         xm = x0;
         x0 = xp;
         xp = xpp;
-        cur = nxt;
+        xh = xhn;
+        bv = bn;
+        code = coden;
     }
 }
 
