@@ -151,3 +151,29 @@ def test_two_stencil_form_matches_the_carried_form(kz, zbc):
     else:
         assert out[3][0].sum() <= 1.02 * out[2][0].sum()
         assert rel_err(out[3][1], out[2][1]) < 1e-5 and rel_err(out[3][2], out[2][2]) < 1e-5
+
+
+@pytest.mark.parametrize("shape,form,kz", [
+    ((256, 256, 32), 1, 8),    # one round of depth-8 workgroups: carried form
+    ((256, 256, 64), 2, 16),   # two rounds at depth 8: two-stencil form, depth 32 would give 256 workgroups
+    ((384, 384, 48), 2, 16),   # depth 32 would pad 16 of 48 planes
+    ((512, 512, 32), 2, 32),
+    ((512, 512, 64), 2, 32),
+    ((64, 48, 16), 1, 2),
+])
+def test_automatic_solver_form_and_tile_depth_by_film_size(shape, form, kz):
+    """The automatic choice (cg_fused = 1) takes the form and depth measured
+    fastest for the film's size; FP64 and cg_fused = 0 run the two-kernel
+    PCG (form 0).  Only the context is created: no film data is needed."""
+    import ctypes as C
+    nx, ny, nz = shape
+    out = {}
+    for prec, fused in ((L.PREC_FP32_REFINED, 1), (L.PREC_FP32_REFINED, 0), (L.PREC_FP64, 1)):
+        cfg = L.default_config(nx=nx, ny=ny, nz=nz, precision=prec, cg_fused=fused)
+        h = C.c_void_p()
+        assert L.lib().hzo_create(C.byref(cfg), C.byref(h)) == 0, L.lib().hzo_last_error()
+        out[(prec, fused)] = (L.lib().hzo_cg_form(h), L.lib().hzo_tile_depth(h))
+        L.lib().hzo_destroy(h)
+    _say(shape, out)
+    assert out[(L.PREC_FP32_REFINED, 1)] == (form, kz)
+    assert out[(L.PREC_FP32_REFINED, 0)][0] == 0 and out[(L.PREC_FP64, 1)][0] == 0
