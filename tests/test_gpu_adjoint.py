@@ -23,9 +23,9 @@ def _say(*a):
     print("[gpu]", *a, flush=True)
 
 
-def _film(inp, precision=L.PREC_FP64, tol=1e-13):
+def _film(inp, precision=L.PREC_FP64, tol=1e-13, cg_fused=1):
     return H.Film(inp, precision=precision, pcg_rule=L.PCG_RULE_RELRES, tol_rel=tol, tol_abs=1e-30,
-                  max_iter=4000, refine_every=20)
+                  max_iter=4000, refine_every=20, cg_fused=cg_fused)
 
 
 def _inputs():
@@ -102,13 +102,18 @@ def test_backward_restores_the_recorded_final_state():
     f.close()
 
 
-def test_fp32_adjoint_matches_fp64_adjoint():
+@pytest.mark.parametrize("fused", [1, 3])
+def test_fp32_adjoint_matches_fp64_adjoint(fused):
+    # fused 3: the adjoint solves run the two-stencil single-reduction CG
+    # (with the solve's own solution vector), the form large films take
     inp, Ez = _inputs()
     S = Ez.size
-    _say("fp32 adjoint vs fp64 adjoint")
+    _say(f"fp32 adjoint vs fp64 adjoint, cg_fused={fused}")
     out = {}
     for prec in (L.PREC_FP64, L.PREC_FP32_REFINED):
-        f = _film(inp, precision=prec, tol=1e-8)
+        f = _film(inp, precision=prec, tol=1e-8, cg_fused=fused)
+        if prec == L.PREC_FP32_REFINED:
+            assert f.cg_form == (2 if fused == 3 else 1)
         f.adjoint_reserve(S, ckpt_every=8, tol=1e-8)
         m = _run(f, inp, Ez, coeff_matrix(inp.phases))
         t = np.linspace(-0.01, 0.01, S)
