@@ -177,3 +177,25 @@ def test_automatic_solver_form_and_tile_depth_by_film_size(shape, form, kz):
     _say(shape, out)
     assert out[(L.PREC_FP32_REFINED, 1)] == (form, kz)
     assert out[(L.PREC_FP32_REFINED, 0)][0] == 0 and out[(L.PREC_FP64, 1)][0] == 0
+
+
+@pytest.mark.parametrize("shape,kz", [((40, 4, 6), 32), ((20, 12, 4), 8), ((70, 9, 5), 16)])
+def test_two_stencil_form_on_films_smaller_than_a_tile(shape, kz):
+    """Films narrower than the 64x8 tile, with fewer rows than it, or
+    shallower than its depth: the two-voxel ring wraps more than once, whole
+    tile rows lie beyond the film and most window planes are clamped.  The
+    two-stencil form must still follow the carried form (depth 2)."""
+    nx, ny, nz = shape
+    inp = film_inputs(nx, ny, nz, 3, fracs=(0.2, 0.6, 0.2), seed=17, jitter=0.2)
+    Ez = inp.field_nd(H.triangle_wave(30, 3.0, 5))
+    out = {}
+    for fused, kf in ((2, 2), (3, kz)):
+        g = H.Film(inp, precision=L.PREC_FP32_REFINED, pcg_rule=L.PCG_RULE_RELRES, tol_rel=1e-8, tol_abs=1e-30,
+                   tile_kz=kf, cg_fused=fused)
+        its = [g.step(float(e)) for e in Ez]
+        assert all(st.status == 0 for st in its)
+        out[fused] = (np.array([st.iters for st in its]), *g.get_state()[:2])
+        g.close()
+    _say(shape, kz, out[2][0].tolist(), out[3][0].tolist())
+    assert np.max(np.abs(out[2][0] - out[3][0])) <= 1
+    assert rel_err(out[3][1], out[2][1]) < 1e-6 and rel_err(out[3][2], out[2][2]) < 1e-6
