@@ -279,7 +279,13 @@ def load_structure(inputs_dir, s, nx, ny, nz):
 
 
 class Film:
-    """One film on one GPU: the device context of libhzo.so."""
+    """One film on one GPU: the device context of libhzo.so.
+
+    FP32-refined solver (``cg_fused``): 0 the two-kernel PCG, 1 the
+    single-reduction CG in the form chosen by film size (default), 2 that CG
+    carrying w = A u and s = A p, 3 that CG forming them by stencils.
+    ``tile_kz`` 0 picks the tile depth (2, 4, 8; 16 and 32 for large films).
+    """
 
     def __init__(self, inp: FilmInputs, precision=L.PREC_FP64, pcg_rule=L.PCG_RULE_REFERENCE,
                  tol_rel=1e-2, tol_abs=1e-3, max_iter=4000, dep_flag=1, zbc=L.ZBC_DIRICHLET,
