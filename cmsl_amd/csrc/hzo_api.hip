@@ -32,6 +32,7 @@ struct hzo_ctx {
 
     bool fp32 = false;  // HZO_PREC_FP32_REFINED
     bool cg_two = false;  // single-reduction CG forming s = A p and w = A u in the launch
+    bool cg_nt = false;   // its iteration stores nontemporal
     double* P[2] = {nullptr, nullptr};
     float* P32[2] = {nullptr, nullptr};
     float* xlo = nullptr;
@@ -442,7 +443,7 @@ int cg32_solve(hzo_ctx* x, const hzo::Fp32Bufs& b32in, const hzo::StepConsts& c,
     while (true) {
         for (int m = 0; m < batch && launches < maxl; ++m, ++launches) {
             const int tt = ++t;
-            rc = timed(x, 4, [&](const hzo::LaunchEv& ev) { hzo::launch_cg32(x->stream, x->g, c, cb, tt, x->cg_two, ev); });
+            rc = timed(x, 4, [&](const hzo::LaunchEv& ev) { hzo::launch_cg32(x->stream, x->g, c, cb, tt, x->cg_two, x->cg_nt, ev); });
             if (rc) return rc;
             fold_after(x, x->stream, tt);
         }
@@ -692,6 +693,11 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
         // (16, 32) of films beyond one round of depth-8 workgroups, the
         // carried form the shallow tiles of smaller ones
         x->cg_two = fused && (cfg->cg_fused == 3 || (cfg->cg_fused == 1 && kz >= 16));
+        // its vectors r, p (twice), x_lo and the codes, 22 bytes per voxel:
+        // beyond the 256 MB last-level cache nothing it stores is read back
+        // from there, and nontemporal stores measured 3 % faster at
+        // 512x512x64 (slower at every film whose vectors fit)
+        x->cg_nt = x->cg_two && kz >= 16 && 22LL * N > (256LL << 20);
     }
     hzo::StepConsts& c = x->c;
     c.nx = cfg->nx;
@@ -1531,7 +1537,7 @@ void group_cg_launches(hzo_batch* B, hzo_batch::Group& G, int K) {
         hzo_ctx* x = B->films[G.single];
         const hzo::CG32Bufs cb = cg_bufs(x, bufs32(x));
         for (int t = 1; t <= K; ++t) {
-            hzo::launch_cg32(B->stream, x->g, x->c, cb, t, x->cg_two);
+            hzo::launch_cg32(B->stream, x->g, x->c, cb, t, x->cg_two, x->cg_nt);
             fold_after(x, B->stream, t);
         }
     } else {

@@ -219,8 +219,10 @@ void launch_diag_f32(hipStream_t s, const Grid& g, const float* P, const uint32_
                  const LaunchEv& ev = {});
 // two_stencil: the form that recomputes s = A p and w = A u in the launch
 // (reads r, p; writes r, p, x_lo) instead of carrying w and s.
+// stream_stores: that form at depth 16 or 32 stores its vectors nontemporal
+// (films whose CG vectors exceed the last-level cache).
 void launch_cg32(hipStream_t s, const Grid& g, const StepConsts& c, const CG32Bufs& b, int t, bool two_stencil,
-                 const LaunchEv& ev = {});
+                 bool stream_stores, const LaunchEv& ev = {});
 // Batched FP32-refined launches over every tile of every film (1-D grid of
 // ntiles workgroups; tiles[] maps each to (film, logical tile)).
 void launch_tdgl_f32_batch(hipStream_t s, int ntiles, const BatchFilm* films, const int2* tiles, int parity);

@@ -472,7 +472,7 @@ struct Plane2 {
 // The mirrored-plane weights of Neumann films are applied at run time (a
 // weight of exactly 1 elsewhere leaves every sum unchanged); a build without
 // them allocates more registers at deep tiles and runs slower.
-template <int KZ, int PM>
+template <int KZ, int PM, bool NTS>
 __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& c, int nb, int ps, int t,
                                            const TileId& T) {
     constexpr int NW = KZ + 4;  // window plane m is z = k0 - 2 + m; the tile's planes are m = 2 .. KZ + 1
@@ -538,6 +538,13 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
     const unsigned o_p = cur * seg, o_pn = (cur ^ 1u) * seg;
     const unsigned cv = G.cofs * 4u, rv = R.has ? rofs * 4u : aend;
     const bool after_true = s.kind == K_SETUP || s.kind == K_RESID;  // no p needed: WCOMP or done
+    // NTS: the iteration's vectors are stored nontemporal (films whose CG
+    // vectors exceed the last-level cache, where nothing they store is read
+    // again from it anyway)
+    auto vst = [](rsrc_t r, unsigned voff, unsigned soff, float v) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff,
+                                              NTS ? 2 : 0);
+    };
 
     Plane2 L[NW];
     auto issue = [&](int m) {
@@ -593,8 +600,8 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
                 if (R.has) sp[m & 1][R.ly + 2][R.lx + 2] = ph[m];
                 if (tile_plane) {
                     const unsigned vo = ok ? cv : aend, sv = so[m] * pb4;
-                    bstore(RB, vo, o_pn + sv, pc[m]);
-                    bstore(RX, vo, sv, L[m].x + alpha * pc[m]);
+                    vst(RB, vo, o_pn + sv, pc[m]);
+                    vst(RX, vo, sv, L[m].x + alpha * pc[m]);
                 }
             } else if (ok) {
                 const long long id = (long long)G.cofs + (long long)c.plane * k;
@@ -634,7 +641,7 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
             }
             if (m >= 2 && m <= KZ + 1) {
                 const bool ok = tp.live && k < nz;
-                if (iter) bstore(RA, ok ? cv : aend, o_rn + so[m] * pb4, rc);
+                if (iter) vst(RA, ok ? cv : aend, o_rn + so[m] * pb4, rc);
                 if (ok) {
                     cg_dot_add<true>(acc[0], rc, uc[m], c, k);
                     acc[2] += (double)rc * (double)rc;
@@ -691,9 +698,9 @@ __global__ __launch_bounds__(NT, cg32_waves(KZ)) void k_cg32(CG32Bufs b, StepCon
 // sixteen and thirty-two.
 constexpr int cg32r_waves(int kz) { return kz <= 2 ? 6 : (kz == 4 ? 5 : 4); }  // 4 at 16 and 32
 
-template <int KZ, int PM>
+template <int KZ, int PM, bool NTS>
 __global__ __launch_bounds__(NT, cg32r_waves(KZ)) void k_cg32r(CG32Bufs b, StepConsts c, int nb, int ps, int t) {
-    cg32r_body<KZ, PM>(b, c, nb, ps, t, tile_id());
+    cg32r_body<KZ, PM, NTS>(b, c, nb, ps, t, tile_id());
 }
 
 // All films of a batch in one launch: the tile table maps each workgroup to
@@ -753,12 +760,22 @@ static void launch_cg32_as(hipStream_t s, const Grid& g, const StepConsts& c, co
 }
 
 static void launch_cg32r(hipStream_t s, const Grid& g, const StepConsts& c, const CG32Bufs& b, int t,
-                         const LaunchEv& ev) {
-#define HZO_X
+                         bool stream_stores, const LaunchEv& ev) {
     const int nb = blocks_for_impl(g), ps = part_stride_impl(nb);
     if (g.kz >= 16) {  // deep tiles for large films
         const int pm = part_pm(b.nb_in);
         const dim3 gr = grid3(g), bl(BX, BY, 1);
+#define HZO_X , true
+        if (stream_stores) {
+            if (g.kz == 32) {
+                HZO_PCG_LAUNCH_PM(k_cg32r, 32, pm, gr, bl, s, b, c, nb, ps, t);
+            } else {
+                HZO_PCG_LAUNCH_PM(k_cg32r, 16, pm, gr, bl, s, b, c, nb, ps, t);
+            }
+            return;
+        }
+#undef HZO_X
+#define HZO_X , false
         if (g.kz == 32) {
             HZO_PCG_LAUNCH_PM(k_cg32r, 32, pm, gr, bl, s, b, c, nb, ps, t);
         } else {
@@ -771,9 +788,9 @@ static void launch_cg32r(hipStream_t s, const Grid& g, const StepConsts& c, cons
 }
 
 void launch_cg32(hipStream_t s, const Grid& g, const StepConsts& c, const CG32Bufs& b, int t, bool two_stencil,
-                 const LaunchEv& ev) {
+                 bool stream_stores, const LaunchEv& ev) {
     if (two_stencil)
-        launch_cg32r(s, g, c, b, t, ev);
+        launch_cg32r(s, g, c, b, t, stream_stores, ev);
     else if (c.zbc != 0)
         launch_cg32_as<true>(s, g, c, b, t, ev);
     else
