@@ -540,10 +540,11 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
     const bool after_true = s.kind == K_SETUP || s.kind == K_RESID;  // no p needed: WCOMP or done
     // NTS: the iteration's vectors are stored nontemporal (films whose CG
     // vectors exceed the last-level cache, where nothing they store is read
-    // again from it anyway)
+    // again from it anyway); otherwise with device scope, like every solver
+    // store
     auto vst = [](rsrc_t r, unsigned voff, unsigned soff, float v) {
         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff,
-                                              NTS ? 2 : 0);
+                                              NTS ? ST_STREAM : ST_DEVICE);
     };
 
     Plane2 L[NW];
