@@ -1,13 +1,15 @@
 // probe.hip -- streaming-traffic probe: the attainable rate of a kernel that
 // moves exactly the per-voxel traffic of the single-reduction CG (five f32
 // arrays and one u16 array read, five f32 arrays written: 42 B per element)
-// with no stencil, halo or reduction.  Used to place the CG kernel's achieved
-// bandwidth between this floor and the HBM peak.
+// with no stencil, halo or reduction, storing with the solver vectors' cache
+// policy.  Used to place the CG kernel's achieved bandwidth between this
+// floor and the HBM peak.
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
 #include "../hzo_internal.h"
 #include "../host/errors.h"
+#include "tile.hip.h"
 
 namespace hzo {
 
@@ -17,21 +19,26 @@ __global__ __launch_bounds__(512) void k_probe_stream(const float* __restrict__ 
                                                       float* __restrict__ oa, float* __restrict__ ob,
                                                       float* __restrict__ oc, float* __restrict__ od,
                                                       float* __restrict__ oe, long long n) {
+    const unsigned bytes = (unsigned)(4 * n);
+    const rsrc_t Ra = make_rsrc(oa, bytes), Rb = make_rsrc(ob, bytes), Rc = make_rsrc(oc, bytes),
+                 Rd = make_rsrc(od, bytes), Re = make_rsrc(oe, bytes);
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const float s = a[i] + b[i] * c[i] + d[i] - e[i] + (float)code[i];
-        oa[i] = s;
-        ob[i] = 2.0f * s;
-        oc[i] = s + 1.0f;
-        od[i] = s - 1.0f;
-        oe[i] = 0.5f * s;
+        const unsigned o = (unsigned)i * 4u;
+        bstore(Ra, o, 0u, s);
+        bstore(Rb, o, 0u, 2.0f * s);
+        bstore(Rc, o, 0u, s + 1.0f);
+        bstore(Rd, o, 0u, s - 1.0f);
+        bstore(Re, o, 0u, 0.5f * s);
     }
 }
 
 }  // namespace hzo
 
 extern "C" int hzo_probe_stream(long long n, int grid, int reps, double* us_per_launch) {
-    if (n < 1 || grid < 1 || reps < 1 || !us_per_launch) return hzo::set_error(HZO_ERR_ARG, "hzo_probe_stream: bad argument");
+    if (n < 1 || n > (1LL << 28) || grid < 1 || reps < 1 || !us_per_launch)
+        return hzo::set_error(HZO_ERR_ARG, "hzo_probe_stream: bad argument");
     float* v[10] = {};
     uint16_t* code = nullptr;
     hipEvent_t t0 = nullptr, t1 = nullptr;
