@@ -64,6 +64,17 @@ def test_create_rejects_solver_and_tile_combinations(kw, msg):
     assert msg in L.lib().hzo_last_error()
 
 
+@pytest.mark.parametrize("n,grid,reps", [(0, 512, 1), ((1 << 28) + 1, 512, 1), (1 << 21, 0, 1), (1 << 21, 512, 0)])
+def test_stream_probe_rejects_bad_sizes(n, grid, reps):
+    # its 32-bit buffer offsets cover at most 2^28 four-byte elements; the
+    # argument check runs before any device call
+    import ctypes as C
+    us = C.c_double(-1.0)
+    rc = L.lib().hzo_probe_stream(C.c_longlong(n), grid, reps, C.byref(us))
+    assert rc == -1 and us.value == -1.0
+    assert b"hzo_probe_stream" in L.lib().hzo_last_error()
+
+
 def test_splitmix64_published_sequence():
     # SplitMix64 seeded with 0 (Steele, Lea, Flood 2014; Vigna's reference code)
     out = F.splitmix64(0, 3)
