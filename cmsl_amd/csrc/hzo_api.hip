@@ -435,8 +435,10 @@ int cg32_solve(hzo_ctx* x, const hzo::Fp32Bufs& b32in, const hzo::StepConsts& c,
     if (rc) return rc;
     fold_after(x, x->stream, 0);
     // launches: one WCOMP, one per iteration, two per residual replacement
-    // at most three launches per iteration (replacement, re-form, iterate)
-    const long long maxl = 3LL * init.max_iter + 8;
+    // and one before it or the end when an iteration's x_lo term is still
+    // deferred; at most four per iteration (add, replacement, re-form,
+    // iterate)
+    const long long maxl = 4LL * init.max_iter + 8;
     long long launches = 0;
     int t = 0;
     int batch = last_iters + 2, next = 4;
@@ -455,7 +457,7 @@ int cg32_solve(hzo_ctx* x, const hzo::Fp32Bufs& b32in, const hzo::StepConsts& c,
     }
     hzo::PcgState& s = *x->st_host;
     if (!s.done) s.status = HZO_PCG_MAXITER;
-    last_iters = s.iters + 2 * s.refinements + 1;
+    last_iters = s.iters + 2 * s.refinements + s.xadds + 1;
     return copy_stats(s, stats);
 }
 
@@ -1623,10 +1625,10 @@ int group_step(hzo_batch* B, hzo_batch::Group& G, const double* Ez, int s, bool 
         hzo::launch_setup_f32_batch(B->stream, G.ntiles, G.dfilms, G.dtiles, (s + 1) & 1);
     }
     HZO_HIP(hipGetLastError());
-    // the most launches any film may need: three per iteration
+    // the most launches any film may need: four per iteration
     int maxl = 0;
     for (int f : G.idx) {
-        const int m = 3 * B->films[f]->cfg.max_iter + 8;
+        const int m = 4 * B->films[f]->cfg.max_iter + 8;
         maxl = m > maxl ? m : maxl;
     }
     int nchunks = (G.need + B->chunk - 1) / B->chunk;

@@ -58,13 +58,18 @@ constexpr int PTAB_ENTRIES = 6;
 // inside the iteration; the host polls `done` between launch batches.  Two
 // copies alternate by launch parity: the kernel launched t-th after the setup
 // reads copy (t-1)&1 and its workgroup 0 writes copy t&1 (see reduce.hip.h).
-enum : int { K_SETUP = 0, K_DIR = 1, K_FOLD = 2, K_UPDATE = 3, K_RESID = 4, K_WCOMP = 5, K_ITER = 6 };
+enum : int { K_SETUP = 0, K_DIR = 1, K_FOLD = 2, K_UPDATE = 3, K_RESID = 4, K_WCOMP = 5, K_ITER = 6, K_XADD = 7 };
 struct PcgState {
     double rz_old, rz0, rr, rr0, bb, alpha, beta, denom;
     int iters, status, done, refinements;
     int max_iter, rule, kind, pbuf;  // kind: K_* of the kernel that wrote this copy
     double tol_abs2, tol_rel2;
-    int refine_every, pad0, pad1, pad2;  // FP32-refined: residual replacement period
+    int refine_every;  // FP32-refined: residual replacement period
+    // single-reduction CG: an iteration may leave its x_lo += alpha p to the
+    // next one (xpend = 1, alpha_prev = its alpha); xadds counts the launches
+    // that only add such a term (before a residual replacement or the end)
+    int xpend, xadds, pad2;
+    double alpha_prev;
 };
 
 // Per-workgroup partial sums: two parity slots of RED_MAXQ arrays each for

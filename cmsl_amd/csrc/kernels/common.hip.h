@@ -201,7 +201,10 @@ __device__ __forceinline__ PcgState load_state(const PcgState* p) {
     s.tol_abs2 = p->tol_abs2;
     s.tol_rel2 = p->tol_rel2;
     s.refine_every = p->refine_every;
-    s.pad0 = s.pad1 = s.pad2 = 0;
+    s.xpend = p->xpend;
+    s.xadds = p->xadds;
+    s.pad2 = 0;
+    s.alpha_prev = p->alpha_prev;
     return s;
 }
 
@@ -225,6 +228,9 @@ __device__ __forceinline__ void store_state(const PcgState& s, PcgState* p) {
     p->tol_abs2 = s.tol_abs2;
     p->tol_rel2 = s.tol_rel2;
     p->refine_every = s.refine_every;
+    p->xpend = s.xpend;
+    p->xadds = s.xadds;
+    p->alpha_prev = s.alpha_prev;
 }
 
 // Copies the control block forward (thread 0 of the film's lead tile).

@@ -93,13 +93,16 @@ __device__ __forceinline__ int cg_decide(PcgState& s, const double (&tot)[3], do
         s.done = 1;
         return 1;
     }
-    if (prev == K_ITER) {
+    // K_XADD: the last launch added a deferred x_lo term before a residual
+    // replacement (its partials are zero; the replacement restates rr)
+    if (prev == K_ITER || prev == K_XADD) {
         if (s.iters >= s.max_iter) {
             s.done = 1;
             s.status = HZO_PCG_MAXITER;
             return 1;
         }
-        if (!keep_going(s, g, rr) || (s.refine_every > 0 && s.iters % s.refine_every == 0)) return 2;
+        if (prev == K_XADD || !keep_going(s, g, rr) || (s.refine_every > 0 && s.iters % s.refine_every == 0))
+            return 2;
     }
     // prev == K_WCOMP (fresh start when iters == 0) or K_ITER
     double den;
@@ -121,6 +124,42 @@ __device__ __forceinline__ int cg_decide(PcgState& s, const double (&tot)[3], do
     s.denom = den;
     s.rz_old = g;
     return 0;
+}
+
+// How an iteration updates x_lo (from the control block before the launch's
+// decision): 0 leaves its alpha p to the next iteration, 1 adds alpha p,
+// 2 adds the deferred term first, (x_lo + alpha_prev p_prev) + alpha p, the
+// same two roundings as two separate updates.  Iterations defer in turn, so
+// half of them neither read nor write x_lo; the one before a periodic
+// residual replacement never defers, so the replacement sees all of x.
+__device__ __forceinline__ int cg_x_mode(const PcgState& s) {
+    if (s.xpend) return 2;
+    return (s.refine_every > 0 && (s.iters + 1) % s.refine_every == 0) ? 1 : 0;
+}
+
+// The control-block update after cg_decide.  A solve that stops, or replaces
+// its residual, while an x_lo term is deferred first spends one launch
+// (mode 4, XADD) adding it; after a replacement that launch hands on to the
+// RESID launch, after a stop it keeps `done`.
+__device__ __forceinline__ int cg_advance(PcgState& s, int mode, int xm, double alpha) {
+    if ((mode == 1 || mode == 2) && s.xpend) {
+        s.kind = K_XADD;
+        s.xpend = 0;
+        s.xadds += 1;
+        return 4;
+    }
+    if (mode == 2) {
+        s.kind = K_RESID;
+    } else if (mode == 3) {
+        s.kind = K_WCOMP;
+    } else if (mode == 0) {
+        s.iters += 1;
+        s.pbuf ^= 1;
+        s.kind = K_ITER;
+        s.xpend = xm == 0;
+        if (xm == 0) s.alpha_prev = alpha;
+    }
+    return mode;
 }
 
 // One plane of a tile as the z-streaming loop holds it: the centre column's
@@ -273,6 +312,11 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
     // an iteration follows unless the last launch set up or replaced the
     // residual (then this launch re-forms w and needs neither w nor s)
     const bool after_true = s.kind == K_SETUP || s.kind == K_RESID;
+    // x_lo is read and written only by iterations that update it; the lanes
+    // of the others address it out of range (no access and no branch)
+    const int xm = cg_x_mode(s);
+    const float aprev = (float)s.alpha_prev;
+    const unsigned xo = (after_true || xm == 0) ? aend : cv;
 
     // Plane loads, two planes ahead of their use.
     Plane32 L[KZ + 2];
@@ -284,7 +328,7 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
         L[q].s = after_true ? 0.0f : bload<float>(RB, cv, o_s + sv);
         if (q >= 1 && q <= KZ) {
             L[q].p = after_true ? 0.0f : bload<float>(RB, cv, o_p + sv);
-            L[q].x = after_true ? 0.0f : bload<float>(RX, cv, sv);
+            L[q].x = bload<float>(RX, xo, sv);
             L[q].rh = bload<float>(RA, hv, o_r + sv);
             L[q].dh = bload<float>(Rdi, ch[q] * 4u, 0u);
             L[q].wh = after_true ? 0.0f : bload<float>(RA, hv, o_w + sv);
@@ -303,16 +347,7 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
     double tot[3];
     wave_lds_total<3>(sred, tot);
     double alpha_d = 0.0, beta_d = 0.0;
-    const int mode = cg_decide(s, tot, alpha_d, beta_d);
-    if (mode == 2) {
-        s.kind = K_RESID;
-    } else if (mode == 3) {
-        s.kind = K_WCOMP;
-    } else if (mode == 0) {
-        s.iters += 1;
-        s.pbuf ^= 1;
-        s.kind = K_ITER;
-    }
+    const int mode = cg_advance(s, cg_decide(s, tot, alpha_d, beta_d), xm, alpha_d);
     // handed on at once: nothing later in the launch reads the block, so it
     // need not stay live in registers through the tile work
     pass_state(s, b.st + io, T.lead);
@@ -320,6 +355,15 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
     double acc[3] = {0.0, 0.0, 0.0};
     if (mode == 2) {
         cg32_resid<KZ, MIR>(b, c, tp, G, k0, RA, o_r, Rsc, Rdi, acc);
+    } else if (mode == 4) {
+        // XADD: the deferred x_lo += alpha_prev p of the last iteration
+#pragma unroll
+        for (int q = 1; q <= KZ; ++q) {
+            const int k = k0 - 1 + q;
+            const unsigned sv = so[q] * pb4;
+            const float xv = bload<float>(RX, cv, sv), pv = bload<float>(RB, cv, o_p + sv);
+            bstore(RX, (tp.live && k < nz) ? cv : aend, sv, xv + aprev * pv);
+        }
     } else {
         // ITER (mode 0): s = w + beta s and r -= alpha s on the tile and its
         // halo, p = M r_old + beta p, x += alpha p, then u = M r and w = A u;
@@ -353,7 +397,8 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
                     const float uo = L[q].r * L[q].d;
                     const float pn = fresh ? uo : uo + beta * L[q].p;
                     bstore(RB, vo, o_p + sv, pn);
-                    bstore(RX, vo, sv, L[q].x + alpha * pn);
+                    const float xa = xm == 2 ? L[q].x + aprev * L[q].p : L[q].x;
+                    bstore(RX, xm != 0 ? vo : aend, sv, xa + alpha * pn);
                 }
             } else if (interior && ok) {
                 const long long id = (long long)G.cofs + (long long)c.plane * k;
@@ -538,6 +583,11 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
     const unsigned o_p = cur * seg, o_pn = (cur ^ 1u) * seg;
     const unsigned cv = G.cofs * 4u, rv = R.has ? rofs * 4u : aend;
     const bool after_true = s.kind == K_SETUP || s.kind == K_RESID;  // no p needed: WCOMP or done
+    // x_lo is read and written only by iterations that update it; the lanes
+    // of the others address it out of range (no access and no branch)
+    const int xm = cg_x_mode(s);
+    const float aprev = (float)s.alpha_prev;
+    const unsigned xo = (after_true || xm == 0) ? aend : cv;
     // NTS: the iteration's vectors are stored nontemporal (films whose CG
     // vectors exceed the last-level cache, where nothing they store is read
     // again from it anyway); otherwise with device scope, like every solver
@@ -556,7 +606,7 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
         L[m].dh = bload<float>(Rdi, (cd[m] >> 16) * 4u, 0u);
         L[m].p = after_true ? 0.0f : bload<float>(RB, cv, o_p + sv);
         L[m].ph = after_true ? 0.0f : bload<float>(RB, rv, o_p + sv);
-        L[m].x = (after_true || m < 2 || m > KZ + 1) ? 0.0f : bload<float>(RX, cv, sv);
+        L[m].x = (m < 2 || m > KZ + 1) ? 0.0f : bload<float>(RX, xo, sv);
     };
     issue(0);
     issue(1);
@@ -567,21 +617,21 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
     double tot[3];
     wave_lds_total<3>(sred, tot);
     double alpha_d = 0.0, beta_d = 0.0;
-    const int mode = cg_decide(s, tot, alpha_d, beta_d);
-    if (mode == 2) {
-        s.kind = K_RESID;
-    } else if (mode == 3) {
-        s.kind = K_WCOMP;
-    } else if (mode == 0) {
-        s.iters += 1;
-        s.pbuf ^= 1;
-        s.kind = K_ITER;
-    }
+    const int mode = cg_advance(s, cg_decide(s, tot, alpha_d, beta_d), xm, alpha_d);
     pass_state(s, b.st + io, T.lead);
     if (mode == 1) return;
     double acc[3] = {0.0, 0.0, 0.0};
     if (mode == 2) {
         cg32_resid<KZ, true>(b, c, tp, G, k0, RA, o_r, Rsc, Rdi, acc);
+    } else if (mode == 4) {
+        // XADD: the deferred x_lo += alpha_prev p of the last iteration
+#pragma unroll
+        for (int m = 2; m <= KZ + 1; ++m) {
+            const int k = k0 - 2 + m;
+            const unsigned sv = so[m] * pb4;
+            const float xv = bload<float>(RX, cv, sv), pv = bload<float>(RB, cv, o_p + sv);
+            vst(RX, (tp.live && k < nz) ? cv : aend, sv, xv + aprev * pv);
+        }
     } else {
         const bool iter = mode == 0;
         const float alpha = (float)alpha_d, beta = (float)beta_d;
@@ -602,7 +652,8 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
                 if (tile_plane) {
                     const unsigned vo = ok ? cv : aend, sv = so[m] * pb4;
                     vst(RB, vo, o_pn + sv, pc[m]);
-                    vst(RX, vo, sv, L[m].x + alpha * pc[m]);
+                    const float xa = xm == 2 ? L[m].x + aprev * L[m].p : L[m].x;
+                    vst(RX, xm != 0 ? vo : aend, sv, xa + alpha * pc[m]);
                 }
             } else if (ok) {
                 const long long id = (long long)G.cofs + (long long)c.plane * k;
@@ -681,11 +732,12 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
     write_block_partials_w<3>(acc, io ? b.part[1] : b.part[0], ps, T.blk, spart);
 }
 
-// Waves per SIMD the register budget is set for: 8 (4 workgroups per CU,
-// <= 64 VGPRs) at two planes per tile, 6 at four and eight, 4 at sixteen.
-// The batch kernel, which also holds its film's descriptor, is set for 5
-// (4 at depth 8).
-constexpr int cg32_waves(int kz) { return kz >= 16 ? 4 : (kz <= 2 ? 8 : 6); }
+// Waves per SIMD the register budget is set for: 6 (3 workgroups per CU,
+// <= 80 VGPRs) at two planes per tile, 5 at four and eight, 4 at sixteen.
+// The automatic choice runs this form only on films of at most 512
+// depth-8 workgroups, two per CU, which 5 waves per SIMD still hold.  The
+// batch kernel, which also holds its film's descriptor, is set for 4.
+constexpr int cg32_waves(int kz) { return kz >= 16 ? 4 : (kz <= 2 ? 6 : 5); }
 
 // MIR: the film has Neumann electrodes (mirrored boundary planes weighted in
 // the CG sums); Dirichlet films run the MIR = false build.
@@ -708,7 +760,7 @@ __global__ __launch_bounds__(NT, cg32r_waves(KZ)) void k_cg32r(CG32Bufs b, StepC
 // (film, logical tile); every film runs its own control block, and the tiles
 // of a film whose solve has finished return at once.
 template <int KZ, int PM, bool MIR>
-__global__ __launch_bounds__(NT, KZ >= 8 ? 4 : 5) void k_cg32_batch(const BatchFilm* __restrict__ films, const int2* __restrict__ tiles,
+__global__ __launch_bounds__(NT, 4) void k_cg32_batch(const BatchFilm* __restrict__ films, const int2* __restrict__ tiles,
                                                                       int t) {
     TileId T;
     const BatchFilm& F = films[batch_tile(tiles, films, T)];
