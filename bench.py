@@ -45,15 +45,17 @@ def algorithmic_bytes_per_cell(kclass: str, precision: str, cg_form: int = 1) ->
           pcg_setup reads Pmx,Pmy,Pmz (12), x_hi, x_lo (12), code (2), writes b (8), r (4) = 38;
           tdgl reads P (24), x_hi + x_lo (12), cell word (4), writes P (24) = 64;
           pcg_fused (single-reduction CG, one launch per iteration) carrying w and
-          s reads r, w, s, p, x_lo (20), code (2), writes s, r, w, p, x_lo (20) = 42;
-          forming s = A p and w = A u by stencils (cg_form 2) it reads r, p, x_lo
-          (12), code (2), writes r, p, x_lo (12) = 26.
+          s reads r, w, s, p (16), code (2), writes s, r, w, p (16) = 34, and
+          every second iteration also reads and writes x_lo (8, two iterations'
+          alpha p at once): 38 on average; forming s = A p and w = A u by
+          stencils (cg_form 2) it reads r, p (8), code (2), writes r, p (8) = 18,
+          with x_lo likewise: 22 on average.
     """
     if kclass == "pcg_fused" and precision == "fp32" and cg_form == 2:
-        return 26.0
+        return 22.0
     table = {
         "fp64": {"tdgl": 108.0, "pcg_setup": 42.0, "pcg_dir_spmv": 26.0, "pcg_update": 42.0, "pcg_fused": 0.0},
-        "fp32": {"tdgl": 64.0, "pcg_setup": 38.0, "pcg_dir_spmv": 14.0, "pcg_update": 22.0, "pcg_fused": 42.0},
+        "fp32": {"tdgl": 64.0, "pcg_setup": 38.0, "pcg_dir_spmv": 14.0, "pcg_update": 22.0, "pcg_fused": 38.0},
     }
     return table[precision][kclass]
 
