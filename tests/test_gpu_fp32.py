@@ -199,3 +199,31 @@ def test_two_stencil_form_on_films_smaller_than_a_tile(shape, kz):
     _say(shape, kz, out[2][0].tolist(), out[3][0].tolist())
     assert np.max(np.abs(out[2][0] - out[3][0])) <= 1
     assert rel_err(out[3][1], out[2][1]) < 1e-6 and rel_err(out[3][2], out[2][2]) < 1e-6
+
+
+@pytest.mark.parametrize("max_iter", [3, 4, 7])
+@pytest.mark.parametrize("fused", [2, 3])
+def test_single_reduction_cg_stopped_early_keeps_every_update(max_iter, fused):
+    """Iterations of the single-reduction CG leave x_lo += alpha p to the
+    next one in turn; a solve that stops with such a term outstanding (odd
+    max_iter here) adds it in one more launch.  Stopped after a few
+    iterations, the FP32 potential must then match the FP64 PCG stopped at
+    the same count: the iterates agree to FP32 rounding, while one dropped
+    alpha p would leave an error of the size of an early CG correction."""
+    inp = film_inputs(64, 48, 16, 12, fracs=(0.2, 0.6, 0.2), seed=21, jitter=0.2)
+    e = float(inp.field_nd(1.5))
+    out = {}
+    for prec, fu in ((L.PREC_FP64, 0), (L.PREC_FP32_REFINED, fused)):
+        g = H.Film(inp, precision=prec, pcg_rule=L.PCG_RULE_RELRES, tol_rel=1e-12, tol_abs=1e-30,
+                   max_iter=max_iter, refine_every=20, cg_fused=fu)
+        st = g.step(e)
+        out[prec] = (st, g.get_state()[1])
+        g.close()
+    s64, phi64 = out[L.PREC_FP64]
+    s32, phi32 = out[L.PREC_FP32_REFINED]
+    err = rel_err(phi32, phi64)
+    _say(f"max_iter={max_iter} fused={fused}: iters {s64.iters}/{s32.iters}, status {s64.status}/{s32.status}, "
+         f"phi rel err {err:.2e}")
+    assert s32.iters == s64.iters
+    assert s32.status == s64.status == 1  # HZO_PCG_MAXITER
+    assert err < 1e-5
