@@ -6,7 +6,8 @@ setup and step loop.
 """
 from . import _lib
 from ._lib import HzoError, device_count, lib
-from .film import Batch, Film, FilmInputs, make_film_inputs, nondim, triangle_wave
+from .film import (Batch, Film, FilmInputs, landau_raw, make_film_inputs, nondim, read_landau_table,
+                   triangle_wave)
 
-__all__ = ["Batch", "Film", "FilmInputs", "HzoError", "device_count", "lib", "make_film_inputs", "nondim",
-           "triangle_wave", "_lib"]
+__all__ = ["Batch", "Film", "FilmInputs", "HzoError", "device_count", "landau_raw", "lib", "make_film_inputs",
+           "nondim", "read_landau_table", "triangle_wave", "_lib"]

@@ -60,6 +60,11 @@ class SweepParams(C.Structure):
                 ("Ez_scale", C.c_double), ("P0_scale", C.c_double)]
 
 
+class LandauRow(C.Structure):
+    _fields_ = [("afe", C.c_double * 5), ("fe", C.c_double * 5), ("de", C.c_double * 2),
+                ("fracs", C.c_double * 3), ("ref_type", C.c_int), ("ncols", C.c_int)]
+
+
 class Deck(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("device_flag", "nx", "ny", "nz", "dep_flag", "num_steps", "out_flag",
                                       "dimx", "dimy", "dimz", "do_pre_relax", "n_layers")] + \
@@ -123,6 +128,8 @@ SIGNATURES = {
     "hzo_read_checkpoint": (_I, [C.c_char_p, _I, _I, _I, _P, _P, _P, _P, _P, _P]),
     "hzo_write_raw_f64": (_I, [C.c_char_p, C.c_longlong, _P]),
     "hzo_read_deck": (_I, [C.c_char_p, _P]),
+    "hzo_read_landau_table": (_I, [C.c_char_p, _I, _P, _P, _P]),
+    "hzo_landau_to_raw": (_I, [_P, _P]),
     "hzo_read_grain_count": (_I, [C.c_char_p, _P]),
     "hzo_read_gdata": (_I, [C.c_char_p, _I, _I, _I, _P, _P]),
     "hzo_read_gchar": (_I, [C.c_char_p, _I, _P, _P]),

@@ -7,6 +7,9 @@
 //   hzo_euler_to_TR        grain_io.cu:101-126 (Bunge ZXZ, passive)
 //   hzo_write_gdata/gchar  voronoi.c:197-223 (generator output formats)
 //   hzo_count_structures   main.cu:430-440 (inputs/struc1, struc2, ... until one is missing)
+//   hzo_read_landau_table  main.cu:112-151 (LoadLandauPool) and the shipped 12- and 16-column tables
+//   hzo_landau_to_raw      main.cu:155-178 (PickLandauSet)
+#include <cctype>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -240,6 +243,110 @@ int hzo_count_structures(const char* inputs_dir, int* n) {
         if (stat(path, &st) != 0 || !S_ISDIR(st.st_mode)) break;
         *n = s;
     }
+    return HZO_OK;
+}
+
+// Landau-coefficient tables.  Three row forms occur in the reference tree:
+// the 13-token pool rows LoadLandauPool reads (12 coefficients and the
+// reference phase label, main.cu:123-151), the 12-column
+// inputs/FreeEnergyParams.txt (coefficients only) and the 16-token
+// src/inputs/SingleL_input.txt (coefficients, AFE/FE/DE fractions in
+// percent, label).  The reference reader accepts only the 13-token form and
+// so loads no row of the shipped tables; this reader takes all three by
+// token count.  As in LoadLandauPool, blank lines and lines whose first
+// non-blank character is '#' are comments, a label other than AFE, FE or DE
+// drops its row, and so does any other token count; dropped rows are
+// counted, not fatal.  A numeric token must parse whole (strtod).
+static int phase_of_label(const char* s) {
+    if (!strcmp(s, "AFE")) return HZO_AFE;
+    if (!strcmp(s, "FE")) return HZO_FE;
+    if (!strcmp(s, "DE")) return HZO_DE;
+    return -1;
+}
+
+static bool parse_num(const char* t, double* v) {
+    char* end = nullptr;
+    *v = strtod(t, &end);
+    return end != t && *end == '\0';
+}
+
+int hzo_read_landau_table(const char* path, int max_rows, hzo_landau_row* rows, int* nrows, int* nskipped) {
+    if (!path || !nrows || max_rows < 0 || (max_rows > 0 && !rows))
+        return set_error(HZO_ERR_ARG, "hzo_read_landau_table: bad argument");
+    FILE* f = fopen(path, "r");
+    if (!f) return set_error(HZO_ERR_IO, "hzo_read_landau_table: cannot open %s", path);
+    int n = 0, skipped = 0;
+    std::string line;
+    std::vector<std::string> tok;
+    int ch = 0;
+    while (ch != EOF) {
+        line.clear();
+        while ((ch = fgetc(f)) != EOF && ch != '\n') line.push_back((char)ch);
+        if (ch == EOF && line.empty()) break;
+        tok.clear();
+        size_t i = 0;
+        while (i < line.size()) {
+            while (i < line.size() && isspace((unsigned char)line[i])) ++i;
+            const size_t j = i;
+            while (i < line.size() && !isspace((unsigned char)line[i])) ++i;
+            if (i > j) tok.emplace_back(line, j, i - j);
+        }
+        if (tok.empty() || tok[0][0] == '#') continue;
+        const int nt = (int)tok.size();
+        hzo_landau_row r;
+        memset(&r, 0, sizeof(r));
+        r.fracs[0] = r.fracs[1] = r.fracs[2] = -1.0;
+        r.ref_type = -1;
+        r.ncols = nt;
+        const int nnum = nt == 12 ? 12 : nt == 13 ? 12 : nt == 16 ? 15 : -1;
+        bool ok = nnum > 0;
+        double v[15];
+        for (int q = 0; ok && q < nnum; ++q) ok = parse_num(tok[q].c_str(), &v[q]);
+        if (ok && nt != 12) {
+            r.ref_type = phase_of_label(tok[nt - 1].c_str());
+            ok = r.ref_type >= 0;
+        }
+        if (!ok) {
+            ++skipped;
+            continue;
+        }
+        for (int q = 0; q < 5; ++q) {
+            r.afe[q] = v[q];
+            r.fe[q] = v[5 + q];
+        }
+        r.de[0] = v[10];
+        r.de[1] = v[11];
+        if (nt == 16)
+            for (int q = 0; q < 3; ++q) r.fracs[q] = v[12 + q];
+        if (n < max_rows) rows[n] = r;
+        ++n;
+    }
+    fclose(f);
+    *nrows = n;
+    if (nskipped) *nskipped = skipped;
+    return HZO_OK;
+}
+
+// PickLandauSet (main.cu:155-178): alpha, beta, gamma, g, a0 of AFE and FE
+// from the row, DE alpha and a0 with beta = gamma = g = 0; L, kappa, eps_r
+// and transition_type stay as given.
+int hzo_landau_to_raw(const hzo_landau_row* row, hzo_phase_raw raw[3]) {
+    if (!row || !raw) return set_error(HZO_ERR_ARG, "hzo_landau_to_raw: null argument");
+    const double* src[2] = {row->afe, row->fe};
+    for (int p = 0; p < 2; ++p) {
+        hzo_phase_raw& d = raw[p == 0 ? HZO_AFE : HZO_FE];
+        d.alpha = src[p][0];
+        d.beta = src[p][1];
+        d.gamma = src[p][2];
+        d.g = src[p][3];
+        d.a0 = src[p][4];
+    }
+    hzo_phase_raw& de = raw[HZO_DE];
+    de.alpha = row->de[0];
+    de.beta = 0.0;
+    de.gamma = 0.0;
+    de.g = 0.0;
+    de.a0 = row->de[1];
     return HZO_OK;
 }
 

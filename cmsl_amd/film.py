@@ -30,6 +30,38 @@ BASE_RAW = {
 }
 
 
+def read_landau_table(path, max_rows=100000):
+    """Rows of a Landau-coefficient table (hzo_read_landau_table): 13-token
+    pool rows (LoadLandauPool, main.cu:112-151), the 12-column
+    FreeEnergyParams.txt and the 16-token SingleL_input.txt.  Returns
+    (rows, skipped): each row a dict of SI coefficients with 'ref_type'
+    (-1 without a label), 'fracs' (percent, None without) and 'ncols'."""
+    n, sk = C.c_int(0), C.c_int(0)
+    L.call("hzo_read_landau_table", str(path).encode(), 0, None, C.byref(n), C.byref(sk))
+    buf = (L.LandauRow * max(n.value, 1))()
+    L.call("hzo_read_landau_table", str(path).encode(), n.value, buf, C.byref(n), C.byref(sk))
+    rows = []
+    for r in buf[: n.value]:
+        rows.append(dict(afe=list(r.afe), fe=list(r.fe), de=list(r.de),
+                         fracs=None if r.ncols != 16 else list(r.fracs),
+                         ref_type=int(r.ref_type), ncols=int(r.ncols), _c=r))
+    return rows, sk.value
+
+
+def landau_raw(row, base=None):
+    """PickLandauSet (main.cu:155-178): the row's alpha, beta, gamma, g, a0 for
+    AFE and FE and (alpha, a0) for DE over base's L, kappa, eps_r."""
+    base = base or BASE_RAW
+    arr = (L.PhaseRaw * 3)()
+    for q in range(3):
+        for k in ("alpha", "beta", "gamma", "g", "a0", "L", "kappa", "eps_r"):
+            setattr(arr[q], k, float(base[q][k]))
+        arr[q].transition_type = int(base[q].get("transition_type", 0))
+    L.call("hzo_landau_to_raw", C.byref(row["_c"]), arr)
+    return {q: {k: getattr(arr[q], k) for k in ("alpha", "beta", "gamma", "g", "a0", "L", "kappa", "eps_r")}
+            for q in range(3)}
+
+
 def splitmix64(seed: int, n: int) -> np.ndarray:
     out = np.zeros(n, np.uint64)
     L.call("hzo_splitmix64", C.c_uint64(seed), C.c_int64(n), L.ptr(out))
@@ -141,7 +173,7 @@ class FilmInputs:
 
 def make_film_inputs(nx, ny, nz, ng, fracs=(0.0, 1.0, 0.0), seed=1, del_h=0.5, del_t=0.01,
                      jitter=0.0, amp_fe_SI=0.01, raw=None, ref_type=L.HZO_FE, layer_counts=None,
-                     phase_seed=123456, label_device=None, gb_dielectric=0.0) -> FilmInputs:
+                     phase_seed=123456, label_device=None, gb_dielectric=0.0, landau=None) -> FilmInputs:
     """Seeded film: Voronoi grains, phase assignment, Landau jitter, initial P.
 
     jitter: alpha, beta, gamma of each phase are multiplied by U(1-jitter, 1+jitter)
@@ -149,8 +181,14 @@ def make_film_inputs(nx, ny, nz, ng, fracs=(0.0, 1.0, 0.0), seed=1, del_h=0.5, d
     label_device: a GPU index labels the Voronoi voxels there (same labels).
     gb_dielectric: share of all cells, taken from grain-boundary cells, that
     carry the dielectric phase (new grains with their parents' orientation).
+    landau: a row of read_landau_table; its coefficients replace those of raw
+    (PickLandauSet) and its label, when it has one, sets ref_type.
     """
     raw = {q: dict(v) for q, v in (raw or BASE_RAW).items()}
+    if landau is not None:
+        raw = landau_raw(landau, raw)
+        if landau["ref_type"] >= 0:
+            ref_type = landau["ref_type"]
     if jitter:
         u = uniform01(seed ^ 0x5EED5EED, 9).reshape(3, 3)
         for q in range(3):

@@ -303,6 +303,29 @@ typedef struct hzo_deck {
 } hzo_deck;
 int hzo_read_deck(const char* path, hzo_deck* out);
 
+/* Landau-coefficient tables (SI values).  Replaces LoadLandauPool
+ * (main.cu:112-151), which reads 13-token rows (12 coefficients and the
+ * reference phase label AFE|FE|DE); this reader also takes the 12-column rows
+ * of inputs/FreeEnergyParams.txt (no label: ref_type = -1) and the 16-token
+ * rows of src/inputs/SingleL_input.txt (12 coefficients, AFE/FE/DE fractions
+ * in percent, label).  Blank and '#' lines are comments; rows of any other
+ * token count or with another label are dropped and counted in *nskipped (may
+ * be NULL), as the reference drops them.  *nrows = rows read; the first
+ * max_rows are stored in rows (rows may be NULL when max_rows is 0). */
+typedef struct hzo_landau_row {
+    double afe[5];    /* AFE alpha, beta, gamma, g, a0 */
+    double fe[5];     /* FE alpha, beta, gamma, g, a0 */
+    double de[2];     /* DE alpha, a0 */
+    double fracs[3];  /* AFE, FE, DE fractions in percent (16-token rows), else -1 */
+    int ref_type;     /* reference phase of the label (HZO_AFE/FE/DE), -1 without one */
+    int ncols;        /* 12, 13 or 16 */
+} hzo_landau_row;
+int hzo_read_landau_table(const char* path, int max_rows, hzo_landau_row* rows, int* nrows, int* nskipped);
+/* PickLandauSet (main.cu:155-178): writes alpha, beta, gamma, g, a0 of raw[AFE]
+ * and raw[FE] from the row and raw[DE] = (alpha, 0, 0, 0, a0); L, kappa, eps_r
+ * and transition_type of raw[] are kept. */
+int hzo_landau_to_raw(const hzo_landau_row* row, hzo_phase_raw raw[3]);
+
 /* Grain count file "gnumb_<tag>.txt" (grain_io.cu:10-25). */
 int hzo_read_grain_count(const char* path, int* ng);
 /* "gdata_<tag>.bin": int32 omega[N] then f64 eta[N], z-fastest; reordered to
