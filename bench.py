@@ -207,14 +207,6 @@ def run_adjoint(args):
             "grad_norm": float(np.linalg.norm(g))}
 
 
-def batch_film_specs(n, seed=2024):
-    """Film shapes of the batched workload: nx = ny from {128, 192, 256, 384},
-    nz from {16, 24, 32, 48}, 50-300 grains, one seed per film."""
-    rng = np.random.default_rng(seed)
-    return [(int(rng.choice([128, 192, 256, 384])), int(rng.choice([16, 24, 32, 48])),
-             int(rng.integers(50, 301)), 100 + f) for f in range(n)]
-
-
 def run_batched(args):
     """Batched workload: args.batch_films films of uneven sizes, grain counts
     and Landau coefficients (+-20 %), FP32-refined, sinusoidal +-3 MV/cm over
@@ -228,7 +220,7 @@ def run_batched(args):
     import cmsl_amd as H
     from cmsl_amd import _lib as L
 
-    specs = batch_film_specs(args.batch_films)
+    specs = H.film.batch_film_specs(args.batch_films)
     t0 = time.perf_counter()
     with ThreadPoolExecutor(max_workers=8) as ex:
         inps = list(ex.map(lambda s: H.make_film_inputs(s[0], s[0], s[1], s[2], fracs=FRACS, seed=s[3],

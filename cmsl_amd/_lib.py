@@ -24,6 +24,9 @@ HZO_AFE, HZO_FE, HZO_DE = 0, 1, 2
 ZBC_DIRICHLET, ZBC_NEUMANN = 0, 1
 PREC_FP64, PREC_FP32_REFINED = 0, 1
 PCG_RULE_REFERENCE, PCG_RULE_RELRES = 0, 1
+# hzo_pcg_stats.status
+HZO_PCG_CONVERGED, HZO_PCG_MAXITER = 0, 1
+HZO_PCG_BAD_INITIAL, HZO_PCG_BREAKDOWN, HZO_PCG_BAD_RZ = -1, -2, -3
 KCLASSES = 5
 KCLASS_NAMES = ("tdgl", "pcg_setup", "pcg_dir_spmv", "pcg_update", "pcg_fused")
 

@@ -213,6 +213,14 @@ def make_film_inputs(nx, ny, nz, ng, fracs=(0.0, 1.0, 0.0), seed=1, del_h=0.5, d
                       meta=dict(seed=seed, fracs=tuple(fracs), jitter=jitter, raw=raw))
 
 
+def batch_film_specs(n, seed=2024):
+    """Film shapes of the batched workload: (nx = ny from {128, 192, 256, 384},
+    nz from {16, 24, 32, 48}, 50-300 grains, film seed) for n films."""
+    rng = np.random.default_rng(seed)
+    return [(int(rng.choice([128, 192, 256, 384])), int(rng.choice([16, 24, 32, 48])),
+             int(rng.integers(50, 301)), 100 + f) for f in range(n)]
+
+
 def triangle_wave(nsteps_period: int, amp: float, nsteps: int, start: int = 0) -> np.ndarray:
     """E(n) = amp*tri(n/T), tri(s) = 4s | 2-4s | 4s-4 on [0,1/4] | (1/4,3/4] | (3/4,1]."""
     s = ((np.arange(start, start + nsteps) % nsteps_period) / nsteps_period).astype(np.float64)
@@ -327,10 +335,10 @@ class Film:
 
     def __init__(self, inp: FilmInputs, precision=L.PREC_FP64, pcg_rule=L.PCG_RULE_REFERENCE,
                  tol_rel=1e-2, tol_abs=1e-3, max_iter=4000, dep_flag=1, zbc=L.ZBC_DIRICHLET,
-                 refine_every=20, phi=None, set_state=True, tile_kz=0, cg_fused=1):
+                 refine_every=20, phi=None, set_state=True, tile_kz=0, cg_fused=1, g_bot=0.0, g_top=0.0):
         self.inp = inp
         cfg = L.default_config(nx=inp.nx, ny=inp.ny, nz=inp.nz, del_h=inp.del_h, del_t=inp.del_t,
-                               dep_flag=dep_flag, zbc=zbc, eps0_nd=inp.scales["eps0_nd"],
+                               dep_flag=dep_flag, zbc=zbc, eps0_nd=inp.scales["eps0_nd"], g_bot=g_bot, g_top=g_top,
                                precision=precision, pcg_rule=pcg_rule, tol_abs=tol_abs, tol_rel=tol_rel,
                                max_iter=max_iter, refine_every=refine_every, tile_kz=tile_kz,
                                cg_fused=cg_fused)

@@ -13,10 +13,11 @@ def film_inputs(nx, ny, nz, ng, fracs=(0.0, 1.0, 0.0), seed=3, jitter=0.0, del_h
                               del_t=del_t)
 
 
-def oracle_film(inp, dep_flag=1, zbc=0, pcg_rule=0, tol_rel=1e-2, tol_abs=1e-3, max_iter=4000):
+def oracle_film(inp, dep_flag=1, zbc=0, pcg_rule=0, tol_rel=1e-2, tol_abs=1e-3, max_iter=4000, g_bot=0.0,
+                g_top=0.0):
     c = O.cfg(inp.nx, inp.ny, inp.nz, inp.del_h, inp.del_t, dep_flag=dep_flag, zbc=zbc,
               eps0_nd=inp.scales["eps0_nd"], pcg_rule=pcg_rule, tol_abs=tol_abs, tol_rel=tol_rel,
-              max_iter=max_iter)
+              max_iter=max_iter, g_bot=g_bot, g_top=g_top)
     return O.Film(c, inp.omega, inp.grain_phase, inp.TR, inp.phases, inp.iface)
 
 
