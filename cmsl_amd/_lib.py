@@ -143,6 +143,7 @@ SIGNATURES = {
     "hzo_write_gchar": (_I, [C.c_char_p, _I, _P, _P]),
     "hzo_voronoi": (_I, [_I, _I, _I, _I, _P, _I, C.c_uint64, _P, _P, _P]),
     "hzo_voronoi_gpu": (_I, [_I, _I, _I, _I, _I, _P, _I, C.c_uint64, _P, _P, _P]),
+    "hzo_voronoi_gpu_noalloc": (_I, [_I, _I, _I, _I, _I, _P, _I, C.c_uint64, _P, _P, _P]),
     "hzo_nondim": (_I, [_P, _I, _P, _P]),
     "hzo_assign_phases": (_I, [_I, _I, _I, _P, _I, _P, _I, _P, _P, C.c_uint, _P]),
     "hzo_interface_flag": (_I, [_I, _I, _I, _P, _I, _P, _P]),

@@ -59,6 +59,8 @@ namespace hzo {
 // GPU labelling of every layer (kernels/voronoi.hip).
 int voronoi_label_gpu(int device, int nx, int ny, int nz, int n_layers, const int* goffs, const int* nls,
                       const int* z0s, const int* spans, const double* cent, int ng, int32_t* omega_host);
+int voronoi_label_gpu_noalloc(int device, int nx, int ny, int n_layers, const int* goffs, const int* nls,
+                              const int* z0s, const int* spans, const double* cent, int ng, int32_t* omega_dev);
 }  // namespace hzo
 
 namespace {
@@ -66,9 +68,10 @@ namespace {
 // Layered periodic Voronoi (voronoi.c:55-153): per-layer grain counts, then
 // per layer the orientations and integer centroids (one seeded SplitMix64
 // stream), then the nearest-centroid labels -- on CPU threads (device < 0)
-// or on the GPU (the same labels bit for bit).
+// or on the GPU (the same labels bit for bit); omega_on_device: omega is the
+// caller's device array and the labelling allocates nothing on the device.
 int voronoi_impl(int device, int nx, int ny, int nz, int n_layers, const int* layer_counts, int ng,
-                 uint64_t seed, int32_t* omega, double* cent, double* euler) {
+                 uint64_t seed, int32_t* omega, double* cent, double* euler, bool omega_on_device = false) {
     if (nx <= 0 || ny <= 0 || nz <= 0 || n_layers <= 0 || n_layers > 16 || !layer_counts || ng <= 0 ||
         !omega || !cent || !euler)
         return set_error(HZO_ERR_ARG, "hzo_voronoi: bad argument");
@@ -114,8 +117,11 @@ int voronoi_impl(int device, int nx, int ny, int nz, int n_layers, const int* la
         z0 += span;
     }
     if (device >= 0)
-        return hzo::voronoi_label_gpu(device, nx, ny, nz, n_layers, goffs.data(), ngl.data(), z0s.data(), layer_counts,
-                                      cent, ng, omega);
+        return omega_on_device
+                   ? hzo::voronoi_label_gpu_noalloc(device, nx, ny, n_layers, goffs.data(), ngl.data(), z0s.data(),
+                                                    layer_counts, cent, ng, omega)
+                   : hzo::voronoi_label_gpu(device, nx, ny, nz, n_layers, goffs.data(), ngl.data(), z0s.data(),
+                                            layer_counts, cent, ng, omega);
     for (int L = 0; L < n_layers; ++L) {
         const int span = layer_counts[L], nL = ngl[L];
         if (nL <= 0) continue;
@@ -346,6 +352,16 @@ int hzo_voronoi_gpu(int device, int nx, int ny, int nz, int n_layers, const int*
                     uint64_t seed, int32_t* omega, double* cent, double* euler) {
     if (device < 0) return set_error(HZO_ERR_ARG, "hzo_voronoi_gpu: device < 0");
     return voronoi_impl(device, nx, ny, nz, n_layers, layer_counts, ng, seed, omega, cent, euler);
+}
+
+int hzo_voronoi_gpu_noalloc(int device, int nx, int ny, int nz, int n_layers, const int* layer_counts, int ng,
+                            uint64_t seed, int32_t* omega_dev, double* cent, double* euler) {
+    if (device < 0) return set_error(HZO_ERR_ARG, "hzo_voronoi_gpu_noalloc: device < 0");
+    if (ng > HZO_VORONOI_NOALLOC_MAX_GRAINS)
+        return set_error(HZO_ERR_ARG, "hzo_voronoi_gpu_noalloc: %d grains exceed the scratch limit of %d", ng,
+                         HZO_VORONOI_NOALLOC_MAX_GRAINS);
+    if (!omega_dev) return set_error(HZO_ERR_ARG, "hzo_voronoi_gpu_noalloc: null label array");
+    return voronoi_impl(device, nx, ny, nz, n_layers, layer_counts, ng, seed, omega_dev, cent, euler, true);
 }
 
 int hzo_grain_boundary_cells(int nx, int ny, int nz, int32_t* omega, int ng, double fraction, uint64_t seed,

@@ -384,6 +384,18 @@ int hzo_voronoi(int nx, int ny, int nz, int n_layers, const int* layer_counts,
  * (integer distances; labels equal to hzo_voronoi's bit for bit). */
 int hzo_voronoi_gpu(int device, int nx, int ny, int nz, int n_layers, const int* layer_counts, int ng,
                     uint64_t seed, int32_t* omega, double* cent, double* euler);
+/* hzo_voronoi_gpu for callers that cannot allocate device memory (code under
+ * graph capture, serving loops, harnesses that forbid hipMalloc); it has no
+ * counterpart in the reference.  The labels go to the caller's device array
+ * omega_dev[N] (C order) on GPU `device`; the centroids are staged in a
+ * file-scope device array of HZO_VORONOI_NOALLOC_MAX_GRAINS grains that each
+ * GPU holds until the process ends, and no device memory is allocated or
+ * freed.  ng above that limit is HZO_ERR_ARG.  Runs on the null stream and
+ * returns when the labels are written; calls that overlap on one GPU share
+ * the staging array and race. */
+#define HZO_VORONOI_NOALLOC_MAX_GRAINS 65536
+int hzo_voronoi_gpu_noalloc(int device, int nx, int ny, int nz, int n_layers, const int* layer_counts, int ng,
+                            uint64_t seed, int32_t* omega_dev, double* cent, double* euler);
 
 /* Non-dimensionalisation (main.cu:200-256, 268-274, 343-349).
  * ref_type selects the reference phase.  out_nd[3] indexed by phase;
