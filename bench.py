@@ -87,23 +87,63 @@ def run_gpu(args, inp, precision):
     return t1 - t0, its, kt, form
 
 
-def run_cpu_baseline(inp, nsteps):
+def run_cpu_baseline(inp, Ez, threads):
     """The FP64 oracle (a restatement of the reference kernels; the reference
-    itself is CUDA-only) on the same film, first `nsteps` steps."""
-    import cmsl_amd as H
+    itself is CUDA-only) on the same film and fields, on `threads` host
+    threads; returns (seconds, PCG iterations per step)."""
     from oracle import pyoracle as O
 
+    O.set_threads(threads)
     c = O.cfg(inp.nx, inp.ny, inp.nz, inp.del_h, inp.del_t, dep_flag=1, zbc=0,
               eps0_nd=inp.scales["eps0_nd"], pcg_rule=1, tol_abs=1e-30, tol_rel=1e-8)
     f = O.Film(c, inp.omega, inp.grain_phase, inp.TR, inp.phases, inp.iface)
     P = inp.P0.copy()
     phi = np.zeros(inp.N)
     E3 = np.zeros(3 * inp.N)
-    Ez = inp.field_nd(H.triangle_wave(PERIOD, AMP, nsteps))
     t0 = time.perf_counter()
-    its, _ = f.run(P, phi, E3, Ez)
+    its, _ = f.run(P, phi, E3, np.ascontiguousarray(Ez, np.float64))
     dt = time.perf_counter() - t0
     return dt, its
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baselines(inp, args):
+    """The oracle on the host cores the process may use (OMP_NUM_THREADS when
+    set: the GPU box allots 16 cores per GPU; else every core) and on one
+    core: a bounded sample of the 256x256x32 workload (its first steps) and
+    the whole 64x64x16, 8-grain, 1 MV/cm, 200-step single film."""
+    import cmsl_amd as H
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    N = inp.N
+    Ez1 = inp.field_nd(H.triangle_wave(PERIOD, AMP, max(args.cpu_steps, args.cpu_single_steps)))
+    dt, its = run_cpu_baseline(inp, Ez1[: args.cpu_steps], threads)
+    dt1, its1 = run_cpu_baseline(inp, Ez1[: args.cpu_single_steps], 1)
+    inp0 = H.make_film_inputs(64, 64, 16, 8, fracs=(0.0, 1.0, 0.0), seed=3)
+    Ez0 = np.full(200, float(inp0.field_nd(1.0)))
+    dt0, its0 = run_cpu_baseline(inp0, Ez0, threads)
+    dt01, _ = run_cpu_baseline(inp0, Ez0, 1)
+    return {"value": N * args.cpu_steps / dt / 1e6, "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle, first {args.cpu_steps} steps of the {inp.nx}x{inp.ny}x{inp.nz} film and field, "
+                      f"{int(np.sum(its))} PCG iterations, {dt:.1f} s on {threads} threads",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "single_core": {"value": N * args.cpu_single_steps / dt1 / 1e6, "unit": "Mcell-steps/s", "cores": 1,
+                            "sample": f"first {args.cpu_single_steps} step(s), {int(np.sum(its1))} PCG "
+                                      f"iterations, {dt1:.1f} s"},
+            "single_film_64x64x16": {"workload": "64x64x16, 8 FE grains, 1 MV/cm, 200 steps, PCG ||r||<=1e-8||b||",
+                                     "value": inp0.N * 200 / dt0 / 1e6, "cores": threads,
+                                     "value_single_core": inp0.N * 200 / dt01 / 1e6,
+                                     "pcg_iters_mean": float(np.mean(its0)), "s": dt0, "s_single_core": dt01}}
 
 
 def loop_observables(E, m):
@@ -323,7 +363,9 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--timing-steps", type=int, default=4)
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=10,
+                    help="steps of the 256x256x32 film timed on the oracle on all allotted cores")
+    ap.add_argument("--cpu-single-steps", type=int, default=1, help="the same on one core")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-kz", type=int, default=0, help="z planes per workgroup (0: automatic)")
     ap.add_argument("--adj-steps", type=int, default=500, help="steps of the reverse-mode workload")
@@ -345,25 +387,26 @@ def main():
         print(f"bench.py measures one GPU only (the film fits one MI355X); --gpus {args.gpus} "
               "is not supported", file=sys.stderr)
         sys.exit(2)
-    if args.stream_probe:
+    if args.stream_probe or args.copy_probe:
         import ctypes as C
-        from cmsl_amd import _lib as L
+        probe = C.CDLL(os.path.join(ROOT, "cmsl_amd", "libhzo_probe.so"))
+        probe.hzo_probe_stream.argtypes = [C.c_longlong, C.c_int, C.c_int, C.c_void_p]
+        probe.hzo_probe_copy.argtypes = [C.c_longlong, C.c_int]
+        probe.hzo_probe_last_error.restype = C.c_char_p
+    if args.stream_probe:
         res = {}
         for n in (1 << 21, 1 << 24):
             for grid in (512, 1024, 2048, 4096):
                 us = C.c_double(0.0)
-                L.call("hzo_probe_stream", n, grid, 200, C.byref(us))
+                if probe.hzo_probe_stream(n, grid, 200, C.byref(us)) != 0:
+                    raise RuntimeError(probe.hzo_probe_last_error().decode())
                 res[f"n{n}_grid{grid}"] = {"us": us.value, "TBs": 42.0 * n / (us.value * 1e-6) / 1e12}
         print(json.dumps({"stream_probe_bytes_per_element": 42, "results": res}))
         return
     if args.copy_probe:
-        import torch
         n = 1 << 30  # larger than the 256 MiB last-level cache: every byte crosses HBM
-        a = torch.ones(n // 4, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
+        if probe.hzo_probe_copy(n, 3) != 0:
+            raise RuntimeError(probe.hzo_probe_last_error().decode())
         print(json.dumps({"copy_probe_bytes_per_launch": n, "launches": 3}))
         return
 
@@ -396,15 +439,7 @@ def main():
     bat = run_batched(args) if args.batch_films > 0 else None
     large = run_large(args) if args.large_steps > 0 else None
 
-    cpu = None
-    if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
-        os.environ["OMP_NUM_THREADS"] = str(threads)
-        cdt, cits = run_cpu_baseline(inp, args.cpu_steps)
-        cpu = {"value": N * args.cpu_steps / cdt / 1e6, "unit": "Mcell-steps/s", "cores": threads,
-               "kind": "port",
-               "sample": f"FP64 oracle, first {args.cpu_steps} steps of the same film and field, "
-                         f"{int(np.sum(cits))} PCG iterations, {cdt:.1f} s"}
+    cpu = None if args.no_cpu else cpu_baselines(inp, args)
 
     line = {
         "metric": "Mcell-steps/s incl. PCG at 256x256x32: FP32-refined vs FP64; adjoint/fwd cost",

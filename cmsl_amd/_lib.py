@@ -120,7 +120,6 @@ SIGNATURES = {
     "hzo_batch_run": (_I, [_P, _P, _I, _P, _P]),
     "hzo_batch_autotune": (_I, [_P, _P, _I, _P, _P]),
     "hzo_batch_counters": (_I, [_P, _P]),
-    "hzo_probe_stream": (_I, [C.c_longlong, _I, _I, _P]),
     "hzo_save_snapshot": (_I, [_P, C.c_char_p, C.c_char_p, _D]),
     "hzo_checkpoint_save": (_I, [_P, C.c_char_p, _I, _I, _D, _D]),
     "hzo_checkpoint_load": (_I, [_P, C.c_char_p, _P, _P, _P, _P, _P]),

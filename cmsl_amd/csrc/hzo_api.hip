@@ -1343,7 +1343,7 @@ struct hzo_batch {
     std::vector<Group> groups;
     double* dez = nullptr;
     double* hez = nullptr;  // pinned
-    int* dinfo = nullptr;   // 4 ints per film: done, iters, status, refinements
+    int* dinfo = nullptr;   // BATCH_INFO ints per film: done, iters, status, refinements, xadds
     int* hinfo = nullptr;   // pinned
     double* dm = nullptr;
     double* hm = nullptr;   // pinned
@@ -1572,7 +1572,8 @@ int group_cg_chunk(hzo_batch* B, hzo_batch::Group& G) {
 }
 
 // Solver statistics of the group's films after a chunk (launch index K) into
-// hinfo[4j..]: done, iterations, status, residual replacements.
+// hinfo[BATCH_INFO*j..]: done, iterations, status, residual replacements,
+// launches that only added a deferred x_lo term.
 int group_poll(hzo_batch* B, hzo_batch::Group& G) {
     const int n = (int)G.idx.size();
     if (G.single >= 0) {
@@ -1584,10 +1585,11 @@ int group_poll(hzo_batch* B, hzo_batch::Group& G) {
         B->hinfo[1] = x->st_host->iters;
         B->hinfo[2] = x->st_host->status;
         B->hinfo[3] = x->st_host->refinements;
+        B->hinfo[4] = x->st_host->xadds;
         return HZO_OK;
     }
     hzo::launch_batch_done(B->stream, G.dfilms, n, B->chunk, B->dinfo);
-    HZO_HIP(hipMemcpyAsync(B->hinfo, B->dinfo, sizeof(int) * 4 * n, hipMemcpyDeviceToHost, B->stream));
+    HZO_HIP(hipMemcpyAsync(B->hinfo, B->dinfo, sizeof(int) * hzo::BATCH_INFO * n, hipMemcpyDeviceToHost, B->stream));
     HZO_HIP(hipStreamSynchronize(B->stream));
     return HZO_OK;
 }
@@ -1595,7 +1597,7 @@ int group_poll(hzo_batch* B, hzo_batch::Group& G) {
 // One step of one group: TDGL of every film, PCG setup, then chunks of
 // single-reduction CG launches until every film's solve has finished.  The
 // launch index t restarts at 1 in each chunk (chunks are even, so the
-// control-block parity continues).  hinfo[4j..] receives film j's solver
+// control-block parity continues).  hinfo[BATCH_INFO*j..] receives film j's solver
 // statistics.
 int group_step(hzo_batch* B, hzo_batch::Group& G, const double* Ez, int s, bool dep, int& launches) {
     const int n = (int)G.idx.size();
@@ -1614,7 +1616,7 @@ int group_step(hzo_batch* B, hzo_batch::Group& G, const double* Ez, int s, bool 
     }
     launches = 0;
     if (!dep) {
-        memset(B->hinfo, 0, sizeof(int) * 4 * n);
+        memset(B->hinfo, 0, sizeof(int) * hzo::BATCH_INFO * n);
         return HZO_OK;
     }
     if (G.single >= 0) {
@@ -1641,15 +1643,16 @@ int group_step(hzo_batch* B, hzo_batch::Group& G, const double* Ez, int s, bool 
         int rc = group_poll(B, G);
         if (rc) return rc;
         bool all = true;
-        for (int j = 0; j < n; ++j) all = all && B->hinfo[4 * j];
+        for (int j = 0; j < n; ++j) all = all && B->hinfo[hzo::BATCH_INFO * j];
         if (all || launches >= maxl) break;
         nchunks = 1;
     }
     // launches the slowest film needed: setup hand-over, one per iteration,
-    // two per residual replacement
+    // two per residual replacement, one per deferred-x_lo addition
     int need = 2;
     for (int j = 0; j < n; ++j) {
-        const int m = B->hinfo[4 * j + 1] + 2 * B->hinfo[4 * j + 3] + 2;
+        const int* h = B->hinfo + hzo::BATCH_INFO * j;
+        const int m = h[1] + 2 * h[3] + h[4] + 2;
         need = m > need ? m : need;
     }
     G.need = need;
@@ -1668,8 +1671,8 @@ int group_run(hzo_batch* B, hzo_batch::Group& G, const double* Ez, int nsteps, i
         if (rc) return rc;
         B->launches_total += l;
         for (int j = 0; j < n; ++j) {
-            if (iters) iters[(size_t)s * B->n + G.idx[j]] = B->hinfo[4 * j + 1];
-            B->unconverged += dep && !B->hinfo[4 * j];
+            if (iters) iters[(size_t)s * B->n + G.idx[j]] = B->hinfo[hzo::BATCH_INFO * j + 1];
+            B->unconverged += dep && !B->hinfo[hzo::BATCH_INFO * j];
         }
         if (m_out) {
             if (G.single >= 0) {
@@ -1722,10 +1725,10 @@ int hzo_batch_create(hzo_ctx* const* films, int n, int tile_kz, hzo_batch** out)
     if (hipSetDevice(B->device) != hipSuccess || hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(set_error(HZO_ERR_HIP, "hzo_batch_create: stream creation failed"));
     if (hipMalloc((void**)&B->dez, sizeof(double) * n) != hipSuccess ||
-        hipMalloc((void**)&B->dinfo, sizeof(int) * 4 * n) != hipSuccess ||
+        hipMalloc((void**)&B->dinfo, sizeof(int) * hzo::BATCH_INFO * n) != hipSuccess ||
         hipMalloc((void**)&B->dm, sizeof(double) * n) != hipSuccess ||
         hipHostMalloc((void**)&B->hez, sizeof(double) * n, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&B->hinfo, sizeof(int) * 4 * n, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&B->hinfo, sizeof(int) * hzo::BATCH_INFO * n, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&B->hm, sizeof(double) * n, hipHostMallocDefault) != hipSuccess)
         return fail(set_error(HZO_ERR_NOMEM, "hzo_batch_create: allocation failed"));
     for (hzo_ctx* x : B->films)

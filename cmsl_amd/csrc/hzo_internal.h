@@ -173,6 +173,10 @@ struct CG32Bufs {
     PcgState* st;
 };
 
+// Solver summary of one batch film after a poll (k_batch_done): done,
+// iterations, status, residual replacements, deferred-x_lo launches.
+constexpr int BATCH_INFO = 5;
+
 // One film of a batch (device table, hzo_batch_*): the buffers of its
 // context by P parity, its single-reduction CG vectors, step constants, tile
 // grid and partial layout (batch-owned partial arrays, common stride).

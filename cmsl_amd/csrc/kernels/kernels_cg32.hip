@@ -863,10 +863,12 @@ __global__ void k_batch_set_ez(BatchFilm* films, const double* ez, int n) {
 __global__ void k_batch_done(const BatchFilm* films, int n, int t, int* info) {
     for (int f = threadIdx.x; f < n; f += blockDim.x) {
         const PcgState& st = films[f].cg.st[t & 1];
-        info[4 * f + 0] = st.done;
-        info[4 * f + 1] = st.iters;
-        info[4 * f + 2] = st.status;
-        info[4 * f + 3] = st.refinements;
+        int* o = info + BATCH_INFO * f;
+        o[0] = st.done;
+        o[1] = st.iters;
+        o[2] = st.status;
+        o[3] = st.refinements;
+        o[4] = st.xadds;
     }
 }
 

@@ -3,12 +3,21 @@
 // arrays and one u16 array read, five f32 arrays written: 42 B per element)
 // with no stencil, halo or reduction, storing with the solver vectors' cache
 // policy.  Used to place the CG kernel's achieved bandwidth between this
-// floor and the HBM peak.
+// floor and the HBM peak.  Also a device-to-device copy of known size that
+// calibrates the FETCH_SIZE / WRITE_SIZE counters.
+//
+// Measurement tooling only: built into its own library (libhzo_probe.so),
+// which bench.py loads for --stream-probe and --copy-probe; the product
+// library libhzo.so does not contain it.  Exports, returning 0 or -1 with the
+// text in hzo_probe_last_error():
+//   int hzo_probe_stream(long long n, int grid, int reps, double* us_per_launch);
+//   int hzo_probe_copy(long long bytes, int reps);
 #include <hip/hip_runtime.h>
+#include <cstdarg>
 #include <cstdint>
+#include <cstdio>
 
 #include "../hzo_internal.h"
-#include "../host/errors.h"
 #include "tile.hip.h"
 
 namespace hzo {
@@ -36,21 +45,34 @@ __global__ __launch_bounds__(512) void k_probe_stream(const float* __restrict__ 
 
 }  // namespace hzo
 
+namespace {
+thread_local char g_probe_err[256] = "";
+int probe_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_probe_err, sizeof(g_probe_err), fmt, ap);
+    va_end(ap);
+    return -1;
+}
+}  // namespace
+
+extern "C" const char* hzo_probe_last_error(void) { return g_probe_err; }
+
 extern "C" int hzo_probe_stream(long long n, int grid, int reps, double* us_per_launch) {
     if (n < 1 || n > (1LL << 28) || grid < 1 || reps < 1 || !us_per_launch)
-        return hzo::set_error(HZO_ERR_ARG, "hzo_probe_stream: bad argument");
+        return probe_error("hzo_probe_stream: bad argument");
     float* v[10] = {};
     uint16_t* code = nullptr;
     hipEvent_t t0 = nullptr, t1 = nullptr;
-    int rc = HZO_OK;
-    auto fail = [&](const char* what) { rc = hzo::set_error(HZO_ERR_HIP, "hzo_probe_stream: %s", what); };
-    for (int q = 0; q < 10 && rc == HZO_OK; ++q)
+    int rc = 0;
+    auto fail = [&](const char* what) { rc = probe_error("hzo_probe_stream: %s", what); };
+    for (int q = 0; q < 10 && rc == 0; ++q)
         if (hipMalloc((void**)&v[q], sizeof(float) * n) != hipSuccess || hipMemset(v[q], 0, sizeof(float) * n) != hipSuccess)
             fail("allocation");
-    if (rc == HZO_OK && (hipMalloc((void**)&code, 2 * n) != hipSuccess || hipMemset(code, 0, 2 * n) != hipSuccess))
+    if (rc == 0 && (hipMalloc((void**)&code, 2 * n) != hipSuccess || hipMemset(code, 0, 2 * n) != hipSuccess))
         fail("allocation");
-    if (rc == HZO_OK && (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess)) fail("events");
-    if (rc == HZO_OK) {
+    if (rc == 0 && (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess)) fail("events");
+    if (rc == 0) {
         auto launch = [&]() {
             hipLaunchKernelGGL(hzo::k_probe_stream, dim3(grid), dim3(512), 0, 0, v[0], v[1], v[2], v[3], v[4], code, v[5],
                                v[6], v[7], v[8], v[9], n);
@@ -67,5 +89,23 @@ extern "C" int hzo_probe_stream(long long n, int grid, int reps, double* us_per_
     if (t1) (void)hipEventDestroy(t1);
     for (float* p : v) (void)hipFree(p);
     (void)hipFree(code);
+    return rc;
+}
+
+// bytes copied device to device reps times (the runtime's copy kernel), then
+// waited for; no timing: the profiler's counters are what is read.
+extern "C" int hzo_probe_copy(long long bytes, int reps) {
+    if (bytes < 4 || reps < 1) return probe_error("hzo_probe_copy: bad argument");
+    void *a = nullptr, *b = nullptr;
+    int rc = 0;
+    if (hipMalloc(&a, (size_t)bytes) != hipSuccess || hipMalloc(&b, (size_t)bytes) != hipSuccess ||
+        hipMemset(a, 1, (size_t)bytes) != hipSuccess)
+        rc = probe_error("hzo_probe_copy: allocation");
+    for (int r = 0; r < reps && rc == 0; ++r)
+        if (hipMemcpyAsync(b, a, (size_t)bytes, hipMemcpyDeviceToDevice, 0) != hipSuccess)
+            rc = probe_error("hzo_probe_copy: copy");
+    if (rc == 0 && hipDeviceSynchronize() != hipSuccess) rc = probe_error("hzo_probe_copy: copy");
+    (void)hipFree(a);
+    (void)hipFree(b);
     return rc;
 }

@@ -219,15 +219,6 @@ int hzo_forward_record(hzo_ctx* ctx, const double* Ez, int nsteps, double* m_out
  * the context at the recorded final state. */
 int hzo_backward(hzo_ctx* ctx, const double* dL_dm, double grad[12], long long* adj_pcg_iters);
 
-/* ---------------------------------------------------------------- measurement */
-/* Streaming-traffic probe (no counterpart in the reference): a kernel that
- * reads five f32 arrays and one u16 array and writes five f32 arrays of n
- * elements (the 42 B per voxel of one single-reduction CG iteration, with no
- * stencil, halo or reduction, and the solver's store cache policy), grid x
- * 512 threads, 1 <= n <= 2^28; us_per_launch receives the mean time of reps
- * launches.  Allocates and frees its own arrays. */
-int hzo_probe_stream(long long n, int grid, int reps, double* us_per_launch);
-
 /* ---------------------------------------------------------------- dumps and restart */
 /* save_snapshot (generate.cu:266-303): Pm and Ps of the current state as
  * interleaved (x, y, z) f64 per voxel, multiplied by P0_scale. */

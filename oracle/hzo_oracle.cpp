@@ -11,6 +11,7 @@
 #include "hzo_oracle.h"
 
 #include <cmath>
+#include <omp.h>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -583,5 +584,13 @@ void hzo_oracle_reorder_zf_to_c(int nx, int ny, int nz, const int32_t* in_zf, in
                 out_c[id_c] = in_zf[id_f];
             }
 }
+
+// Thread count of the oracle's parallel loops (the CPU baseline times it on
+// all cores and on one); n <= 0 leaves the OpenMP default.
+void hzo_oracle_set_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
+}
+
+int hzo_oracle_get_threads(void) { return omp_get_max_threads(); }
 
 }  // extern "C"

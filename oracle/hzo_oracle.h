@@ -132,6 +132,10 @@ void hzo_oracle_voronoi_layer(int nx, int ny, int nz, int z0, int span,
 /* z-fastest (generator file order) -> C order (grain_io.cu:61-69). */
 void hzo_oracle_reorder_zf_to_c(int nx, int ny, int nz, const int32_t* in_zf, int32_t* out_c);
 
+/* OpenMP threads of the oracle's loops (n <= 0: unchanged) and the current count. */
+void hzo_oracle_set_threads(int n);
+int hzo_oracle_get_threads(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,6 +47,8 @@ _SIG = {
     "hzo_oracle_diagnostics": [_P, _P, _P, _P, _P, _P],
     "hzo_oracle_voronoi_layer": [C.c_int] * 5 + [_P, _P, _P, C.c_int, C.c_int, _P],
     "hzo_oracle_reorder_zf_to_c": [C.c_int, C.c_int, C.c_int, _P, _P],
+    "hzo_oracle_set_threads": [C.c_int],
+    "hzo_oracle_get_threads": [],
 }
 
 _lib = None
@@ -61,6 +63,7 @@ def lib():
         for k, a in _SIG.items():
             getattr(l, k).restype = None
             getattr(l, k).argtypes = a
+        l.hzo_oracle_get_threads.restype = C.c_int
         _lib = l
     return _lib
 
@@ -137,6 +140,12 @@ class Film:
         lib().hzo_oracle_diagnostics(C.byref(self.c), _p(P), _p(Plocal), _p(self.omega),
                                      _p(self.grain_phase), _p(out))
         return out
+
+
+def set_threads(n: int) -> int:
+    """Threads of the oracle's OpenMP loops; returns the count now in force."""
+    lib().hzo_oracle_set_threads(int(n))
+    return int(lib().hzo_oracle_get_threads())
 
 
 def apply_A(c: OCfg, eps, x):

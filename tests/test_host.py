@@ -66,13 +66,22 @@ def test_create_rejects_solver_and_tile_combinations(kw, msg):
 
 @pytest.mark.parametrize("n,grid,reps", [(0, 512, 1), ((1 << 28) + 1, 512, 1), (1 << 21, 0, 1), (1 << 21, 512, 0)])
 def test_stream_probe_rejects_bad_sizes(n, grid, reps):
-    # its 32-bit buffer offsets cover at most 2^28 four-byte elements; the
+    # the measurement library bench.py loads (not the product library): its
+    # 32-bit buffer offsets cover at most 2^28 four-byte elements; the
     # argument check runs before any device call
     import ctypes as C
+    import os
+    probe = C.CDLL(os.path.join(os.path.dirname(L.LIB_PATH), "libhzo_probe.so"))
+    probe.hzo_probe_stream.argtypes = [C.c_longlong, C.c_int, C.c_int, C.c_void_p]
+    probe.hzo_probe_last_error.restype = C.c_char_p
     us = C.c_double(-1.0)
-    rc = L.lib().hzo_probe_stream(C.c_longlong(n), grid, reps, C.byref(us))
+    rc = probe.hzo_probe_stream(C.c_longlong(n), grid, reps, C.byref(us))
     assert rc == -1 and us.value == -1.0
-    assert b"hzo_probe_stream" in L.lib().hzo_last_error()
+    assert b"hzo_probe_stream" in probe.hzo_probe_last_error()
+
+
+def test_product_library_holds_no_measurement_probe():
+    assert not hasattr(L.lib(), "hzo_probe_stream")
 
 
 def test_splitmix64_published_sequence():
