@@ -27,8 +27,8 @@ PCG_RULE_REFERENCE, PCG_RULE_RELRES = 0, 1
 # hzo_pcg_stats.status
 HZO_PCG_CONVERGED, HZO_PCG_MAXITER = 0, 1
 HZO_PCG_BAD_INITIAL, HZO_PCG_BREAKDOWN, HZO_PCG_BAD_RZ = -1, -2, -3
-KCLASSES = 5
-KCLASS_NAMES = ("tdgl", "pcg_setup", "pcg_dir_spmv", "pcg_update", "pcg_fused")
+KCLASSES = 6
+KCLASS_NAMES = ("tdgl", "pcg_setup", "pcg_dir_spmv", "pcg_update", "pcg_fused", "pcg_resident")
 
 
 class PhaseND(C.Structure):
@@ -108,6 +108,7 @@ SIGNATURES = {
     "hzo_sweep_run": (_I, [_P, _P, C.c_char_p, _I, _P, _P]),
     "hzo_set_timing": (_I, [_P, _I]),
     "hzo_kernel_times": (_I, [_P, _P, _P]),
+    "hzo_cg_counters": (_I, [_P, _P]),
     "hzo_adjoint_reserve": (_I, [_P, _I, _I, _D]),
     "hzo_forward_record": (_I, [_P, _P, _I, _P]),
     "hzo_backward": (_I, [_P, _P, _P, _P]),

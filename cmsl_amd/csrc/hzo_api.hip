@@ -33,6 +33,16 @@ struct hzo_ctx {
     bool fp32 = false;  // HZO_PREC_FP32_REFINED
     bool cg_two = false;  // single-reduction CG forming s = A p and w = A u in the launch
     bool cg_nt = false;   // its iteration stores nontemporal
+    // register-resident single-reduction CG: the whole solve in one
+    // cooperative launch over 16 x 16 column tiles (films whose tiles all fit
+    // the device at once)
+    bool cg_res = false;
+    int res_gxt = 0, res_G = 0;  // tiles in x, tiles
+    float* res_xch = nullptr;    // edge columns [2][G][4][nz][16]
+    double* res_part = nullptr;  // partial sums [2][G][4]
+    unsigned* res_sync = nullptr;  // barrier counters, then the timeout flag
+    int* res_err_host = nullptr;   // pinned copy of the timeout flag
+    long long res_counts[3] = {0, 0, 0};  // solves, iterations, grid barriers (since hzo_set_timing)
     double* P[2] = {nullptr, nullptr};
     float* P32[2] = {nullptr, nullptr};
     float* xlo = nullptr;
@@ -73,7 +83,7 @@ struct hzo_ctx {
 
     bool timing = false;
     std::vector<hipEvent_t> ev[HZO_KCLASSES][2];
-    int nt[HZO_KCLASSES] = {0, 0, 0, 0, 0};
+    int nt[HZO_KCLASSES] = {};
 
     // reverse mode: checkpoints, recompute segment and adjoint vectors
     // (hzo_adjoint_reserve / hzo_forward_record / hzo_backward)
@@ -422,6 +432,52 @@ int follow_up_round(int& next, int last_iters) {
     return r;
 }
 
+// The resident solve after the setup: one cooperative launch runs every
+// iteration and residual replacement; the host waits for it once.
+int cgres_solve(hzo_ctx* x, const hzo::StepConsts& c, const hzo::CG32Bufs& cb, int& last_iters,
+                hzo_pcg_stats* stats) {
+    hzo::ResBufs rb;
+    rb.r0 = x->r32;
+    rb.xlo = cb.xlo;
+    rb.xhi = cb.xhi;
+    rb.b = cb.b;
+    rb.sc = cb.sc;
+    rb.lut = cb.lut;
+    rb.lut32 = cb.lut32;
+    rb.pin = cb.pin[0];
+    rb.nb_in = cb.nb_in;
+    rb.ps_in = cb.ps_in;
+    rb.st = cb.st;
+    rb.xch = x->res_xch;
+    rb.part = x->res_part;
+    rb.sync = x->res_sync;
+    rb.err = reinterpret_cast<int*>(x->res_sync + hzo::RES_SYNC_WORDS);
+    // counters and flag start at zero (a fill kernel, not a memset node, in
+    // case the caller captures the stream)
+    hzo::launch_fill_u32(x->stream, x->res_sync, hzo::RES_SYNC_WORDS + 64, 0u);
+    HZO_HIP(hipGetLastError());
+    hipError_t le = hipSuccess;
+    int rc = timed(x, 5, [&](const hzo::LaunchEv& ev) {
+        if (ev.start) (void)hipEventRecord(ev.start, x->stream);
+        le = hzo::launch_cgres(x->stream, c, rb, x->res_gxt, x->res_G);
+        if (ev.stop) (void)hipEventRecord(ev.stop, x->stream);
+    });
+    if (rc) return rc;
+    if (le != hipSuccess)
+        return set_error(HZO_ERR_HIP, "resident CG launch failed: %s", hipGetErrorString(le));
+    HZO_HIP(hipMemcpyAsync(x->st_host, cb.st, sizeof(hzo::PcgState), hipMemcpyDeviceToHost, x->stream));
+    HZO_HIP(hipMemcpyAsync(x->res_err_host, rb.err, sizeof(int), hipMemcpyDeviceToHost, x->stream));
+    HZO_HIP(hipStreamSynchronize(x->stream));
+    if (*x->res_err_host) return set_error(HZO_ERR_HIP, "resident CG: a grid barrier timed out");
+    hzo::PcgState& s = *x->st_host;
+    if (!s.done) s.status = HZO_PCG_MAXITER;
+    x->res_counts[0] += 1;
+    x->res_counts[1] += s.iters;
+    x->res_counts[2] += s.passes;
+    last_iters = s.iters + 2 * s.refinements + 1;
+    return copy_stats(s, stats);
+}
+
 // FP32-refined solve by the single-reduction CG: the setup (r = b - A x,
 // partials r.z, r.r, b.b) and then one fused launch per iteration, the
 // device deciding from the control block whether a launch iterates,
@@ -434,6 +490,7 @@ int cg32_solve(hzo_ctx* x, const hzo::Fp32Bufs& b32in, const hzo::StepConsts& c,
     int rc = timed(x, 1, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_setup_f32(x->stream, x->g, c, b32, init, rhs, ev); });
     if (rc) return rc;
     fold_after(x, x->stream, 0);
+    if (x->cg_res) return cgres_solve(x, c, cb, last_iters, stats);
     // launches: one WCOMP, one per iteration, two per residual replacement
     // and one before it or the end when an iteration's x_lo term is still
     // deferred; at most four per iteration (add, replacement, re-form,
@@ -561,6 +618,10 @@ void free_all(hzo_ctx* x) {
     (void)hipFree(x->ptab32);
     (void)hipFree(x->lut32);
     if (x->st_host) (void)hipHostFree(x->st_host);
+    (void)hipFree(x->res_xch);
+    (void)hipFree(x->res_part);
+    (void)hipFree(x->res_sync);
+    if (x->res_err_host) (void)hipHostFree(x->res_err_host);
     for (int c = 0; c < HZO_KCLASSES; ++c)
         for (int q = 0; q < 2; ++q)
             for (auto e : x->ev[c][q]) (void)hipEventDestroy(e);
@@ -617,7 +678,8 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
     if (cfg->refine_every < 0) return set_error(HZO_ERR_ARG, "hzo_create: refine_every < 0");
     if (cfg->tile_kz != 0 && cfg->tile_kz != 2 && cfg->tile_kz != 4 && cfg->tile_kz != 8 &&
         !(cfg->tile_kz == 16 && cfg->precision == HZO_PREC_FP32_REFINED && cfg->cg_fused) &&
-        !(cfg->tile_kz == 32 && cfg->precision == HZO_PREC_FP32_REFINED && (cfg->cg_fused == 1 || cfg->cg_fused == 3)))
+        !(cfg->tile_kz == 32 && cfg->precision == HZO_PREC_FP32_REFINED && (cfg->cg_fused == 1 || cfg->cg_fused == 3 ||
+                                                                             cfg->cg_fused == 4)))
         return set_error(HZO_ERR_ARG, "hzo_create: tile_kz must be 0, 2, 4 or 8 (16: single-reduction FP32 CG only; "
                                       "32: its two-stencil form only)");
     if (cfg->pcg_rule != 0 && cfg->pcg_rule != 1) return set_error(HZO_ERR_ARG, "hzo_create: pcg_rule %d", cfg->pcg_rule);
@@ -627,7 +689,7 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
     // the single-reduction CG addresses four of its vectors through one
     // buffer range of 16 N bytes, which must stay below 2^31: larger films
     // run the two-kernel FP32 PCG
-    if (cfg->cg_fused < 0 || cfg->cg_fused > 3) return set_error(HZO_ERR_ARG, "hzo_create: cg_fused %d", cfg->cg_fused);
+    if (cfg->cg_fused < 0 || cfg->cg_fused > 4) return set_error(HZO_ERR_ARG, "hzo_create: cg_fused %d", cfg->cg_fused);
     const bool cg_fused = cfg->cg_fused && 16 * N < (1LL << 31);
     if (cfg->tile_kz >= 16 && cfg->precision == HZO_PREC_FP32_REFINED && !cg_fused)
         return set_error(HZO_ERR_ARG, "hzo_create: tile_kz %d needs the single-reduction CG (at most 2^27 - 1 voxels)",
@@ -701,6 +763,26 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
         // 512x512x64 (slower at every film whose vectors fit)
         x->cg_nt = x->cg_two && kz >= 16 && 22LL * N > (256LL << 20);
     }
+    (void)hipGetDevice(&x->device);
+    // Register-resident CG: automatic (cg_fused 1, tile depth automatic) or
+    // forced (4) when every 16 x 16 column tile of the film fits the device
+    // at once, for film depths with a kernel instance (16, 24, 32).
+    {
+        const bool fused = cfg->precision == HZO_PREC_FP32_REFINED && x->cfg.cg_fused;
+        const bool shape = fused && cfg->nx % 16 == 0 && cfg->ny % 16 == 0 &&
+                           (cfg->nz == 16 || cfg->nz == 24 || cfg->nz == 32);
+        const int G = (cfg->nx / 16) * (cfg->ny / 16);
+        const int cap = shape ? hzo::cgres_capacity(x->device, cfg->nz, cfg->zbc != 0) : 0;
+        x->cg_res = ((cfg->cg_fused == 1 && cfg->tile_kz == 0) || cfg->cg_fused == 4) && shape && G <= cap;
+        if (cfg->cg_fused == 4 && !x->cg_res) {
+            delete x;
+            return set_error(HZO_ERR_ARG, "hzo_create: the resident CG needs FP32-refined, nx and ny multiples of 16, "
+                                          "nz of 16, 24 or 32 and at most %d tiles of 16 x 16 columns (%d here)",
+                             cap, G);
+        }
+        x->res_gxt = cfg->nx / 16;
+        x->res_G = G;
+    }
     hzo::StepConsts& c = x->c;
     c.nx = cfg->nx;
     c.ny = cfg->ny;
@@ -723,7 +805,6 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
     c.nvox = (unsigned)N;
     c.ref_id = (long long)(cfg->nx / 2) + (long long)cfg->nx * ((long long)(cfg->ny / 2) + (long long)cfg->ny * (cfg->nz / 2));
 
-    (void)hipGetDevice(&x->device);
     x->nb = hzo::blocks_for(x->g);
     x->ps = hzo::part_stride(x->g);
     auto fail = [&](int rc) {
@@ -781,6 +862,13 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
     HZO_ALLOC(x->tdgl_part, sizeof(double) * hzo::TDGL_Q * (size_t)x->nb);
     HZO_ALLOC(x->diag_part, sizeof(double) * hzo::DIAG_Q * (size_t)x->nb);
     HZO_ALLOC(x->st, sizeof(hzo::PcgState) * 2);
+    if (x->cg_res) {
+        HZO_ALLOC(x->res_xch, sizeof(float) * 2 * (size_t)x->res_G * 4 * cfg->nz * 16);
+        HZO_ALLOC(x->res_part, sizeof(double) * 2 * (size_t)x->res_G * 4);
+        HZO_ALLOC(x->res_sync, sizeof(unsigned) * (hzo::RES_SYNC_WORDS + 64));
+        if (hipHostMalloc((void**)&x->res_err_host, sizeof(int), hipHostMallocDefault) != hipSuccess)
+            return fail(set_error(HZO_ERR_NOMEM, "hzo_create: pinned host allocation failed"));
+    }
 #undef HZO_ALLOC
     if (hipHostMalloc((void**)&x->st_host, sizeof(hzo::PcgState), hipHostMallocDefault) != hipSuccess)
         return fail(set_error(HZO_ERR_NOMEM, "hzo_create: pinned host allocation failed"));
@@ -1024,7 +1112,9 @@ int hzo_grid_dims(hzo_ctx* x, int dims[3]) {
 }
 
 int hzo_tile_depth(hzo_ctx* x) { return x ? x->g.kz : 0; }
-int hzo_cg_form(hzo_ctx* x) { return !x ? -1 : (!x->fp32 || !x->cfg.cg_fused) ? 0 : (x->cg_two ? 2 : 1); }
+int hzo_cg_form(hzo_ctx* x) {
+    return !x ? -1 : (!x->fp32 || !x->cfg.cg_fused) ? 0 : x->cg_res ? 3 : (x->cg_two ? 2 : 1);
+}
 
 int hzo_set_timing(hzo_ctx* x, int enable) {
     if (!x) return set_error(HZO_ERR_ARG, "null context");
@@ -1032,6 +1122,13 @@ int hzo_set_timing(hzo_ctx* x, int enable) {
     HZO_HIP(hipStreamSynchronize(x->stream));
     x->timing = enable != 0;
     for (int c = 0; c < HZO_KCLASSES; ++c) x->nt[c] = 0;
+    for (long long& v : x->res_counts) v = 0;
+    return HZO_OK;
+}
+
+int hzo_cg_counters(hzo_ctx* x, long long out[3]) {
+    if (!x || !out) return set_error(HZO_ERR_ARG, "hzo_cg_counters: null argument");
+    for (int i = 0; i < 3; ++i) out[i] = x->res_counts[i];
     return HZO_OK;
 }
 

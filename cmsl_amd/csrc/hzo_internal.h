@@ -68,7 +68,8 @@ struct PcgState {
     // single-reduction CG: an iteration may leave its x_lo += alpha p to the
     // next one (xpend = 1, alpha_prev = its alpha); xadds counts the launches
     // that only add such a term (before a residual replacement or the end)
-    int xpend, xadds, pad2;
+    int xpend, xadds;
+    int passes;  // resident solve: grid barriers passed
     double alpha_prev;
 };
 
@@ -172,6 +173,29 @@ struct CG32Bufs {
     int nb_in, ps_in;      // count and stride of pin (1 and 512 when folded)
     PcgState* st;
 };
+
+// Register-resident single-reduction CG (kernels_cgres.hip): the whole
+// FP32-refined solve in one cooperative launch over 16 x 16 column tiles.
+struct ResBufs {
+    const float* r0;       // r = b - A x after the setup kernel
+    float* xlo;            // x_lo: read at the start, written at the end
+    double* xhi;           // x_hi: x_lo is folded into it at every re-formation of w
+    const double* b;       // right-hand side
+    const uint16_t* sc;    // stencil codes
+    const double* lut;     // FP64 face means (residual replacement)
+    const float* lut32;    // FP32 face means and D^-1
+    const double* pin;     // the setup's partial sums (r.z, r.r, b.b): nb_in each, stride ps_in
+    int nb_in, ps_in;
+    PcgState* st;          // in: the setup's control block; out: the final one
+    float* xch;            // edge columns: [2][tiles][4][nz][16]
+    double* part;          // workgroup partial sums: [2][tiles][4]
+    unsigned* sync;        // barrier counters (global, then one per XCD, 64 apart), zero at launch
+    int* err;              // barrier timeout flag, zero at launch
+};
+constexpr int RES_SYNC_WORDS = 64 * 9;
+hipError_t launch_cgres(hipStream_t s, const StepConsts& c, const ResBufs& b, int gxt, int G);
+int cgres_capacity(int device, int nz, bool mir);
+void launch_fill_u32(hipStream_t s, unsigned* p, int n, unsigned v);
 
 // Solver summary of one batch film after a poll (k_batch_done): done,
 // iterations, status, residual replacements, deferred-x_lo launches.

@@ -1,0 +1,89 @@
+// cg_ctl.hip.h -- pieces shared by the single-reduction CG kernels of the
+// FP32-refined form (kernels_cg32.hip: one launch per iteration;
+// kernels_cgres.hip: one persistent launch per solve): the FP32 operator row
+// from a stencil code and the loop decision after a completed reduction.
+#pragma once
+#include "../hzo_internal.h"
+#include "common.hip.h"
+#include "tile.hip.h"
+
+namespace hzo {
+
+// One row of the operator (-(sum of face fluxes)/h^2) from the stencil code:
+// the paired harmonic face means of the three axes come from the 64-entry
+// HPair table (see load_hpair).
+__device__ __forceinline__ float row_Ax32(unsigned code, const HPair<float>* sHP, float ih2, float xc, float xe,
+                                          float xw, float xn, float xs, float xu, float xd) {
+    const unsigned cb = (code & 3u) << 4;
+    const HPair<float> hx = sHP[cb | ((code >> 2) & 15u)];
+    const HPair<float> hy = sHP[cb | ((code >> 6) & 15u)];
+    const HPair<float> hz = sHP[cb | ((code >> 10) & 15u)];
+    const float num = hx.p * (xe - xc) - hx.m * (xc - xw) + hy.p * (xn - xc) - hy.m * (xc - xs) +
+                      hz.p * (xu - xc) - hz.m * (xc - xd);
+    return -(num * ih2);
+}
+
+// The loop logic after a completed reduction (every lane of every
+// workgroup computes the same values).  Returns the work mode: 0 ITER,
+// 1 done, 2 RESID, 3 WCOMP.  Sets alpha/beta for ITER.
+__device__ __forceinline__ int cg_decide(PcgState& s, const double (&tot)[3], double& alpha, double& beta) {
+    const int prev = s.kind;
+    if (prev == K_SETUP || prev == K_RESID) {
+        const double g = tot[0], rr = tot[1];
+        if (prev == K_SETUP) {
+            s.rz0 = g;
+            s.rr0 = rr;
+            s.bb = tot[2];
+        } else {
+            s.refinements += 1;
+        }
+        s.rr = rr;
+        if (!finite(g) || g < 0.0) {
+            s.status = prev == K_SETUP ? HZO_PCG_BAD_INITIAL : HZO_PCG_BAD_RZ;
+            s.done = 1;
+            return 1;
+        }
+        judge(s, g, rr);
+        return s.done ? 1 : 3;
+    }
+    const double g = tot[0], dl = tot[1], rr = tot[2];
+    s.rr = rr;
+    if (!finite(g) || g < 0.0) {
+        s.status = HZO_PCG_BAD_RZ;
+        s.done = 1;
+        return 1;
+    }
+    // K_XADD: the last launch added a deferred x_lo term before a residual
+    // replacement (its partials are zero; the replacement restates rr)
+    if (prev == K_ITER || prev == K_XADD) {
+        if (s.iters >= s.max_iter) {
+            s.done = 1;
+            s.status = HZO_PCG_MAXITER;
+            return 1;
+        }
+        if (prev == K_XADD || !keep_going(s, g, rr) || (s.refine_every > 0 && s.iters % s.refine_every == 0))
+            return 2;
+    }
+    // prev == K_WCOMP (fresh start when iters == 0) or K_ITER
+    double den;
+    if (prev == K_WCOMP && s.iters == 0) {
+        beta = 0.0;
+        den = dl;
+    } else {
+        beta = g / s.rz_old;
+        den = dl - beta * g / s.alpha;
+    }
+    if (!finite(den) || !(den > 0.0)) {
+        s.status = HZO_PCG_BREAKDOWN;
+        s.done = 1;
+        return 1;
+    }
+    alpha = g / den;
+    s.alpha = alpha;
+    s.beta = beta;
+    s.denom = den;
+    s.rz_old = g;
+    return 0;
+}
+
+}  // namespace hzo

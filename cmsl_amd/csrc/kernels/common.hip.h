@@ -203,7 +203,7 @@ __device__ __forceinline__ PcgState load_state(const PcgState* p) {
     s.refine_every = p->refine_every;
     s.xpend = p->xpend;
     s.xadds = p->xadds;
-    s.pad2 = 0;
+    s.passes = p->passes;
     s.alpha_prev = p->alpha_prev;
     return s;
 }
@@ -230,6 +230,7 @@ __device__ __forceinline__ void store_state(const PcgState& s, PcgState* p) {
     p->refine_every = s.refine_every;
     p->xpend = s.xpend;
     p->xadds = s.xadds;
+    p->passes = s.passes;
     p->alpha_prev = s.alpha_prev;
 }
 

@@ -1,0 +1,547 @@
+// kernels_cgres.hip -- register-resident single-reduction CG: the whole
+// FP32-refined Poisson solve in one persistent launch.
+//
+// The single-reduction CG of kernels_cg32.hip streams r, w, s, p and x_lo
+// through memory every iteration (38 bytes per voxel); on a film that fits
+// the GPU's register files that traffic is not needed at all.  Here each
+// workgroup owns a 16 x 16 column tile over the film's full depth, two
+// threads per column (the lower and the upper half of the planes), and
+// keeps the tile's r, w, s, p, x_lo and Jacobi D^-1 in registers for the
+// whole solve (an MI355X holds 256 x 512 KB of vector registers; a
+// 256 x 256 x 32 film needs 50 MB of them).  Per
+// iteration only the tile's edge columns of w cross to the neighbouring
+// tiles, and one grid-wide reduction of (gamma, delta, r.r) ends it:
+//   after the barrier: every workgroup sums the published partials in the
+//   same fixed order, runs the same loop decision (cg_decide), and updates
+//   s = w + beta s, r -= alpha s, p = M r_old + beta p, x_lo += alpha p on
+//   its columns and, for s and r, on its one-column ring (the ring keeps
+//   the neighbours' s and r by the same recurrence and the same arithmetic,
+//   given their published w, so it holds their values bit for bit);
+//   then u = M r on tile and ring (LDS), w = A u on the tile, partials,
+//   publication of the edge columns of w, and the barrier.
+// The residual replacement of the FP32-refined form (every refine_every
+// iterations and when the recursive residual claims convergence) takes
+// three barriers: the edge columns of x_lo go out; r = b - A (x_hi + x_lo)
+// in FP64 on the tile (plane by plane through LDS) and its edge columns go
+// out; x_lo is folded into x_hi, u and w are re-formed from the true
+// residual (WCOMP).  The iterates are those of the streaming kernel's
+// recurrence; only the grouping of the inner products differs.
+//
+// Grid barrier: cooperative launch (every workgroup resident), one arrival
+// counter per XCD (workgroup w runs on XCD w % 8) whose last arriver bumps a
+// global counter that all workgroups poll.  Data crossing workgroups is
+// stored with device scope (written through the XCD's L2) and loaded with
+// device-scope coherence, so no cache-wide write-back or invalidate is
+// needed; every wait is bounded and a timeout raises an error flag that
+// ends every workgroup at its next barrier.
+#include "../hzo_internal.h"
+#include "common.hip.h"
+#include "tile.hip.h"
+#include "reduce.hip.h"
+#include "cg_ctl.hip.h"
+
+namespace hzo {
+
+namespace {
+
+constexpr int RT = 16;        // tile edge in x and y
+constexpr int RTH = RT + 2;   // with the one-column ring
+constexpr int RNT = 2 * RT * RT;  // threads per workgroup: two per column (z halves)
+constexpr int RNW = RNT / 64;     // waves per workgroup
+constexpr unsigned RES_SPIN_LIMIT = 1u << 22;
+constexpr int LD_COH = 17;  // buffer-load cache policy: device-scope coherent (SC0 | SC1)
+
+__device__ __forceinline__ float ld_coh(rsrc_t R, unsigned voff, unsigned soff = 0u) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(R, (int)voff, (int)soff, LD_COH));
+}
+__device__ __forceinline__ double ld_coh_d(rsrc_t R, unsigned voff, unsigned soff = 0u) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(R, (int)voff, (int)soff, LD_COH));
+}
+
+// Grid barrier number q (0, 1, ...) of G workgroups.  The caller's stores
+// have all been issued; each wave waits for them to complete before the
+// workgroup arrives.  Returns false after a timeout (the error flag is set).
+__device__ __forceinline__ bool res_barrier(unsigned* sync, int* err, int q, int G, int& sok) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int ngrp = G < N_XCD ? G : N_XCD;
+        const int xg = blockIdx.x % N_XCD;
+        const unsigned nxg = (unsigned)((G - xg + N_XCD - 1) / N_XCD);  // workgroups of this group
+        const unsigned old = __hip_atomic_fetch_add(sync + 64 * (1 + xg), 1u, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+        if (old + 1u == (unsigned)(q + 1) * nxg)
+            __hip_atomic_fetch_add(sync, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned target = (unsigned)(q + 1) * (unsigned)ngrp;
+        unsigned spins = 0;
+        int ok = 1;
+        while (__hip_atomic_load(sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (++spins > RES_SPIN_LIMIT ||
+                ((spins & 1023u) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        sok = ok;
+    }
+    __syncthreads();
+    return sok != 0;
+}
+
+// The control-block update after cg_decide for the resident solve: an
+// iteration counts, the other modes only record what the pass does (x_lo
+// lives in registers, so no x_lo term is ever deferred).
+__device__ __forceinline__ int res_advance(PcgState& s, int mode) {
+    if (mode == 0) {
+        s.iters += 1;
+        s.kind = K_ITER;
+    } else if (mode == 2) {
+        s.kind = K_RESID;
+    } else if (mode == 3) {
+        s.kind = K_WCOMP;
+    }
+    return mode;
+}
+
+}  // namespace
+
+// NZ = nz of the film (each thread holds NZ/2 planes of a column); MIR:
+// Neumann electrodes (mirrored-plane weights in the inner products).
+template <int NZ, bool MIR>
+__global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int gxt, int G) {
+    static_assert(NZ % 8 == 0, "ring cells are spread over the workgroup in whole rounds");
+    constexpr int NH = NZ / 2;       // planes per thread
+    constexpr int NR = 4 * NZ * RT;  // ring cells: 4 sides x NZ planes x 16
+    constexpr int RPT = NR / RNT;    // ring cells per thread
+    __shared__ float uT[NZ][RTH][RTH];
+    // x_lo of the tile stays in LDS (each thread touches only its own
+    // voxels, once per iteration), which leaves the registers to r, w, s, p,
+    // D^-1 and the codes; eT stages the vector whose edge columns a pass
+    // publishes, so that the stores go out coalesced and branch-free
+    __shared__ float eT[NZ][RT][RT], xT[NZ][RT][RT];
+    __shared__ float rR[NR], sR[NR], dR[NR];
+    __shared__ HPair<float> sHP[64];
+    __shared__ double sH64[16];
+    __shared__ double xP[2][2][RTH][RTH];  // RESID: [half][plane parity]
+    __shared__ double sredA[3][RNW], sredB[3][RNW];
+    __shared__ int sok;
+
+    const int tid = threadIdx.x, tx = tid & (RT - 1), ty = (tid / RT) & (RT - 1), tz = tid / (RT * RT);
+    // a wave lies in one z half: its first plane is wave-uniform, so plane
+    // offsets can go in the buffer instructions' scalar offset
+    const int wv = tid >> 6, k0 = __builtin_amdgcn_readfirstlane(tz * NH);
+    const int blk = blockIdx.x, bxt = blk % gxt, byt = blk / gxt, gyt = G / gxt;
+    const int nx = c.nx, ny = c.ny;
+    const unsigned pl = c.plane, nv = c.nvox;
+    const int i0 = bxt * RT, j0 = byt * RT;
+    const unsigned col = (unsigned)(i0 + tx) + (unsigned)nx * (unsigned)(j0 + ty);
+    // neighbour tiles across the x-, x+, y-, y+ sides (periodic)
+    int nbt[4];
+    nbt[0] = (bxt + gxt - 1) % gxt + byt * gxt;
+    nbt[1] = (bxt + 1) % gxt + byt * gxt;
+    nbt[2] = bxt + ((byt + gyt - 1) % gyt) * gxt;
+    nbt[3] = bxt + ((byt + 1) % gyt) * gxt;
+
+    const rsrc_t Rsc = make_rsrc(b.sc, nv * 2u);
+    const rsrc_t Rr0 = make_rsrc(b.r0, nv * 4u);
+    const rsrc_t Rxl = make_rsrc(b.xlo, nv * 4u);
+    const rsrc_t Rxh = make_rsrc(b.xhi, nv * 8u);
+    const rsrc_t Rb = make_rsrc(b.b, nv * 8u);
+    const rsrc_t Rdi = make_rsrc(b.lut32 + LUT_DINV, SC_LUT_SIZE * 4u);
+    const rsrc_t RX = make_rsrc(b.xch, 2u * (unsigned)G * NR * 4u);
+    const rsrc_t RP = make_rsrc(b.part, 2u * (unsigned)G * 4u * 8u);
+    // edge-column slot: [parity][tile][side][plane][position]
+    auto xoff = [&](unsigned slot, int tile, int side, int k, int pos) {
+        return ((((slot * (unsigned)G + (unsigned)tile) * 4u + (unsigned)side) * NZ + (unsigned)k) * RT +
+                (unsigned)pos) * 4u;
+    };
+    // ring cell e = (side * NZ + k) * 16 + pos: its LDS place and film column
+    auto ring_lds = [&](int e, int& k, int& ly, int& lx) {
+        const int side = e / (NZ * RT), pos = e % RT;
+        k = (e / RT) % NZ;
+        ly = side < 2 ? pos + 1 : (side == 2 ? 0 : RTH - 1);
+        lx = side < 2 ? (side == 0 ? 0 : RTH - 1) : pos + 1;
+    };
+    auto ring_col = [&](int e) {
+        const int side = e / (NZ * RT), pos = e % RT;
+        int gi, gj;
+        if (side < 2) {
+            gi = side == 0 ? (i0 + nx - 1) % nx : (i0 + RT) % nx;
+            gj = j0 + pos;
+        } else {
+            gi = i0 + pos;
+            gj = side == 2 ? (j0 + ny - 1) % ny : (j0 + RT) % ny;
+        }
+        return (unsigned)gi + (unsigned)nx * (unsigned)gj;
+    };
+
+    if (tid < 64) sHP[tid] = load_hpair(b.lut32);
+    if (tid < 16) sH64[tid] = b.lut[LUT_HARM + tid];
+
+    // The thread's planes of its column: vectors and codes (two per register).
+    float r[NH], w[NH], s[NH], p[NH], d[NH];
+    unsigned cdp[NH / 2];
+#pragma unroll
+    for (int kk = 0; kk < NH; ++kk) {
+        const unsigned so = (unsigned)(k0 + kk) * pl;
+        const unsigned code = bload_code(Rsc, col * 2u, so * 2u);
+        if (kk & 1) cdp[kk / 2] |= code << 16;
+        else cdp[kk / 2] = code;
+        d[kk] = bload<float>(Rdi, code * 4u, 0u);
+        r[kk] = bload<float>(Rr0, col * 4u, so * 4u);
+        xT[k0 + kk][ty][tx] = bload<float>(Rxl, col * 4u, so * 4u);
+        p[kk] = 0.0f;
+        w[kk] = 0.0f;
+        s[kk] = 0.0f;
+    }
+    auto code_of = [&](int kk) { return (cdp[kk / 2] >> (16 * (kk & 1))) & 0xFFFFu; };
+#pragma unroll
+    for (int m = 0; m < RPT; ++m) {
+        const int e = tid + RNT * m;
+        int k, ly, lx;
+        ring_lds(e, k, ly, lx);
+        const unsigned rv = ring_col(e) + (unsigned)k * pl;  // the plane differs across lanes
+        const unsigned code = bload_code(Rsc, rv * 2u, 0u);
+        dR[e] = bload<float>(Rdi, code * 4u, 0u);
+        rR[e] = bload<float>(Rr0, rv * 4u, 0u);
+        sR[e] = 0.0f;
+    }
+
+    // Workgroup totals of per-thread values in a fixed order (waves, then
+    // the wave slots in order): the same bits in every thread.
+    auto wg_total = [&](double (&a)[3], double (&S)[3][RNW], double (&t)[3]) {
+#pragma unroll
+        for (int qq = 0; qq < 3; ++qq) {
+            const double ws = wave_sum(a[qq]);
+            if ((tid & 63) == 0) S[qq][wv] = ws;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int qq = 0; qq < 3; ++qq) {
+            double v = S[qq][0];
+#pragma unroll
+            for (int u = 1; u < RNW; ++u) v += S[qq][u];
+            t[qq] = v;
+        }
+    };
+
+    // The control block lives in LDS between decisions: wave 0 runs the
+    // (deterministic) decision on totals every thread holds, so the block
+    // is live in registers only around it.
+    __shared__ PcgState sst;
+    __shared__ double sdec[2];  // alpha, beta of the next pass
+    __shared__ int smode;
+    if (tid == 0) store_state(load_state(b.st), &sst);
+    int mode = 1;
+    float af = 0.0f, bf = 0.0f;
+    bool fresh = false;
+    auto decide = [&](const double (&t)[3], bool cap) {
+        if (tid < 64) {
+            PcgState st = load_state(&sst);
+            double a = 0.0, bb = 0.0;
+            int m = res_advance(st, cg_decide(st, t, a, bb));
+            if (m != 1 && cap) {
+                st.done = 1;
+                st.status = HZO_PCG_MAXITER;
+                m = 1;
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (tid == 0) {
+                store_state(st, &sst);
+                sdec[0] = a;
+                sdec[1] = bb;
+                smode = m;
+            }
+        }
+        __syncthreads();
+        mode = smode;
+        af = (float)sdec[0];
+        bf = (float)sdec[1];
+        fresh = sdec[1] == 0.0;  // first iteration of a solve: no previous p, s
+    };
+
+    // The setup's partial sums (r.z, r.r, b.b) and the first decision.
+    double tot[3];
+    {
+        double a[3] = {0.0, 0.0, 0.0};
+        for (int t = tid; t < b.nb_in; t += RNT) {
+#pragma unroll
+            for (int qq = 0; qq < 3; ++qq) a[qq] += b.pin[(long long)qq * b.ps_in + t];
+        }
+        wg_total(a, sredB, tot);
+    }
+    decide(tot, false);
+    const float ih2 = (float)c.inv_h2;
+    const int qcap = 4 * sst.max_iter + 16;
+    int q = 0;             // grid barriers passed
+    bool ring_r = false;   // the ring's r comes from the last pass's edge columns (after RESID)
+    bool resid_x = false;  // RESID: the edge columns of x_lo are out
+    // the ring cells' values the neighbours published in the last pass
+    // (loaded right after each grid barrier, beside the partial sums)
+    float rin[RPT];
+#pragma unroll
+    for (int m = 0; m < RPT; ++m) rin[m] = 0.0f;
+
+    while (mode != 1) {
+        // the codes and the thread index enter each pass as fresh values, so
+        // the addresses derived from them are formed where they are used
+        // instead of being hoisted out of the loop and held live through it
+#pragma unroll
+        for (int i = 0; i < NH / 2; ++i) asm volatile("" : "+v"(cdp[i]));
+        int tidp = tid, txp = tx, typ = ty;
+        unsigned colp = col;
+        asm volatile("" : "+v"(tidp), "+v"(txp), "+v"(typ), "+v"(colp));
+        const unsigned slot = (unsigned)q & 1u;
+        // the ring cells' values the neighbours published in pass slot sl
+        auto ring_load = [&](unsigned sl) {
+#pragma unroll
+            for (int m = 0; m < RPT; ++m) {
+                const int e = tidp + RNT * m;
+                const int side = e / (NZ * RT), k = (e / RT) % NZ, pos = e % RT;
+                rin[m] = ld_coh(RX, xoff(sl, nbt[side], side ^ 1, k, pos));
+            }
+        };
+        // The tile's edge columns of the vector staged in eT go out, one
+        // ring-ordered slot per thread and round (contiguous stores).
+        auto publish_staged = [&](unsigned sl) {
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < RPT; ++m) {
+                const int e = tidp + RNT * m;
+                const int side = e / (NZ * RT), k = (e / RT) % NZ, pos = e % RT;
+                const int ex = side < 2 ? (side == 0 ? 0 : RT - 1) : pos;
+                const int ey = side < 2 ? pos : (side == 2 ? 0 : RT - 1);
+                bstore(RX, (((sl * (unsigned)G + (unsigned)blk) * (unsigned)NR) + (unsigned)e) * 4u, 0u, eT[k][ey][ex]);
+            }
+        };
+
+        double acc[3] = {0.0, 0.0, 0.0};
+        bool partials = true;
+        if (mode == 0 || mode == 3) {
+            // ITER (0): s = w + beta s and r -= alpha s on tile and ring,
+            // p = M r_old + beta p, x_lo += alpha p; WCOMP (3): x_lo folded
+            // into x_hi.  Then u = M r (LDS), w = A u, partials (gamma, delta,
+            // r.r), edge columns of w.
+            const bool iter = mode == 0;
+#pragma unroll
+            for (int m = 0; m < RPT; ++m) {
+                const int e = tidp + RNT * m;
+                int k, ly, lx;
+                ring_lds(e, k, ly, lx);
+                float rn = rR[e];
+                if (iter) {
+                    const float wn = rin[m];
+                    const float sn = fresh ? wn : wn + bf * sR[e];
+                    sR[e] = sn;
+                    rn = rn - af * sn;
+                    rR[e] = rn;
+                } else if (ring_r) {
+                    rn = rin[m];
+                    rR[e] = rn;
+                }
+                uT[k][ly][lx] = rn * dR[e];
+            }
+            if (iter) {
+#pragma unroll
+                for (int kk = 0; kk < NH; ++kk) {
+                    const float sn = fresh ? w[kk] : w[kk] + bf * s[kk];
+                    s[kk] = sn;
+                    const float uo = r[kk] * d[kk];
+                    const float pn = fresh ? uo : uo + bf * p[kk];
+                    p[kk] = pn;
+                    float& xv = xT[k0 + kk][typ][txp];
+                    xv = xv + af * pn;
+                    r[kk] = r[kk] - af * sn;
+                }
+            } else {
+#pragma unroll
+                for (int kk = 0; kk < NH; ++kk) {
+                    const unsigned o = colp * 8u, so = (unsigned)(k0 + kk) * pl * 8u;
+                    float& xv = xT[k0 + kk][typ][txp];
+                    bstore(Rxh, o, so, ld_coh_d(Rxh, o, so) + (double)xv);
+                    xv = 0.0f;
+                }
+            }
+#pragma unroll
+            for (int kk = 0; kk < NH; ++kk) uT[k0 + kk][typ + 1][txp + 1] = r[kk] * d[kk];
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < NH; ++kk) {
+                const int k = k0 + kk;
+                // u of the column's own planes from registers (the same
+                // product that went to LDS), of the other half's from LDS
+                const float xc = r[kk] * d[kk];
+                float xu = kk + 1 < NH ? r[kk + 1] * d[kk + 1] : (k + 1 < NZ ? uT[k + 1][typ + 1][txp + 1] : xc);
+                float xd = kk > 0 ? r[kk - 1] * d[kk - 1] : (k > 0 ? uT[k - 1][typ + 1][txp + 1] : xc);
+                if (c.zbc != 0) {  // mirror: planes 1 and nz - 2 stand on both sides of the faces
+                    if (k == 0) xd = xu;
+                    else if (k == NZ - 1) xu = xd;
+                }
+                const unsigned code = code_of(kk);
+                const float qs = row_Ax32(code, sHP, ih2, xc, uT[k][typ + 1][txp + 2], uT[k][typ + 1][txp],
+                                          uT[k][typ + 2][txp + 1], uT[k][typ][txp + 1], xu, xd);
+                const float wvl = (code & SC_IDENTITY) ? op_identity(xc, k, c) : qs;
+                w[kk] = wvl;
+                cg_dot_add<MIR>(acc[0], r[kk], xc, c, k);
+                cg_dot_add<MIR>(acc[1], wvl, xc, c, k);
+                acc[2] += (double)r[kk] * (double)r[kk];
+                eT[k][typ][txp] = wvl;
+            }
+            publish_staged(slot);
+        } else if (!resid_x) {
+            // RESID, first pass: the edge columns of x_lo go out
+#pragma unroll
+            for (int m = 0; m < RPT; ++m) {
+                const int e = tidp + RNT * m;
+                const int side = e / (NZ * RT), k = (e / RT) % NZ, pos = e % RT;
+                const int ex = side < 2 ? (side == 0 ? 0 : RT - 1) : pos;
+                const int ey = side < 2 ? pos : (side == 2 ? 0 : RT - 1);
+                bstore(RX, (((slot * (unsigned)G + (unsigned)blk) * (unsigned)NR) + (unsigned)e) * 4u, 0u, xT[k][ey][ex]);
+            }
+            partials = false;
+        } else {
+            // RESID: r = b - A (x_hi + x_lo) in FP64.  The ring's x_lo comes
+            // from the neighbours' edge columns (staged in rR, which WCOMP
+            // refills); both halves walk their planes together, each plane's
+            // potential on tile and ring staged in LDS (xP[half][plane
+            // parity], one barrier per plane), the column's planes below and
+            // above from memory and xT.
+#pragma unroll
+            for (int m = 0; m < RPT; ++m) rR[tidp + RNT * m] = rin[m];
+            __syncthreads();
+            const unsigned o8 = colp * 8u;
+            auto X = [&](int k) {  // own column, clamped into the film
+                const int kc = k < 0 ? 0 : (k >= NZ ? NZ - 1 : k);
+                return ld_coh_d(Rxh, o8, (unsigned)kc * pl * 8u) + (double)xT[kc][typ][txp];
+            };
+            double xm = X(k0 - 1), x0 = X(k0), xp = X(k0 + 1);
+            const int rl = tidp & (RT * RT - 1);  // ring loaders: 64 threads of each half
+#pragma unroll
+            for (int kk = 0; kk < NH; ++kk) {
+                const int k = k0 + kk;
+                double (*S)[RTH] = xP[tz][kk & 1];
+                S[typ + 1][txp + 1] = x0;
+                if (rl < 4 * RT) {  // the plane's 64 ring cells
+                    const int side = rl / RT, pos = rl % RT;
+                    const int e = (side * NZ + k) * RT + pos;
+                    int kq, ly, lx;
+                    ring_lds(e, kq, ly, lx);
+                    S[ly][lx] = ld_coh_d(Rxh, ring_col(e) * 8u, (unsigned)k * pl * 8u) + (double)rR[e];
+                }
+                const double bv = bload<double>(Rb, o8, (unsigned)k * pl * 8u);
+                const double xpp = X(k + 2);
+                __syncthreads();
+                double xu = xp, xd = xm;
+                if (c.zbc != 0) {
+                    if (k == 0) xd = xu;
+                    else if (k == NZ - 1) xu = xd;
+                }
+                const unsigned code = code_of(kk);
+                double Ax;
+                if (code & SC_IDENTITY) {
+                    Ax = op_identity(x0, k, c);
+                } else {
+                    const double* Hr = sH64 + (code & 3u) * 4u;
+                    const double num = Hr[(code >> 2) & 3u] * (S[typ + 1][txp + 2] - x0) -
+                                       Hr[(code >> 4) & 3u] * (x0 - S[typ + 1][txp]) +
+                                       Hr[(code >> 6) & 3u] * (S[typ + 2][txp + 1] - x0) -
+                                       Hr[(code >> 8) & 3u] * (x0 - S[typ][txp + 1]) + Hr[(code >> 10) & 3u] * (xu - x0) -
+                                       Hr[(code >> 12) & 3u] * (x0 - xd);
+                    Ax = -(num * c.inv_h2);
+                }
+                const float rv = (float)(bv - Ax);
+                r[kk] = rv;
+                cg_dot_add<MIR>(acc[0], rv, rv * d[kk], c, k);
+                acc[1] += (double)rv * (double)rv;
+                xm = x0;
+                x0 = xp;
+                xp = xpp;
+                eT[k][typ][txp] = rv;
+            }
+            publish_staged(slot);
+        }
+        if (partials) {
+            double t[3];
+            wg_total(acc, sredA, t);
+            if (tid == 0) {
+#pragma unroll
+                for (int qq = 0; qq < 3; ++qq)
+                    bstore(RP, ((slot * (unsigned)G + (unsigned)blk) * 4u + (unsigned)qq) * 8u, 0u, t[qq]);
+            }
+        }
+        if (!res_barrier(b.sync, b.err, q, G, sok)) break;
+        ++q;
+        ring_load(slot);  // in flight while the partials are summed and the decision runs
+        if (!partials) {  // RESID continues with its second pass
+            resid_x = true;
+            continue;
+        }
+        // every workgroup adds the partials in the same order
+        {
+            double a[3] = {0.0, 0.0, 0.0};
+            if (tid < G) {
+#pragma unroll
+                for (int qq = 0; qq < 3; ++qq)
+                    a[qq] = ld_coh_d(RP, ((slot * (unsigned)G + (unsigned)tid) * 4u + (unsigned)qq) * 8u);
+            }
+            wg_total(a, sredB, tot);
+        }
+        ring_r = mode == 2;
+        resid_x = false;
+        if (mode == 2) tot[2] = 0.0;
+        decide(tot, q >= qcap);
+    }
+    // the solution's correction goes back; x_hi is already in memory
+#pragma unroll
+    for (int kk = 0; kk < NH; ++kk) bstore(Rxl, col * 4u, (unsigned)(k0 + kk) * pl * 4u, xT[k0 + kk][ty][tx]);
+    if (blk == 0 && tid == 0) {
+        PcgState fin = load_state(&sst);
+        fin.passes = q;
+        store_state(fin, b.st);
+    }
+}
+
+__global__ void k_fill_u32(unsigned* p, int n, unsigned v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void launch_fill_u32(hipStream_t s, unsigned* p, int n, unsigned v) {
+    hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
+}
+
+// Resident solve of the film in one cooperative launch on stream s (the
+// setup kernel has run).  Returns the HIP status of the launch.
+hipError_t launch_cgres(hipStream_t s, const StepConsts& c, const ResBufs& b, int gxt, int G) {
+    void* args[] = {(void*)&b, (void*)&c, (void*)&gxt, (void*)&G};
+    const void* fn = nullptr;
+    const bool mir = c.zbc != 0;
+    switch (c.nz) {
+        case 16: fn = mir ? (const void*)k_cgres<16, true> : (const void*)k_cgres<16, false>; break;
+        case 24: fn = mir ? (const void*)k_cgres<24, true> : (const void*)k_cgres<24, false>; break;
+        case 32: fn = mir ? (const void*)k_cgres<32, true> : (const void*)k_cgres<32, false>; break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipLaunchCooperativeKernel(fn, dim3(G), dim3(RNT), args, 0, s);
+}
+
+// Workgroups of the resident kernel one device can hold at once (0 if the
+// film's depth has no instance or the device has no cooperative launch).
+int cgres_capacity(int device, int nz, bool mir) {
+    int coop = 0, ncu = 0, occ = 0;
+    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device) != hipSuccess || !coop) return 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+    const void* fn = nullptr;
+    switch (nz) {
+        case 16: fn = mir ? (const void*)k_cgres<16, true> : (const void*)k_cgres<16, false>; break;
+        case 24: fn = mir ? (const void*)k_cgres<24, true> : (const void*)k_cgres<24, false>; break;
+        case 32: fn = mir ? (const void*)k_cgres<32, true> : (const void*)k_cgres<32, false>; break;
+        default: return 0;
+    }
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, RNT, 0) != hipSuccess) return 0;
+    return occ * ncu;
+}
+
+}  // namespace hzo

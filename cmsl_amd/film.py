@@ -359,7 +359,8 @@ class Film:
     @property
     def cg_form(self) -> int:
         """0 two-kernel PCG, 1 single-reduction CG carrying w and s, 2 the
-        single-reduction CG forming them by stencils."""
+        single-reduction CG forming them by stencils, 3 the register-resident
+        single-reduction CG (one launch per solve)."""
         return L.lib().hzo_cg_form(self.h)
 
     def close(self):
@@ -471,6 +472,12 @@ class Film:
 
     def set_timing(self, on: bool):
         L.call("hzo_set_timing", self.h, 1 if on else 0)
+
+    def cg_counters(self):
+        """Resident solves since set_timing: (solves, CG iterations, grid barriers)."""
+        out = np.zeros(3, np.int64)
+        L.call("hzo_cg_counters", self.h, L.ptr(out))
+        return tuple(int(v) for v in out)
 
     def kernel_times(self):
         cnt = np.zeros(L.KCLASSES, np.int32)

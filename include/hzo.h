@@ -85,11 +85,14 @@ typedef struct hzo_config {
     int tile_kz;         /* z planes per workgroup of the stencil kernels: 2, 4, 8 (16: single-
                             reduction CG only; 32: its two-stencil form only); 0 = automatic */
     int cg_fused;        /* FP32-refined: 0 = the two-kernel PCG of the FP64 form; otherwise the
-                            single-reduction CG (Chronopoulos-Gear), one fused kernel per
-                            iteration, in one of two forms: 2 = carrying w = A u and s = A p
-                            in memory (42 bytes per voxel and iteration), 3 = forming both by
-                            stencils in the launch (26 bytes, more arithmetic), 1 = automatic
-                            (form 3 when the tiles are 16 planes deep, else form 2).
+                            single-reduction CG (Chronopoulos-Gear): 2 = one fused kernel per
+                            iteration carrying w = A u and s = A p in memory (42 bytes per voxel
+                            and iteration), 3 = the same forming both by stencils in the launch
+                            (26 bytes, more arithmetic), 4 = the whole solve in one cooperative
+                            launch with the vectors held in registers and LDS (16 x 16 column
+                            tiles: nx, ny multiples of 16, nz of 16, 24 or 32, every tile
+                            resident at once), 1 = automatic (4 where it applies and tile_kz is
+                            0; else 3 when the tiles are 16 planes deep, else 2).
                             Films of 2^27 voxels or more always run the two-kernel PCG. */
 } hzo_config;
 
@@ -155,8 +158,9 @@ int hzo_grid_dims(hzo_ctx* ctx, int dims[3]);
 int hzo_tile_depth(hzo_ctx* ctx);
 /* Poisson solver the context runs: 0 two-kernel PCG (FP64 form or FP32
  * cg_fused = 0), 1 single-reduction CG carrying w and s, 2 single-reduction
- * CG forming them by stencils; -1 for a null context.  No reference
- * counterpart (the reference has one solver). */
+ * CG forming them by stencils, 3 the register-resident single-reduction CG
+ * (one launch per solve); -1 for a null context.  No reference counterpart
+ * (the reference has one solver). */
 int hzo_cg_form(hzo_ctx* ctx);
 
 /* ---------------------------------------------------------------- sweep driver
@@ -188,11 +192,15 @@ int hzo_sweep_run(hzo_ctx* ctx, const hzo_sweep_params* p, const char* path, int
  * (launches that found the solve already finished, under a quarter of the
  * class median, are left out).  Classes: 0 tdgl, 1 pcg_setup, 2 pcg_dir_spmv,
  * 3 pcg_update (two-kernel PCG), 4 pcg_fused (single-reduction CG launch of
- * the FP32-refined form). */
+ * the FP32-refined form), 5 pcg_resident (one resident solve). */
 #define HZO_TIMING_MAX 8192
-#define HZO_KCLASSES 5
+#define HZO_KCLASSES 6
 int hzo_set_timing(hzo_ctx* ctx, int enable);
 int hzo_kernel_times(hzo_ctx* ctx, int counts[HZO_KCLASSES], double mean_ms[HZO_KCLASSES]);
+/* Resident solves since hzo_set_timing: out[0] solves, out[1] CG iterations,
+ * out[2] grid barriers (one per iteration, three per residual replacement,
+ * one to re-form w after the setup). */
+int hzo_cg_counters(hzo_ctx* ctx, long long out[3]);
 
 /* ---------------------------------------------------------------- reverse mode
  * Gradients of a P-E-loop loss with respect to the per-phase Landau and

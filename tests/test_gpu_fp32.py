@@ -154,17 +154,20 @@ def test_two_stencil_form_matches_the_carried_form(kz, zbc):
 
 
 @pytest.mark.parametrize("shape,form,kz", [
-    ((256, 256, 32), 1, 8),    # one round of depth-8 workgroups: carried form
+    ((256, 256, 32), 3, 8),    # 256 column tiles, all resident at once: the resident CG
     ((256, 256, 64), 2, 16),   # two rounds at depth 8: two-stencil form, depth 32 would give 256 workgroups
     ((384, 384, 48), 2, 16),   # depth 32 would pad 16 of 48 planes
-    ((512, 512, 32), 2, 32),
+    ((512, 512, 32), 2, 32),   # 1024 column tiles exceed one per CU
     ((512, 512, 64), 2, 32),
-    ((64, 48, 16), 1, 2),
+    ((64, 48, 16), 3, 2),
+    ((96, 40, 12), 1, 2),      # depth 12 has no resident instance: carried form
 ])
 def test_automatic_solver_form_and_tile_depth_by_film_size(shape, form, kz):
     """The automatic choice (cg_fused = 1) takes the form and depth measured
-    fastest for the film's size; FP64 and cg_fused = 0 run the two-kernel
-    PCG (form 0).  Only the context is created: no film data is needed."""
+    fastest for the film's size (the resident CG wherever every 16 x 16
+    column tile fits the device at once); FP64 and cg_fused = 0 run the
+    two-kernel PCG (form 0).  Only the context is created: no film data is
+    needed."""
     import ctypes as C
     nx, ny, nz = shape
     out = {}

@@ -43,7 +43,7 @@ def test_device_count_and_create_without_gpu_fail_cleanly():
 
 
 @pytest.mark.parametrize("kw,msg", [
-    (dict(cg_fused=4), b"cg_fused"),
+    (dict(cg_fused=5), b"cg_fused"),
     (dict(cg_fused=-1), b"cg_fused"),
     (dict(tile_kz=32, cg_fused=2), b"tile_kz"),
     (dict(tile_kz=32, cg_fused=0), b"tile_kz"),
@@ -54,6 +54,7 @@ def test_device_count_and_create_without_gpu_fail_cleanly():
 def test_create_rejects_solver_and_tile_combinations(kw, msg):
     # argument checks run before any device call, so they hold without a GPU:
     # depth 16 needs the single-reduction CG, depth 32 its two-stencil form
+    # (cg_fused 4, the resident form, is checked against the device)
     import ctypes as C
     base = dict(nx=8, ny=8, nz=4, precision=L.PREC_FP32_REFINED)
     base.update(kw)
