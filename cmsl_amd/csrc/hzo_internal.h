@@ -179,7 +179,8 @@ struct CG32Bufs {
 struct ResBufs {
     const float* r0;       // r = b - A x after the setup kernel
     float* xlo;            // x_lo: read at the start, written at the end
-    double* xhi;           // x_hi: x_lo is folded into it at every re-formation of w
+    double* xhi;           // x_hi: read at the start, holds x_hi at the end
+    double* xhi2;          // second copy of x_hi: replacements alternate between the two
     const double* b;       // right-hand side
     const uint16_t* sc;    // stencil codes
     const double* lut;     // FP64 face means (residual replacement)

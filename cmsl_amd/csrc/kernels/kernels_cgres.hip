@@ -87,7 +87,7 @@ __device__ __forceinline__ bool res_barrier(unsigned* sync, int* err, int q, int
         sok = ok;
     }
     __syncthreads();
-    return sok != 0;
+    return __builtin_amdgcn_readfirstlane(sok) != 0;
 }
 
 // The control-block update after cg_decide for the resident solve: an
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int g
     constexpr int NH = NZ / 2;       // planes per thread
     constexpr int NR = 4 * NZ * RT;  // ring cells: 4 sides x NZ planes x 16
     constexpr int RPT = NR / RNT;    // ring cells per thread
-    __shared__ float uT[NZ][RTH][RTH];
+    __shared__ __align__(16) float uT[NZ][RTH][RTH];  // RESID's second pass holds the ring's x here
     // x_lo of the tile stays in LDS (each thread touches only its own
     // voxels, once per iteration), which leaves the registers to r, w, s, p,
     // D^-1 and the codes; eT stages the vector whose edge columns a pass
@@ -148,6 +148,7 @@ __global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int g
     const rsrc_t Rr0 = make_rsrc(b.r0, nv * 4u);
     const rsrc_t Rxl = make_rsrc(b.xlo, nv * 4u);
     const rsrc_t Rxh = make_rsrc(b.xhi, nv * 8u);
+    const rsrc_t Rxh2 = make_rsrc(b.xhi2, nv * 8u);
     const rsrc_t Rb = make_rsrc(b.b, nv * 8u);
     const rsrc_t Rdi = make_rsrc(b.lut32 + LUT_DINV, SC_LUT_SIZE * 4u);
     const rsrc_t RX = make_rsrc(b.xch, 2u * (unsigned)G * NR * 4u);
@@ -227,39 +228,35 @@ __global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int g
         }
     };
 
-    // The control block lives in LDS between decisions: wave 0 runs the
-    // (deterministic) decision on totals every thread holds, so the block
-    // is live in registers only around it.
-    __shared__ PcgState sst;
-    __shared__ double sdec[2];  // alpha, beta of the next pass
-    __shared__ int smode;
-    if (tid == 0) store_state(load_state(b.st), &sst);
+    // The control block lives in LDS between decisions, double-buffered:
+    // every thread runs the (deterministic) decision on the same totals from
+    // copy dpar, thread 0 writes the result to the other copy, which the
+    // next decision reads after at least one workgroup barrier.  So no
+    // broadcast barrier follows a decision, and the block is live in
+    // registers only around it.
+    __shared__ PcgState sst[2];
+    int dpar = 0;
+    if (tid == 0) store_state(load_state(b.st), &sst[0]);
     int mode = 1;
     float af = 0.0f, bf = 0.0f;
     bool fresh = false;
     auto decide = [&](const double (&t)[3], bool cap) {
-        if (tid < 64) {
-            PcgState st = load_state(&sst);
-            double a = 0.0, bb = 0.0;
-            int m = res_advance(st, cg_decide(st, t, a, bb));
-            if (m != 1 && cap) {
-                st.done = 1;
-                st.status = HZO_PCG_MAXITER;
-                m = 1;
-            }
-            __builtin_amdgcn_wave_barrier();
-            if (tid == 0) {
-                store_state(st, &sst);
-                sdec[0] = a;
-                sdec[1] = bb;
-                smode = m;
-            }
+        PcgState st = load_state(&sst[dpar]);
+        double a = 0.0, bb = 0.0;
+        int m = res_advance(st, cg_decide(st, t, a, bb));
+        if (m != 1 && cap) {
+            st.done = 1;
+            st.status = HZO_PCG_MAXITER;
+            m = 1;
         }
-        __syncthreads();
-        mode = smode;
-        af = (float)sdec[0];
-        bf = (float)sdec[1];
-        fresh = sdec[1] == 0.0;  // first iteration of a solve: no previous p, s
+        if (tid == 0) store_state(st, &sst[dpar ^ 1]);
+        dpar ^= 1;
+        // every thread holds the same decision: made wave-uniform, the
+        // branches on it are scalar and alpha, beta sit in scalar registers
+        mode = __builtin_amdgcn_readfirstlane(m);
+        af = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, (float)a)));
+        bf = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, (float)bb)));
+        fresh = __builtin_amdgcn_readfirstlane(bb == 0.0 ? 1 : 0) != 0;  // first iteration of a solve: no previous p, s
     };
 
     // The setup's partial sums (r.z, r.r, b.b) and the first decision.
@@ -274,10 +271,13 @@ __global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int g
     }
     decide(tot, false);
     const float ih2 = (float)c.inv_h2;
-    const int qcap = 4 * sst.max_iter + 16;
+    const int qcap = 4 * sst[0].max_iter + 16;
     int q = 0;             // grid barriers passed
     bool ring_r = false;   // the ring's r comes from the last pass's edge columns (after RESID)
     bool resid_x = false;  // RESID: the edge columns of x_lo are out
+    // x_hi in use: b.xhi (0) or b.xhi2 (1).  Each replacement writes
+    // x_hi + x_lo to the other copy, so neighbours read the old one meanwhile.
+    int hsel = 0;
     // the ring cells' values the neighbours published in the last pass
     // (loaded right after each grid barrier, beside the partial sums)
     float rin[RPT];
@@ -355,12 +355,15 @@ __global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int g
                     xv = xv + af * pn;
                     r[kk] = r[kk] - af * sn;
                 }
-            } else {
+            } else if (!ring_r) {
+                // WCOMP after the setup: x_lo is folded into x_hi (after a
+                // replacement the RESID pass has folded it already)
+                const rsrc_t Rc = hsel ? Rxh2 : Rxh;
 #pragma unroll
                 for (int kk = 0; kk < NH; ++kk) {
                     const unsigned o = colp * 8u, so = (unsigned)(k0 + kk) * pl * 8u;
                     float& xv = xT[k0 + kk][typ][txp];
-                    bstore(Rxh, o, so, ld_coh_d(Rxh, o, so) + (double)xv);
+                    bstore(Rc, o, so, ld_coh_d(Rc, o, so) + (double)xv);
                     xv = 0.0f;
                 }
             }
@@ -402,21 +405,29 @@ __global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int g
             }
             partials = false;
         } else {
-            // RESID: r = b - A (x_hi + x_lo) in FP64.  The ring's x_lo comes
-            // from the neighbours' edge columns (staged in rR, which WCOMP
-            // refills); both halves walk their planes together, each plane's
-            // potential on tile and ring staged in LDS (xP[half][plane
-            // parity], one barrier per plane), the column's planes below and
-            // above from memory and xT.
+            // RESID: r = b - A (x_hi + x_lo) in FP64, and x_hi + x_lo goes to
+            // the other x_hi copy (x_lo = 0 after the pass).  The ring's
+            // potential (x_hi from memory plus the neighbours' published
+            // x_lo) is staged in LDS as doubles over uT, which this pass does
+            // not use; both halves walk their planes together, each plane's
+            // potential on tile and ring staged in xP[half][plane parity] (one
+            // barrier per plane), the column's planes below and above from
+            // memory and xT.
+            const rsrc_t Rc = hsel ? Rxh2 : Rxh, Rn = hsel ? Rxh : Rxh2;
+            double* xR = reinterpret_cast<double*>(&uT[0][0][0]);
 #pragma unroll
-            for (int m = 0; m < RPT; ++m) rR[tidp + RNT * m] = rin[m];
-            __syncthreads();
+            for (int m = 0; m < RPT; ++m) {
+                const int e = tidp + RNT * m;
+                const int k = (e / RT) % NZ;
+                xR[e] = ld_coh_d(Rc, (ring_col(e) + (unsigned)k * pl) * 8u) + (double)rin[m];
+            }
             const unsigned o8 = colp * 8u;
             auto X = [&](int k) {  // own column, clamped into the film
                 const int kc = k < 0 ? 0 : (k >= NZ ? NZ - 1 : k);
-                return ld_coh_d(Rxh, o8, (unsigned)kc * pl * 8u) + (double)xT[kc][typ][txp];
+                return ld_coh_d(Rc, o8, (unsigned)kc * pl * 8u) + (double)xT[kc][typ][txp];
             };
             double xm = X(k0 - 1), x0 = X(k0), xp = X(k0 + 1);
+            __syncthreads();
             const int rl = tidp & (RT * RT - 1);  // ring loaders: 64 threads of each half
 #pragma unroll
             for (int kk = 0; kk < NH; ++kk) {
@@ -428,8 +439,9 @@ __global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int g
                     const int e = (side * NZ + k) * RT + pos;
                     int kq, ly, lx;
                     ring_lds(e, kq, ly, lx);
-                    S[ly][lx] = ld_coh_d(Rxh, ring_col(e) * 8u, (unsigned)k * pl * 8u) + (double)rR[e];
+                    S[ly][lx] = xR[e];
                 }
+                bstore(Rn, o8, (unsigned)k * pl * 8u, x0);
                 const double bv = bload<double>(Rb, o8, (unsigned)k * pl * 8u);
                 const double xpp = X(k + 2);
                 __syncthreads();
@@ -460,7 +472,9 @@ __global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int g
                 xp = xpp;
                 eT[k][typ][txp] = rv;
             }
-            publish_staged(slot);
+            publish_staged(slot);  // its barrier ends every read of xT
+#pragma unroll
+            for (int kk = 0; kk < NH; ++kk) xT[k0 + kk][typ][txp] = 0.0f;
         }
         if (partials) {
             double t[3];
@@ -473,12 +487,14 @@ __global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int g
         }
         if (!res_barrier(b.sync, b.err, q, G, sok)) break;
         ++q;
-        ring_load(slot);  // in flight while the partials are summed and the decision runs
         if (!partials) {  // RESID continues with its second pass
+            ring_load(slot);
             resid_x = true;
             continue;
         }
-        // every workgroup adds the partials in the same order
+        // every workgroup adds the partials in the same order; the ring's
+        // values for the next pass are loaded behind them, in flight while
+        // the partials are summed and the decision runs
         {
             double a[3] = {0.0, 0.0, 0.0};
             if (tid < G) {
@@ -486,9 +502,11 @@ __global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int g
                 for (int qq = 0; qq < 3; ++qq)
                     a[qq] = ld_coh_d(RP, ((slot * (unsigned)G + (unsigned)tid) * 4u + (unsigned)qq) * 8u);
             }
+            ring_load(slot);
             wg_total(a, sredB, tot);
         }
         ring_r = mode == 2;
+        if (mode == 2) hsel ^= 1;
         resid_x = false;
         if (mode == 2) tot[2] = 0.0;
         decide(tot, q >= qcap);
@@ -496,8 +514,15 @@ __global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int g
     // the solution's correction goes back; x_hi is already in memory
 #pragma unroll
     for (int kk = 0; kk < NH; ++kk) bstore(Rxl, col * 4u, (unsigned)(k0 + kk) * pl * 4u, xT[k0 + kk][ty][tx]);
+    if (hsel) {  // the last replacement left x_hi in the second copy
+#pragma unroll
+        for (int kk = 0; kk < NH; ++kk) {
+            const unsigned so = (unsigned)(k0 + kk) * pl * 8u;
+            bstore(Rxh, col * 8u, so, ld_coh_d(Rxh2, col * 8u, so));
+        }
+    }
     if (blk == 0 && tid == 0) {
-        PcgState fin = load_state(&sst);
+        PcgState fin = load_state(&sst[dpar]);
         fin.passes = q;
         store_state(fin, b.st);
     }
