@@ -39,7 +39,7 @@ struct hzo_ctx {
     bool cg_res = false;
     int res_gxt = 0, res_G = 0;  // tiles in x, tiles
     float* res_xch = nullptr;    // edge columns [2][G][4][nz][16]
-    double* res_xhi2 = nullptr;  // second copy of x_hi (one per voxel)
+    double* res_xchd = nullptr;  // FP64 edge columns [G][4][nz][16]
     double* res_part = nullptr;  // partial sums [2][G][4]
     unsigned* res_sync = nullptr;  // barrier counters, then the timeout flag
     int* res_err_host = nullptr;   // pinned copy of the timeout flag
@@ -441,7 +441,7 @@ int cgres_solve(hzo_ctx* x, const hzo::StepConsts& c, const hzo::CG32Bufs& cb, i
     rb.r0 = x->r32;
     rb.xlo = cb.xlo;
     rb.xhi = cb.xhi;
-    rb.xhi2 = x->res_xhi2;
+    rb.xchd = x->res_xchd;
     rb.b = cb.b;
     rb.sc = cb.sc;
     rb.lut = cb.lut;
@@ -621,7 +621,7 @@ void free_all(hzo_ctx* x) {
     (void)hipFree(x->lut32);
     if (x->st_host) (void)hipHostFree(x->st_host);
     (void)hipFree(x->res_xch);
-    (void)hipFree(x->res_xhi2);
+    (void)hipFree(x->res_xchd);
     (void)hipFree(x->res_part);
     (void)hipFree(x->res_sync);
     if (x->res_err_host) (void)hipHostFree(x->res_err_host);
@@ -867,7 +867,7 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
     HZO_ALLOC(x->st, sizeof(hzo::PcgState) * 2);
     if (x->cg_res) {
         HZO_ALLOC(x->res_xch, sizeof(float) * 2 * (size_t)x->res_G * 4 * cfg->nz * 16);
-        HZO_ALLOC(x->res_xhi2, sizeof(double) * (size_t)cfg->nx * cfg->ny * cfg->nz);
+        HZO_ALLOC(x->res_xchd, sizeof(double) * (size_t)x->res_G * 4 * cfg->nz * 16);
         HZO_ALLOC(x->res_part, sizeof(double) * 2 * (size_t)x->res_G * 4);
         HZO_ALLOC(x->res_sync, sizeof(unsigned) * (hzo::RES_SYNC_WORDS + 64));
         if (hipHostMalloc((void**)&x->res_err_host, sizeof(int), hipHostMallocDefault) != hipSuccess)

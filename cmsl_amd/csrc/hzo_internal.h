@@ -179,8 +179,7 @@ struct CG32Bufs {
 struct ResBufs {
     const float* r0;       // r = b - A x after the setup kernel
     float* xlo;            // x_lo: read at the start, written at the end
-    double* xhi;           // x_hi: read at the start, holds x_hi at the end
-    double* xhi2;          // second copy of x_hi: replacements alternate between the two
+    double* xhi;           // x_hi: x_lo is folded into it at every residual replacement
     const double* b;       // right-hand side
     const uint16_t* sc;    // stencil codes
     const double* lut;     // FP64 face means (residual replacement)
@@ -189,6 +188,7 @@ struct ResBufs {
     int nb_in, ps_in;
     PcgState* st;          // in: the setup's control block; out: the final one
     float* xch;            // edge columns: [2][tiles][4][nz][16]
+    double* xchd;          // FP64 edge columns of x (residual replacement): [tiles][4][nz][16]
     double* part;          // workgroup partial sums: [2][tiles][4]
     unsigned* sync;        // barrier counters (global, then one per XCD, 64 apart), zero at launch
     int* err;              // barrier timeout flag, zero at launch

@@ -148,7 +148,7 @@ __global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int g
     const rsrc_t Rr0 = make_rsrc(b.r0, nv * 4u);
     const rsrc_t Rxl = make_rsrc(b.xlo, nv * 4u);
     const rsrc_t Rxh = make_rsrc(b.xhi, nv * 8u);
-    const rsrc_t Rxh2 = make_rsrc(b.xhi2, nv * 8u);
+    const rsrc_t RXD = make_rsrc(b.xchd, (unsigned)G * NR * 8u);
     const rsrc_t Rb = make_rsrc(b.b, nv * 8u);
     const rsrc_t Rdi = make_rsrc(b.lut32 + LUT_DINV, SC_LUT_SIZE * 4u);
     const rsrc_t RX = make_rsrc(b.xch, 2u * (unsigned)G * NR * 4u);
@@ -157,6 +157,10 @@ __global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int g
     auto xoff = [&](unsigned slot, int tile, int side, int k, int pos) {
         return ((((slot * (unsigned)G + (unsigned)tile) * 4u + (unsigned)side) * NZ + (unsigned)k) * RT +
                 (unsigned)pos) * 4u;
+    };
+    // FP64 edge-column slot of the residual replacement: [tile][side][plane][position]
+    auto xdoff = [&](int tile, int side, int k, int pos) {
+        return ((((unsigned)tile * 4u + (unsigned)side) * NZ + (unsigned)k) * RT + (unsigned)pos) * 8u;
     };
     // ring cell e = (side * NZ + k) * 16 + pos: its LDS place and film column
     auto ring_lds = [&](int e, int& k, int& ly, int& lx) {
@@ -275,9 +279,6 @@ __global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int g
     int q = 0;             // grid barriers passed
     bool ring_r = false;   // the ring's r comes from the last pass's edge columns (after RESID)
     bool resid_x = false;  // RESID: the edge columns of x_lo are out
-    // x_hi in use: b.xhi (0) or b.xhi2 (1).  Each replacement writes
-    // x_hi + x_lo to the other copy, so neighbours read the old one meanwhile.
-    int hsel = 0;
     // the ring cells' values the neighbours published in the last pass
     // (loaded right after each grid barrier, beside the partial sums)
     float rin[RPT];
@@ -314,6 +315,22 @@ __global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int g
                 const int ex = side < 2 ? (side == 0 ? 0 : RT - 1) : pos;
                 const int ey = side < 2 ? pos : (side == 2 ? 0 : RT - 1);
                 bstore(RX, (((sl * (unsigned)G + (unsigned)blk) * (unsigned)NR) + (unsigned)e) * 4u, 0u, eT[k][ey][ex]);
+            }
+        };
+
+        // x_hi + x_lo of the thread's voxels, x_lo = 0.  Every load is
+        // issued before the first store: a load behind a store waits for
+        // the store's write-through to complete.
+        auto fold_load = [&](rsrc_t Rfrom, double (&xh)[NH]) {
+#pragma unroll
+            for (int kk = 0; kk < NH; ++kk) xh[kk] = ld_coh_d(Rfrom, colp * 8u, (unsigned)(k0 + kk) * pl * 8u);
+        };
+        auto fold_store = [&](rsrc_t Rto, const double (&xh)[NH]) {
+#pragma unroll
+            for (int kk = 0; kk < NH; ++kk) {
+                float& xv = xT[k0 + kk][typ][txp];
+                bstore(Rto, colp * 8u, (unsigned)(k0 + kk) * pl * 8u, xh[kk] + (double)xv);
+                xv = 0.0f;
             }
         };
 
@@ -358,14 +375,9 @@ __global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int g
             } else if (!ring_r) {
                 // WCOMP after the setup: x_lo is folded into x_hi (after a
                 // replacement the RESID pass has folded it already)
-                const rsrc_t Rc = hsel ? Rxh2 : Rxh;
-#pragma unroll
-                for (int kk = 0; kk < NH; ++kk) {
-                    const unsigned o = colp * 8u, so = (unsigned)(k0 + kk) * pl * 8u;
-                    float& xv = xT[k0 + kk][typ][txp];
-                    bstore(Rc, o, so, ld_coh_d(Rc, o, so) + (double)xv);
-                    xv = 0.0f;
-                }
+                double xh[NH];
+                fold_load(Rxh, xh);
+                fold_store(Rxh, xh);
             }
 #pragma unroll
             for (int kk = 0; kk < NH; ++kk) uT[k0 + kk][typ + 1][txp + 1] = r[kk] * d[kk];
@@ -394,45 +406,60 @@ __global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int g
             }
             publish_staged(slot);
         } else if (!resid_x) {
-            // RESID, first pass: the edge columns of x_lo go out
+            // RESID, first pass: x_lo is folded into x_hi in place (no other
+            // thread touches these voxels of x_hi during the solve) and the
+            // edge columns of x = x_hi go out in FP64
+            double xh[NH];
+            fold_load(Rxh, xh);
+#pragma unroll
+            for (int kk = 0; kk < NH; ++kk) {
+                const int k = k0 + kk;
+                float& xl = xT[k][typ][txp];
+                const double xv = xh[kk] + (double)xl;
+                xl = 0.0f;
+                bstore(Rxh, colp * 8u, (unsigned)k * pl * 8u, xv);
+                if (txp == 0) bstore(RXD, xdoff(blk, 0, k, typ), 0u, xv);
+                if (txp == RT - 1) bstore(RXD, xdoff(blk, 1, k, typ), 0u, xv);
+                if (typ == 0) bstore(RXD, xdoff(blk, 2, k, txp), 0u, xv);
+                if (typ == RT - 1) bstore(RXD, xdoff(blk, 3, k, txp), 0u, xv);
+            }
+            partials = false;
+        } else {
+            // RESID, second pass: r = b - A x in FP64 (x = x_hi after the
+            // first pass's fold).  The ring's x (the neighbours' published
+            // edge columns) is staged in LDS as doubles over uT, which this
+            // pass does not use; the column's x on planes k0 - 1 .. k0 + NH
+            // (clamped into the film) is loaded up front, b two planes ahead.
+            // Both halves walk their planes together, each plane's x on tile
+            // and ring staged in xP[half][plane parity] (one barrier per
+            // plane).
+            double* xR = reinterpret_cast<double*>(&uT[0][0][0]);
+            const unsigned o8 = colp * 8u;
+            double rx[RPT];
 #pragma unroll
             for (int m = 0; m < RPT; ++m) {
                 const int e = tidp + RNT * m;
                 const int side = e / (NZ * RT), k = (e / RT) % NZ, pos = e % RT;
-                const int ex = side < 2 ? (side == 0 ? 0 : RT - 1) : pos;
-                const int ey = side < 2 ? pos : (side == 2 ? 0 : RT - 1);
-                bstore(RX, (((slot * (unsigned)G + (unsigned)blk) * (unsigned)NR) + (unsigned)e) * 4u, 0u, xT[k][ey][ex]);
+                rx[m] = ld_coh_d(RXD, xdoff(nbt[side], side ^ 1, k, pos));
             }
-            partials = false;
-        } else {
-            // RESID: r = b - A (x_hi + x_lo) in FP64, and x_hi + x_lo goes to
-            // the other x_hi copy (x_lo = 0 after the pass).  The ring's
-            // potential (x_hi from memory plus the neighbours' published
-            // x_lo) is staged in LDS as doubles over uT, which this pass does
-            // not use; both halves walk their planes together, each plane's
-            // potential on tile and ring staged in xP[half][plane parity] (one
-            // barrier per plane), the column's planes below and above from
-            // memory and xT.
-            const rsrc_t Rc = hsel ? Rxh2 : Rxh, Rn = hsel ? Rxh : Rxh2;
-            double* xR = reinterpret_cast<double*>(&uT[0][0][0]);
+            double xq[NH + 2];
 #pragma unroll
-            for (int m = 0; m < RPT; ++m) {
-                const int e = tidp + RNT * m;
-                const int k = (e / RT) % NZ;
-                xR[e] = ld_coh_d(Rc, (ring_col(e) + (unsigned)k * pl) * 8u) + (double)rin[m];
+            for (int i = 0; i < NH + 2; ++i) {
+                const int k = k0 - 1 + i, kc = k < 0 ? 0 : (k >= NZ ? NZ - 1 : k);
+                xq[i] = ld_coh_d(Rxh, o8, (unsigned)kc * pl * 8u);
             }
-            const unsigned o8 = colp * 8u;
-            auto X = [&](int k) {  // own column, clamped into the film
-                const int kc = k < 0 ? 0 : (k >= NZ ? NZ - 1 : k);
-                return ld_coh_d(Rc, o8, (unsigned)kc * pl * 8u) + (double)xT[kc][typ][txp];
-            };
-            double xm = X(k0 - 1), x0 = X(k0), xp = X(k0 + 1);
+            double bq[NH];
+            bq[0] = bload<double>(Rb, o8, (unsigned)k0 * pl * 8u);
+            bq[1] = bload<double>(Rb, o8, (unsigned)(k0 + 1) * pl * 8u);
+#pragma unroll
+            for (int m = 0; m < RPT; ++m) xR[tidp + RNT * m] = rx[m];
             __syncthreads();
             const int rl = tidp & (RT * RT - 1);  // ring loaders: 64 threads of each half
 #pragma unroll
             for (int kk = 0; kk < NH; ++kk) {
                 const int k = k0 + kk;
                 double (*S)[RTH] = xP[tz][kk & 1];
+                const double xm = xq[kk], x0 = xq[kk + 1], xp = xq[kk + 2];
                 S[typ + 1][txp + 1] = x0;
                 if (rl < 4 * RT) {  // the plane's 64 ring cells
                     const int side = rl / RT, pos = rl % RT;
@@ -441,9 +468,8 @@ __global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int g
                     ring_lds(e, kq, ly, lx);
                     S[ly][lx] = xR[e];
                 }
-                bstore(Rn, o8, (unsigned)k * pl * 8u, x0);
-                const double bv = bload<double>(Rb, o8, (unsigned)k * pl * 8u);
-                const double xpp = X(k + 2);
+                if (kk + 2 < NH) bq[kk + 2] = bload<double>(Rb, o8, (unsigned)(k + 2) * pl * 8u);
+                const double bv = bq[kk];
                 __syncthreads();
                 double xu = xp, xd = xm;
                 if (c.zbc != 0) {
@@ -467,14 +493,9 @@ __global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int g
                 r[kk] = rv;
                 cg_dot_add<MIR>(acc[0], rv, rv * d[kk], c, k);
                 acc[1] += (double)rv * (double)rv;
-                xm = x0;
-                x0 = xp;
-                xp = xpp;
                 eT[k][typ][txp] = rv;
             }
-            publish_staged(slot);  // its barrier ends every read of xT
-#pragma unroll
-            for (int kk = 0; kk < NH; ++kk) xT[k0 + kk][typ][txp] = 0.0f;
+            publish_staged(slot);
         }
         if (partials) {
             double t[3];
@@ -488,7 +509,6 @@ __global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int g
         if (!res_barrier(b.sync, b.err, q, G, sok)) break;
         ++q;
         if (!partials) {  // RESID continues with its second pass
-            ring_load(slot);
             resid_x = true;
             continue;
         }
@@ -506,7 +526,6 @@ __global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int g
             wg_total(a, sredB, tot);
         }
         ring_r = mode == 2;
-        if (mode == 2) hsel ^= 1;
         resid_x = false;
         if (mode == 2) tot[2] = 0.0;
         decide(tot, q >= qcap);
@@ -514,13 +533,7 @@ __global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int g
     // the solution's correction goes back; x_hi is already in memory
 #pragma unroll
     for (int kk = 0; kk < NH; ++kk) bstore(Rxl, col * 4u, (unsigned)(k0 + kk) * pl * 4u, xT[k0 + kk][ty][tx]);
-    if (hsel) {  // the last replacement left x_hi in the second copy
-#pragma unroll
-        for (int kk = 0; kk < NH; ++kk) {
-            const unsigned so = (unsigned)(k0 + kk) * pl * 8u;
-            bstore(Rxh, col * 8u, so, ld_coh_d(Rxh2, col * 8u, so));
-        }
-    }
+
     if (blk == 0 && tid == 0) {
         PcgState fin = load_state(&sst[dpar]);
         fin.passes = q;
