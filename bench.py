@@ -67,7 +67,7 @@ def run_gpu(args, inp, precision):
     prec = L.PREC_FP64 if precision == "fp64" else L.PREC_FP32_REFINED
     kz = args.tile_kz if (precision == "fp32" or args.tile_kz != 16) else 0
     film = H.Film(inp, precision=prec, pcg_rule=L.PCG_RULE_RELRES, tol_rel=1e-8, tol_abs=1e-30,
-                  refine_every=20, tile_kz=kz)
+                  refine_every=20, tile_kz=kz, cg_fused=args.cg_fused if precision == "fp32" else 1)
     Ez = inp.field_nd(H.triangle_wave(PERIOD, AMP, args.warmup + args.steps))
     film.run(Ez[: args.warmup])
     film.synchronize()
@@ -79,12 +79,52 @@ def run_gpu(args, inp, precision):
     # events on the film's stream
     film.set_timing(True)
     Ez2 = inp.field_nd(H.triangle_wave(PERIOD, AMP, args.timing_steps, start=args.warmup + args.steps))
-    film.run(Ez2)
+    its2 = film.run(Ez2)
     kt = film.kernel_times()
+    cnt = film.cg_counters()
     film.set_timing(False)
     form = film.cg_form
     film.close()
-    return t1 - t0, its, kt, form
+    return t1 - t0, its, kt, form, its2, cnt
+
+
+def cg_iteration_roofline(kt, its_timed, counters, form, N, precision="fp32"):
+    """Time and algorithmic bytes of one CG iteration of the FP32-refined
+    solve, amortizing every launch or pass of the solver class over the
+    iterations of the timing pass (residual replacements, re-formations and
+    the launches that only add a deferred x_lo term included).  The bytes are
+    those the iteration's vectors need when streamed from HBM (38 per voxel
+    for the carried single-reduction CG, 22 for its two-stencil form); the
+    resident form holds them in registers and LDS and streams only edge
+    columns and partial sums, so its fraction says how fast it does the
+    iteration relative to a kernel that would have to move them."""
+    if form == 3:
+        cls, iters, bpc = "pcg_resident", counters[1], 38.0
+    else:
+        cls, iters = "pcg_fused", int(np.sum(its_timed))
+        bpc = algorithmic_bytes_per_cell("pcg_fused", precision, 2 if form == 2 else 1)
+    c, ms = kt[cls]
+    ipl = iters / max(c, 1)  # iterations per launch
+    bytes_per_launch = bpc * N * ipl
+    achieved = bytes_per_launch / (ms * 1e-3) / 1e9
+    return {"kernel": cls, "achieved": achieved, "frac": achieved / HBM_PEAK_GBS, "bytes_per_cell": bpc,
+            "launch_ms": ms, "bytes_per_launch": bytes_per_launch, "iterations_per_launch": ipl,
+            "ms_per_iteration": ms / max(ipl, 1e-30), "launches_timed": c, "iterations_timed": iters}
+
+
+def measured_traffic(kernel, ipl):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE per dispatch, calibrated), rescaled from the
+    profiled run's iterations per dispatch to `ipl`; None without it."""
+    prof = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(prof):
+        return None
+    with open(prof) as fh:
+        pm = json.load(fh).get("fp32", {})
+    per_dispatch, its = pm.get(kernel), pm.get(kernel + "_iterations_per_dispatch")
+    if per_dispatch is None or not its:
+        return None
+    return per_dispatch / its * ipl
 
 
 def run_cpu_baseline(inp, Ez, threads):
@@ -339,19 +379,17 @@ def run_large(args):
         film.synchronize()
         dt = time.perf_counter() - t0
         film.set_timing(True)
-        film.run(Ez[W + S:])
+        its2 = film.run(Ez[W + S:])
         kt = film.kernel_times()
+        cnt = film.cg_counters()
         form, kz = film.cg_form, film.tile_depth
         film.close()
         entry = {"value": inp.N * S / dt / 1e6, "unit": "Mcell-steps/s", "ms_per_step": dt / S * 1e3,
                  "pcg_iters_mean": float(np.mean(its)), "kernel_ms": {k: v[1] for k, v in kt.items()},
                  "tile_kz": kz, "cg_form": form}
         if name == "fp32":
-            ms = kt["pcg_fused"][1]
-            bpc = algorithmic_bytes_per_cell("pcg_fused", "fp32", form)
-            achieved = bpc * inp.N / (ms * 1e-3) / 1e9
-            entry["roofline_pcg_fused"] = {"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                           "frac": achieved / HBM_PEAK_GBS, "bytes_per_cell": bpc}
+            rl = cg_iteration_roofline(kt, its2, cnt, form, inp.N)
+            entry["roofline_cg"] = dict(rl, peak=HBM_PEAK_GBS, unit="GB/s")
         out[name] = entry
     out["speedup_fp32_refined_vs_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
     return out
@@ -368,6 +406,8 @@ def main():
     ap.add_argument("--cpu-single-steps", type=int, default=1, help="the same on one core")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-kz", type=int, default=0, help="z planes per workgroup (0: automatic)")
+    ap.add_argument("--cg-fused", type=int, default=1,
+                    help="FP32-refined solver: 1 automatic, 2 carried, 3 two-stencil, 4 resident, 0 two-kernel")
     ap.add_argument("--adj-steps", type=int, default=500, help="steps of the reverse-mode workload")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--batch-films", type=int, default=64, help="films of the batched workload (0: skip it)")
@@ -414,25 +454,15 @@ def main():
     N = inp.N
     res = {}
     for prec in ("fp64", "fp32"):
-        dt, its, kt, form = run_gpu(args, inp, prec)
-        res[prec] = dict(dt=dt, its=its, kt=kt, form=form, value=N * args.steps / dt / 1e6)
+        dt, its, kt, form, its2, cnt = run_gpu(args, inp, prec)
+        res[prec] = dict(dt=dt, its=its, kt=kt, form=form, value=N * args.steps / dt / 1e6, its2=its2, cnt=cnt)
     head = res["fp32"]
     value = head["value"]
     kt = head["kt"]
-
-    # dominant kernel of the FP32-refined step: largest total time in the timing pass
-    tot = {k: c * ms for k, (c, ms) in kt.items() if c > 0}
-    dom = max(tot, key=tot.get)
-    cnt, ms = kt[dom]
-    bpc = algorithmic_bytes_per_cell(dom, "fp32", head["form"])
-    achieved = bpc * N / (ms * 1e-3) / 1e9
-
-    traffic = None
-    prof = os.path.join(ROOT, "profiles", "pmc_traffic.json")
-    if os.path.exists(prof):
-        with open(prof) as fh:
-            pm = json.load(fh)
-        traffic = pm.get("fp32", {}).get(dom)
+    rl = cg_iteration_roofline(kt, head["its2"], head["cnt"], head["form"], N)
+    bpc = rl["bytes_per_cell"]
+    achieved = rl["achieved"]
+    traffic = measured_traffic(rl["kernel"], rl["iterations_per_launch"])
 
     adj = None if args.no_adjoint else run_adjoint(args)
     loop = None if args.no_loop else run_loop_errors(inp)
@@ -457,7 +487,9 @@ def main():
         "config": {"workload": "256x256x32 film, 200 grains FE/AFE/DE 0.6/0.2/0.2, +-20% Landau, "
                                "triangle +-3 MV/cm / 2000 steps, PCG ||r||<=1e-8||b||",
                    "nx": NX, "ny": NY, "nz": NZ, "grains": NG,
-                   "precision": "FP32 stencil+CG vectors, FP64 inner products, FP64 residual every 20 its, single-reduction CG (one fused kernel per iteration)"},
+                   "precision": "FP32 stencil+CG vectors, FP64 inner products, FP64 residual every 20 its, "
+                                "single-reduction CG (cg_form 3: one register-resident launch per solve; "
+                                "1/2: one streaming launch per iteration)"},
         "pcg_iters_mean": float(np.mean(head["its"])),
         "fp64_plain": {"value": res["fp64"]["value"], "ms_per_step": res["fp64"]["dt"] / args.steps * 1e3,
                        "pcg_iters_mean": float(np.mean(res["fp64"]["its"])),
@@ -468,9 +500,15 @@ def main():
         "fp32_vs_fp64_loop": loop,
         "batched": bat,
         "large_film": large,
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": rl["kernel"], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "bytes_per_cell": bpc, "launch_ms": ms, "launches_timed": cnt},
+                     "bytes_per_cell": bpc, "bytes_per_launch": rl["bytes_per_launch"],
+                     "launch_ms": rl["launch_ms"], "iterations_per_launch": rl["iterations_per_launch"],
+                     "ms_per_iteration": rl["ms_per_iteration"],
+                     "launches_timed": rl["launches_timed"], "iterations_timed": rl["iterations_timed"],
+                     "per": "launch; algorithmic bytes = bytes_per_cell x voxels x CG iterations of the "
+                            "launch (replacement and re-formation passes amortized)"},
+        "cg_form": head["form"],
         "kernel_ms": {k: v[1] for k, v in kt.items()},
         "cpu_baseline": cpu,
     }
