@@ -1849,6 +1849,22 @@ void hzo_batch_destroy(hzo_batch* B) {
     delete B;
 }
 
+// Graph replay is refused while a rocprofiler-sdk tool is attached to the
+// process (rocprofv3 names its tool library in ROCP_TOOL_LIBRARIES): a
+// kernel trace of the batch with replay on ended in a segmentation fault
+// inside the runtime's graph launch, called from the tool's interception,
+// while the same replays run clean without a tool.  Plain launches (use_graphs
+// = 0) run the same kernels and are traced normally.
+static int refuse_traced_replay(const hzo_batch* B, const char* fn) {
+    const char* tool = std::getenv("ROCP_TOOL_LIBRARIES");
+    if (B->graphs && tool && *tool)
+        return set_error(HZO_ERR_ARG,
+                         "%s: hipGraph replay is refused under a profiler tool (ROCP_TOOL_LIBRARIES is set); "
+                         "call hzo_batch_config(batch, 0, chunk) for plain launches",
+                         fn);
+    return HZO_OK;
+}
+
 int hzo_batch_config(hzo_batch* B, int use_graphs, int chunk) {
     if (!B || chunk < 2 || (chunk & 1)) return set_error(HZO_ERR_ARG, "hzo_batch_config: chunk must be even and >= 2");
     B->graphs = use_graphs != 0;
@@ -1876,6 +1892,7 @@ int hzo_batch_shape(const hzo_batch* B, int out[3]) {
 
 int hzo_batch_run(hzo_batch* B, const double* Ez, int nsteps, int* iters, double* m_out) {
     if (!B || nsteps < 0 || (nsteps > 0 && !Ez)) return set_error(HZO_ERR_ARG, "hzo_batch_run: bad argument");
+    if (int rc = refuse_traced_replay(B, "hzo_batch_run")) return rc;
     HZO_HIP(hipSetDevice(B->device));
     for (hzo_ctx* x : B->films) HZO_HIP(hipStreamSynchronize(x->stream));
     for (hzo_batch::Group& G : B->groups) {
@@ -1887,6 +1904,7 @@ int hzo_batch_run(hzo_batch* B, const double* Ez, int nsteps, int* iters, double
 
 int hzo_batch_autotune(hzo_batch* B, const double* Ez, int reps, int* kz_out, double ms[12]) {
     if (!B || !Ez || reps < 1) return set_error(HZO_ERR_ARG, "hzo_batch_autotune: bad argument");
+    if (int rc = refuse_traced_replay(B, "hzo_batch_autotune")) return rc;
     HZO_HIP(hipSetDevice(B->device));
     for (hzo_ctx* x : B->films) HZO_HIP(hipStreamSynchronize(x->stream));
     // A trial step reads P[cur] and writes only P[cur^1], the potential and

@@ -151,3 +151,21 @@ def test_batch_rejects_fp64_and_two_kernel_films():
             H.Batch([bad])
     f64.close()
     two.close()
+
+
+def test_graph_replay_is_refused_under_a_profiler_tool(monkeypatch):
+    inps = _inputs()[:2]
+    Ez = _fields(inps, 2)
+    films = [_film(inp) for inp in inps]
+    batch = H.Batch(films, tile_kz=2, use_graphs=True, chunk=16)
+    monkeypatch.setenv("ROCP_TOOL_LIBRARIES", "librocprofiler-sdk-tool.so")
+    with pytest.raises(L.HzoError, match="hipGraph replay is refused"):
+        batch.run(Ez[:1])
+    # plain launches run under the tool, and the refused call left no step behind
+    L.call("hzo_batch_config", batch.h, 0, 16)
+    its = batch.run(Ez[:1])
+    assert its.shape == (1, 2) and np.all(its > 0)
+    monkeypatch.delenv("ROCP_TOOL_LIBRARIES")
+    batch.close()
+    for f in films:
+        f.close()
