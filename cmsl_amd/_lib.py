@@ -116,6 +116,7 @@ SIGNATURES = {
     "hzo_batch_destroy": (None, [_P]),
     "hzo_batch_config": (_I, [_P, _I, _I]),
     "hzo_batch_tile_depth": (_I, [_P]),
+    "hzo_batch_film_group": (_I, [_P, _I, _P]),
     "hzo_batch_set_shape": (_I, [_P, _I, _D]),
     "hzo_batch_shape": (_I, [_P, _P]),
     "hzo_batch_run": (_I, [_P, _P, _I, _P, _P]),

@@ -435,8 +435,9 @@ int follow_up_round(int& next, int last_iters) {
 
 // The resident solve after the setup: one cooperative launch runs every
 // iteration and residual replacement; the host waits for it once.
-int cgres_solve(hzo_ctx* x, const hzo::StepConsts& c, const hzo::CG32Bufs& cb, int& last_iters,
-                hzo_pcg_stats* stats) {
+// The resident solve's buffers for film x after its setup kernel (cb: the
+// film's CG buffers), with the film's own barrier words and timeout flag.
+hzo::ResBufs res_bufs(hzo_ctx* x, const hzo::CG32Bufs& cb) {
     hzo::ResBufs rb;
     rb.r0 = x->r32;
     rb.xlo = cb.xlo;
@@ -454,6 +455,22 @@ int cgres_solve(hzo_ctx* x, const hzo::StepConsts& c, const hzo::CG32Bufs& cb, i
     rb.part = x->res_part;
     rb.sync = x->res_sync;
     rb.err = reinterpret_cast<int*>(x->res_sync + hzo::RES_SYNC_WORDS);
+    return rb;
+}
+
+// Control-block bookkeeping after a resident solve of film x (its state in
+// x->st_host): a solve the kernel ended at its pass cap reports max_iter.
+void res_after(hzo_ctx* x) {
+    hzo::PcgState& s = *x->st_host;
+    if (!s.done) s.status = HZO_PCG_MAXITER;
+    x->res_counts[0] += 1;
+    x->res_counts[1] += s.iters;
+    x->res_counts[2] += s.passes;
+}
+
+int cgres_solve(hzo_ctx* x, const hzo::StepConsts& c, const hzo::CG32Bufs& cb, int& last_iters,
+                hzo_pcg_stats* stats) {
+    const hzo::ResBufs rb = res_bufs(x, cb);
     // counters and flag start at zero (a fill kernel, not a memset node, in
     // case the caller captures the stream)
     hzo::launch_fill_u32(x->stream, x->res_sync, hzo::RES_SYNC_WORDS + 64, 0u);
@@ -471,11 +488,8 @@ int cgres_solve(hzo_ctx* x, const hzo::StepConsts& c, const hzo::CG32Bufs& cb, i
     HZO_HIP(hipMemcpyAsync(x->res_err_host, rb.err, sizeof(int), hipMemcpyDeviceToHost, x->stream));
     HZO_HIP(hipStreamSynchronize(x->stream));
     if (*x->res_err_host) return set_error(HZO_ERR_HIP, "resident CG: a grid barrier timed out");
-    hzo::PcgState& s = *x->st_host;
-    if (!s.done) s.status = HZO_PCG_MAXITER;
-    x->res_counts[0] += 1;
-    x->res_counts[1] += s.iters;
-    x->res_counts[2] += s.passes;
+    res_after(x);
+    const hzo::PcgState& s = *x->st_host;
     last_iters = s.iters + 2 * s.refinements + 1;
     return copy_stats(s, stats);
 }
@@ -1434,6 +1448,13 @@ struct hzo_batch {
         int g_chunk = -1;
         int need = 16;  // CG launches the group's last solve needed
         bool mirrored = false;  // some film has Neumann electrodes
+        // a resident group: films of one depth whose tiles fit the device
+        // together, each stepped by its own TDGL and setup kernels and solved
+        // by one cooperative launch for all of them
+        bool res = false;
+        int res_nz = 0, res_tiles = 0;
+        hzo::ResFilm* dres = nullptr;  // per-film descriptors (device)
+        unsigned* rsync = nullptr;     // per-film barrier words, then one timeout flag
     };
     std::vector<hzo_ctx*> films;
     int n = 0;
@@ -1471,8 +1492,12 @@ void group_free(hzo_batch::Group& G) {
     G.diag_part.clear();
     (void)hipFree(G.dfilms);
     (void)hipFree(G.dtiles);
+    (void)hipFree(G.dres);
+    (void)hipFree(G.rsync);
     G.dfilms = nullptr;
     G.dtiles = nullptr;
+    G.dres = nullptr;
+    G.rsync = nullptr;
 }
 
 void batch_free_groups(hzo_batch* B) {
@@ -1504,6 +1529,33 @@ hzo::Fp32Bufs film_bufs32(hzo_ctx* x, int cur) {
 // A group of one film needs none of them: it runs the single-film kernels.
 int group_tables(hzo_batch* B, hzo_batch::Group& G, int kz) {
     const int n = (int)G.idx.size();
+    if (G.res) {
+        std::vector<hzo::ResFilm> rf(n);
+        const size_t words = (size_t)n * hzo::RES_SYNC_WORDS + 64;
+        HZO_HIP(hipMalloc((void**)&G.rsync, sizeof(unsigned) * words));
+        int blk0 = 0;
+        G.mirrored = false;
+        for (int j = 0; j < n; ++j) {
+            hzo_ctx* x = B->films[G.idx[j]];
+            hzo::Fp32Bufs b32 = bufs32(x);
+            b32.r = x->r32;
+            rf[j].b = res_bufs(x, cg_bufs(x, b32));
+            rf[j].b.sync = G.rsync + (size_t)j * hzo::RES_SYNC_WORDS;
+            rf[j].b.err = reinterpret_cast<int*>(G.rsync + (size_t)n * hzo::RES_SYNC_WORDS);
+            rf[j].c = x->c;
+            rf[j].gxt = x->res_gxt;
+            rf[j].G = x->res_G;
+            rf[j].blk0 = blk0;
+            rf[j].pad = 0;
+            blk0 += x->res_G;
+            G.mirrored = G.mirrored || x->c.zbc != 0;
+        }
+        G.res_nz = B->films[G.idx[0]]->cfg.nz;
+        G.res_tiles = blk0;
+        HZO_HIP(hipMalloc((void**)&G.dres, sizeof(hzo::ResFilm) * n));
+        HZO_HIP(hipMemcpy(G.dres, rf.data(), sizeof(hzo::ResFilm) * n, hipMemcpyHostToDevice));
+        return HZO_OK;
+    }
     if (n == 1) {
         G.single = G.idx[0];
         G.ntiles = B->films[G.single]->nb;
@@ -1592,8 +1644,43 @@ int batch_tables(hzo_batch* B, int kz, double budget) {
     batch_free_groups(B);
     B->kz = kz;
     B->budget = budget;
-    std::vector<int> order(B->n);
-    for (int f = 0; f < B->n; ++f) order[f] = f;
+    // Films that take the resident solve on their own are packed, by depth
+    // and first-fit decreasing on tiles, into groups that fit the device at
+    // once (the Neumann build's capacity when a group may hold such a film);
+    // the others go by CG working set as below.
+    std::vector<int> order;
+    std::vector<std::vector<int>> rgroups;
+    std::vector<int> rnz, rload;
+    {
+        std::vector<int> rfilms;
+        for (int f = 0; f < B->n; ++f) {
+            if (B->films[f]->cg_res) rfilms.push_back(f);
+            else order.push_back(f);
+        }
+        std::stable_sort(rfilms.begin(), rfilms.end(),
+                         [&](int a, int b) { return B->films[a]->res_G > B->films[b]->res_G; });
+        for (int f : rfilms) {
+            hzo_ctx* x = B->films[f];
+            const int nz = x->cfg.nz;
+            const int cap = std::min(hzo::cgres_capacity(B->device, nz, false), hzo::cgres_capacity(B->device, nz, true));
+            size_t g = 0;
+            while (g < rgroups.size() && (rnz[g] != nz || rload[g] + x->res_G > cap)) ++g;
+            if (g == rgroups.size()) {
+                rgroups.emplace_back();
+                rnz.push_back(nz);
+                rload.push_back(0);
+            }
+            rgroups[g].push_back(f);
+            rload[g] += x->res_G;
+        }
+    }
+    for (std::vector<int>& idx : rgroups) {
+        B->groups.emplace_back();
+        std::sort(idx.begin(), idx.end());
+        B->groups.back().idx = idx;
+        B->groups.back().res = true;
+    }
+    const size_t nres = B->groups.size();
     auto bytes = [&](int f) { return CG_SET_BYTES * (double)B->films[f]->g.N; };
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return bytes(a) > bytes(b); });
     std::vector<double> load;
@@ -1606,7 +1693,7 @@ int batch_tables(hzo_batch* B, int kz, double budget) {
             B->groups.emplace_back();
         }
         load[g] += bytes(f);
-        B->groups[g].idx.push_back(f);
+        B->groups[nres + g].idx.push_back(f);
     }
     for (hzo_batch::Group& G : B->groups) {
         std::sort(G.idx.begin(), G.idx.end());
@@ -1619,7 +1706,7 @@ int batch_tables(hzo_batch* B, int kz, double budget) {
 // Re-reads each film's P parity into the group's descriptor table (contexts
 // may have been stepped on their own since the last batched run).
 int group_sync_parity(hzo_batch* B, hzo_batch::Group& G) {
-    if (G.single >= 0) return HZO_OK;
+    if (G.single >= 0 || G.res) return HZO_OK;
     for (size_t j = 0; j < G.idx.size(); ++j) {
         hzo_ctx* x = B->films[G.idx[j]];
         hzo::BatchFilm& F = G.hf[j];
@@ -1700,7 +1787,56 @@ int group_poll(hzo_batch* B, hzo_batch::Group& G) {
 // launch index t restarts at 1 in each chunk (chunks are even, so the
 // control-block parity continues).  hinfo[BATCH_INFO*j..] receives film j's solver
 // statistics.
+int group_step_res(hzo_batch* B, hzo_batch::Group& G, const double* Ez, int s, bool dep, int& launches) {
+    const int n = (int)G.idx.size();
+    for (int f : G.idx) {
+        hzo_ctx* x = B->films[f];
+        hzo::launch_tdgl_f32(B->stream, x->g, x->c, Ez[(size_t)s * B->n + f], bufs32(x));
+        x->cur ^= 1;
+    }
+    HZO_HIP(hipGetLastError());
+    launches = 0;
+    if (!dep) {
+        memset(B->hinfo, 0, sizeof(int) * hzo::BATCH_INFO * n);
+        return HZO_OK;
+    }
+    for (int f : G.idx) {
+        hzo_ctx* x = B->films[f];
+        hzo::launch_pcg_setup_f32(B->stream, x->g, x->c, bufs32(x), pcg_init(x), nullptr);
+        fold_after(x, B->stream, 0);
+    }
+    // barrier words and the timeout flag start at zero (a fill kernel, not a
+    // memset node, in case the caller captures the stream)
+    hzo::launch_fill_u32(B->stream, G.rsync, n * hzo::RES_SYNC_WORDS + 64, 0u);
+    HZO_HIP(hipGetLastError());
+    const hipError_t le = hzo::launch_cgres_films(B->stream, G.dres, n, G.res_tiles, G.res_nz, G.mirrored);
+    if (le != hipSuccess) return set_error(HZO_ERR_HIP, "resident CG launch failed: %s", hipGetErrorString(le));
+    launches = 1;
+    hzo_ctx* x0 = B->films[G.idx[0]];
+    for (int f : G.idx) {
+        hzo_ctx* x = B->films[f];
+        HZO_HIP(hipMemcpyAsync(x->st_host, x->st, sizeof(hzo::PcgState), hipMemcpyDeviceToHost, B->stream));
+    }
+    HZO_HIP(hipMemcpyAsync(x0->res_err_host, G.rsync + (size_t)n * hzo::RES_SYNC_WORDS, sizeof(int),
+                           hipMemcpyDeviceToHost, B->stream));
+    HZO_HIP(hipStreamSynchronize(B->stream));
+    if (*x0->res_err_host) return set_error(HZO_ERR_HIP, "resident CG: a grid barrier timed out");
+    for (int j = 0; j < n; ++j) {
+        hzo_ctx* x = B->films[G.idx[j]];
+        res_after(x);
+        const hzo::PcgState& st = *x->st_host;
+        int* h = B->hinfo + hzo::BATCH_INFO * j;
+        h[0] = st.done;
+        h[1] = st.iters;
+        h[2] = st.status;
+        h[3] = st.refinements;
+        h[4] = st.xadds;
+    }
+    return HZO_OK;
+}
+
 int group_step(hzo_batch* B, hzo_batch::Group& G, const double* Ez, int s, bool dep, int& launches) {
+    if (G.res) return group_step_res(B, G, Ez, s, dep, launches);
     const int n = (int)G.idx.size();
     if (G.single >= 0) {
         hzo_ctx* x = B->films[G.single];
@@ -1776,10 +1912,12 @@ int group_run(hzo_batch* B, hzo_batch::Group& G, const double* Ez, int nsteps, i
             B->unconverged += dep && !B->hinfo[hzo::BATCH_INFO * j];
         }
         if (m_out) {
-            if (G.single >= 0) {
-                hzo_ctx* x = B->films[G.single];
-                hzo::launch_diag_f32(B->stream, x->g, x->P32[x->cur], x->cw, x->diag_part);
-                hzo::launch_sum_column(B->stream, x->g, x->diag_part, B->dm);
+            if (G.single >= 0 || G.res) {
+                for (int j = 0; j < n; ++j) {
+                    hzo_ctx* x = B->films[G.idx[j]];
+                    hzo::launch_diag_f32(B->stream, x->g, x->P32[x->cur], x->cw, x->diag_part);
+                    hzo::launch_sum_column(B->stream, x->g, x->diag_part, B->dm + j);
+                }
             } else {
                 hzo::launch_diag_f32_batch(B->stream, G.ntiles, G.dfilms, G.dtiles, (s + 1) & 1);
                 hzo::launch_batch_sum_pmz(B->stream, G.dfilms, n, B->dm);
@@ -1792,7 +1930,7 @@ int group_run(hzo_batch* B, hzo_batch::Group& G, const double* Ez, int nsteps, i
         }
     }
     HZO_HIP(hipStreamSynchronize(B->stream));
-    if (G.single < 0)
+    if (G.single < 0 && !G.res)
         for (int f : G.idx) B->films[f]->cur = (B->films[f]->cur + nsteps) & 1;
     return HZO_OK;
 }
@@ -1873,6 +2011,21 @@ int hzo_batch_config(hzo_batch* B, int use_graphs, int chunk) {
 }
 
 int hzo_batch_tile_depth(hzo_batch* B) { return B ? B->kz : 0; }
+
+int hzo_batch_film_group(const hzo_batch* B, int film, int out[3]) {
+    if (!B || !out || film < 0 || film >= B->n) return set_error(HZO_ERR_ARG, "hzo_batch_film_group: bad argument");
+    for (size_t g = 0; g < B->groups.size(); ++g) {
+        const hzo_batch::Group& G = B->groups[g];
+        for (int f : G.idx) {
+            if (f != film) continue;
+            out[0] = (int)g;
+            out[1] = (int)G.idx.size();
+            out[2] = G.res ? 2 : (G.single >= 0 ? 1 : 0);
+            return HZO_OK;
+        }
+    }
+    return set_error(HZO_ERR_STATE, "hzo_batch_film_group: film %d is in no group", film);
+}
 
 int hzo_batch_set_shape(hzo_batch* B, int tile_kz, double group_mib) {
     if (!B || (tile_kz != 2 && tile_kz != 4 && tile_kz != 8) || !(group_mib >= 0.0))

@@ -195,6 +195,14 @@ struct ResBufs {
 };
 constexpr int RES_SYNC_WORDS = 64 * 9;
 hipError_t launch_cgres(hipStream_t s, const StepConsts& c, const ResBufs& b, int gxt, int G);
+// One film of a multi-film resident launch: its buffers and constants, tiles
+// in x, tiles, and the first workgroup of its block range.
+struct ResFilm {
+    ResBufs b;
+    StepConsts c;
+    int gxt, G, blk0, pad;
+};
+hipError_t launch_cgres_films(hipStream_t s, const ResFilm* films, int nf, int tiles, int nz, bool mir);
 int cgres_capacity(int device, int nz, bool mir);
 void launch_fill_u32(hipStream_t s, unsigned* p, int n, unsigned v);
 

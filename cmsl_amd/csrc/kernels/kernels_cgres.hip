@@ -58,16 +58,18 @@ __device__ __forceinline__ double ld_coh_d(rsrc_t R, unsigned voff, unsigned sof
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(R, (int)voff, (int)soff, LD_COH));
 }
 
-// Grid barrier number q (0, 1, ...) of G workgroups.  The caller's stores
-// have all been issued; each wave waits for them to complete before the
-// workgroup arrives.  Returns false after a timeout (the error flag is set).
-__device__ __forceinline__ bool res_barrier(unsigned* sync, int* err, int q, int G, int& sok) {
+// Grid barrier number q (0, 1, ...) of the G workgroups blk0 .. blk0 + G - 1
+// of the launch.  The caller's stores have all been issued; each wave waits
+// for them to complete before the workgroup arrives.  Returns false after a
+// timeout (the error flag is set).
+__device__ __forceinline__ bool res_barrier(unsigned* sync, int* err, int q, int G, int blk0, int& sok) {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
-        const int ngrp = G < N_XCD ? G : N_XCD;
+        const int ngrp = G < N_XCD ? G : N_XCD;  // G consecutive workgroups cover min(G, 8) XCDs
         const int xg = blockIdx.x % N_XCD;
-        const unsigned nxg = (unsigned)((G - xg + N_XCD - 1) / N_XCD);  // workgroups of this group
+        // workgroups of the range on this XCD: those of [0, blk0 + G) less those of [0, blk0)
+        const unsigned nxg = (unsigned)((blk0 + G - xg + N_XCD - 1) / N_XCD - (blk0 - xg + N_XCD - 1) / N_XCD);
         const unsigned old = __hip_atomic_fetch_add(sync + 64 * (1 + xg), 1u, __ATOMIC_RELAXED,
                                                     __HIP_MEMORY_SCOPE_AGENT);
         if (old + 1u == (unsigned)(q + 1) * nxg)
@@ -107,10 +109,14 @@ __device__ __forceinline__ int res_advance(PcgState& s, int mode) {
 
 }  // namespace
 
-// NZ = nz of the film (each thread holds NZ/2 planes of a column); MIR:
-// Neumann electrodes (mirrored-plane weights in the inner products).
+// The solve of one film by the G workgroups blk0 .. blk0 + G - 1 of the
+// launch; this workgroup owns tile blk (0 .. G - 1).  NZ = nz of the film
+// (each thread holds NZ/2 planes of a column); MIR: the build that weights
+// mirrored planes in the inner products (Neumann electrodes; exact weight 1
+// on Dirichlet films).
 template <int NZ, bool MIR>
-__global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int gxt, int G) {
+__device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c, int gxt, int G, int blk,
+                                           int blk0) {
     static_assert(NZ % 8 == 0, "ring cells are spread over the workgroup in whole rounds");
     constexpr int NH = NZ / 2;       // planes per thread
     constexpr int NR = 4 * NZ * RT;  // ring cells: 4 sides x NZ planes x 16
@@ -132,7 +138,7 @@ __global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int g
     // a wave lies in one z half: its first plane is wave-uniform, so plane
     // offsets can go in the buffer instructions' scalar offset
     const int wv = tid >> 6, k0 = __builtin_amdgcn_readfirstlane(tz * NH);
-    const int blk = blockIdx.x, bxt = blk % gxt, byt = blk / gxt, gyt = G / gxt;
+    const int bxt = blk % gxt, byt = blk / gxt, gyt = G / gxt;
     const int nx = c.nx, ny = c.ny;
     const unsigned pl = c.plane, nv = c.nvox;
     const int i0 = bxt * RT, j0 = byt * RT;
@@ -506,7 +512,7 @@ __global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int g
                     bstore(RP, ((slot * (unsigned)G + (unsigned)blk) * 4u + (unsigned)qq) * 8u, 0u, t[qq]);
             }
         }
-        if (!res_barrier(b.sync, b.err, q, G, sok)) break;
+        if (!res_barrier(b.sync, b.err, q, G, blk0, sok)) break;
         ++q;
         if (!partials) {  // RESID continues with its second pass
             resid_x = true;
@@ -541,6 +547,24 @@ __global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int g
     }
 }
 
+template <int NZ, bool MIR>
+__global__ __launch_bounds__(RNT, 1) void k_cgres(ResBufs b, StepConsts c, int gxt, int G) {
+    cgres_body<NZ, MIR>(b, c, gxt, G, blockIdx.x, 0);
+}
+
+// Several films of one depth in one launch: each takes the block range
+// films[f].blk0 .. + G - 1 (ascending, contiguous) and runs its own solve,
+// with its own barriers, partial sums and control block.
+template <int NZ, bool MIR>
+__global__ __launch_bounds__(RNT, 1) void k_cgres_films(const ResFilm* __restrict__ films, int nf) {
+    int f = 0;
+    while (f + 1 < nf && films[f + 1].blk0 <= (int)blockIdx.x) ++f;
+    const ResBufs b = films[f].b;
+    const StepConsts c = films[f].c;
+    const int blk0 = films[f].blk0;
+    cgres_body<NZ, MIR>(b, c, films[f].gxt, films[f].G, (int)blockIdx.x - blk0, blk0);
+}
+
 __global__ void k_fill_u32(unsigned* p, int n, unsigned v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -563,6 +587,21 @@ hipError_t launch_cgres(hipStream_t s, const StepConsts& c, const ResBufs& b, in
         default: return hipErrorInvalidValue;
     }
     return hipLaunchCooperativeKernel(fn, dim3(G), dim3(RNT), args, 0, s);
+}
+
+// The films' resident solves in one cooperative launch of their total tiles
+// (films on device, nf of them, every one of depth nz; mir: some film has
+// Neumann electrodes).
+hipError_t launch_cgres_films(hipStream_t s, const ResFilm* films, int nf, int tiles, int nz, bool mir) {
+    void* args[] = {(void*)&films, (void*)&nf};
+    const void* fn = nullptr;
+    switch (nz) {
+        case 16: fn = mir ? (const void*)k_cgres_films<16, true> : (const void*)k_cgres_films<16, false>; break;
+        case 24: fn = mir ? (const void*)k_cgres_films<24, true> : (const void*)k_cgres_films<24, false>; break;
+        case 32: fn = mir ? (const void*)k_cgres_films<32, true> : (const void*)k_cgres_films<32, false>; break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipLaunchCooperativeKernel(fn, dim3(tiles), dim3(RNT), args, 0, s);
 }
 
 // Workgroups of the resident kernel one device can hold at once (0 if the

@@ -534,6 +534,14 @@ class Batch:
         """Tile depth (2, 4 or 8) and group budget in MiB of CG working set (0: one group)."""
         L.call("hzo_batch_set_shape", self.h, int(tile_kz), float(group_mib))
 
+    def film_group(self, f: int):
+        """(group index, films in the group, kind) of film f; kind 0: batch
+        kernels over several films, 1: the film's own kernels, 2: the
+        resident solve, one cooperative launch for the group."""
+        out = np.zeros(3, np.int32)
+        L.call("hzo_batch_film_group", self.h, int(f), L.ptr(out))
+        return int(out[0]), int(out[1]), int(out[2])
+
     @property
     def shape(self):
         out = np.zeros(3, np.int32)

@@ -254,12 +254,18 @@ int hzo_checkpoint_load(hzo_ctx* ctx, const char* path, int* peloop, int* count,
  * reference, which steps one film per process (evolve.cu:220-237 in a loop
  * over decks); it serves parameter sweeps over many microstructures.
  *
- * The films must be FP32-refined (HZO_PREC_FP32_REFINED, cg_fused = 1), on
+ * The films must be FP32-refined (HZO_PREC_FP32_REFINED, cg_fused != 0), on
  * one device, with the same dep_flag, and ready (microstructure, phase table
  * and state set).  A batch holds the films' device buffers by reference: the
- * films must outlive it and must not be stepped concurrently with it.  Each
- * film ends a batched run in exactly the state its own hzo_run leaves at the
- * same tile depth. */
+ * films must outlive it and must not be stepped concurrently with it.
+ * Films whose own solve is the register-resident one (hzo_cg_form 3) are
+ * packed by depth into groups that fit the device together, and each such
+ * group's solves run as one cooperative launch; each of these films ends a
+ * batched run in exactly the state its own hzo_run leaves.  The other films
+ * go into groups by CG working set: a film alone in its group runs its own
+ * kernels, the films of a larger group run the batch kernels at the batch's
+ * tile depth, and each ends in exactly the state its own hzo_run leaves at
+ * that tile depth. */
 typedef struct hzo_batch hzo_batch;
 /* tile_kz: 2, 4 or 8 (planes per workgroup tile); all films in one group. */
 int hzo_batch_create(hzo_ctx* const* films, int n, int tile_kz, hzo_batch** out);
@@ -273,6 +279,10 @@ int hzo_batch_set_shape(hzo_batch* batch, int tile_kz, double group_mib);
 /* out[3] = tile depth, group budget (MiB), number of groups. */
 int hzo_batch_shape(const hzo_batch* batch, int out[3]);
 int hzo_batch_tile_depth(hzo_batch* batch);
+/* out[3] = group index, films in that group, and how the group runs: 0 batch
+ * kernels over several films, 1 the film's own kernels, 2 the resident solve
+ * (one cooperative launch for the group). */
+int hzo_batch_film_group(const hzo_batch* batch, int film, int out[3]);
 /* nsteps steps; Ez[s*n + f] is the applied field of film f at step s.
  * iters[s*n + f] (may be NULL): PCG iterations of film f at step s.
  * m_out[s*n + f] (may be NULL): <Pmz> of film f after step s. */

@@ -169,3 +169,41 @@ def test_graph_replay_is_refused_under_a_profiler_tool(monkeypatch):
     batch.close()
     for f in films:
         f.close()
+
+
+def test_resident_groups_match_their_own_runs():
+    # films at automatic tile depth whose shapes the resident solve covers
+    # share one cooperative launch per depth (the nz 16 group holds a Neumann
+    # film, so it runs the build with mirrored-plane weights, exactly 1 on the
+    # Dirichlet films); the 40-wide film is not covered and runs its own
+    # streaming kernels; every film ends bit-identical to its own run
+    shapes = [(64, 48, 16, 12), (32, 32, 16, 5), (48, 32, 16, 7), (32, 48, 24, 6), (40, 72, 12, 9)]
+    zbc = [L.ZBC_DIRICHLET, L.ZBC_NEUMANN, L.ZBC_DIRICHLET, L.ZBC_DIRICHLET, L.ZBC_DIRICHLET]
+    inps = [film_inputs(nx, ny, nz, ng, fracs=(0.3, 0.5, 0.2), seed=41 + f, jitter=0.2)
+            for f, (nx, ny, nz, ng) in enumerate(shapes)]
+    S = 8
+    amps = [3.0, 2.0, 2.5, 1.5, 2.2]
+    Ez = np.stack([inp.field_nd(H.triangle_wave(S, amps[f], S)) for f, inp in enumerate(inps)], axis=1)
+    _say("resident groups: 3 films of depth 16 (one Neumann), 1 of depth 24, 1 streaming film,", S, "steps")
+    films = [_film(inp, 0, z) for inp, z in zip(inps, zbc)]
+    assert [f.cg_form for f in films] == [3, 3, 3, 3, 1]
+    batch = H.Batch(films, tile_kz=2)
+    groups = [batch.film_group(f) for f in range(len(films))]
+    _say("groups", groups)
+    assert [g[1:] for g in groups] == [(3, 2), (3, 2), (3, 2), (1, 2), (1, 1)]
+    assert groups[0][0] == groups[1][0] == groups[2][0] != groups[3][0]
+    its_b, m_b = batch.run(Ez, want_m=True)
+    assert batch.counters()["unconverged"] == 0
+    for f, inp in enumerate(inps):
+        single = _film(inp, 0, zbc[f])
+        its_s = single.run(Ez[:, f])
+        Pb, phib, _ = films[f].get_state()
+        Ps, phis, _ = single.get_state()
+        assert np.array_equal(its_b[:, f], its_s), (f, its_b[:, f], its_s)
+        assert np.array_equal(Pb, Ps) and np.array_equal(phib, phis), f
+        d = single.diagnostics()
+        assert abs(m_b[-1, f] - d[0] / inp.N) <= 1e-12 * max(1.0, abs(d[0] / inp.N))
+        single.close()
+    batch.close()
+    for f in films:
+        f.close()

@@ -85,7 +85,6 @@ def test_batched_64_films_converge_and_match_their_own_runs():
     cnt = batch.counters()
     _say("shape", batch.shape, "counters", cnt, "iterations min/max", int(its.min()), int(its.max()))
     assert cnt["unconverged"] == 0 and np.all(its > 0)
-    batch.close()
     order = np.argsort([inp.N for inp in inps])
     sample = [int(order[0]), int(order[1]), int(order[-1]), int(order[len(order) // 2])]
     for f in sample:
@@ -93,10 +92,20 @@ def test_batched_64_films_converge_and_match_their_own_runs():
         rr = _true_relres(oracle_film(inps[f]), P, phi)
         _say("film", f, specs[f], "true relative residual", rr)
         assert rr <= 1.0e-8 * 1.001
-    # the two smallest films share multi-film groups (tile depth 8); the
-    # largest exceeds the group budget and runs its own kernels at its own
-    # automatic tile depth
-    for f, kz in ((int(order[0]), 8), (int(order[1]), 8), (int(order[-1]), 0)):
+    # each film ends as its own run: at automatic tile depth where it runs
+    # its own kernels or the resident solve (groups of kind 1 and 2), at the
+    # batch's depth 8 in a multi-film streaming group (kind 0); one film of
+    # each kind is checked, the smallest of it, and the largest film
+    kinds = [batch.film_group(f)[2] for f in range(len(films))]
+    _say("films per group kind", [kinds.count(k) for k in (0, 1, 2)])
+    assert set(kinds) == {0, 1, 2}
+    picks = {}
+    for f in [int(i) for i in order]:
+        picks.setdefault(kinds[f], f)
+    picks.setdefault("largest", int(order[-1]))
+    batch.close()
+    for f in picks.values():
+        kz = 8 if kinds[f] == 0 else 0
         single = H.Film(inps[f], precision=L.PREC_FP32_REFINED, tile_kz=kz, **RELRES)
         its_s = single.run(Ez[:, f])
         Pb, phib, _ = films[f].get_state()
