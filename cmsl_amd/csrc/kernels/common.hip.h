@@ -155,6 +155,12 @@ __device__ __forceinline__ T op_identity(T xc, int k, const StepConsts& c) {
 // weighted sum and no product is held beside it.  r.r and b.b, which the
 // relative-residual rule reads, stay unweighted.  MIR = false compiles the
 // plain sum, for kernels launched only on films with Dirichlet electrodes.
+// v as the wave's first active lane holds it: for values every lane
+// computes alike, which the compiler then keeps in a scalar register.
+__device__ __forceinline__ float uniform_f(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+
 template <bool MIR>
 __device__ __forceinline__ void cg_dot_add(double& acc, float x, float y, const StepConsts& c, int k) {
     if constexpr (MIR) {

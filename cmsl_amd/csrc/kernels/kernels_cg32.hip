@@ -274,7 +274,9 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
     double tot[3];
     wave_lds_total<3>(sred, tot);
     double alpha_d = 0.0, beta_d = 0.0;
-    const int mode = cg_advance(s, cg_decide(s, tot, alpha_d, beta_d), xm, alpha_d);
+    // every lane holds the same decision; made wave-uniform, the branches on
+    // it are scalar and alpha, beta sit in scalar registers
+    const int mode = __builtin_amdgcn_readfirstlane(cg_advance(s, cg_decide(s, tot, alpha_d, beta_d), xm, alpha_d));
     // handed on at once: nothing later in the launch reads the block, so it
     // need not stay live in registers through the tile work
     pass_state(s, b.st + io, T.lead);
@@ -299,8 +301,8 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
         // streamed bottom to top: plane q's update and u, then the stencil of
         // plane q - 1, whose z neighbours are u of planes q - 2 and q.
         const bool iter = mode == 0;
-        const float alpha = (float)alpha_d, beta = (float)beta_d;
-        const bool fresh = beta_d == 0.0;  // first iteration of a solve: no previous p, s
+        const float alpha = uniform_f((float)alpha_d), beta = uniform_f((float)beta_d);
+        const bool fresh = __builtin_amdgcn_readfirstlane(beta_d == 0.0 ? 1 : 0) != 0;  // first iteration: no previous p, s
         const unsigned o_r2 = (cur ^ 1u) * seg, o_s2 = (cur ^ 1u) * seg;
         // w goes to the other buffer after an iteration, in place after WCOMP
         const unsigned o_wo = iter ? (3u - cur) * seg : o_w;
@@ -480,13 +482,19 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
     const HPair<float> hpv = load_hpair(b.lut32);
     Partials<3, PM> P;
     load_partials<3, PM>(ip ? b.pin[1] : b.pin[0], b.nb_in, b.ps_in, P);
-    // plane offsets; stencil codes (centre | ring << 16), each loaded two
-    // phases before its plane's data, whose D^-1 gathers need it
-    unsigned so[NW], cd[NW];
-#pragma unroll
-    for (int m = 0; m < NW; ++m) so[m] = (unsigned)min(max(k0 - 2 + m, 0), nz - 1);
+    // Plane z of window plane m, clamped into the film.  It is formed from a
+    // copy of k0 that each phase of the plane loop renews (an empty asm that
+    // claims to change it), so each plane offset is computed where it is
+    // used: formed all at once, the 36 offsets of a depth-32 window and their
+    // products were held in scalar registers, which spilled (125 of them at
+    // depth 32) and were read back with a VALU instruction per use.
+    int k0p = k0;
+    auto so = [&](int m) { return (unsigned)min(max(k0p - 2 + m, 0), nz - 1); };
+    // stencil codes (centre | ring << 16), each loaded two phases before its
+    // plane's data, whose D^-1 gathers need it
+    unsigned cd[NW];
     auto load_code = [&](int m) {
-        cd[m] = bload_code(Rsc, G.cofs * 2u, so[m] * pb2) | (bload_code(Rsc, rofs * 2u, so[m] * pb2) << 16);
+        cd[m] = bload_code(Rsc, G.cofs * 2u, so(m) * pb2) | (bload_code(Rsc, rofs * 2u, so(m) * pb2) << 16);
     };
 #pragma unroll
     for (int m = 0; m < 4; ++m) load_code(m);
@@ -526,7 +534,7 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
 
     Plane2 L[NW];
     auto issue = [&](int m) {
-        const unsigned sv = so[m] * pb4;
+        const unsigned sv = so(m) * pb4;
         L[m].r = bload<float>(RA, cv, o_r + sv);
         L[m].d = bload<float>(Rdi, (cd[m] & 0xFFFFu) * 4u, 0u);
         L[m].rh = bload<float>(RA, rv, o_r + sv);
@@ -544,7 +552,9 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
     double tot[3];
     wave_lds_total<3>(sred, tot);
     double alpha_d = 0.0, beta_d = 0.0;
-    const int mode = cg_advance(s, cg_decide(s, tot, alpha_d, beta_d), xm, alpha_d);
+    // every lane holds the same decision; made wave-uniform, the branches on
+    // it are scalar and alpha, beta sit in scalar registers
+    const int mode = __builtin_amdgcn_readfirstlane(cg_advance(s, cg_decide(s, tot, alpha_d, beta_d), xm, alpha_d));
     pass_state(s, b.st + io, T.lead);
     if (mode == 1) return;
     double acc[3] = {0.0, 0.0, 0.0};
@@ -555,14 +565,14 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
 #pragma unroll
         for (int m = 2; m <= KZ + 1; ++m) {
             const int k = k0 - 2 + m;
-            const unsigned sv = so[m] * pb4;
+            const unsigned sv = so(m) * pb4;
             const float xv = bload<float>(RX, cv, sv), pv = bload<float>(RB, cv, o_p + sv);
             vst(RX, (tp.live && k < nz) ? cv : aend, sv, xv + aprev * pv);
         }
     } else {
         const bool iter = mode == 0;
-        const float alpha = (float)alpha_d, beta = (float)beta_d;
-        const bool fresh = beta_d == 0.0;
+        const float alpha = uniform_f((float)alpha_d), beta = uniform_f((float)beta_d);
+        const bool fresh = __builtin_amdgcn_readfirstlane(beta_d == 0.0 ? 1 : 0) != 0;
         float pc[NW], ph[NW], uc[NW];
         // A(m): p of plane m (tile and two-voxel ring) into LDS; on the tile
         // p_i and x_lo += alpha p_i are stored (WCOMP: x_lo folded instead)
@@ -577,7 +587,7 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
                 sp[m & 1][ty + 2][tx + 2] = pc[m];
                 if (R.has) sp[m & 1][R.ly + 2][R.lx + 2] = ph[m];
                 if (tile_plane) {
-                    const unsigned vo = ok ? cv : aend, sv = so[m] * pb4;
+                    const unsigned vo = ok ? cv : aend, sv = so(m) * pb4;
                     vst(RB, vo, o_pn + sv, pc[m]);
                     const float xa = xm == 2 ? L[m].x + aprev * L[m].p : L[m].x;
                     vst(RX, xm != 0 ? vo : aend, sv, xa + alpha * pc[m]);
@@ -620,7 +630,7 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
             }
             if (m >= 2 && m <= KZ + 1) {
                 const bool ok = tp.live && k < nz;
-                if (iter) vst(RA, ok ? cv : aend, o_rn + so[m] * pb4, rc);
+                if (iter) vst(RA, ok ? cv : aend, o_rn + so(m) * pb4, rc);
                 if (ok) {
                     cg_dot_add<true>(acc[0], rc, uc[m], c, k);
                     acc[2] += (double)rc * (double)rc;
@@ -648,6 +658,8 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
         // plane's LDS buffer is rewritten two phases after it was filled.
 #pragma unroll
         for (int n = 0; n < NW; ++n) {
+            k0p = k0;
+            asm volatile("" : "+s"(k0p));
             if (n + 4 < NW) load_code(n + 4);
             if (n + 2 < NW) issue(n + 2);
             stageA(n);
