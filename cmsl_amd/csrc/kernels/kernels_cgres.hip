@@ -168,15 +168,18 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
     auto xdoff = [&](int tile, int side, int k, int pos) {
         return ((((unsigned)tile * 4u + (unsigned)side) * NZ + (unsigned)k) * RT + (unsigned)pos) * 8u;
     };
-    // ring cell e = (side * NZ + k) * 16 + pos: its LDS place and film column
-    auto ring_lds = [&](int e, int& k, int& ly, int& lx) {
-        const int side = e / (NZ * RT), pos = e % RT;
-        k = (e / RT) % NZ;
+    // ring cell e = (side * NZ + k) * 16 + pos: its LDS place and film
+    // column.  Unsigned arithmetic: with the thread index known to be below
+    // RNT, the side of each of a thread's cells folds to a constant where
+    // NZ * 16 divides RNT.
+    auto ring_lds = [&](unsigned e, int& k, int& ly, int& lx) {
+        const int side = (int)(e / (NZ * RT)), pos = (int)(e % RT);
+        k = (int)((e / RT) % NZ);
         ly = side < 2 ? pos + 1 : (side == 2 ? 0 : RTH - 1);
         lx = side < 2 ? (side == 0 ? 0 : RTH - 1) : pos + 1;
     };
-    auto ring_col = [&](int e) {
-        const int side = e / (NZ * RT), pos = e % RT;
+    auto ring_col = [&](unsigned e) {
+        const int side = (int)(e / (NZ * RT)), pos = (int)(e % RT);
         int gi, gj;
         if (side < 2) {
             gi = side == 0 ? (i0 + nx - 1) % nx : (i0 + RT) % nx;
@@ -210,7 +213,7 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
     auto code_of = [&](int kk) { return (cdp[kk / 2] >> (16 * (kk & 1))) & 0xFFFFu; };
 #pragma unroll
     for (int m = 0; m < RPT; ++m) {
-        const int e = tid + RNT * m;
+        const unsigned e = (unsigned)tid + RNT * m;
         int k, ly, lx;
         ring_lds(e, k, ly, lx);
         const unsigned rv = ring_col(e) + (unsigned)k * pl;  // the plane differs across lanes
@@ -297,16 +300,18 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
         // instead of being hoisted out of the loop and held live through it
 #pragma unroll
         for (int i = 0; i < NH / 2; ++i) asm volatile("" : "+v"(cdp[i]));
-        int tidp = tid, txp = tx, typ = ty;
+        unsigned tidp = (unsigned)tid;
+        int txp = tx, typ = ty;
         unsigned colp = col;
         asm volatile("" : "+v"(tidp), "+v"(txp), "+v"(typ), "+v"(colp));
+        __builtin_assume(tidp < (unsigned)RNT);
         const unsigned slot = (unsigned)q & 1u;
         // the ring cells' values the neighbours published in pass slot sl
         auto ring_load = [&](unsigned sl) {
 #pragma unroll
             for (int m = 0; m < RPT; ++m) {
-                const int e = tidp + RNT * m;
-                const int side = e / (NZ * RT), k = (e / RT) % NZ, pos = e % RT;
+                const unsigned e = tidp + RNT * m;
+                const int side = (int)(e / (NZ * RT)), k = (int)((e / RT) % NZ), pos = (int)(e % RT);
                 rin[m] = ld_coh(RX, xoff(sl, nbt[side], side ^ 1, k, pos));
             }
         };
@@ -316,8 +321,8 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
             __syncthreads();
 #pragma unroll
             for (int m = 0; m < RPT; ++m) {
-                const int e = tidp + RNT * m;
-                const int side = e / (NZ * RT), k = (e / RT) % NZ, pos = e % RT;
+                const unsigned e = tidp + RNT * m;
+                const int side = (int)(e / (NZ * RT)), k = (int)((e / RT) % NZ), pos = (int)(e % RT);
                 const int ex = side < 2 ? (side == 0 ? 0 : RT - 1) : pos;
                 const int ey = side < 2 ? pos : (side == 2 ? 0 : RT - 1);
                 bstore(RX, (((sl * (unsigned)G + (unsigned)blk) * (unsigned)NR) + (unsigned)e) * 4u, 0u, eT[k][ey][ex]);
@@ -350,7 +355,7 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
             const bool iter = mode == 0;
 #pragma unroll
             for (int m = 0; m < RPT; ++m) {
-                const int e = tidp + RNT * m;
+                const unsigned e = tidp + RNT * m;
                 int k, ly, lx;
                 ring_lds(e, k, ly, lx);
                 float rn = rR[e];
@@ -444,8 +449,8 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
             double rx[RPT];
 #pragma unroll
             for (int m = 0; m < RPT; ++m) {
-                const int e = tidp + RNT * m;
-                const int side = e / (NZ * RT), k = (e / RT) % NZ, pos = e % RT;
+                const unsigned e = tidp + RNT * m;
+                const int side = (int)(e / (NZ * RT)), k = (int)((e / RT) % NZ), pos = (int)(e % RT);
                 rx[m] = ld_coh_d(RXD, xdoff(nbt[side], side ^ 1, k, pos));
             }
             double xq[NH + 2];
@@ -460,7 +465,7 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
 #pragma unroll
             for (int m = 0; m < RPT; ++m) xR[tidp + RNT * m] = rx[m];
             __syncthreads();
-            const int rl = tidp & (RT * RT - 1);  // ring loaders: 64 threads of each half
+            const int rl = (int)(tidp & (RT * RT - 1));  // ring loaders: 64 threads of each half
 #pragma unroll
             for (int kk = 0; kk < NH; ++kk) {
                 const int k = k0 + kk;
@@ -469,7 +474,7 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
                 S[typ + 1][txp + 1] = x0;
                 if (rl < 4 * RT) {  // the plane's 64 ring cells
                     const int side = rl / RT, pos = rl % RT;
-                    const int e = (side * NZ + k) * RT + pos;
+                    const unsigned e = (unsigned)((side * NZ + k) * RT + pos);
                     int kq, ly, lx;
                     ring_lds(e, kq, ly, lx);
                     S[ly][lx] = xR[e];
