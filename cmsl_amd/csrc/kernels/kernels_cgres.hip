@@ -408,7 +408,13 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
                 const unsigned code = code_of(kk);
                 const float qs = row_Ax32(code, sHP, ih2, xc, uT[k][typ + 1][txp + 2], uT[k][typ + 1][txp],
                                           uT[k][typ + 2][txp + 1], uT[k][typ][txp + 1], xu, xd);
-                const float wvl = (code & SC_IDENTITY) ? op_identity(xc, k, c) : qs;
+                // identity rows lie on the electrode planes (Dirichlet) or
+                // at the gauge voxel (Neumann): interior planes of a
+                // Dirichlet film skip the test (a scalar branch)
+                float wvl = qs;
+                if (c.zbc != 0 || k == 0 || k == NZ - 1) {
+                    if (code & SC_IDENTITY) wvl = op_identity(xc, k, c);
+                }
                 w[kk] = wvl;
                 cg_dot_add<MIR>(acc[0], r[kk], xc, c, k);
                 cg_dot_add<MIR>(acc[1], wvl, xc, c, k);
@@ -488,17 +494,17 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
                     else if (k == NZ - 1) xu = xd;
                 }
                 const unsigned code = code_of(kk);
-                double Ax;
-                if (code & SC_IDENTITY) {
-                    Ax = op_identity(x0, k, c);
-                } else {
-                    const double* Hr = sH64 + (code & 3u) * 4u;
-                    const double num = Hr[(code >> 2) & 3u] * (S[typ + 1][txp + 2] - x0) -
-                                       Hr[(code >> 4) & 3u] * (x0 - S[typ + 1][txp]) +
-                                       Hr[(code >> 6) & 3u] * (S[typ + 2][txp + 1] - x0) -
-                                       Hr[(code >> 8) & 3u] * (x0 - S[typ][txp + 1]) + Hr[(code >> 10) & 3u] * (xu - x0) -
-                                       Hr[(code >> 12) & 3u] * (x0 - xd);
-                    Ax = -(num * c.inv_h2);
+                // the row as a stencil (an identity row's code reads table
+                // entry 0, and its value is replaced below)
+                const double* Hr = sH64 + (code & 3u) * 4u;
+                const double num = Hr[(code >> 2) & 3u] * (S[typ + 1][txp + 2] - x0) -
+                                   Hr[(code >> 4) & 3u] * (x0 - S[typ + 1][txp]) +
+                                   Hr[(code >> 6) & 3u] * (S[typ + 2][txp + 1] - x0) -
+                                   Hr[(code >> 8) & 3u] * (x0 - S[typ][txp + 1]) + Hr[(code >> 10) & 3u] * (xu - x0) -
+                                   Hr[(code >> 12) & 3u] * (x0 - xd);
+                double Ax = -(num * c.inv_h2);
+                if (c.zbc != 0 || k == 0 || k == NZ - 1) {  // planes that can hold identity rows
+                    if (code & SC_IDENTITY) Ax = op_identity(x0, k, c);
                 }
                 const float rv = (float)(bv - Ax);
                 r[kk] = rv;
