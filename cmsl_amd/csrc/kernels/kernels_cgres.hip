@@ -20,12 +20,14 @@
 //   then u = M r on tile and ring (LDS), w = A u on the tile, partials,
 //   publication of the edge columns of w, and the barrier.
 // The residual replacement of the FP32-refined form (every refine_every
-// iterations and when the recursive residual claims convergence) takes
-// three barriers: the edge columns of x_lo go out; r = b - A (x_hi + x_lo)
-// in FP64 on the tile (plane by plane through LDS) and its edge columns go
-// out; x_lo is folded into x_hi, u and w are re-formed from the true
-// residual (WCOMP).  The iterates are those of the streaming kernel's
-// recurrence; only the grouping of the inner products differs.
+// iterations and when the recursive residual claims convergence): x_lo is
+// folded into x_hi and the tile's edge columns of x go out in FP64 (done by
+// the iteration itself when the replacement is the periodic one, otherwise
+// by a pass of its own); then r = b - A x in FP64 on the tile (plane by
+// plane through LDS) and its edge columns go out; then u and w are
+// re-formed from the true residual (WCOMP).  The iterates are those of the
+// streaming kernel's recurrence; only the grouping of the inner products
+// differs.
 //
 // Grid barrier: cooperative launch (every workgroup resident), one arrival
 // counter per XCD (workgroup w runs on XCD w % 8) whose last arriver bumps a
@@ -253,6 +255,7 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
     int mode = 1;
     float af = 0.0f, bf = 0.0f;
     bool fresh = false;
+    bool fold_now = false;  // an iteration that a periodic residual replacement follows
     auto decide = [&](const double (&t)[3], bool cap) {
         PcgState st = load_state(&sst[dpar]);
         double a = 0.0, bb = 0.0;
@@ -270,6 +273,10 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
         af = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, (float)a)));
         bf = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, (float)bb)));
         fresh = __builtin_amdgcn_readfirstlane(bb == 0.0 ? 1 : 0) != 0;  // first iteration of a solve: no previous p, s
+        // the decision after this iteration will be a replacement (or the
+        // end) when its count is a multiple of the replacement period
+        fold_now = __builtin_amdgcn_readfirstlane(
+                       (m == 0 && st.refine_every > 0 && st.iters % st.refine_every == 0) ? 1 : 0) != 0;
     };
 
     // The setup's partial sums (r.z, r.r, b.b) and the first decision.
@@ -287,7 +294,7 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
     const int qcap = 4 * sst[0].max_iter + 16;
     int q = 0;             // grid barriers passed
     bool ring_r = false;   // the ring's r comes from the last pass's edge columns (after RESID)
-    bool resid_x = false;  // RESID: the edge columns of x_lo are out
+    bool resid_x = false;  // RESID: x_lo is folded and the edge columns of x are out
     // the ring cells' values the neighbours published in the last pass
     // (loaded right after each grid barrier, beside the partial sums)
     float rin[RPT];
@@ -342,6 +349,23 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
                 float& xv = xT[k0 + kk][typ][txp];
                 bstore(Rto, colp * 8u, (unsigned)(k0 + kk) * pl * 8u, xh[kk] + (double)xv);
                 xv = 0.0f;
+            }
+        };
+        // x_lo folded into x_hi in place (no other thread touches these
+        // voxels of x_hi during the solve), and the tile's edge columns of
+        // x = x_hi go out in FP64 for the residual replacement
+        auto fold_publish_x = [&](const double (&xh)[NH]) {
+#pragma unroll
+            for (int kk = 0; kk < NH; ++kk) {
+                const int k = k0 + kk;
+                float& xl = xT[k][typ][txp];
+                const double xv = xh[kk] + (double)xl;
+                xl = 0.0f;
+                bstore(Rxh, colp * 8u, (unsigned)k * pl * 8u, xv);
+                if (txp == 0) bstore(RXD, xdoff(blk, 0, k, typ), 0u, xv);
+                if (txp == RT - 1) bstore(RXD, xdoff(blk, 1, k, typ), 0u, xv);
+                if (typ == 0) bstore(RXD, xdoff(blk, 2, k, txp), 0u, xv);
+                if (typ == RT - 1) bstore(RXD, xdoff(blk, 3, k, txp), 0u, xv);
             }
         };
 
@@ -421,25 +445,24 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
                 acc[2] += (double)r[kk] * (double)r[kk];
                 eT[k][typ][txp] = wvl;
             }
-            publish_staged(slot);
+            if (fold_now) {
+                // the iteration a periodic replacement follows: the
+                // replacement's first pass is done here (x_hi loaded ahead
+                // of the edge stores), so the replacement starts with its
+                // second pass
+                double xh[NH];
+                fold_load(Rxh, xh);
+                publish_staged(slot);
+                fold_publish_x(xh);
+            } else {
+                publish_staged(slot);
+            }
         } else if (!resid_x) {
-            // RESID, first pass: x_lo is folded into x_hi in place (no other
-            // thread touches these voxels of x_hi during the solve) and the
-            // edge columns of x = x_hi go out in FP64
+            // RESID, first pass (a replacement the convergence test asked
+            // for): x_lo folded and the edges of x published
             double xh[NH];
             fold_load(Rxh, xh);
-#pragma unroll
-            for (int kk = 0; kk < NH; ++kk) {
-                const int k = k0 + kk;
-                float& xl = xT[k][typ][txp];
-                const double xv = xh[kk] + (double)xl;
-                xl = 0.0f;
-                bstore(Rxh, colp * 8u, (unsigned)k * pl * 8u, xv);
-                if (txp == 0) bstore(RXD, xdoff(blk, 0, k, typ), 0u, xv);
-                if (txp == RT - 1) bstore(RXD, xdoff(blk, 1, k, typ), 0u, xv);
-                if (typ == 0) bstore(RXD, xdoff(blk, 2, k, txp), 0u, xv);
-                if (typ == RT - 1) bstore(RXD, xdoff(blk, 3, k, txp), 0u, xv);
-            }
+            fold_publish_x(xh);
             partials = false;
         } else {
             // RESID, second pass: r = b - A x in FP64 (x = x_hi after the
@@ -543,9 +566,10 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
             wg_total(a, sredB, tot);
         }
         ring_r = mode == 2;
-        resid_x = false;
+        const bool folded = fold_now;  // the pass just ended did a replacement's first pass
         if (mode == 2) tot[2] = 0.0;
         decide(tot, q >= qcap);
+        resid_x = mode == 2 && folded;
     }
     // the solution's correction goes back; x_hi is already in memory
 #pragma unroll
