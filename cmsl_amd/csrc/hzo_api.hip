@@ -33,6 +33,7 @@ struct hzo_ctx {
     bool fp32 = false;  // HZO_PREC_FP32_REFINED
     bool cg_two = false;  // single-reduction CG forming s = A p and w = A u in the launch
     bool cg_nt = false;   // its iteration stores nontemporal
+    bool pcg_ntl = false; // FP64 deep-tile PCG: streamed loads nontemporal
     // register-resident single-reduction CG: the whole solve in one
     // cooperative launch over 16 x 16 column tiles (films whose tiles all fit
     // the device at once)
@@ -157,6 +158,7 @@ hzo::Fp64Bufs bufs(hzo_ctx* x) {
     set_pin(x, b);
     b.st = x->st;
     b.tdgl_part = x->tdgl_part;
+    b.stream_loads = x->pcg_ntl;
     return b;
 }
 
@@ -694,11 +696,13 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
         return set_error(HZO_ERR_ARG, "hzo_create: precision %d", cfg->precision);
     if (cfg->refine_every < 0) return set_error(HZO_ERR_ARG, "hzo_create: refine_every < 0");
     if (cfg->tile_kz != 0 && cfg->tile_kz != 2 && cfg->tile_kz != 4 && cfg->tile_kz != 8 &&
-        !(cfg->tile_kz == 16 && cfg->precision == HZO_PREC_FP32_REFINED && cfg->cg_fused) &&
-        !(cfg->tile_kz == 32 && cfg->precision == HZO_PREC_FP32_REFINED && (cfg->cg_fused == 1 || cfg->cg_fused == 3 ||
-                                                                             cfg->cg_fused == 4)))
-        return set_error(HZO_ERR_ARG, "hzo_create: tile_kz must be 0, 2, 4 or 8 (16: single-reduction FP32 CG only; "
-                                      "32: its two-stencil form only)");
+        !(cfg->tile_kz == 16 && (cfg->precision == HZO_PREC_FP64 ||
+                                 (cfg->precision == HZO_PREC_FP32_REFINED && cfg->cg_fused))) &&
+        !(cfg->tile_kz == 32 && (cfg->precision == HZO_PREC_FP64 ||
+                                 (cfg->precision == HZO_PREC_FP32_REFINED &&
+                                  (cfg->cg_fused == 1 || cfg->cg_fused == 3 || cfg->cg_fused == 4)))))
+        return set_error(HZO_ERR_ARG, "hzo_create: tile_kz must be 0, 2, 4 or 8 (16: FP64 or the single-reduction "
+                                      "FP32 CG; 32: FP64 or that CG's two-stencil form)");
     if (cfg->pcg_rule != 0 && cfg->pcg_rule != 1) return set_error(HZO_ERR_ARG, "hzo_create: pcg_rule %d", cfg->pcg_rule);
     if (cfg->max_iter < 0) return set_error(HZO_ERR_ARG, "hzo_create: max_iter < 0");
     const long long N = (long long)cfg->nx * cfg->ny * cfg->nz;
@@ -732,9 +736,14 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
     // deepest of {8, 4} giving at least 512 workgroups, else 2, and 16 when
     // depth 8 would leave more than 2048 partial sums.  Two-kernel FP32 PCG:
     // the smallest depth giving at most 512 workgroups (one partial per
-    // consuming thread, two workgroups per CU), else 8.  FP64: the smallest
-    // of {2, 4} giving at most 1024 workgroups, else 4 (its depth-8 tiles
-    // exceed 128 VGPRs).
+    // consuming thread, two workgroups per CU), else 8.  FP64: films that
+    // fill 512 workgroups of depth 8 run the z-streaming PCG kernels at the
+    // deepest of {32, 16, 8} that still gives 512 workgroups and pads at
+    // most an eighth of the planes (measured per CG iteration: 256x256x32
+    // depth 8 27.7 us against 32.3 at the shallow kernels' depth 4;
+    // 512x512x64 depth 32 about 200 against 300 us); smaller films the
+    // shallow kernels at the smallest of {2, 4} giving at most 1024
+    // workgroups, else 4.
     {
         const long long txy = (long long)((cfg->nx + 63) / 64) * ((cfg->ny + 7) / 8);
         const bool f32 = cfg->precision == HZO_PREC_FP32_REFINED && !x->cfg.cg_fused;
@@ -757,6 +766,13 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
             // very large films: depth 16 keeps the partial sums at most
             // 2048 (four per consuming thread) instead of a fold launch
             if (kz == 8 && tiles_at(8) > 2048 && tiles_at(16) >= 512) kz = 16;
+        } else if (kz == 0 && !f32 && tiles_at(8) >= 512) {
+            for (int cand : {32, 16, 8})
+                if (tiles_at(cand) >= 512 && 8LL * ((cfg->nz + cand - 1) / cand) * cand <= 9LL * cfg->nz) {
+                    kz = cand;
+                    break;
+                }
+            if (kz == 0) kz = 8;
         } else if (kz == 0) {
             kz = f32 ? 8 : 4;
             const long long cap = f32 ? 512 : 1024;
@@ -779,6 +795,9 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
         // from there, and nontemporal stores measured 3 % faster at
         // 512x512x64 (slower at every film whose vectors fit)
         x->cg_nt = x->cg_two && kz >= 16 && 22LL * N > (256LL << 20);
+        // FP64 deep tiles: r, p (twice), x and the codes, 34 bytes per voxel;
+        // beyond the last-level cache their streamed loads are nontemporal
+        x->pcg_ntl = cfg->precision == HZO_PREC_FP64 && kz >= 8 && 34LL * N > (256LL << 20);
     }
     (void)hipGetDevice(&x->device);
     // Register-resident CG: automatic (cg_fused 1, tile depth automatic) or

@@ -122,6 +122,7 @@ struct Fp64Bufs {
     int nb_in, ps_in;      // count and stride of pin (1 and 512 when folded)
     PcgState* st;       // [2] control blocks by launch parity
     double* tdgl_part;  // TDGL_Q * nb: grain-frame z sums of the last TDGL call
+    int stream_loads;   // deep-tile PCG kernels load their streamed vectors nontemporal
 };
 
 // FP32-refined form: FP32 polarisation and CG vectors, FP64 potential x_hi

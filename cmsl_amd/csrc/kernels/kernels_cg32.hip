@@ -39,20 +39,6 @@
 namespace hzo {
 
 
-// An empty asm statement that takes v as an operand: the value must exist
-// at this point, which keeps the load that produces it ahead of a branch.
-template <class V>
-__device__ __forceinline__ void keep_live(V v) {
-    asm volatile("" ::"v"(v));
-}
-template <int NQ, int PM>
-__device__ __forceinline__ void keep_live_partials(const Partials<NQ, PM>& P) {
-#pragma unroll
-    for (int q = 0; q < NQ; ++q)
-#pragma unroll
-        for (int m = 0; m < (PM > 0 ? PM : 1); ++m) keep_live(P.v[q][m]);
-}
-
 // How an iteration updates x_lo (from the control block before the launch's
 // decision): 0 leaves its alpha p to the next iteration, 1 adds alpha p,
 // 2 adds the deferred term first, (x_lo + alpha_prev p_prev) + alpha p, the

@@ -284,6 +284,52 @@ struct Bcast {
     int flag;     // 0 continue, 1 done, 2 fold (FP32-refined)
 };
 
+// The direction kernels' loop test on the totals of the previous launch
+// (setup: r.z, r.r, b.b; update: r.z, r.r) and beta (depol_varEPS.cu:459-476,
+// 506-521).  A stopping solve keeps s.done = 1; a continuing one counts the
+// iteration and flips the direction buffer.
+__device__ __forceinline__ void dir_decide(PcgState& s, const double (&tot)[3], bool first) {
+    if (first) {
+        s.rz0 = s.rz_old = tot[0];
+        s.rr0 = s.rr = tot[1];
+        s.bb = tot[2];
+        if (!finite(tot[0]) || tot[0] < 0.0) {  // :466-469
+            s.status = HZO_PCG_BAD_INITIAL;
+            s.done = 1;
+        } else {
+            judge(s, tot[0], tot[1]);
+        }
+    } else {
+        s.rr = tot[1];
+        if (!finite(tot[0]) || tot[0] < 0.0) {  // :508-513
+            s.status = HZO_PCG_BAD_RZ;
+            s.rz_old = tot[0];
+            s.done = 1;
+        } else {
+            s.beta = tot[0] / s.rz_old;  // :515
+            s.rz_old = tot[0];
+            judge(s, tot[0], tot[1]);
+        }
+    }
+    if (!s.done) {
+        s.iters += 1;  // ++kiter (:476)
+        s.pbuf ^= 1;
+        s.kind = K_DIR;
+    }
+}
+
+// The update kernels' breakdown test on p.Ap and alpha (:485-492).
+__device__ __forceinline__ void update_decide(PcgState& s, double pap) {
+    s.denom = pap;
+    if (!finite(pap) || !(pap > 0.0)) {  // :486-490
+        s.status = HZO_PCG_BREAKDOWN;
+        s.done = 1;
+    } else {
+        s.alpha = s.rz_old / pap;  // :492
+        s.kind = K_UPDATE;
+    }
+}
+
 // ---------------------------------------------------------------- PCG direction + SpMV
 // p_new = z + beta*p_old (first iteration: p_new = z), q = A p_new, p_new.q.
 // z = D^-1 r is formed from r and the exact per-code D^-1 table wherever it
@@ -375,35 +421,8 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_dir_f64(Fp64Bufs b, StepConsts c,
         sHP[tid] = hpv;
         double tot[3];
         wave0_total<3>(sraw, tot);
-        if (first) {
-            s.rz0 = s.rz_old = tot[0];
-            s.rr0 = s.rr = tot[1];
-            s.bb = tot[2];
-            if (!finite(tot[0]) || tot[0] < 0.0) {  // :466-469
-                s.status = HZO_PCG_BAD_INITIAL;
-                s.done = 1;
-            } else {
-                judge(s, tot[0], tot[1]);
-            }
-        } else {
-            s.rr = tot[1];
-            if (!finite(tot[0]) || tot[0] < 0.0) {  // :508-513
-                s.status = HZO_PCG_BAD_RZ;
-                s.rz_old = tot[0];
-                s.done = 1;
-            } else {
-                s.beta = tot[0] / s.rz_old;  // :515
-                s.rz_old = tot[0];
-                judge(s, tot[0], tot[1]);
-            }
-        }
-        if (s.done) {
-            pass_state(s, b.st + io, T.lead);
-        } else {
-            s.iters += 1;  // ++kiter (:476)
-            s.pbuf ^= 1;
-            s.kind = K_DIR;
-        }
+        dir_decide(s, tot, first);
+        if (s.done) pass_state(s, b.st + io, T.lead);
         if (tid == 0) {
             bc.coef = s.beta;
             bc.flag = s.done;
@@ -513,15 +532,8 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_update_f64(Fp64Bufs b, StepConsts
         sHP[tid] = hpv;
         double tot[1];
         wave0_total<1>(*reinterpret_cast<RawLDS<1>*>(&sraw), tot);  // p.Ap (:485)
-        s.denom = tot[0];
-        if (!finite(tot[0]) || !(tot[0] > 0.0)) {  // :486-490
-            s.status = HZO_PCG_BREAKDOWN;
-            s.done = 1;
-            pass_state(s, b.st + io, T.lead);
-        } else {
-            s.alpha = s.rz_old / tot[0];  // :492
-            s.kind = K_UPDATE;
-        }
+        update_decide(s, tot[0]);
+        if (s.done) pass_state(s, b.st + io, T.lead);
         if (tid == 0) {
             bc.coef = s.alpha;
             bc.flag = s.done;
@@ -562,6 +574,339 @@ __global__ __launch_bounds__(NT, 4) void k_pcg_update_f64(Fp64Bufs b, StepConsts
             acc[0] += r * z;
             acc[1] += r * r;
         }
+    }
+    write_block_partials<2>(acc, io ? b.part[1] : b.part[0], ps, T.blk, sraw);
+    pass_state(s, b.st + io, T.lead);
+}
+
+// ---------------------------------------------------------------- PCG, z-streaming deep tiles
+// The direction and update kernels above stage all KZ planes of a tile in
+// LDS at once, which limits them to shallow tiles: a large film then needs
+// thousands of workgroups (8192 at 512x512x64, depth 4), each spending most
+// of its life on the load-compute-store latency of its few planes, plus a
+// one-workgroup fold of the partial sums after every launch.  These kernels
+// run the same per-voxel arithmetic, operand for operand, over deep tiles
+// (KZ 16 or 32, at most 2048 workgroups, so no fold) that walk their planes
+// bottom to top.  Window plane m is z = k0 - 1 + m (m = 0 and KZ + 1 are the
+// z-ghost planes, which feed only the centre column's u/d neighbours).
+// Phase n forms plane n's operand of the stencil on the tile and its
+// one-voxel x-y halo into LDS buffer n % 2, then evaluates the stencil of
+// plane n - 1 (in-plane neighbours from buffer (n - 1) % 2, z neighbours
+// from registers); one barrier ends each phase, so a buffer is rewritten
+// only after every wave has finished reading it.  Loads run two planes
+// ahead, stencil codes three.  Only the grouping of the inner products'
+// partial sums follows the tile depth.
+
+// Loads run F64Z_LA planes ahead of their use, stencil codes F64Z_CL (two
+// and three planes; three and five measured the same at 512x512x64).
+constexpr int F64Z_LA = 2, F64Z_CL = 3;
+
+// A streamed FP64 load of the tile's own column: NTL (films whose PCG
+// vectors exceed the last-level cache) with the nontemporal hint, which took
+// the 512x512x64 film's direction and update launches from 97.3 and 120.2 to
+// 93.9 and 114.8 us; on films that fit the cache the hint cost 20-25 %.
+template <bool NTL>
+__device__ __forceinline__ double bload_s(rsrc_t r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, NTL ? 2 : 0));
+}
+
+// D^-1 of a row from the paired face table in LDS: the operations of
+// dinv_of and of the host's per-code table (depol_varEPS.cu:177-179), so the
+// same double.  The direction kernel forms D^-1 of its tile and halo cells
+// this way instead of gathering them from the 16385-entry table: 122.9 ->
+// 99.4 us per launch at 512x512x64 (the update kernel, which needs D^-1 of
+// the centre only, keeps the gather: forming it there took 128 -> 443 us).
+template <bool P2>
+__device__ __forceinline__ double dinv_pairs(unsigned code, const HPair<double>* sHP, const StepConsts& c) {
+    const unsigned cb = (code & 3u) << 4;
+    const HPair<double> hx = sHP[cb | ((code >> 2) & 15u)];
+    const HPair<double> hy = sHP[cb | ((code >> 6) & 15u)];
+    const HPair<double> hz = sHP[cb | ((code >> 10) & 15u)];
+    const double sum = hx.p + hx.m + hy.p + hy.m + hz.p + hz.m;
+    const double D = P2 ? sum * c.inv_h2 : sum / c.h2;
+    const double d = D > 1e-30 ? 1.0 / D : 1.0;
+    return (code & SC_IDENTITY) ? 1.0 : d;
+}
+
+// p_new = z + beta p_old, q = A p_new, p_new.q (as k_pcg_dir_f64).
+template <int KZ, int PM, bool P2, bool NTL>
+__global__ __launch_bounds__(NT, 4) void k_pcg_dir_f64z(Fp64Bufs b, StepConsts c, int nb, int ps, int t) {
+    constexpr int NW = KZ + 2;
+    const int ip = (t - 1) & 1, io = t & 1;
+    const TileId T = tile_id();
+    __shared__ HPair<double> sHP[64];
+    __shared__ double pl[2][HY][HX];
+    __shared__ RawLDS<3> sraw;
+    __shared__ Bcast bc;
+    const int nz = c.nz;
+    const int tid = linear_tid();
+    const int k0 = T.bz * KZ;
+    const TilePos tp = tile_pos(c, T);
+    const TileCols G = tile_cols(tp, c);
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const unsigned pb8 = c.plane * 8u, pb2 = c.plane * 2u;
+    const unsigned cv = G.cofs * 8u, hv = G.hofs * 8u;
+    const rsrc_t Rr = make_rsrc(b.r, c.nvox * 8u);
+    const rsrc_t Rsc = make_rsrc(b.sc, c.nvox * 2u);
+    const double* pin = ip ? b.pin[1] : b.pin[0];
+
+    // What does not depend on the control block goes out first: the
+    // face-pair table, the partials (all three columns; after an update
+    // launch the third is not used), the first planes' codes and r.
+    const HPair<double> hpv = load_hpair(b.lut);
+    Partials<3, PM> P;
+    load_partials<3, PM>(pin, b.nb_in, b.ps_in, P);
+    // Plane z of window plane m, clamped into the film, formed from a copy of
+    // k0 that each phase renews (an empty asm that claims to change it): the
+    // loads of a phase then cannot be hoisted above it, which would hold the
+    // whole window's loads in registers.
+    int k0p = k0;
+    auto so = [&](int m) { return (unsigned)min(max(k0p - 1 + m, 0), nz - 1); };
+    unsigned cc[NW], ch[NW];
+    auto load_code = [&](int m) {
+        cc[m] = bload_code(Rsc, G.cofs * 2u, so(m) * pb2);
+        ch[m] = (m == 0 || m == NW - 1) ? 0u : bload_code(Rsc, G.hofs * 2u, so(m) * pb2);
+    };
+    double rc[NW], rh[NW], pc_old[NW], ph_old[NW], dc[NW], dh[NW];
+    auto load_r = [&](int m) {
+        rc[m] = bload_s<NTL>(Rr, cv, so(m) * pb8);
+        rh[m] = (m == 0 || m == NW - 1) ? 0.0 : bload<double>(Rr, hv, so(m) * pb8);
+    };
+#pragma unroll
+    for (int m = 0; m < F64Z_CL; ++m) load_code(m);
+#pragma unroll
+    for (int m = 0; m < F64Z_LA; ++m) load_r(m);
+
+    PcgState s = load_state(b.st + ip);
+    if (s.done) {
+        keep_live(hpv.p);
+        keep_live_partials(P);
+#pragma unroll
+        for (int m = 0; m < F64Z_LA; ++m) {
+            keep_live(rc[m]);
+            keep_live(rh[m]);
+        }
+#pragma unroll
+        for (int m = F64Z_LA; m < F64Z_CL; ++m) {
+            keep_live(cc[m]);
+            keep_live(ch[m]);
+        }
+        pass_state(s, b.st + io, T.lead);
+        return;
+    }
+    const bool first = s.kind == K_SETUP;
+    const rsrc_t Rpo = make_rsrc(s.pbuf ? b.p[1] : b.p[0], c.nvox * 8u);
+    const rsrc_t Rpn = make_rsrc(s.pbuf ? b.p[0] : b.p[1], c.nvox * 8u);
+    auto load_p = [&](int m) {
+        pc_old[m] = first ? 0.0 : bload_s<NTL>(Rpo, cv, so(m) * pb8);
+        ph_old[m] = (first || m == 0 || m == NW - 1) ? 0.0 : bload<double>(Rpo, hv, so(m) * pb8);
+    };
+#pragma unroll
+    for (int m = 0; m < F64Z_LA; ++m) load_p(m);
+
+    stage_partials<3, PM>(P, sraw);
+    __syncthreads();
+    if (tid < 64) {
+        sHP[tid] = hpv;
+        double tot[3];
+        wave0_total<3>(sraw, tot);
+        dir_decide(s, tot, first);
+        if (s.done) pass_state(s, b.st + io, T.lead);
+        if (tid == 0) {
+            bc.coef = s.beta;
+            bc.flag = s.done;
+        }
+    }
+    __syncthreads();
+    if (bc.flag) return;
+    const double beta = bc.coef;
+    const unsigned oob = c.nvox * 8u;
+    double pc[NW];
+    double acc[1] = {0.0};
+#pragma unroll
+    for (int n = 0; n < NW; ++n) {
+        k0p = k0;
+        asm volatile("" : "+s"(k0p));
+        if (n + F64Z_CL < NW) load_code(n + F64Z_CL);
+        if (n + F64Z_LA < NW) {
+            load_r(n + F64Z_LA);
+            load_p(n + F64Z_LA);
+        }
+        // plane n: p_new on the tile (and, on tile planes, its halo)
+        dc[n] = dinv_pairs<P2>(cc[n], sHP, c);
+        if (n >= 1 && n <= KZ) dh[n] = dinv_pairs<P2>(ch[n], sHP, c);
+        const double z = rc[n] * dc[n];  // :502
+        pc[n] = first ? z : z + beta * pc_old[n];  // :518-519
+        if (n >= 1 && n <= KZ) {
+            pl[n & 1][ty + 1][tx + 1] = pc[n];
+            const double zh = rh[n] * dh[n];
+            const double v = first ? zh : zh + beta * ph_old[n];
+            if (tp.has_h) pl[n & 1][tp.hy][tp.hx] = v;
+            const bool ok = tp.live && k0 - 1 + n < nz;
+            bstore(Rpn, ok ? cv : oob, so(n) * pb8, pc[n]);
+        }
+        // plane n - 1: q = A p_new and p_new.q
+        if (n >= 2) {
+            const int m = n - 1, k = k0 - 1 + m;
+            const double xc = pc[m];
+            const unsigned code = cc[m];
+            double xu = pc[m + 1], xd = pc[m - 1];
+            if (c.zbc != 0) {  // mirror ghosts (depol_varEPS.cu:94-98)
+                if (k == 0) xd = xu;
+                else if (k == nz - 1) xu = xd;
+            }
+            const double(*S)[HX] = pl[m & 1];
+            const double Ax = row_num<P2>(code, sHP, c, xc, S[ty + 1][tx + 2], S[ty + 1][tx], S[ty + 2][tx + 1],
+                                          S[ty][tx + 1], xu, xd);
+            const double qv = (code & SC_IDENTITY) ? op_identity(xc, k, c) : -Ax;
+            if (tp.live && k < nz) acc[0] += xc * qv;  // p.Ap (:485)
+            // the sum must exist here: otherwise the compiler keeps every
+            // phase's LDS operands (in scratch) and evaluates all stencils
+            // after the loop, where their only use is
+            asm volatile("" : "+v"(acc[0]));
+        }
+        if (n >= 1 && n + 1 < NW) __syncthreads();
+    }
+    RawLDS<1>& sacc = *reinterpret_cast<RawLDS<1>*>(&sraw);
+    write_block_partials<1>(acc, io ? b.part[1] : b.part[0], ps, T.blk, sacc);
+    pass_state(s, b.st + io, T.lead);
+}
+
+// x += alpha p, r -= alpha A p, z = D^-1 r, r.z and r.r (as k_pcg_update_f64).
+template <int KZ, int PM, bool P2, bool NTL>
+__global__ __launch_bounds__(NT, 4) void k_pcg_update_f64z(Fp64Bufs b, StepConsts c, int nb, int ps, int t) {
+    constexpr int NW = KZ + 2;
+    const int ip = (t - 1) & 1, io = t & 1;
+    const TileId T = tile_id();
+    __shared__ HPair<double> sHP[64];
+    __shared__ double pl[2][HY][HX];
+    __shared__ RawLDS<2> sraw;
+    __shared__ Bcast bc;
+    const int nz = c.nz;
+    const int tid = linear_tid();
+    const int k0 = T.bz * KZ;
+    const TilePos tp = tile_pos(c, T);
+    const TileCols G = tile_cols(tp, c);
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const unsigned pb8 = c.plane * 8u, pb2 = c.plane * 2u;
+    const unsigned cv = G.cofs * 8u, hv = G.hofs * 8u;
+    const rsrc_t Rr = make_rsrc(b.r, c.nvox * 8u);
+    const rsrc_t Rx = make_rsrc(b.phi, c.nvox * 8u);
+    const rsrc_t Rsc = make_rsrc(b.sc, c.nvox * 2u);
+    const rsrc_t Rdi = make_rsrc(b.lut + LUT_DINV, SC_LUT_SIZE * 8u);
+
+    const HPair<double> hpv = load_hpair(b.lut);
+    Partials<1, PM> P;
+    load_partials<1, PM>(ip ? b.pin[1] : b.pin[0], b.nb_in, b.ps_in, P);
+    // Plane z of window plane m, clamped into the film, formed from a copy of
+    // k0 that each phase renews (an empty asm that claims to change it): the
+    // loads of a phase then cannot be hoisted above it, which would hold the
+    // whole window's loads in registers.
+    int k0p = k0;
+    auto so = [&](int m) { return (unsigned)min(max(k0p - 1 + m, 0), nz - 1); };
+    // the centre column's code, D^-1, x and r of tile planes
+    unsigned cc[NW];
+    double dc[NW], xv[NW], rv[NW];
+    auto load_code = [&](int m) {
+        if (m >= 1 && m <= KZ) cc[m] = bload_code(Rsc, G.cofs * 2u, so(m) * pb2);
+    };
+    auto load_xr = [&](int m) {
+        if (m >= 1 && m <= KZ) {
+            dc[m] = bload<double>(Rdi, cc[m] * 8u, 0u);
+            xv[m] = bload_s<NTL>(Rx, cv, so(m) * pb8);
+            rv[m] = bload_s<NTL>(Rr, cv, so(m) * pb8);
+        }
+    };
+#pragma unroll
+    for (int m = 0; m < F64Z_CL; ++m) load_code(m);
+#pragma unroll
+    for (int m = 0; m < F64Z_LA; ++m) load_xr(m);
+
+    PcgState s = load_state(b.st + ip);
+    if (s.done) {
+        keep_live(hpv.p);
+        keep_live_partials(P);
+#pragma unroll
+        for (int m = 1; m < F64Z_LA; ++m) {
+            keep_live(dc[m]);
+            keep_live(xv[m]);
+            keep_live(rv[m]);
+        }
+#pragma unroll
+        for (int m = F64Z_LA; m < F64Z_CL; ++m)
+            if (m >= 1 && m <= KZ) keep_live(cc[m]);
+        pass_state(s, b.st + io, T.lead);
+        return;
+    }
+    const rsrc_t Rp = make_rsrc(s.pbuf ? b.p[1] : b.p[0], c.nvox * 8u);
+    double pc[NW], ph[NW];
+    auto load_p = [&](int m) {
+        pc[m] = bload_s<NTL>(Rp, cv, so(m) * pb8);
+        ph[m] = (m == 0 || m == NW - 1) ? 0.0 : bload<double>(Rp, hv, so(m) * pb8);
+    };
+#pragma unroll
+    for (int m = 0; m < F64Z_LA; ++m) load_p(m);
+
+    {
+        RawLDS<1>& s1 = *reinterpret_cast<RawLDS<1>*>(&sraw);
+        stage_partials<1, PM>(P, s1);
+    }
+    __syncthreads();
+    if (tid < 64) {
+        sHP[tid] = hpv;
+        double tot[1];
+        wave0_total<1>(*reinterpret_cast<RawLDS<1>*>(&sraw), tot);  // p.Ap (:485)
+        update_decide(s, tot[0]);
+        if (s.done) pass_state(s, b.st + io, T.lead);
+        if (tid == 0) {
+            bc.coef = s.alpha;
+            bc.flag = s.done;
+        }
+    }
+    __syncthreads();
+    if (bc.flag) return;
+    const double alpha = bc.coef;
+    const double nalpha = -alpha;
+    const unsigned oob = c.nvox * 8u;
+    double acc[2] = {0.0, 0.0};  // r.z, r.r
+#pragma unroll
+    for (int n = 0; n < NW; ++n) {
+        k0p = k0;
+        asm volatile("" : "+s"(k0p));
+        if (n + F64Z_CL < NW) load_code(n + F64Z_CL);
+        if (n + F64Z_LA < NW) {
+            load_xr(n + F64Z_LA);
+            load_p(n + F64Z_LA);
+        }
+        if (n >= 1 && n <= KZ) {
+            pl[n & 1][ty + 1][tx + 1] = pc[n];
+            if (tp.has_h) pl[n & 1][tp.hy][tp.hx] = ph[n];
+        }
+        if (n >= 2) {
+            const int m = n - 1, k = k0 - 1 + m;
+            const double xc = pc[m];
+            const unsigned code = cc[m];
+            double xu = pc[m + 1], xd = pc[m - 1];
+            if (c.zbc != 0) {
+                if (k == 0) xd = xu;
+                else if (k == nz - 1) xu = xd;
+            }
+            const double(*S)[HX] = pl[m & 1];
+            const double Ax = row_num<P2>(code, sHP, c, xc, S[ty + 1][tx + 2], S[ty + 1][tx], S[ty + 2][tx + 1],
+                                          S[ty][tx + 1], xu, xd);
+            const double qv = (code & SC_IDENTITY) ? op_identity(xc, k, c) : -Ax;
+            const bool ok = tp.live && k < nz;
+            const unsigned vo = ok ? cv : oob;
+            bstore(Rx, vo, so(m) * pb8, alpha * xc + xv[m]);  // Daxpy x += alpha p (:495)
+            const double r = nalpha * qv + rv[m];               // Daxpy r -= alpha Ap (:499)
+            bstore(Rr, vo, so(m) * pb8, r);
+            const double z = r * dc[m];  // :502
+            if (ok) {
+                acc[0] += r * z;
+                acc[1] += r * r;
+            }
+        }
+        if (n >= 1 && n + 1 < NW) __syncthreads();
     }
     write_block_partials<2>(acc, io ? b.part[1] : b.part[0], ps, T.blk, sraw);
     pass_state(s, b.st + io, T.lead);
@@ -617,14 +962,54 @@ void launch_pcg_setup_f64(hipStream_t s, const Grid& g, const StepConsts& c, con
                           rhs);
 }
 
-#define HZO_X , P2
+// Tile depths 8, 16 and 32 run the z-streaming kernels.
 template <bool P2>
 static void launch_dir64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t, const LaunchEv& ev) {
+    if (g.kz >= 8) {
+        const int pm = part_pm(b.nb_in);
+        const dim3 gr = grid3(g), bl(BX, BY, 1);
+        if (b.stream_loads) {
+#define HZO_X , P2, true
+            if (g.kz == 32) { HZO_PCG_LAUNCH_PM(k_pcg_dir_f64z, 32, pm, gr, bl, s, b, c, blocks_for(g), part_stride(g), t); }
+            else if (g.kz == 16) { HZO_PCG_LAUNCH_PM(k_pcg_dir_f64z, 16, pm, gr, bl, s, b, c, blocks_for(g), part_stride(g), t); }
+            else { HZO_PCG_LAUNCH_PM(k_pcg_dir_f64z, 8, pm, gr, bl, s, b, c, blocks_for(g), part_stride(g), t); }
+#undef HZO_X
+        } else {
+#define HZO_X , P2, false
+            if (g.kz == 32) { HZO_PCG_LAUNCH_PM(k_pcg_dir_f64z, 32, pm, gr, bl, s, b, c, blocks_for(g), part_stride(g), t); }
+            else if (g.kz == 16) { HZO_PCG_LAUNCH_PM(k_pcg_dir_f64z, 16, pm, gr, bl, s, b, c, blocks_for(g), part_stride(g), t); }
+            else { HZO_PCG_LAUNCH_PM(k_pcg_dir_f64z, 8, pm, gr, bl, s, b, c, blocks_for(g), part_stride(g), t); }
+#undef HZO_X
+        }
+        return;
+    }
+#define HZO_X , P2
     HZO_PCG_LAUNCH_X(k_pcg_dir_f64, g, b.nb_in, s, b, c, blocks_for(g), part_stride(g), t);
+#undef HZO_X
 }
 template <bool P2>
 static void launch_update64(hipStream_t s, const Grid& g, const StepConsts& c, const Fp64Bufs& b, int t, const LaunchEv& ev) {
+    if (g.kz >= 8) {
+        const int pm = part_pm(b.nb_in);
+        const dim3 gr = grid3(g), bl(BX, BY, 1);
+        if (b.stream_loads) {
+#define HZO_X , P2, true
+            if (g.kz == 32) { HZO_PCG_LAUNCH_PM(k_pcg_update_f64z, 32, pm, gr, bl, s, b, c, blocks_for(g), part_stride(g), t); }
+            else if (g.kz == 16) { HZO_PCG_LAUNCH_PM(k_pcg_update_f64z, 16, pm, gr, bl, s, b, c, blocks_for(g), part_stride(g), t); }
+            else { HZO_PCG_LAUNCH_PM(k_pcg_update_f64z, 8, pm, gr, bl, s, b, c, blocks_for(g), part_stride(g), t); }
+#undef HZO_X
+        } else {
+#define HZO_X , P2, false
+            if (g.kz == 32) { HZO_PCG_LAUNCH_PM(k_pcg_update_f64z, 32, pm, gr, bl, s, b, c, blocks_for(g), part_stride(g), t); }
+            else if (g.kz == 16) { HZO_PCG_LAUNCH_PM(k_pcg_update_f64z, 16, pm, gr, bl, s, b, c, blocks_for(g), part_stride(g), t); }
+            else { HZO_PCG_LAUNCH_PM(k_pcg_update_f64z, 8, pm, gr, bl, s, b, c, blocks_for(g), part_stride(g), t); }
+#undef HZO_X
+        }
+        return;
+    }
+#define HZO_X , P2
     HZO_PCG_LAUNCH_X(k_pcg_update_f64, g, b.nb_in, s, b, c, blocks_for(g), part_stride(g), t);
+#undef HZO_X
 }
 
 #undef HZO_X

@@ -110,6 +110,20 @@ struct Partials {
     double v[NQ][PM > 0 ? PM : 1];
 };
 
+// An empty asm statement that takes v as an operand: the value must exist
+// at this point, which keeps the load that produces it ahead of a branch.
+template <class V>
+__device__ __forceinline__ void keep_live(V v) {
+    asm volatile("" ::"v"(v));
+}
+template <int NQ, int PM>
+__device__ __forceinline__ void keep_live_partials(const Partials<NQ, PM>& P) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int m = 0; m < (PM > 0 ? PM : 1); ++m) keep_live(P.v[q][m]);
+}
+
 template <int NQ, int PM>
 __device__ __forceinline__ void load_partials(const double* in, int nb, int ps, Partials<NQ, PM>& P) {
     const int tid = linear_tid();

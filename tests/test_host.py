@@ -48,12 +48,13 @@ def test_device_count_and_create_without_gpu_fail_cleanly():
     (dict(tile_kz=32, cg_fused=2), b"tile_kz"),
     (dict(tile_kz=32, cg_fused=0), b"tile_kz"),
     (dict(tile_kz=16, cg_fused=0), b"tile_kz"),
-    (dict(tile_kz=32, precision=L.PREC_FP64), b"tile_kz"),
+    (dict(tile_kz=64, precision=L.PREC_FP64), b"tile_kz"),
     (dict(tile_kz=12), b"tile_kz"),
 ])
 def test_create_rejects_solver_and_tile_combinations(kw, msg):
     # argument checks run before any device call, so they hold without a GPU:
-    # depth 16 needs the single-reduction CG, depth 32 its two-stencil form
+    # FP32-refined depth 16 needs the single-reduction CG, depth 32 its
+    # two-stencil form; FP64 takes 16 and 32 (its z-streaming kernels)
     # (cg_fused 4, the resident form, is checked against the device)
     import ctypes as C
     base = dict(nx=8, ny=8, nz=4, precision=L.PREC_FP32_REFINED)
