@@ -6,21 +6,38 @@
 #include "../hzo_internal.h"
 #include "common.hip.h"
 #include "tile.hip.h"
+#include <type_traits>
 
 namespace hzo {
 
 // One row of the operator (-(sum of face fluxes)/h^2) from the stencil code:
 // the paired harmonic face means of the three axes come from the 64-entry
-// HPair table (see load_hpair).
+// HPair table (see load_hpair).  coef_of reads them, row_apply32 applies
+// them, so a kernel that evaluates one row on two operands reads the
+// coefficients once.
+struct Coef3 {
+    HPair<float> x, y, z;
+};
+
+__device__ __forceinline__ Coef3 coef_of(unsigned code, const HPair<float>* sHP) {
+    const unsigned cb = (code & 3u) << 4;
+    Coef3 h;
+    h.x = sHP[cb | ((code >> 2) & 15u)];
+    h.y = sHP[cb | ((code >> 6) & 15u)];
+    h.z = sHP[cb | ((code >> 10) & 15u)];
+    return h;
+}
+
+__device__ __forceinline__ float row_apply32(const Coef3& h, float ih2, float xc, float xe, float xw, float xn,
+                                             float xs, float xu, float xd) {
+    const float num = h.x.p * (xe - xc) - h.x.m * (xc - xw) + h.y.p * (xn - xc) - h.y.m * (xc - xs) +
+                      h.z.p * (xu - xc) - h.z.m * (xc - xd);
+    return -(num * ih2);
+}
+
 __device__ __forceinline__ float row_Ax32(unsigned code, const HPair<float>* sHP, float ih2, float xc, float xe,
                                           float xw, float xn, float xs, float xu, float xd) {
-    const unsigned cb = (code & 3u) << 4;
-    const HPair<float> hx = sHP[cb | ((code >> 2) & 15u)];
-    const HPair<float> hy = sHP[cb | ((code >> 6) & 15u)];
-    const HPair<float> hz = sHP[cb | ((code >> 10) & 15u)];
-    const float num = hx.p * (xe - xc) - hx.m * (xc - xw) + hy.p * (xn - xc) - hy.m * (xc - xs) +
-                      hz.p * (xu - xc) - hz.m * (xc - xd);
-    return -(num * ih2);
+    return row_apply32(coef_of(code, sHP), ih2, xc, xe, xw, xn, xs, xu, xd);
 }
 
 // The loop logic after a completed reduction (every lane of every

@@ -556,103 +556,124 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
             vst(RX, (tp.live && k < nz) ? cv : aend, sv, xv + aprev * pv);
         }
     } else {
-        const bool iter = mode == 0;
         const float alpha = uniform_f((float)alpha_d), beta = uniform_f((float)beta_d);
         const bool fresh = __builtin_amdgcn_readfirstlane(beta_d == 0.0 ? 1 : 0) != 0;
-        float pc[NW], ph[NW], uc[NW];
-        // A(m): p of plane m (tile and two-voxel ring) into LDS; on the tile
-        // p_i and x_lo += alpha p_i are stored (WCOMP: x_lo folded instead)
-        auto stageA = [&](int m) {
-            const int k = k0 - 2 + m;
-            const bool tile_plane = m >= 2 && m <= KZ + 1;
-            const bool ok = tile_plane && tp.live && k < nz;
-            if (iter) {
-                const float uo = L[m].r * L[m].d, uoh = L[m].rh * L[m].dh;
-                pc[m] = fresh ? uo : uo + beta * L[m].p;
-                ph[m] = fresh ? uoh : uoh + beta * L[m].ph;
-                sp[m & 1][ty + 2][tx + 2] = pc[m];
-                if (R.has) sp[m & 1][R.ly + 2][R.lx + 2] = ph[m];
-                if (tile_plane) {
-                    const unsigned vo = ok ? cv : aend, sv = so(m) * pb4;
-                    vst(RB, vo, o_pn + sv, pc[m]);
-                    const float xa = xm == 2 ? L[m].x + aprev * L[m].p : L[m].x;
-                    vst(RX, xm != 0 ? vo : aend, sv, xa + alpha * pc[m]);
+        // The plane loop, built twice: an iteration (ITER) and the
+        // re-formation after a residual replacement (WCOMP), so neither
+        // carries the other's selects.
+        auto planes = [&](auto iter_c) {
+            constexpr bool iter = decltype(iter_c)::value;
+            float pc[NW], ph[NW], uc[NW];
+            Coef3 cf[NW];
+            // planes that can hold identity rows: the electrode planes under
+            // Dirichlet electrodes, the gauge node's plane under Neumann
+            auto ident_plane = [&](int k) { return c.zbc == 0 ? (k == 0 || k == nz - 1) : (k == nz / 2); };
+            auto mirrored = [&](int k) { return c.zbc != 0 && (k == 0 || k == nz - 1); };
+            // A(m): p of plane m (tile and two-voxel ring) into LDS; on the
+            // tile p_i and x_lo += alpha p_i are stored (WCOMP: x_lo folded)
+            auto stageA = [&](int m) {
+                const int k = k0 - 2 + m;
+                const bool tile_plane = m >= 2 && m <= KZ + 1;
+                const bool ok = tile_plane && tp.live && k < nz;
+                if constexpr (iter) {
+                    const float uo = L[m].r * L[m].d, uoh = L[m].rh * L[m].dh;
+                    pc[m] = fresh ? uo : uo + beta * L[m].p;
+                    ph[m] = fresh ? uoh : uoh + beta * L[m].ph;
+                    sp[m & 1][ty + 2][tx + 2] = pc[m];
+                    if (R.has) sp[m & 1][R.ly + 2][R.lx + 2] = ph[m];
+                    if (tile_plane) {
+                        const unsigned vo = ok ? cv : aend, sv = so(m) * pb4;
+                        vst(RB, vo, o_pn + sv, pc[m]);
+                        const float xa = xm == 2 ? L[m].x + aprev * L[m].p : L[m].x;
+                        vst(RX, xm != 0 ? vo : aend, sv, xa + alpha * pc[m]);
+                    }
+                } else if (ok) {
+                    const long long id = (long long)G.cofs + (long long)c.plane * k;
+                    b.xhi[id] += (double)b.xlo[id];
+                    b.xlo[id] = 0.0f;
                 }
-            } else if (ok) {
-                const long long id = (long long)G.cofs + (long long)c.plane * k;
-                b.xhi[id] += (double)b.xlo[id];
-                b.xlo[id] = 0.0f;
-            }
-        };
-        // B(m): s = A p, r -= alpha s, u = M r on the tile and the one-voxel
-        // ring of plane m (in-plane p from LDS, z neighbours from registers);
-        // u into LDS, the new r stored on the tile
-        auto stageB = [&](int m) {
-            const int k = k0 - 2 + m;
-            float sc = 0.0f, sh = 0.0f;
-            if (iter) {
-                const unsigned cc = cd[m] & 0xFFFFu, ch = cd[m] >> 16;
-                float xu = pc[m + 1], xd = pc[m - 1], hu = ph[m + 1], hd = ph[m - 1];
+            };
+            // B(m): s = A p, r -= alpha s, u = M r on the tile and the
+            // one-voxel ring of plane m (in-plane p from LDS, z neighbours
+            // from registers); u into LDS, the new r stored on the tile.  The
+            // centre row's face coefficients are kept for C(m).
+            auto stageB = [&](int m) {
+                const int k = k0 - 2 + m;
+                const unsigned cc = cd[m] & 0xFFFFu;
+                cf[m] = coef_of(cc, sHP);
+                float sc = 0.0f, sh = 0.0f;
+                if constexpr (iter) {
+                    const unsigned ch = cd[m] >> 16;
+                    float xu = pc[m + 1], xd = pc[m - 1], hu = ph[m + 1], hd = ph[m - 1];
+                    if (c.zbc != 0) {
+                        if (k == 0) xd = xu, hd = hu;
+                        else if (k == nz - 1) xu = xd, hu = hd;
+                    }
+                    const float(*Pm)[PX] = sp[m & 1];
+                    sc = row_apply32(cf[m], ih2, pc[m], Pm[ty + 2][tx + 3], Pm[ty + 2][tx + 1], Pm[ty + 3][tx + 2],
+                                     Pm[ty + 1][tx + 2], xu, xd);
+                    if (winner)
+                        sh = row_Ax32(ch, sHP, ih2, ph[m], Pm[by + 2][bx + 3], Pm[by + 2][bx + 1], Pm[by + 3][bx + 2],
+                                      Pm[by + 1][bx + 2], hu, hd);
+                    if (ident_plane(k)) {
+                        if (cc & SC_IDENTITY) sc = op_identity(pc[m], k, c);
+                        if (ch & SC_IDENTITY) sh = op_identity(ph[m], k, c);
+                    }
+                }
+                const float rc = iter ? L[m].r - alpha * sc : L[m].r;
+                uc[m] = rc * L[m].d;
+                su[m & 1][ty + 1][tx + 1] = uc[m];
+                if (R.inner) {
+                    const float rh = iter ? L[m].rh - alpha * sh : L[m].rh;
+                    su[m & 1][R.ly + 1][R.lx + 1] = rh * L[m].dh;
+                }
+                if (m >= 2 && m <= KZ + 1) {
+                    const bool ok = tp.live && k < nz;
+                    if constexpr (iter) vst(RA, ok ? cv : aend, o_rn + so(m) * pb4, rc);
+                    if (ok) {
+                        if (mirrored(k)) acc[0] = __builtin_fma((double)rc, (double)uc[m], acc[0] * 2.0) * 0.5;
+                        else acc[0] = __builtin_fma((double)rc, (double)uc[m], acc[0]);
+                        acc[2] += (double)rc * (double)rc;
+                    }
+                }
+            };
+            // C(m): w = A u on the tile of plane m and its term of w.u
+            auto stageC = [&](int m) {
+                const int k = k0 - 2 + m;
+                float xu = uc[m + 1], xd = uc[m - 1];
                 if (c.zbc != 0) {
-                    if (k == 0) xd = xu, hd = hu;
-                    else if (k == nz - 1) xu = xd, hu = hd;
+                    if (k == 0) xd = xu;
+                    else if (k == nz - 1) xu = xd;
                 }
-                const float(*Pm)[PX] = sp[m & 1];
-                const float qc = row_Ax32(cc, sHP, ih2, pc[m], Pm[ty + 2][tx + 3], Pm[ty + 2][tx + 1],
-                                          Pm[ty + 3][tx + 2], Pm[ty + 1][tx + 2], xu, xd);
-                sc = (cc & SC_IDENTITY) ? op_identity(pc[m], k, c) : qc;
-                if (winner) {
-                    const float qh = row_Ax32(ch, sHP, ih2, ph[m], Pm[by + 2][bx + 3], Pm[by + 2][bx + 1],
-                                              Pm[by + 3][bx + 2], Pm[by + 1][bx + 2], hu, hd);
-                    sh = (ch & SC_IDENTITY) ? op_identity(ph[m], k, c) : qh;
+                const float(*Um)[HX] = su[m & 1];
+                float wv = row_apply32(cf[m], ih2, uc[m], Um[ty + 1][tx + 2], Um[ty + 1][tx], Um[ty + 2][tx + 1],
+                                       Um[ty][tx + 1], xu, xd);
+                if (ident_plane(k) && (cd[m] & SC_IDENTITY)) wv = op_identity(uc[m], k, c);
+                if (tp.live && k < nz) {
+                    if (mirrored(k)) acc[1] = __builtin_fma((double)wv, (double)uc[m], acc[1] * 2.0) * 0.5;
+                    else acc[1] = __builtin_fma((double)wv, (double)uc[m], acc[1]);
                 }
-            }
-            const float rc = iter ? L[m].r - alpha * sc : L[m].r;
-            uc[m] = rc * L[m].d;
-            su[m & 1][ty + 1][tx + 1] = uc[m];
-            if (R.inner) {
-                const float rh = iter ? L[m].rh - alpha * sh : L[m].rh;
-                su[m & 1][R.ly + 1][R.lx + 1] = rh * L[m].dh;
-            }
-            if (m >= 2 && m <= KZ + 1) {
-                const bool ok = tp.live && k < nz;
-                if (iter) vst(RA, ok ? cv : aend, o_rn + so(m) * pb4, rc);
-                if (ok) {
-                    cg_dot_add<true>(acc[0], rc, uc[m], c, k);
-                    acc[2] += (double)rc * (double)rc;
-                }
-            }
-        };
-        // C(m): w = A u on the tile of plane m and its term of w.u
-        auto stageC = [&](int m) {
-            const int k = k0 - 2 + m;
-            const unsigned cc = cd[m] & 0xFFFFu;
-            float xu = uc[m + 1], xd = uc[m - 1];
-            if (c.zbc != 0) {
-                if (k == 0) xd = xu;
-                else if (k == nz - 1) xu = xd;
-            }
-            const float(*Um)[HX] = su[m & 1];
-            const float qs = row_Ax32(cc, sHP, ih2, uc[m], Um[ty + 1][tx + 2], Um[ty + 1][tx], Um[ty + 2][tx + 1],
-                                      Um[ty][tx + 1], xu, xd);
-            const float wv = (cc & SC_IDENTITY) ? op_identity(uc[m], k, c) : qs;
-            if (tp.live && k < nz) cg_dot_add<true>(acc[1], wv, uc[m], c, k);
-        };
-        // Phase n: A(n); B(n - 1), whose in-plane p was written in phase
-        // n - 1 and whose upper z neighbour A(n) has just formed; C(n - 2),
-        // whose u was written in phase n - 1.  One barrier per phase; a
-        // plane's LDS buffer is rewritten two phases after it was filled.
+            };
+            // Phase n: A(n); B(n - 1), whose in-plane p was written in phase
+            // n - 1 and whose upper z neighbour A(n) has just formed; C(n - 2),
+            // whose u was written in phase n - 1.  One barrier per phase; a
+            // plane's LDS buffer is rewritten two phases after it was filled.
 #pragma unroll
-        for (int n = 0; n < NW; ++n) {
-            k0p = k0;
-            asm volatile("" : "+s"(k0p));
-            if (n + 4 < NW) load_code(n + 4);
-            if (n + 2 < NW) issue(n + 2);
-            stageA(n);
-            if (n >= 2) stageB(n - 1);
-            if (n >= 4) stageC(n - 2);
-            if (n + 1 < NW) __syncthreads();
-        }
+            for (int n = 0; n < NW; ++n) {
+                k0p = k0;
+                asm volatile("" : "+s"(k0p));
+                if (n + 4 < NW) load_code(n + 4);
+                if (n + 2 < NW) issue(n + 2);
+                stageA(n);
+                if (n >= 2) stageB(n - 1);
+                if (n >= 4) stageC(n - 2);
+                if (n + 1 < NW) __syncthreads();
+            }
+        };
+        if (mode == 0)
+            planes(std::true_type{});
+        else
+            planes(std::false_type{});
     }
     write_block_partials_w<3>(acc, io ? b.part[1] : b.part[0], ps, T.blk, spart);
 }
