@@ -39,6 +39,16 @@
 namespace hzo {
 
 
+// acc += w x y with the mirrored-plane weight w = 1/2 taken as
+// (2 acc + x y) / 2 (the terms of cg_dot_add), chosen by a uniform branch
+// instead of multiplications by 1 on every other plane.
+__device__ __forceinline__ void dot_w(double& acc, float x, float y, bool mirrored) {
+    if (mirrored)
+        acc = __builtin_fma((double)x, (double)y, acc * 2.0) * 0.5;
+    else
+        acc = __builtin_fma((double)x, (double)y, acc);
+}
+
 // How an iteration updates x_lo (from the control block before the launch's
 // decision): 0 leaves its alpha p to the next iteration, 1 adds alpha p,
 // 2 adds the deferred term first, (x_lo + alpha_prev p_prev) + alpha p, the
@@ -286,79 +296,93 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
         // x_hi, u = M r, w = A u; partials (r.u, w.u, r.r).  The planes are
         // streamed bottom to top: plane q's update and u, then the stencil of
         // plane q - 1, whose z neighbours are u of planes q - 2 and q.
-        const bool iter = mode == 0;
         const float alpha = uniform_f((float)alpha_d), beta = uniform_f((float)beta_d);
         const bool fresh = __builtin_amdgcn_readfirstlane(beta_d == 0.0 ? 1 : 0) != 0;  // first iteration: no previous p, s
         const unsigned o_r2 = (cur ^ 1u) * seg, o_s2 = (cur ^ 1u) * seg;
-        // w goes to the other buffer after an iteration, in place after WCOMP
-        const unsigned o_wo = iter ? (3u - cur) * seg : o_w;
-        float u[KZ + 2];
+        // The plane loop, built twice: an iteration (ITER) and the
+        // re-formation after a residual replacement (WCOMP), so neither
+        // carries the other's selects.
+        auto planes = [&](auto iter_c) {
+            constexpr bool iter = decltype(iter_c)::value;
+            // w goes to the other buffer after an iteration, in place after WCOMP
+            const unsigned o_wo = iter ? (3u - cur) * seg : o_w;
+            // planes that can hold identity rows: the electrode planes under
+            // Dirichlet electrodes, the gauge node's plane under Neumann
+            auto ident_plane = [&](int kk) { return c.zbc == 0 ? (kk == 0 || kk == nz - 1) : (kk == nz / 2); };
+            auto mirrored = [&](int kk) { return MIR && c.zbc != 0 && (kk == 0 || kk == nz - 1); };
+            float u[KZ + 2];
 #pragma unroll
-        for (int q = 0; q < KZ + 2; ++q) {
-            if (q + 2 < KZ + 2) issue(q + 2);
-            const int k = k0 - 1 + q;
-            const bool interior = q >= 1 && q <= KZ;
-            const bool ok = interior && tp.live && k < nz;
-            const unsigned vo = ok ? cv : aend;
-            const unsigned sv = so[q] * pb4;
-            // A(q): update and u of plane q
-            float rn = L[q].r;
-            if (iter) {
-                const float sn = fresh ? L[q].w : L[q].w + beta * L[q].s;
-                rn = L[q].r - alpha * sn;
-                if (interior) {
-                    bstore(RB, vo, o_s2 + sv, sn);
-                    bstore(RA, vo, o_r2 + sv, rn);
-                    const float uo = L[q].r * L[q].d;
-                    const float pn = fresh ? uo : uo + beta * L[q].p;
-                    bstore(RB, vo, o_p + sv, pn);
-                    const float xa = xm == 2 ? L[q].x + aprev * L[q].p : L[q].x;
-                    bstore(RX, xm != 0 ? vo : aend, sv, xa + alpha * pn);
-                }
-            } else if (interior && ok) {
-                const long long id = (long long)G.cofs + (long long)c.plane * k;
-                b.xhi[id] += (double)b.xlo[id];
-                b.xlo[id] = 0.0f;
-            }
-            u[q] = rn * L[q].d;
-            if (ok) {
-                cg_dot_add<MIR>(acc[0], rn, u[q], c, k);
-                acc[2] += (double)rn * (double)rn;
-            }
-            if (interior) {
-                pl[q % 6][ty + 1][tx + 1] = u[q];
-                float rhn = L[q].rh;
-                if (iter) {
-                    const float shn = fresh ? L[q].wh : L[q].wh + beta * L[q].sh;
-                    rhn = L[q].rh - alpha * shn;
-                }
-                if (tp.has_h) pl[q % 6][tp.hy][tp.hx] = rhn * L[q].dh;
-            }
-            // B(q - 2), B(q - 1): stencils of planes q - 2 and q - 1 after
-            // every second plane (KZ is even, so the last plane q = KZ + 1 is
-            // odd and closes the loop)
-            if (q >= 3 && (q & 1)) {
-                __syncthreads();
-#pragma unroll
-                for (int m = q - 2; m <= q - 1; ++m) {
-                    const int km = k0 - 1 + m;
-                    const float xc = u[m];
-                    const unsigned code = cc[m];
-                    float xu = u[m + 1], xd = u[m - 1];
-                    if (c.zbc != 0) {
-                        if (km == 0) xd = xu;
-                        else if (km == nz - 1) xu = xd;
+            for (int q = 0; q < KZ + 2; ++q) {
+                if (q + 2 < KZ + 2) issue(q + 2);
+                const int k = k0 - 1 + q;
+                const bool interior = q >= 1 && q <= KZ;
+                const bool ok = interior && tp.live && k < nz;
+                const unsigned vo = ok ? cv : aend;
+                const unsigned sv = so[q] * pb4;
+                // A(q): update and u of plane q
+                float rn = L[q].r;
+                if constexpr (iter) {
+                    const float sn = fresh ? L[q].w : L[q].w + beta * L[q].s;
+                    rn = L[q].r - alpha * sn;
+                    if (interior) {
+                        bstore(RB, vo, o_s2 + sv, sn);
+                        bstore(RA, vo, o_r2 + sv, rn);
+                        const float uo = L[q].r * L[q].d;
+                        const float pn = fresh ? uo : uo + beta * L[q].p;
+                        bstore(RB, vo, o_p + sv, pn);
+                        const float xa = xm == 2 ? L[q].x + aprev * L[q].p : L[q].x;
+                        bstore(RX, xm != 0 ? vo : aend, sv, xa + alpha * pn);
                     }
-                    const float(*pm)[HX] = pl[m % 6];
-                    const float qs = row_Ax32(code, sHP, ih2, xc, pm[ty + 1][tx + 2], pm[ty + 1][tx],
-                                              pm[ty + 2][tx + 1], pm[ty][tx + 1], xu, xd);
-                    const float wv = (code & SC_IDENTITY) ? op_identity(xc, km, c) : qs;
-                    const bool okm = tp.live && km < nz;
-                    bstore(RA, okm ? cv : aend, o_wo + so[m] * pb4, wv);
-                    if (okm) cg_dot_add<MIR>(acc[1], wv, xc, c, km);
+                } else if (interior && ok) {
+                    const long long id = (long long)G.cofs + (long long)c.plane * k;
+                    b.xhi[id] += (double)b.xlo[id];
+                    b.xlo[id] = 0.0f;
+                }
+                u[q] = rn * L[q].d;
+                if (ok) {
+                    dot_w(acc[0], rn, u[q], mirrored(k));
+                    acc[2] += (double)rn * (double)rn;
+                }
+                if (interior) {
+                    pl[q % 6][ty + 1][tx + 1] = u[q];
+                    float rhn = L[q].rh;
+                    if constexpr (iter) {
+                        const float shn = fresh ? L[q].wh : L[q].wh + beta * L[q].sh;
+                        rhn = L[q].rh - alpha * shn;
+                    }
+                    if (tp.has_h) pl[q % 6][tp.hy][tp.hx] = rhn * L[q].dh;
+                }
+                // B(q - 2), B(q - 1): stencils of planes q - 2 and q - 1 after
+                // every second plane (KZ is even, so the last plane q = KZ + 1 is
+                // odd and closes the loop)
+                if (q >= 3 && (q & 1)) {
+                    __syncthreads();
+#pragma unroll
+                    for (int m = q - 2; m <= q - 1; ++m) {
+                        const int km = k0 - 1 + m;
+                        const float xc = u[m];
+                        const unsigned code = cc[m];
+                        float xu = u[m + 1], xd = u[m - 1];
+                        if (c.zbc != 0) {
+                            if (km == 0) xd = xu;
+                            else if (km == nz - 1) xu = xd;
+                        }
+                        const float(*pm)[HX] = pl[m % 6];
+                        const float qs = row_Ax32(code, sHP, ih2, xc, pm[ty + 1][tx + 2], pm[ty + 1][tx],
+                                                  pm[ty + 2][tx + 1], pm[ty][tx + 1], xu, xd);
+                        float wv = qs;
+                        if (ident_plane(km) && (code & SC_IDENTITY)) wv = op_identity(xc, km, c);
+                        const bool okm = tp.live && km < nz;
+                        bstore(RA, okm ? cv : aend, o_wo + so[m] * pb4, wv);
+                        if (okm) dot_w(acc[1], wv, xc, mirrored(km));
+                    }
                 }
             }
-        }
+        };
+        if (mode == 0)
+            planes(std::true_type{});
+        else
+            planes(std::false_type{});
     }
     write_block_partials_w<3>(acc, io ? b.part[1] : b.part[0], ps, T.blk, spart);
 }
@@ -631,8 +655,7 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
                     const bool ok = tp.live && k < nz;
                     if constexpr (iter) vst(RA, ok ? cv : aend, o_rn + so(m) * pb4, rc);
                     if (ok) {
-                        if (mirrored(k)) acc[0] = __builtin_fma((double)rc, (double)uc[m], acc[0] * 2.0) * 0.5;
-                        else acc[0] = __builtin_fma((double)rc, (double)uc[m], acc[0]);
+                        dot_w(acc[0], rc, uc[m], mirrored(k));
                         acc[2] += (double)rc * (double)rc;
                     }
                 }
@@ -650,8 +673,7 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
                                        Um[ty][tx + 1], xu, xd);
                 if (ident_plane(k) && (cd[m] & SC_IDENTITY)) wv = op_identity(uc[m], k, c);
                 if (tp.live && k < nz) {
-                    if (mirrored(k)) acc[1] = __builtin_fma((double)wv, (double)uc[m], acc[1] * 2.0) * 0.5;
-                    else acc[1] = __builtin_fma((double)wv, (double)uc[m], acc[1]);
+                    dot_w(acc[1], wv, uc[m], mirrored(k));
                 }
             };
             // Phase n: A(n); B(n - 1), whose in-plane p was written in phase
