@@ -22,6 +22,9 @@ using hzo::set_error;
                              hipGetErrorString(e_));                                           \
     } while (0)
 
+// Steps hzo_run enqueues on a resident film before it waits for them.
+constexpr int RUN_RING = 64;
+
 struct hzo_ctx {
     hzo_config cfg;
     hzo::Grid g;
@@ -44,6 +47,11 @@ struct hzo_ctx {
     double* res_part = nullptr;  // partial sums [2][G][4]
     unsigned* res_sync = nullptr;  // barrier counters, then the timeout flag
     int* res_err_host = nullptr;   // pinned copy of the timeout flag
+    // hzo_run on a resident film: each step's control block and timeout flag
+    // land in a pinned ring (RUN_RING entries) that the host reads once per
+    // RUN_RING steps instead of waiting for every solve
+    hzo::PcgState* res_ring = nullptr;
+    int* res_err_ring = nullptr;
     long long res_counts[3] = {0, 0, 0};  // solves, iterations, grid barriers (since hzo_set_timing)
     double* P[2] = {nullptr, nullptr};
     float* P32[2] = {nullptr, nullptr};
@@ -470,8 +478,10 @@ void res_after(hzo_ctx* x) {
     x->res_counts[2] += s.passes;
 }
 
-int cgres_solve(hzo_ctx* x, const hzo::StepConsts& c, const hzo::CG32Bufs& cb, int& last_iters,
-                hzo_pcg_stats* stats) {
+// Enqueues the resident solve of film x on its stream; its control block
+// and timeout flag are copied to st_out and err_out (pinned) when it ends.
+int cgres_enqueue(hzo_ctx* x, const hzo::StepConsts& c, const hzo::CG32Bufs& cb, hzo::PcgState* st_out,
+                  int* err_out) {
     const hzo::ResBufs rb = res_bufs(x, cb);
     // counters and flag start at zero (a fill kernel, not a memset node, in
     // case the caller captures the stream)
@@ -486,14 +496,28 @@ int cgres_solve(hzo_ctx* x, const hzo::StepConsts& c, const hzo::CG32Bufs& cb, i
     if (rc) return rc;
     if (le != hipSuccess)
         return set_error(HZO_ERR_HIP, "resident CG launch failed: %s", hipGetErrorString(le));
-    HZO_HIP(hipMemcpyAsync(x->st_host, cb.st, sizeof(hzo::PcgState), hipMemcpyDeviceToHost, x->stream));
-    HZO_HIP(hipMemcpyAsync(x->res_err_host, rb.err, sizeof(int), hipMemcpyDeviceToHost, x->stream));
-    HZO_HIP(hipStreamSynchronize(x->stream));
-    if (*x->res_err_host) return set_error(HZO_ERR_HIP, "resident CG: a grid barrier timed out");
+    HZO_HIP(hipMemcpyAsync(st_out, cb.st, sizeof(hzo::PcgState), hipMemcpyDeviceToHost, x->stream));
+    HZO_HIP(hipMemcpyAsync(err_out, rb.err, sizeof(int), hipMemcpyDeviceToHost, x->stream));
+    return HZO_OK;
+}
+
+// The host's side of a finished resident solve whose control block and
+// timeout flag are st and err.
+int cgres_collect(hzo_ctx* x, const hzo::PcgState& st, int err, int& last_iters, hzo_pcg_stats* stats) {
+    if (err) return set_error(HZO_ERR_HIP, "resident CG: a grid barrier timed out");
+    if (&st != x->st_host) *x->st_host = st;
     res_after(x);
     const hzo::PcgState& s = *x->st_host;
     last_iters = s.iters + 2 * s.refinements + 1;
     return copy_stats(s, stats);
+}
+
+int cgres_solve(hzo_ctx* x, const hzo::StepConsts& c, const hzo::CG32Bufs& cb, int& last_iters,
+                hzo_pcg_stats* stats) {
+    int rc = cgres_enqueue(x, c, cb, x->st_host, x->res_err_host);
+    if (rc) return rc;
+    HZO_HIP(hipStreamSynchronize(x->stream));
+    return cgres_collect(x, *x->st_host, *x->res_err_host, last_iters, stats);
 }
 
 // FP32-refined solve by the single-reduction CG: the setup (r = b - A x,
@@ -501,13 +525,16 @@ int cgres_solve(hzo_ctx* x, const hzo::StepConsts& c, const hzo::CG32Bufs& cb, i
 // device deciding from the control block whether a launch iterates,
 // recomputes the true residual in FP64, or re-forms w after it.
 int cg32_solve(hzo_ctx* x, const hzo::Fp32Bufs& b32in, const hzo::StepConsts& c, const hzo::PcgState& init,
-               const double* rhs, int& last_iters, hzo_pcg_stats* stats) {
+               const double* rhs, int& last_iters, hzo_pcg_stats* stats, int ring_slot = -1) {
     hzo::Fp32Bufs b32 = b32in;
     b32.r = x->r32;
     const hzo::CG32Bufs cb = cg_bufs(x, b32);
     int rc = timed(x, 1, [&](const hzo::LaunchEv& ev) { hzo::launch_pcg_setup_f32(x->stream, x->g, c, b32, init, rhs, ev); });
     if (rc) return rc;
     fold_after(x, x->stream, 0);
+    // ring_slot >= 0: a resident solve is only enqueued (hzo_run collects it)
+    if (x->cg_res && ring_slot >= 0)
+        return cgres_enqueue(x, c, cb, x->res_ring + ring_slot, x->res_err_ring + ring_slot);
     if (x->cg_res) return cgres_solve(x, c, cb, last_iters, stats);
     // launches: one WCOMP, one per iteration, two per residual replacement
     // and one before it or the end when an iteration's x_lo term is still
@@ -641,6 +668,8 @@ void free_all(hzo_ctx* x) {
     (void)hipFree(x->res_part);
     (void)hipFree(x->res_sync);
     if (x->res_err_host) (void)hipHostFree(x->res_err_host);
+    if (x->res_ring) (void)hipHostFree(x->res_ring);
+    if (x->res_err_ring) (void)hipHostFree(x->res_err_ring);
     for (int c = 0; c < HZO_KCLASSES; ++c)
         for (int q = 0; q < 2; ++q)
             for (auto e : x->ev[c][q]) (void)hipEventDestroy(e);
@@ -903,7 +932,9 @@ int hzo_create(const hzo_config* cfg, hzo_ctx** out) {
         HZO_ALLOC(x->res_xchd, sizeof(double) * (size_t)x->res_G * 4 * cfg->nz * 16);
         HZO_ALLOC(x->res_part, sizeof(double) * 2 * (size_t)x->res_G * 4);
         HZO_ALLOC(x->res_sync, sizeof(unsigned) * (hzo::RES_SYNC_WORDS + 64));
-        if (hipHostMalloc((void**)&x->res_err_host, sizeof(int), hipHostMallocDefault) != hipSuccess)
+        if (hipHostMalloc((void**)&x->res_err_host, sizeof(int), hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void**)&x->res_ring, sizeof(hzo::PcgState) * RUN_RING, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void**)&x->res_err_ring, sizeof(int) * RUN_RING, hipHostMallocDefault) != hipSuccess)
             return fail(set_error(HZO_ERR_NOMEM, "hzo_create: pinned host allocation failed"));
     }
 #undef HZO_ALLOC
@@ -1087,6 +1118,31 @@ int hzo_run(hzo_ctx* x, const double* Ez, int nsteps, int* its) {
     if (rc) return rc;
     if (nsteps < 0 || (nsteps > 0 && !Ez)) return set_error(HZO_ERR_ARG, "hzo_run: bad argument");
     HZO_HIP(hipSetDevice(x->device));
+    if (x->cg_res && x->cfg.dep_flag) {
+        // Resident film: the steps are enqueued back to back (TDGL, setup,
+        // resident solve, the copies of its control block and timeout flag
+        // into a ring slot), and the host waits once per RUN_RING steps
+        // instead of after every solve.  A timed-out grid barrier is
+        // reported when its group of steps is collected.
+        for (int s0 = 0; s0 < nsteps; s0 += RUN_RING) {
+            const int m = nsteps - s0 < RUN_RING ? nsteps - s0 : RUN_RING;
+            for (int j = 0; j < m; ++j) {
+                rc = tdgl(x, Ez[s0 + j]);
+                if (rc) return rc;
+                rc = cg32_solve(x, bufs32(x), x->c, pcg_init(x), nullptr, x->last_iters, nullptr, j);
+                if (rc) return rc;
+            }
+            HZO_HIP(hipStreamSynchronize(x->stream));
+            for (int j = 0; j < m; ++j) {
+                hzo_pcg_stats st;
+                memset(&st, 0, sizeof(st));
+                rc = cgres_collect(x, x->res_ring[j], x->res_err_ring[j], x->last_iters, &st);
+                if (rc) return rc;
+                if (its) its[s0 + j] = st.iters;
+            }
+        }
+        return HZO_OK;
+    }
     for (int s = 0; s < nsteps; ++s) {
         hzo_pcg_stats st;
         memset(&st, 0, sizeof(st));

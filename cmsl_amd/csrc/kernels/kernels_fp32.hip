@@ -86,10 +86,11 @@ __device__ __forceinline__ void tdgl_f32_body(const Fp32Bufs& b, const StepConst
             for (int q = 0; q < 6; ++q) sP[sb][q][tp.hy][tp.hx] = ring.P[q];
             sphi[sb][tp.hy][tp.hx] = ring.phi;
         }
-        // the next plane: its halo cell, the column two planes up, its cell word
-        Col32 ring_n, above_n;
+        // the next plane: the column two planes up and its cell word now, its
+        // halo cell (an L2 hit, loaded after this plane's evaluation so its
+        // registers are free during it)
+        Col32 above_n;
         const int kn = min(k + 1, k1 - 1);
-        ld(hc, kn, ring_n);
         ld(cc, min(k + 2, nz - 1), above_n);
         const uint32_t cw_n = b.cw[cc + plane * kn];
         float T9[9];
@@ -161,10 +162,10 @@ __device__ __forceinline__ void tdgl_f32_body(const Fp32Bufs& b, const StepConst
                 b.Pnew[5 * N + id] = sz * om + A * sum[5] - cL * dsz;
             }
         }
+        ld(hc, kn, ring);
         below = cur;
         cur = above;
         above = above_n;
-        ring = ring_n;
         cw = cw_n;
         {
             const float* Tr = b.TR32 + (long long)(cw & CW_GID_MASK) * 9;
