@@ -40,6 +40,20 @@ __device__ __forceinline__ float row_Ax32(unsigned code, const HPair<float>* sHP
     return row_apply32(coef_of(code, sHP), ih2, xc, xe, xw, xn, xs, xu, xd);
 }
 
+// a is a multiple of b (a >= 0, b > 0), the replacement-period test of the
+// single-reduction CG: a quotient from the hardware reciprocal, corrected by
+// one step (exact while a and b stay below 2^22, where the reciprocal's
+// error moves the quotient by at most one), instead of the integer-division
+// sequence a run-time divisor costs.
+__device__ __forceinline__ bool multiple_of(int a, int b) {
+    if (a >= (1 << 22) || b >= (1 << 22)) return a % b == 0;
+    int q = (int)((float)a * __builtin_amdgcn_rcpf((float)b));
+    int r = a - q * b;
+    if (r < 0) r += b;
+    else if (r >= b) r -= b;
+    return r == 0;
+}
+
 // The loop logic after a completed reduction (every lane of every
 // workgroup computes the same values).  Returns the work mode: 0 ITER,
 // 1 done, 2 RESID, 3 WCOMP.  Sets alpha/beta for ITER.
@@ -78,7 +92,7 @@ __device__ __forceinline__ int cg_decide(PcgState& s, const double (&tot)[3], do
             s.status = HZO_PCG_MAXITER;
             return 1;
         }
-        if (prev == K_XADD || !keep_going(s, g, rr) || (s.refine_every > 0 && s.iters % s.refine_every == 0))
+        if (prev == K_XADD || !keep_going(s, g, rr) || (s.refine_every > 0 && multiple_of(s.iters, s.refine_every)))
             return 2;
     }
     // prev == K_WCOMP (fresh start when iters == 0) or K_ITER

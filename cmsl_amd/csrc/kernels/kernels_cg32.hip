@@ -57,7 +57,7 @@ __device__ __forceinline__ void dot_w(double& acc, float x, float y, bool mirror
 // residual replacement never defers, so the replacement sees all of x.
 __device__ __forceinline__ int cg_x_mode(const PcgState& s) {
     if (s.xpend) return 2;
-    return (s.refine_every > 0 && (s.iters + 1) % s.refine_every == 0) ? 1 : 0;
+    return (s.refine_every > 0 && multiple_of(s.iters + 1, s.refine_every)) ? 1 : 0;
 }
 
 // The control-block update after cg_decide.  A solve that stops, or replaces

@@ -225,6 +225,18 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
         sR[e] = 0.0f;
     }
 
+    // A pass's partial sums: each wave's sums go to its LDS slots here; the
+    // workgroup barrier that starts the edge-column publication makes them
+    // visible, and thread 0 adds the slots in wave order after it (the
+    // operations of wg_total below), so the pass spends no barrier of its own
+    // on them.
+    auto wave_slots = [&](double (&a)[3], double (&S)[3][RNW]) {
+#pragma unroll
+        for (int qq = 0; qq < 3; ++qq) {
+            const double ws = wave_sum(a[qq]);
+            if ((tid & 63) == 0) S[qq][wv] = ws;
+        }
+    };
     // Workgroup totals of per-thread values in a fixed order (waves, then
     // the wave slots in order): the same bits in every thread.
     auto wg_total = [&](double (&a)[3], double (&S)[3][RNW], double (&t)[3]) {
@@ -276,7 +288,7 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
         // the decision after this iteration will be a replacement (or the
         // end) when its count is a multiple of the replacement period
         fold_now = __builtin_amdgcn_readfirstlane(
-                       (m == 0 && st.refine_every > 0 && st.iters % st.refine_every == 0) ? 1 : 0) != 0;
+                       (m == 0 && st.refine_every > 0 && multiple_of(st.iters, st.refine_every)) ? 1 : 0) != 0;
     };
 
     // The setup's partial sums (r.z, r.r, b.b) and the first decision.
@@ -445,6 +457,7 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
                 acc[2] += (double)r[kk] * (double)r[kk];
                 eT[k][typ][txp] = wvl;
             }
+            wave_slots(acc, sredA);
             if (fold_now) {
                 // the iteration a periodic replacement follows: the
                 // replacement's first pass is done here (x_hi loaded ahead
@@ -535,15 +548,16 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
                 acc[1] += (double)rv * (double)rv;
                 eT[k][typ][txp] = rv;
             }
+            wave_slots(acc, sredA);
             publish_staged(slot);
         }
-        if (partials) {
-            double t[3];
-            wg_total(acc, sredA, t);
-            if (tid == 0) {
+        if (partials && tid == 0) {
 #pragma unroll
-                for (int qq = 0; qq < 3; ++qq)
-                    bstore(RP, ((slot * (unsigned)G + (unsigned)blk) * 4u + (unsigned)qq) * 8u, 0u, t[qq]);
+            for (int qq = 0; qq < 3; ++qq) {
+                double v = sredA[qq][0];
+#pragma unroll
+                for (int u = 1; u < RNW; ++u) v += sredA[qq][u];
+                bstore(RP, ((slot * (unsigned)G + (unsigned)blk) * 4u + (unsigned)qq) * 8u, 0u, v);
             }
         }
         if (!res_barrier(b.sync, b.err, q, G, blk0, sok)) break;
