@@ -145,7 +145,10 @@ int hzo_depol_solve(hzo_ctx* ctx, hzo_pcg_stats* stats);
 int hzo_step(hzo_ctx* ctx, double Ez_ext, hzo_pcg_stats* stats);
 /* nsteps steps with applied field Ez[s]; its[s] (may be NULL) receives the PCG
  * iteration count of step s.  Issues no host synchronisation per step beyond
- * what the PCG stopping test needs. */
+ * what the PCG stopping test needs: a film that runs the register-resident
+ * solve enqueues up to 64 steps before it waits, so a failure of one of them
+ * (a grid barrier that timed out) is returned when its group of steps is
+ * collected, with the film's state then undefined. */
 int hzo_run(hzo_ctx* ctx, const double* Ez, int nsteps, int* its);
 /* Reference diagnostics (evolve.cu:259-357): out[10] = sum Pmz, sum Psz,
  * sum Pmz_local, sum Psz_local (grain frame, from the last TDGL call's input),
