@@ -251,6 +251,8 @@ void launch_diag_f64(hipStream_t s, const Grid& g, const double* P, const uint32
                  const LaunchEv& ev = {});
 void launch_tdgl_f32(hipStream_t s, const Grid& g, const StepConsts& c, double Ez_ext, const Fp32Bufs& b,
                  const LaunchEv& ev = {});
+// The FP32 TDGL with the applied field at ez (device memory), for captured steps.
+void launch_tdgl_f32_dev(hipStream_t s, const Grid& g, const StepConsts& c, const double* ez, const Fp32Bufs& b);
 void launch_pcg_setup_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b,
                           const PcgState& init, const double* rhs,
                  const LaunchEv& ev = {});

@@ -180,6 +180,13 @@ __global__ __launch_bounds__(NT, 4) void k_tdgl_f32(Fp32Bufs b, StepConsts c, fl
     tdgl_f32_body(b, c, Ez_ext, nb, tile_id());
 }
 
+// The same launch with the applied field read from device memory (*ez), so
+// one captured launch serves every step of a run.
+__global__ __launch_bounds__(NT, 4) void k_tdgl_f32_dev(Fp32Bufs b, StepConsts c, const double* __restrict__ ez,
+                                                     int nb) {
+    tdgl_f32_body(b, c, (float)*ez, nb, tile_id());
+}
+
 __global__ __launch_bounds__(NT, 4) void k_tdgl_f32_batch(const BatchFilm* __restrict__ films, const int2* __restrict__ tiles,
                                                        int parity) {
     TileId T;
@@ -658,6 +665,9 @@ __global__ __launch_bounds__(NT) void k_diag_f32_batch(const BatchFilm* __restri
 void launch_tdgl_f32(hipStream_t s, const Grid& g, const StepConsts& c, double Ez_ext, const Fp32Bufs& b,
                   const LaunchEv& ev) {
     hipExtLaunchKernelGGL(k_tdgl_f32, grid3(g), dim3(BX, BY, 1), 0, s, ev.start, ev.stop, 0, b, c, (float)Ez_ext, blocks_for_impl(g));
+}
+void launch_tdgl_f32_dev(hipStream_t s, const Grid& g, const StepConsts& c, const double* ez, const Fp32Bufs& b) {
+    hipLaunchKernelGGL(k_tdgl_f32_dev, grid3(g), dim3(BX, BY, 1), 0, s, b, c, ez, blocks_for_impl(g));
 }
 void launch_pcg_setup_f32(hipStream_t s, const Grid& g, const StepConsts& c, const Fp32Bufs& b,
                           const PcgState& init, const double* rhs,
