@@ -249,8 +249,9 @@ int hzo_checkpoint_load(hzo_ctx* ctx, const char* path, int* peloop, int* count,
  * kernel of the FP32-refined step (TDGL, PCG setup, single-reduction CG
  * iteration, diagnostics) runs once per group of films over a tile table
  * that maps every workgroup to (film, tile); each film keeps its own solver
- * control block, and the CG launches are replayed from a captured hipGraph
- * in chunks until every film of the group has converged.  Films are packed
+ * control block.  A group's step (TDGL, PCG setup, first chunk of CG
+ * launches) is replayed from a captured hipGraph, the further CG launches
+ * from a chunk graph until every film of the group has converged.  Films are packed
  * into groups whose CG working set fits a cache budget, and each group runs
  * all steps of a call before the next group starts; a group of one film runs
  * that film's own single-film kernels at its own tile depth.  No counterpart in the
@@ -273,8 +274,8 @@ typedef struct hzo_batch hzo_batch;
 /* tile_kz: 2, 4 or 8 (planes per workgroup tile); all films in one group. */
 int hzo_batch_create(hzo_ctx* const* films, int n, int tile_kz, hzo_batch** out);
 void hzo_batch_destroy(hzo_batch* batch);
-/* use_graphs: replay CG launches from a captured hipGraph (default 1);
- * chunk: CG launches per replay, even, >= 2 (default 16). */
+/* use_graphs: replay each step and its CG launches from captured hipGraphs
+ * (default 1); chunk: CG launches per replay, even, >= 2 (default 16). */
 int hzo_batch_config(hzo_batch* batch, int use_graphs, int chunk);
 /* Tile depth (2, 4 or 8) of the groups of several films, and group budget in
  * MiB of CG working set (34 bytes per voxel; 0: one group of all films). */
