@@ -135,3 +135,24 @@ def test_resident_refused_for_shapes_it_does_not_cover():
     inp2 = film_inputs(32, 32, 16, 5, seed=2)
     assert H.Film(inp2, precision=L.PREC_FP32_REFINED, **RELRES).cg_form == 3
     assert H.Film(inp2, precision=L.PREC_FP32_REFINED, tile_kz=8, **RELRES).cg_form in (1, 2)
+
+
+def test_run_enqueues_resident_steps_and_matches_single_steps():
+    # hzo_run on a resident film enqueues its steps 64 at a time and reads
+    # their control blocks afterwards; 70 steps cross one such group.  The
+    # same kernels run as with one hzo_step per step, so the states and the
+    # iteration counts are bit-identical.
+    inp = film_inputs(32, 32, 16, 6, fracs=(0.2, 0.6, 0.2), seed=17, jitter=0.2)
+    Ez = inp.field_nd(H.triangle_wave(40, 3.0, 70))
+    _say("resident 32x32x16: hzo_run of 70 steps against 70 hzo_step calls")
+    a = H.Film(inp, precision=L.PREC_FP32_REFINED, **RELRES)
+    b = H.Film(inp, precision=L.PREC_FP32_REFINED, **RELRES)
+    assert a.cg_form == 3 and b.cg_form == 3
+    its_a = a.run(Ez)
+    its_b = [b.step(float(e)).iters for e in Ez]
+    Pa, phia, Ea = a.get_state()
+    Pb, phib, Eb = b.get_state()
+    a.close()
+    b.close()
+    assert list(its_a) == its_b
+    assert np.array_equal(Pa, Pb) and np.array_equal(phia, phib) and np.array_equal(Ea, Eb)
