@@ -290,11 +290,13 @@ def run_adjoint(args):
 def run_batched(args):
     """Batched workload: args.batch_films films of uneven sizes, grain counts
     and Landau coefficients (+-20 %), FP32-refined, sinusoidal +-3 MV/cm over
-    1000 steps; all films advance together (one launch per kernel for the
-    whole batch or per cache-sized group of films, CG launches replayed from
-    a captured hipGraph; tile depth and group budget picked by the
-    autotuner).  Times args.batch_steps steps of the batch, then
-    the same number of following steps with the films stepped one by one."""
+    1000 steps; all films advance together (resident groups of small films
+    in one cooperative launch each, the other films in cache-sized groups
+    whose steps replay captured hipGraphs, shared over args.batch_lanes host
+    threads with a stream each; tile depth and group budget picked by the
+    autotuner unless args.batch_shape names them).  Times args.batch_steps
+    steps of the batch, then (unless args.batch_no_seq) the same number of
+    following steps with the films stepped one by one."""
     from concurrent.futures import ThreadPoolExecutor
 
     import cmsl_amd as H
@@ -314,8 +316,13 @@ def run_batched(args):
     Ez = np.stack([inp.field_nd(wave) for inp in inps], axis=1)  # (steps, films)
     # under a profiler tool the library refuses graph replay; the plain
     # launches run the same kernels
-    batch = H.Batch(films, tile_kz=2, use_graphs=not os.environ.get("ROCP_TOOL_LIBRARIES"))
-    shape, tune_ms = batch.autotune(Ez[0], reps=2)
+    batch = H.Batch(films, tile_kz=2, use_graphs=not os.environ.get("ROCP_TOOL_LIBRARIES"), lanes=args.batch_lanes)
+    if args.batch_shape:
+        kz_b, mib_b = (float(v) for v in args.batch_shape.split(","))
+        batch.set_shape(int(kz_b), mib_b)
+        shape, tune_ms = batch.shape, {}
+    else:
+        shape, tune_ms = batch.autotune(Ez[0], reps=2)
     batch.run(Ez[:W])
     cnt0 = batch.counters()
     t0 = time.perf_counter()
@@ -324,11 +331,13 @@ def run_batched(args):
     cnt = batch.counters()
     cnt = {k: cnt[k] - cnt0[k] for k in cnt}
     batch.close()
-    t0 = time.perf_counter()
-    its_seq = np.stack([f.run(Ez[W + S:W + 2 * S, q]) for q, f in enumerate(films)], axis=1)
-    for f in films:
-        f.synchronize()
-    dt_seq = time.perf_counter() - t0
+    its_seq, dt_seq = None, None
+    if not args.batch_no_seq:
+        t0 = time.perf_counter()
+        its_seq = np.stack([f.run(Ez[W + S:W + 2 * S, q]) for q, f in enumerate(films)], axis=1)
+        for f in films:
+            f.synchronize()
+        dt_seq = time.perf_counter() - t0
     # algorithmic bytes: TDGL + PCG setup per voxel, plus one fused CG
     # launch per iteration
     per_vox = algorithmic_bytes_per_cell("tdgl", "fp32") + algorithmic_bytes_per_cell("pcg_setup", "fp32")
@@ -345,11 +354,12 @@ def run_batched(args):
             "films": args.batch_films, "cells_total": ntot, "film_shapes": sizes,
             "value": ntot * S / dt_batch / 1e6, "unit": "Mcell-steps/s",
             "ms_per_step": dt_batch / S * 1e3,
-            "sequential_value": ntot * S / dt_seq / 1e6,
-            "speedup_vs_sequential": dt_seq / dt_batch,
-            "pcg_iters_mean": float(its.mean()), "pcg_iters_mean_sequential": float(its_seq.mean()),
+            "sequential_value": ntot * S / dt_seq / 1e6 if dt_seq else None,
+            "speedup_vs_sequential": dt_seq / dt_batch if dt_seq else None,
+            "pcg_iters_mean": float(its.mean()),
+            "pcg_iters_mean_sequential": float(its_seq.mean()) if its_seq is not None else None,
             "algorithmic_GBs": bytes_batch / dt_batch / 1e9,
-            "shape": shape,
+            "shape": shape, "lanes": args.batch_lanes,
             "autotune_ms_per_step": {f"kz{k}_group{m}MiB": v for (k, m), v in tune_ms.items()},
             "cg_launches_timed": cnt["cg_launches"],
             "unconverged_solves": cnt["unconverged"], "t_inputs_s": t_inputs}
@@ -417,6 +427,11 @@ def main():
                     help="timed steps of the 512x512x64 large-film workload (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=8)
     ap.add_argument("--batch-warmup", type=int, default=2)
+    ap.add_argument("--batch-lanes", type=int, default=2,
+                    help="host threads (own streams) that run the batch's streaming groups side by side")
+    ap.add_argument("--batch-shape", default="",
+                    help="'KZ,MIB': tile depth and group budget of the batch instead of the autotuner's choice")
+    ap.add_argument("--batch-no-seq", action="store_true", help="skip the film-by-film leg of the batched workload")
     ap.add_argument("--no-loop", action="store_true", help="skip the full-loop FP32-vs-FP64 error run")
     ap.add_argument("--stream-probe", action="store_true",
                     help="only time a plain kernel with the single-reduction CG's per-voxel traffic "

@@ -115,6 +115,7 @@ SIGNATURES = {
     "hzo_batch_create": (_I, [_P, _I, _I, _P]),
     "hzo_batch_destroy": (None, [_P]),
     "hzo_batch_config": (_I, [_P, _I, _I]),
+    "hzo_batch_set_lanes": (_I, [_P, _I]),
     "hzo_batch_tile_depth": (_I, [_P]),
     "hzo_batch_film_group": (_I, [_P, _I, _P]),
     "hzo_batch_set_shape": (_I, [_P, _I, _D]),

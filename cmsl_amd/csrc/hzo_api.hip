@@ -7,6 +7,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "host/errors.h"
@@ -1510,6 +1512,10 @@ int hzo_checkpoint_load(hzo_ctx* x, const char* path, int* peloop, int* count, d
 // each group's kernels run as one launch over a (film, tile) table, and the
 // groups run one after another, each through all steps of a call, so that a
 // group's CG vectors stay in the 256 MB last-level cache across iterations.
+// Host threads (each with its own stream) a batch may run its streaming
+// groups on side by side.
+constexpr int BATCH_MAX_LANES = 4;
+
 struct hzo_batch {
     struct Group {
         std::vector<int> idx;  // batch indices of the films
@@ -1538,20 +1544,29 @@ struct hzo_batch {
         hzo::ResFilm* dres = nullptr;  // per-film descriptors (device)
         unsigned* rsync = nullptr;     // per-film barrier words, then one timeout flag
     };
+    // A lane: a stream and the staging buffers with which one host thread
+    // runs its share of the streaming groups; lane 0 also runs the resident
+    // groups and the batch's own copies.
+    struct Lane {
+        hipStream_t stream = nullptr;
+        double* dez = nullptr;
+        double* hez = nullptr;  // pinned
+        int* dinfo = nullptr;   // BATCH_INFO ints per film: done, iters, status, refinements, xadds
+        int* hinfo = nullptr;   // pinned
+        double* dm = nullptr;
+        double* hm = nullptr;   // pinned
+        long long launches = 0, unconverged = 0;  // of the lane's groups in the current call
+    };
     std::vector<hzo_ctx*> films;
     int n = 0;
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // lanes[0].stream
     int kz = 2;
     double budget = 0.0;  // bytes of CG working set per group (0: one group)
     std::vector<Group> groups;
-    double* dez = nullptr;
-    double* hez = nullptr;  // pinned
-    int* dinfo = nullptr;   // BATCH_INFO ints per film: done, iters, status, refinements, xadds
-    int* hinfo = nullptr;   // pinned
-    double* dm = nullptr;
-    double* hm = nullptr;   // pinned
-    int chunk = 16;         // CG launches per graph replay (even)
+    std::vector<Lane> lanes;  // BATCH_MAX_LANES of them
+    int nlanes = 2;           // lanes that run streaming groups side by side
+    int chunk = 16;           // CG launches per graph replay (even)
     bool graphs = true;
     long long launches_total = 0;  // CG launches of all batched runs so far
     long long unconverged = 0;     // film solves that hit the launch cap
@@ -1591,13 +1606,17 @@ void batch_free_groups(hzo_batch* B) {
 
 void batch_free(hzo_batch* B) {
     batch_free_groups(B);
-    (void)hipFree(B->dez);
-    (void)hipFree(B->dinfo);
-    (void)hipFree(B->dm);
-    if (B->hez) (void)hipHostFree(B->hez);
-    if (B->hinfo) (void)hipHostFree(B->hinfo);
-    if (B->hm) (void)hipHostFree(B->hm);
-    if (B->stream) (void)hipStreamDestroy(B->stream);
+    for (hzo_batch::Lane& Ln : B->lanes) {
+        (void)hipFree(Ln.dez);
+        (void)hipFree(Ln.dinfo);
+        (void)hipFree(Ln.dm);
+        if (Ln.hez) (void)hipHostFree(Ln.hez);
+        if (Ln.hinfo) (void)hipHostFree(Ln.hinfo);
+        if (Ln.hm) (void)hipHostFree(Ln.hm);
+        if (Ln.stream) (void)hipStreamDestroy(Ln.stream);
+    }
+    B->lanes.clear();
+    B->stream = nullptr;
 }
 
 hzo::Fp32Bufs film_bufs32(hzo_ctx* x, int cur) {
@@ -1789,7 +1808,7 @@ int batch_tables(hzo_batch* B, int kz, double budget) {
 
 // Re-reads each film's P parity into the group's descriptor table (contexts
 // may have been stepped on their own since the last batched run).
-int group_sync_parity(hzo_batch* B, hzo_batch::Group& G) {
+int group_sync_parity(hzo_batch* B, hzo_batch::Group& G, hipStream_t st) {
     if (G.single >= 0 || G.res) return HZO_OK;
     for (size_t j = 0; j < G.idx.size(); ++j) {
         hzo_ctx* x = B->films[G.idx[j]];
@@ -1801,43 +1820,44 @@ int group_sync_parity(hzo_batch* B, hzo_batch::Group& G) {
             F.b[q].Pcur = b.Pcur;
         }
     }
-    HZO_HIP(hipMemcpy(G.dfilms, G.hf.data(), sizeof(hzo::BatchFilm) * G.idx.size(), hipMemcpyHostToDevice));
+    // ordered on the lane's stream (G.hf lives as long as the group)
+    HZO_HIP(hipMemcpyAsync(G.dfilms, G.hf.data(), sizeof(hzo::BatchFilm) * G.idx.size(), hipMemcpyHostToDevice, st));
     return HZO_OK;
 }
 
 // K launches of the single-reduction CG for the group (launch index 1..K).
-void group_cg_launches(hzo_batch* B, hzo_batch::Group& G, int K) {
+void group_cg_launches(hzo_batch* B, hzo_batch::Group& G, int K, hipStream_t st) {
     if (G.single >= 0) {
         hzo_ctx* x = B->films[G.single];
         const hzo::CG32Bufs cb = cg_bufs(x, bufs32(x));
         for (int t = 1; t <= K; ++t) {
-            hzo::launch_cg32(B->stream, x->g, x->c, cb, t, x->cg_two, x->cg_nt);
-            fold_after(x, B->stream, t);
+            hzo::launch_cg32(st, x->g, x->c, cb, t, x->cg_two, x->cg_nt);
+            fold_after(x, st, t);
         }
     } else {
         for (int t = 1; t <= K; ++t)
-            hzo::launch_cg32_batch(B->stream, G.ntiles, B->kz, G.pm, G.mirrored, G.dfilms, G.dtiles, t);
+            hzo::launch_cg32_batch(st, G.ntiles, B->kz, G.pm, G.mirrored, G.dfilms, G.dtiles, t);
     }
 }
 
-int group_cg_chunk(hzo_batch* B, hzo_batch::Group& G) {
+int group_cg_chunk(hzo_batch* B, hzo_batch::Group& G, hipStream_t st) {
     const int K = B->chunk;
     if (B->graphs) {
         if (!G.gexec || G.g_chunk != K) {
             if (G.gexec) (void)hipGraphExecDestroy(G.gexec);
             G.gexec = nullptr;
             hipGraph_t g;
-            HZO_HIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
-            group_cg_launches(B, G, K);
-            HZO_HIP(hipStreamEndCapture(B->stream, &g));
+            HZO_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            group_cg_launches(B, G, K, st);
+            HZO_HIP(hipStreamEndCapture(st, &g));
             const hipError_t e = hipGraphInstantiate(&G.gexec, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             HZO_HIP(e);
             G.g_chunk = K;
         }
-        HZO_HIP(hipGraphLaunch(G.gexec, B->stream));
+        HZO_HIP(hipGraphLaunch(G.gexec, st));
     } else {
-        group_cg_launches(B, G, K);
+        group_cg_launches(B, G, K, st);
         HZO_HIP(hipGetLastError());
     }
     return HZO_OK;
@@ -1846,23 +1866,23 @@ int group_cg_chunk(hzo_batch* B, hzo_batch::Group& G) {
 // Solver statistics of the group's films after a chunk (launch index K) into
 // hinfo[BATCH_INFO*j..]: done, iterations, status, residual replacements,
 // launches that only added a deferred x_lo term.
-int group_poll(hzo_batch* B, hzo_batch::Group& G) {
+int group_poll(hzo_batch* B, hzo_batch::Group& G, hzo_batch::Lane& Ln) {
     const int n = (int)G.idx.size();
     if (G.single >= 0) {
         hzo_ctx* x = B->films[G.single];
         HZO_HIP(hipMemcpyAsync(x->st_host, x->st + (B->chunk & 1), sizeof(hzo::PcgState), hipMemcpyDeviceToHost,
-                               B->stream));
-        HZO_HIP(hipStreamSynchronize(B->stream));
-        B->hinfo[0] = x->st_host->done;
-        B->hinfo[1] = x->st_host->iters;
-        B->hinfo[2] = x->st_host->status;
-        B->hinfo[3] = x->st_host->refinements;
-        B->hinfo[4] = x->st_host->xadds;
+                               Ln.stream));
+        HZO_HIP(hipStreamSynchronize(Ln.stream));
+        Ln.hinfo[0] = x->st_host->done;
+        Ln.hinfo[1] = x->st_host->iters;
+        Ln.hinfo[2] = x->st_host->status;
+        Ln.hinfo[3] = x->st_host->refinements;
+        Ln.hinfo[4] = x->st_host->xadds;
         return HZO_OK;
     }
-    hzo::launch_batch_done(B->stream, G.dfilms, n, B->chunk, B->dinfo);
-    HZO_HIP(hipMemcpyAsync(B->hinfo, B->dinfo, sizeof(int) * hzo::BATCH_INFO * n, hipMemcpyDeviceToHost, B->stream));
-    HZO_HIP(hipStreamSynchronize(B->stream));
+    hzo::launch_batch_done(Ln.stream, G.dfilms, n, B->chunk, Ln.dinfo);
+    HZO_HIP(hipMemcpyAsync(Ln.hinfo, Ln.dinfo, sizeof(int) * hzo::BATCH_INFO * n, hipMemcpyDeviceToHost, Ln.stream));
+    HZO_HIP(hipStreamSynchronize(Ln.stream));
     return HZO_OK;
 }
 
@@ -1871,45 +1891,46 @@ int group_poll(hzo_batch* B, hzo_batch::Group& G) {
 // launch index t restarts at 1 in each chunk (chunks are even, so the
 // control-block parity continues).  hinfo[BATCH_INFO*j..] receives film j's solver
 // statistics.
-int group_step_res(hzo_batch* B, hzo_batch::Group& G, const double* Ez, int s, bool dep, int& launches) {
+int group_step_res(hzo_batch* B, hzo_batch::Group& G, hzo_batch::Lane& Ln, const double* Ez, int s, bool dep,
+                   int& launches) {
     const int n = (int)G.idx.size();
     for (int f : G.idx) {
         hzo_ctx* x = B->films[f];
-        hzo::launch_tdgl_f32(B->stream, x->g, x->c, Ez[(size_t)s * B->n + f], bufs32(x));
+        hzo::launch_tdgl_f32(Ln.stream, x->g, x->c, Ez[(size_t)s * B->n + f], bufs32(x));
         x->cur ^= 1;
     }
     HZO_HIP(hipGetLastError());
     launches = 0;
     if (!dep) {
-        memset(B->hinfo, 0, sizeof(int) * hzo::BATCH_INFO * n);
+        memset(Ln.hinfo, 0, sizeof(int) * hzo::BATCH_INFO * n);
         return HZO_OK;
     }
     for (int f : G.idx) {
         hzo_ctx* x = B->films[f];
-        hzo::launch_pcg_setup_f32(B->stream, x->g, x->c, bufs32(x), pcg_init(x), nullptr);
-        fold_after(x, B->stream, 0);
+        hzo::launch_pcg_setup_f32(Ln.stream, x->g, x->c, bufs32(x), pcg_init(x), nullptr);
+        fold_after(x, Ln.stream, 0);
     }
     // barrier words and the timeout flag start at zero (a fill kernel, not a
     // memset node, in case the caller captures the stream)
-    hzo::launch_fill_u32(B->stream, G.rsync, n * hzo::RES_SYNC_WORDS + 64, 0u);
+    hzo::launch_fill_u32(Ln.stream, G.rsync, n * hzo::RES_SYNC_WORDS + 64, 0u);
     HZO_HIP(hipGetLastError());
-    const hipError_t le = hzo::launch_cgres_films(B->stream, G.dres, n, G.res_tiles, G.res_nz, G.mirrored);
+    const hipError_t le = hzo::launch_cgres_films(Ln.stream, G.dres, n, G.res_tiles, G.res_nz, G.mirrored);
     if (le != hipSuccess) return set_error(HZO_ERR_HIP, "resident CG launch failed: %s", hipGetErrorString(le));
     launches = 1;
     hzo_ctx* x0 = B->films[G.idx[0]];
     for (int f : G.idx) {
         hzo_ctx* x = B->films[f];
-        HZO_HIP(hipMemcpyAsync(x->st_host, x->st, sizeof(hzo::PcgState), hipMemcpyDeviceToHost, B->stream));
+        HZO_HIP(hipMemcpyAsync(x->st_host, x->st, sizeof(hzo::PcgState), hipMemcpyDeviceToHost, Ln.stream));
     }
     HZO_HIP(hipMemcpyAsync(x0->res_err_host, G.rsync + (size_t)n * hzo::RES_SYNC_WORDS, sizeof(int),
-                           hipMemcpyDeviceToHost, B->stream));
-    HZO_HIP(hipStreamSynchronize(B->stream));
+                           hipMemcpyDeviceToHost, Ln.stream));
+    HZO_HIP(hipStreamSynchronize(Ln.stream));
     if (*x0->res_err_host) return set_error(HZO_ERR_HIP, "resident CG: a grid barrier timed out");
     for (int j = 0; j < n; ++j) {
         hzo_ctx* x = B->films[G.idx[j]];
         res_after(x);
         const hzo::PcgState& st = *x->st_host;
-        int* h = B->hinfo + hzo::BATCH_INFO * j;
+        int* h = Ln.hinfo + hzo::BATCH_INFO * j;
         h[0] = st.done;
         h[1] = st.iters;
         h[2] = st.status;
@@ -1920,31 +1941,32 @@ int group_step_res(hzo_batch* B, hzo_batch::Group& G, const double* Ez, int s, b
 }
 
 // The launches of one step of a streaming group whose films' P parity is
-// par: TDGL of every film (the applied fields read from B->dez), then, with
+// par: TDGL of every film (the applied fields read from Ln.dez), then, with
 // depolarisation, the PCG setup and nchunks chunks of single-reduction CG
 // launches.  Captured into a graph or issued as they are; a single film's
 // context is left at its new parity either way.
-void group_step_launches(hzo_batch* B, hzo_batch::Group& G, int par, bool dep, int nchunks) {
+void group_step_launches(hzo_batch* B, hzo_batch::Group& G, hzo_batch::Lane& Ln, int par, bool dep, int nchunks) {
     const int n = (int)G.idx.size();
     if (G.single >= 0) {
         hzo_ctx* x = B->films[G.single];
-        hzo::launch_tdgl_f32_dev(B->stream, x->g, x->c, B->dez, film_bufs32(x, par));
+        hzo::launch_tdgl_f32_dev(Ln.stream, x->g, x->c, Ln.dez, film_bufs32(x, par));
         x->cur = par ^ 1;
         if (dep) {
-            hzo::launch_pcg_setup_f32(B->stream, x->g, x->c, bufs32(x), pcg_init(x), nullptr);
-            fold_after(x, B->stream, 0);
+            hzo::launch_pcg_setup_f32(Ln.stream, x->g, x->c, bufs32(x), pcg_init(x), nullptr);
+            fold_after(x, Ln.stream, 0);
         }
     } else {
-        hzo::launch_batch_set_ez(B->stream, G.dfilms, B->dez, n);
-        hzo::launch_tdgl_f32_batch(B->stream, G.ntiles, G.dfilms, G.dtiles, par);
-        if (dep) hzo::launch_setup_f32_batch(B->stream, G.ntiles, G.dfilms, G.dtiles, par ^ 1);
+        hzo::launch_batch_set_ez(Ln.stream, G.dfilms, Ln.dez, n);
+        hzo::launch_tdgl_f32_batch(Ln.stream, G.ntiles, G.dfilms, G.dtiles, par);
+        if (dep) hzo::launch_setup_f32_batch(Ln.stream, G.ntiles, G.dfilms, G.dtiles, par ^ 1);
     }
-    for (int q = 0; q < nchunks; ++q) group_cg_launches(B, G, B->chunk);
+    for (int q = 0; q < nchunks; ++q) group_cg_launches(B, G, B->chunk, Ln.stream);
 }
 
 // The executable graph of a streaming group's step at parity par with
 // nchunks CG chunks, captured on first use (at most eight kept per group).
-int group_step_graph(hzo_batch* B, hzo_batch::Group& G, int par, bool dep, int nchunks, hipGraphExec_t& out) {
+int group_step_graph(hzo_batch* B, hzo_batch::Group& G, hzo_batch::Lane& Ln, int par, bool dep, int nchunks,
+                     hipGraphExec_t& out) {
     for (const hzo_batch::Group::StepGraph& sg : G.sgraphs)
         if (sg.par == par && sg.nchunks == nchunks && sg.chunk == B->chunk) {
             out = sg.exec;
@@ -1955,9 +1977,9 @@ int group_step_graph(hzo_batch* B, hzo_batch::Group& G, int par, bool dep, int n
         G.sgraphs.erase(G.sgraphs.begin());
     }
     hipGraph_t g;
-    HZO_HIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
-    group_step_launches(B, G, par, dep, nchunks);
-    HZO_HIP(hipStreamEndCapture(B->stream, &g));
+    HZO_HIP(hipStreamBeginCapture(Ln.stream, hipStreamCaptureModeThreadLocal));
+    group_step_launches(B, G, Ln, par, dep, nchunks);
+    HZO_HIP(hipStreamEndCapture(Ln.stream, &g));
     hipGraphExec_t ex = nullptr;
     const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -1974,12 +1996,13 @@ int group_step_graph(hzo_batch* B, hzo_batch::Group& G, int par, bool dep, int n
 // launch index t restarts at 1 in each chunk (chunks are even, so the
 // control-block parity continues).  hinfo[BATCH_INFO*j..] receives film j's
 // solver statistics.
-int group_step(hzo_batch* B, hzo_batch::Group& G, const double* Ez, int s, bool dep, int& launches) {
-    if (G.res) return group_step_res(B, G, Ez, s, dep, launches);
+int group_step(hzo_batch* B, hzo_batch::Group& G, hzo_batch::Lane& Ln, const double* Ez, int s, bool dep,
+               int& launches) {
+    if (G.res) return group_step_res(B, G, Ln, Ez, s, dep, launches);
     const int n = (int)G.idx.size();
-    HZO_HIP(hipStreamSynchronize(B->stream));  // the previous step's copy out of hez has run
-    for (int j = 0; j < n; ++j) B->hez[j] = Ez[(size_t)s * B->n + G.idx[j]];
-    HZO_HIP(hipMemcpyAsync(B->dez, B->hez, sizeof(double) * n, hipMemcpyHostToDevice, B->stream));
+    HZO_HIP(hipStreamSynchronize(Ln.stream));  // the previous step's copy out of hez has run
+    for (int j = 0; j < n; ++j) Ln.hez[j] = Ez[(size_t)s * B->n + G.idx[j]];
+    HZO_HIP(hipMemcpyAsync(Ln.dez, Ln.hez, sizeof(double) * n, hipMemcpyHostToDevice, Ln.stream));
     const int par = G.single >= 0 ? B->films[G.single]->cur : (s & 1);
     const int nchunks = dep ? (G.need + B->chunk - 1) / B->chunk : 0;
     // The step graph holds the step's first chunk; the others replay the
@@ -1989,21 +2012,21 @@ int group_step(hzo_batch* B, hzo_batch::Group& G, const double* Ez, int s, bool 
     const int first = nchunks < 1 ? nchunks : 1;
     if (B->graphs) {
         hipGraphExec_t ex = nullptr;
-        int rc = group_step_graph(B, G, par, dep, first, ex);
+        int rc = group_step_graph(B, G, Ln, par, dep, first, ex);
         if (rc) return rc;
-        HZO_HIP(hipGraphLaunch(ex, B->stream));
+        HZO_HIP(hipGraphLaunch(ex, Ln.stream));
     } else {
-        group_step_launches(B, G, par, dep, first);
+        group_step_launches(B, G, Ln, par, dep, first);
         HZO_HIP(hipGetLastError());
     }
     if (G.single >= 0) B->films[G.single]->cur = par ^ 1;
     for (int q = first; q < nchunks; ++q) {
-        int rc = group_cg_chunk(B, G);
+        int rc = group_cg_chunk(B, G, Ln.stream);
         if (rc) return rc;
     }
     launches = nchunks * B->chunk;
     if (!dep) {
-        memset(B->hinfo, 0, sizeof(int) * hzo::BATCH_INFO * n);
+        memset(Ln.hinfo, 0, sizeof(int) * hzo::BATCH_INFO * n);
         return HZO_OK;
     }
     // the most launches any film may need: four per iteration
@@ -2013,12 +2036,12 @@ int group_step(hzo_batch* B, hzo_batch::Group& G, const double* Ez, int s, bool 
         maxl = m > maxl ? m : maxl;
     }
     while (true) {
-        int rc = group_poll(B, G);
+        int rc = group_poll(B, G, Ln);
         if (rc) return rc;
         bool all = true;
-        for (int j = 0; j < n; ++j) all = all && B->hinfo[hzo::BATCH_INFO * j];
+        for (int j = 0; j < n; ++j) all = all && Ln.hinfo[hzo::BATCH_INFO * j];
         if (all || launches >= maxl) break;
-        rc = group_cg_chunk(B, G);
+        rc = group_cg_chunk(B, G, Ln.stream);
         if (rc) return rc;
         launches += B->chunk;
     }
@@ -2026,7 +2049,7 @@ int group_step(hzo_batch* B, hzo_batch::Group& G, const double* Ez, int s, bool 
     // two per residual replacement, one per deferred-x_lo addition
     int need = 2;
     for (int j = 0; j < n; ++j) {
-        const int* h = B->hinfo + hzo::BATCH_INFO * j;
+        const int* h = Ln.hinfo + hzo::BATCH_INFO * j;
         const int m = h[1] + 2 * h[3] + h[4] + 2;
         need = m > need ? m : need;
     }
@@ -2035,41 +2058,123 @@ int group_step(hzo_batch* B, hzo_batch::Group& G, const double* Ez, int s, bool 
 }
 
 // nsteps steps of one group; leaves each film's P parity advanced by nsteps.
-int group_run(hzo_batch* B, hzo_batch::Group& G, const double* Ez, int nsteps, int* iters, double* m_out) {
-    int rc = group_sync_parity(B, G);
+int group_run(hzo_batch* B, hzo_batch::Group& G, hzo_batch::Lane& Ln, const double* Ez, int nsteps, int* iters,
+              double* m_out) {
+    int rc = group_sync_parity(B, G, Ln.stream);
     if (rc) return rc;
     const int n = (int)G.idx.size();
     const bool dep = B->films[0]->cfg.dep_flag != 0;
     for (int s = 0; s < nsteps; ++s) {
         int l = 0;
-        rc = group_step(B, G, Ez, s, dep, l);
+        rc = group_step(B, G, Ln, Ez, s, dep, l);
         if (rc) return rc;
-        B->launches_total += l;
+        Ln.launches += l;
         for (int j = 0; j < n; ++j) {
-            if (iters) iters[(size_t)s * B->n + G.idx[j]] = B->hinfo[hzo::BATCH_INFO * j + 1];
-            B->unconverged += dep && !B->hinfo[hzo::BATCH_INFO * j];
+            if (iters) iters[(size_t)s * B->n + G.idx[j]] = Ln.hinfo[hzo::BATCH_INFO * j + 1];
+            Ln.unconverged += dep && !Ln.hinfo[hzo::BATCH_INFO * j];
         }
         if (m_out) {
             if (G.single >= 0 || G.res) {
                 for (int j = 0; j < n; ++j) {
                     hzo_ctx* x = B->films[G.idx[j]];
-                    hzo::launch_diag_f32(B->stream, x->g, x->P32[x->cur], x->cw, x->diag_part);
-                    hzo::launch_sum_column(B->stream, x->g, x->diag_part, B->dm + j);
+                    hzo::launch_diag_f32(Ln.stream, x->g, x->P32[x->cur], x->cw, x->diag_part);
+                    hzo::launch_sum_column(Ln.stream, x->g, x->diag_part, Ln.dm + j);
                 }
             } else {
-                hzo::launch_diag_f32_batch(B->stream, G.ntiles, G.dfilms, G.dtiles, (s + 1) & 1);
-                hzo::launch_batch_sum_pmz(B->stream, G.dfilms, n, B->dm);
+                hzo::launch_diag_f32_batch(Ln.stream, G.ntiles, G.dfilms, G.dtiles, (s + 1) & 1);
+                hzo::launch_batch_sum_pmz(Ln.stream, G.dfilms, n, Ln.dm);
             }
             HZO_HIP(hipGetLastError());
-            HZO_HIP(hipMemcpyAsync(B->hm, B->dm, sizeof(double) * n, hipMemcpyDeviceToHost, B->stream));
-            HZO_HIP(hipStreamSynchronize(B->stream));
+            HZO_HIP(hipMemcpyAsync(Ln.hm, Ln.dm, sizeof(double) * n, hipMemcpyDeviceToHost, Ln.stream));
+            HZO_HIP(hipStreamSynchronize(Ln.stream));
             for (int j = 0; j < n; ++j)
-                m_out[(size_t)s * B->n + G.idx[j]] = B->hm[j] / (double)B->films[G.idx[j]]->g.N;
+                m_out[(size_t)s * B->n + G.idx[j]] = Ln.hm[j] / (double)B->films[G.idx[j]]->g.N;
         }
     }
-    HZO_HIP(hipStreamSynchronize(B->stream));
+    HZO_HIP(hipStreamSynchronize(Ln.stream));
     if (G.single < 0 && !G.res)
         for (int f : G.idx) B->films[f]->cur = (B->films[f]->cur + nsteps) & 1;
+    return HZO_OK;
+}
+
+// nsteps steps of every group.  The resident groups run first, one after
+// another on lane 0: a cooperative launch needs every workgroup of its grid
+// resident at once, so nothing else runs beside it.  The streaming groups
+// are then shared out over the lanes, longest first to the least loaded
+// lane (cost: voxels times the CG launches of the group's last step), and
+// each lane runs its share on its own stream from its own host thread
+// (lane 0 from the calling thread), so one group's launches fill the CUs
+// that another's leave idle (the tail of a launch, the gaps between
+// launches and while the host polls).  Groups share no buffers and each
+// film's work stays in order on one stream, so every film ends in the same
+// state whatever the number of lanes.
+int run_groups(hzo_batch* B, const double* Ez, int nsteps, int* iters, double* m_out) {
+    for (hzo_batch::Lane& Ln : B->lanes) Ln.launches = Ln.unconverged = 0;
+    auto tally = [&]() {
+        for (hzo_batch::Lane& Ln : B->lanes) {
+            B->launches_total += Ln.launches;
+            B->unconverged += Ln.unconverged;
+            Ln.launches = Ln.unconverged = 0;
+        }
+    };
+    std::vector<int> streaming;
+    for (size_t g = 0; g < B->groups.size(); ++g) {
+        hzo_batch::Group& G = B->groups[g];
+        if (!G.res) {
+            streaming.push_back((int)g);
+            continue;
+        }
+        int rc = group_run(B, G, B->lanes[0], Ez, nsteps, iters, m_out);
+        if (rc) {
+            tally();
+            return rc;
+        }
+    }
+    const int nl = std::max(1, std::min(B->nlanes, (int)streaming.size()));
+    std::vector<double> cost(B->groups.size(), 0.0), load(nl, 0.0);
+    for (int g : streaming) {
+        double v = 0.0;
+        for (int f : B->groups[g].idx) v += (double)B->films[f]->g.N;
+        cost[g] = v * (double)std::max(B->groups[g].need, 1);
+    }
+    std::stable_sort(streaming.begin(), streaming.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<std::vector<int>> share(nl);
+    for (int g : streaming) {
+        const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        share[l].push_back(g);
+        load[l] += cost[g];
+    }
+    std::vector<int> rcs(nl, HZO_OK);
+    std::vector<std::string> msgs(nl);
+    auto run_lane = [&](int l) {
+        for (int g : share[l]) {
+            rcs[l] = group_run(B, B->groups[g], B->lanes[l], Ez, nsteps, iters, m_out);
+            if (rcs[l]) {
+                msgs[l] = hzo::last_error();
+                return;
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int l = 1; l < nl; ++l) {
+        try {
+            th.emplace_back([&, l]() {
+                if (hipSetDevice(B->device) != hipSuccess) {
+                    rcs[l] = HZO_ERR_HIP;
+                    msgs[l] = "batch lane: hipSetDevice failed";
+                    return;
+                }
+                run_lane(l);
+            });
+        } catch (...) {
+            run_lane(l);  // no thread: the lane's share runs here, after the others have been issued
+        }
+    }
+    run_lane(0);
+    for (std::thread& t : th) t.join();
+    tally();
+    for (int l = 0; l < nl; ++l)
+        if (rcs[l]) return set_error(rcs[l], "%s", msgs[l].c_str());
     return HZO_OK;
 }
 
@@ -2099,15 +2204,21 @@ int hzo_batch_create(hzo_ctx* const* films, int n, int tile_kz, hzo_batch** out)
         delete B;
         return rc;
     };
-    if (hipSetDevice(B->device) != hipSuccess || hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess)
-        return fail(set_error(HZO_ERR_HIP, "hzo_batch_create: stream creation failed"));
-    if (hipMalloc((void**)&B->dez, sizeof(double) * n) != hipSuccess ||
-        hipMalloc((void**)&B->dinfo, sizeof(int) * hzo::BATCH_INFO * n) != hipSuccess ||
-        hipMalloc((void**)&B->dm, sizeof(double) * n) != hipSuccess ||
-        hipHostMalloc((void**)&B->hez, sizeof(double) * n, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&B->hinfo, sizeof(int) * hzo::BATCH_INFO * n, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&B->hm, sizeof(double) * n, hipHostMallocDefault) != hipSuccess)
-        return fail(set_error(HZO_ERR_NOMEM, "hzo_batch_create: allocation failed"));
+    if (hipSetDevice(B->device) != hipSuccess)
+        return fail(set_error(HZO_ERR_HIP, "hzo_batch_create: hipSetDevice failed"));
+    B->lanes.resize(BATCH_MAX_LANES);
+    for (hzo_batch::Lane& Ln : B->lanes) {
+        if (hipStreamCreateWithFlags(&Ln.stream, hipStreamNonBlocking) != hipSuccess)
+            return fail(set_error(HZO_ERR_HIP, "hzo_batch_create: stream creation failed"));
+        if (hipMalloc((void**)&Ln.dez, sizeof(double) * n) != hipSuccess ||
+            hipMalloc((void**)&Ln.dinfo, sizeof(int) * hzo::BATCH_INFO * n) != hipSuccess ||
+            hipMalloc((void**)&Ln.dm, sizeof(double) * n) != hipSuccess ||
+            hipHostMalloc((void**)&Ln.hez, sizeof(double) * n, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void**)&Ln.hinfo, sizeof(int) * hzo::BATCH_INFO * n, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void**)&Ln.hm, sizeof(double) * n, hipHostMallocDefault) != hipSuccess)
+            return fail(set_error(HZO_ERR_NOMEM, "hzo_batch_create: allocation failed"));
+    }
+    B->stream = B->lanes[0].stream;
     for (hzo_ctx* x : B->films)
         if (hipStreamSynchronize(x->stream) != hipSuccess)
             return fail(set_error(HZO_ERR_HIP, "hzo_batch_create: film stream synchronisation failed"));
@@ -2120,7 +2231,7 @@ int hzo_batch_create(hzo_ctx* const* films, int n, int tile_kz, hzo_batch** out)
 void hzo_batch_destroy(hzo_batch* B) {
     if (!B) return;
     (void)hipSetDevice(B->device);
-    (void)hipStreamSynchronize(B->stream);
+    for (hzo_batch::Lane& Ln : B->lanes) (void)hipStreamSynchronize(Ln.stream);
     batch_free(B);
     delete B;
 }
@@ -2145,6 +2256,13 @@ int hzo_batch_config(hzo_batch* B, int use_graphs, int chunk) {
     if (!B || chunk < 2 || (chunk & 1)) return set_error(HZO_ERR_ARG, "hzo_batch_config: chunk must be even and >= 2");
     B->graphs = use_graphs != 0;
     B->chunk = chunk;
+    return HZO_OK;
+}
+
+int hzo_batch_set_lanes(hzo_batch* B, int lanes) {
+    if (!B || lanes < 1 || lanes > BATCH_MAX_LANES)
+        return set_error(HZO_ERR_ARG, "hzo_batch_set_lanes: lanes must be 1 .. %d", BATCH_MAX_LANES);
+    B->nlanes = lanes;
     return HZO_OK;
 }
 
@@ -2186,11 +2304,7 @@ int hzo_batch_run(hzo_batch* B, const double* Ez, int nsteps, int* iters, double
     if (int rc = refuse_traced_replay(B, "hzo_batch_run")) return rc;
     HZO_HIP(hipSetDevice(B->device));
     for (hzo_ctx* x : B->films) HZO_HIP(hipStreamSynchronize(x->stream));
-    for (hzo_batch::Group& G : B->groups) {
-        int rc = group_run(B, G, Ez, nsteps, iters, m_out);
-        if (rc) return rc;
-    }
-    return HZO_OK;
+    return run_groups(B, Ez, nsteps, iters, m_out);
 }
 
 int hzo_batch_autotune(hzo_batch* B, const double* Ez, int reps, int* kz_out, double ms[12]) {
@@ -2244,9 +2358,7 @@ int hzo_batch_autotune(hzo_batch* B, const double* Ez, int reps, int* kz_out, do
         for (int r = 0; r <= reps && rc == HZO_OK; ++r) {  // r = 0 warms up (code objects, graphs)
             if ((rc = copy(false)) != HZO_OK) break;
             const auto t0 = std::chrono::steady_clock::now();
-            for (hzo_batch::Group& G : B->groups)
-                if ((rc = group_run(B, G, Ez, 1, nullptr, nullptr)) != HZO_OK) break;
-            if (rc) break;
+            if ((rc = run_groups(B, Ez, 1, nullptr, nullptr)) != HZO_OK) break;
             const double t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             if (r > 0 && t < best_ms[c]) best_ms[c] = t;
         }

@@ -488,11 +488,13 @@ class Film:
 
 class Batch:
     """Films of different sizes and coefficients stepped together on one GPU
-    (FP32-refined, single-reduction CG): one launch per kernel for the whole
-    batch, CG launches replayed from a captured hipGraph.  The films stay
-    owned by the caller and must outlive the batch."""
+    (FP32-refined, single-reduction CG): small films share one cooperative
+    resident launch per group, the others run in cache-sized groups whose
+    steps replay captured hipGraphs, shared out over `lanes` host threads
+    with a stream each.  The films stay owned by the caller and must outlive
+    the batch."""
 
-    def __init__(self, films, tile_kz: int = 2, use_graphs: bool = True, chunk: int = 16):
+    def __init__(self, films, tile_kz: int = 2, use_graphs: bool = True, chunk: int = 16, lanes: int = 2):
         self.films = list(films)
         arr = (C.c_void_p * len(self.films))(*[f.h.value for f in self.films])
         self._arr = arr
@@ -500,6 +502,7 @@ class Batch:
         L.call("hzo_batch_create", arr, len(self.films), int(tile_kz), C.byref(h))
         self.h = h
         L.call("hzo_batch_config", h, 1 if use_graphs else 0, int(chunk))
+        L.call("hzo_batch_set_lanes", h, int(lanes))
 
     @property
     def n(self) -> int:
@@ -529,6 +532,11 @@ class Batch:
         m = np.zeros((S, self.n)) if want_m else None
         L.call("hzo_batch_run", self.h, L.ptr(Ez), S, L.ptr(its), L.ptr(m))
         return (its, m) if want_m else its
+
+    def set_lanes(self, lanes: int):
+        """Host threads (1 .. 4), each with its own stream, that run the
+        streaming groups side by side; the resident groups run alone."""
+        L.call("hzo_batch_set_lanes", self.h, int(lanes))
 
     def set_shape(self, tile_kz: int, group_mib: float = 0.0):
         """Tile depth (2, 4 or 8) and group budget in MiB of CG working set (0: one group)."""

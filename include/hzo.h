@@ -277,6 +277,11 @@ void hzo_batch_destroy(hzo_batch* batch);
 /* use_graphs: replay each step and its CG launches from captured hipGraphs
  * (default 1); chunk: CG launches per replay, even, >= 2 (default 16). */
 int hzo_batch_config(hzo_batch* batch, int use_graphs, int chunk);
+/* lanes (1 .. 4, default 2): host threads, each with its own stream, over
+ * which the groups that do not run the resident solve are shared out, so
+ * that their launches overlap on the GPU (resident groups always run alone).
+ * Every film ends in the same state whatever the number of lanes. */
+int hzo_batch_set_lanes(hzo_batch* batch, int lanes);
 /* Tile depth (2, 4 or 8) of the groups of several films, and group budget in
  * MiB of CG working set (34 bytes per voxel; 0: one group of all films). */
 int hzo_batch_set_shape(hzo_batch* batch, int tile_kz, double group_mib);
