@@ -504,7 +504,7 @@ def main():
         "config": {"workload": "256x256x32 film, 200 grains FE/AFE/DE 0.6/0.2/0.2, +-20% Landau, "
                                "triangle +-3 MV/cm / 2000 steps, PCG ||r||<=1e-8||b||",
                    "nx": NX, "ny": NY, "nz": NZ, "grains": NG,
-                   "precision": "FP32 stencil+CG vectors, FP64 inner products, FP64 residual every 20 its, "
+                   "precision": "FP32 stencil+CG vectors, per-thread FP32 partial sums, FP64 global inner products, FP64 residual every 20 its, "
                                 "single-reduction CG (cg_form 3: one register-resident launch per solve; "
                                 "1/2: one streaming launch per iteration)"},
         "pcg_iters_mean": float(np.mean(head["its"])),

@@ -155,12 +155,10 @@ __device__ __forceinline__ T op_identity(T xc, int k, const StepConsts& c) {
 // weighted sum and no product is held beside it.  r.r and b.b, which the
 // relative-residual rule reads, stay unweighted.  MIR = false compiles the
 // plain sum, for kernels launched only on films with Dirichlet electrodes.
-// v as the wave's first active lane holds it: for values every lane
-// computes alike, which the compiler then keeps in a scalar register.
-__device__ __forceinline__ float uniform_f(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
-}
-
+// The FP64 form accumulates in FP64 (setup, residual replacement, the
+// two-kernel PCG); cg_dot_addf accumulates a thread's own planes in FP32
+// (the single-reduction CG iterations, at most 32 terms per thread), whose
+// sums are then added across threads, waves and workgroups in FP64.
 template <bool MIR>
 __device__ __forceinline__ void cg_dot_add(double& acc, float x, float y, const StepConsts& c, int k) {
     if constexpr (MIR) {
@@ -170,6 +168,23 @@ __device__ __forceinline__ void cg_dot_add(double& acc, float x, float y, const 
     } else {
         acc = __builtin_fma((double)x, (double)y, acc);
     }
+}
+
+template <bool MIR>
+__device__ __forceinline__ void cg_dot_addf(float& acc, float x, float y, const StepConsts& c, int k) {
+    if constexpr (MIR) {
+        const bool mirrored = c.zbc != 0 && (k == 0 || k == c.nz - 1);
+        const float up = mirrored ? 2.0f : 1.0f, down = mirrored ? 0.5f : 1.0f;  // uniform
+        acc = __builtin_fmaf(x, y, acc * up) * down;
+    } else {
+        acc = __builtin_fmaf(x, y, acc);
+    }
+}
+
+// v as the wave's first active lane holds it: for values every lane
+// computes alike, which the compiler then keeps in a scalar register.
+__device__ __forceinline__ float uniform_f(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
 }
 
 __device__ __forceinline__ bool keep_going(const PcgState& st, double rz, double rr) {

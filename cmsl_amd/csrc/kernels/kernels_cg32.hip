@@ -39,14 +39,15 @@
 namespace hzo {
 
 
-// acc += w x y with the mirrored-plane weight w = 1/2 taken as
-// (2 acc + x y) / 2 (the terms of cg_dot_add), chosen by a uniform branch
+// acc += w x y in FP32 over a thread's planes (the launch's sums across
+// threads are FP64), with the mirrored-plane weight w = 1/2 taken as
+// (2 acc + x y) / 2 (the terms of cg_dot_addf), chosen by a uniform branch
 // instead of multiplications by 1 on every other plane.
-__device__ __forceinline__ void dot_w(double& acc, float x, float y, bool mirrored) {
+__device__ __forceinline__ void dot_w(float& acc, float x, float y, bool mirrored) {
     if (mirrored)
-        acc = __builtin_fma((double)x, (double)y, acc * 2.0) * 0.5;
+        acc = __builtin_fmaf(x, y, acc * 2.0f) * 0.5f;
     else
-        acc = __builtin_fma((double)x, (double)y, acc);
+        acc = __builtin_fmaf(x, y, acc);
 }
 
 // How an iteration updates x_lo (from the control block before the launch's
@@ -299,6 +300,7 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
         const float alpha = uniform_f((float)alpha_d), beta = uniform_f((float)beta_d);
         const bool fresh = __builtin_amdgcn_readfirstlane(beta_d == 0.0 ? 1 : 0) != 0;  // first iteration: no previous p, s
         const unsigned o_r2 = (cur ^ 1u) * seg, o_s2 = (cur ^ 1u) * seg;
+        float af[3] = {0.0f, 0.0f, 0.0f};  // the thread's sums over its planes (FP32)
         // The plane loop, built twice: an iteration (ITER) and the
         // re-formation after a residual replacement (WCOMP), so neither
         // carries the other's selects.
@@ -340,8 +342,8 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
                 }
                 u[q] = rn * L[q].d;
                 if (ok) {
-                    dot_w(acc[0], rn, u[q], mirrored(k));
-                    acc[2] += (double)rn * (double)rn;
+                    dot_w(af[0], rn, u[q], mirrored(k));
+                    af[2] = __builtin_fmaf(rn, rn, af[2]);
                 }
                 if (interior) {
                     pl[q % 6][ty + 1][tx + 1] = u[q];
@@ -374,7 +376,7 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
                         if (ident_plane(km) && (code & SC_IDENTITY)) wv = op_identity(xc, km, c);
                         const bool okm = tp.live && km < nz;
                         bstore(RA, okm ? cv : aend, o_wo + so[m] * pb4, wv);
-                        if (okm) dot_w(acc[1], wv, xc, mirrored(km));
+                        if (okm) dot_w(af[1], wv, xc, mirrored(km));
                     }
                 }
             }
@@ -383,6 +385,8 @@ __device__ __forceinline__ void cg32_body(const CG32Bufs& b, const StepConsts& c
             planes(std::true_type{});
         else
             planes(std::false_type{});
+#pragma unroll
+        for (int qq = 0; qq < 3; ++qq) acc[qq] = af[qq];
     }
     write_block_partials_w<3>(acc, io ? b.part[1] : b.part[0], ps, T.blk, spart);
 }
@@ -582,6 +586,7 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
     } else {
         const float alpha = uniform_f((float)alpha_d), beta = uniform_f((float)beta_d);
         const bool fresh = __builtin_amdgcn_readfirstlane(beta_d == 0.0 ? 1 : 0) != 0;
+        float af[3] = {0.0f, 0.0f, 0.0f};  // the thread's sums over its planes (FP32)
         // The plane loop, built twice: an iteration (ITER) and the
         // re-formation after a residual replacement (WCOMP), so neither
         // carries the other's selects.
@@ -655,8 +660,8 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
                     const bool ok = tp.live && k < nz;
                     if constexpr (iter) vst(RA, ok ? cv : aend, o_rn + so(m) * pb4, rc);
                     if (ok) {
-                        dot_w(acc[0], rc, uc[m], mirrored(k));
-                        acc[2] += (double)rc * (double)rc;
+                        dot_w(af[0], rc, uc[m], mirrored(k));
+                        af[2] = __builtin_fmaf(rc, rc, af[2]);
                     }
                 }
             };
@@ -673,7 +678,7 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
                                        Um[ty][tx + 1], xu, xd);
                 if (ident_plane(k) && (cd[m] & SC_IDENTITY)) wv = op_identity(uc[m], k, c);
                 if (tp.live && k < nz) {
-                    dot_w(acc[1], wv, uc[m], mirrored(k));
+                    dot_w(af[1], wv, uc[m], mirrored(k));
                 }
             };
             // Phase n: A(n); B(n - 1), whose in-plane p was written in phase
@@ -689,6 +694,10 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
                 stageA(n);
                 if (n >= 2) stageB(n - 1);
                 if (n >= 4) stageC(n - 2);
+                // the sums are formed phase by phase (left to the compiler,
+                // their FMAs were gathered late in the loop and the kernel
+                // spilled at depth 32)
+                asm volatile("" : "+v"(af[0]), "+v"(af[1]), "+v"(af[2]));
                 if (n + 1 < NW) __syncthreads();
             }
         };
@@ -696,6 +705,8 @@ __device__ __forceinline__ void cg32r_body(const CG32Bufs& b, const StepConsts& 
             planes(std::true_type{});
         else
             planes(std::false_type{});
+#pragma unroll
+        for (int qq = 0; qq < 3; ++qq) acc[qq] = af[qq];
     }
     write_block_partials_w<3>(acc, io ? b.part[1] : b.part[0], ps, T.blk, spart);
 }

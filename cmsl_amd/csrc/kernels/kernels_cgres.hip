@@ -429,6 +429,8 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
 #pragma unroll
             for (int kk = 0; kk < NH; ++kk) uT[k0 + kk][typ + 1][txp + 1] = r[kk] * d[kk];
             __syncthreads();
+            // the thread's sums over its planes in FP32, across threads in FP64
+            float af[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int kk = 0; kk < NH; ++kk) {
                 const int k = k0 + kk;
@@ -452,11 +454,13 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
                     if (code & SC_IDENTITY) wvl = op_identity(xc, k, c);
                 }
                 w[kk] = wvl;
-                cg_dot_add<MIR>(acc[0], r[kk], xc, c, k);
-                cg_dot_add<MIR>(acc[1], wvl, xc, c, k);
-                acc[2] += (double)r[kk] * (double)r[kk];
+                cg_dot_addf<MIR>(af[0], r[kk], xc, c, k);
+                cg_dot_addf<MIR>(af[1], wvl, xc, c, k);
+                af[2] = __builtin_fmaf(r[kk], r[kk], af[2]);
                 eT[k][typ][txp] = wvl;
             }
+#pragma unroll
+            for (int qq = 0; qq < 3; ++qq) acc[qq] = af[qq];
             wave_slots(acc, sredA);
             if (fold_now) {
                 // the iteration a periodic replacement follows: the
