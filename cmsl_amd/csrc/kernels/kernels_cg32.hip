@@ -133,7 +133,8 @@ __device__ __forceinline__ void cg32_resid(const CG32Bufs& b, const StepConsts& 
         sx[sb][ty + 1][tx + 1] = x0;
         if (tp.has_h) sx[sb][tp.hy][tp.hx] = xh;
         const int kn = min(k + 1, k1 - 1);
-        const double xpp = X(oc, min(k + 2, nz - 1)), xhn = X(oh, kn), bn = B(kn);
+        // (the last plane reloads the ghost plane above the tile, unused)
+        const double xpp = X(oc, min(k + 2, min(k1, nz - 1))), xhn = X(oh, kn), bn = B(kn);
         const unsigned coden = CODE(kn);
         __syncthreads();
         double xu = xp, xd = xm;

@@ -88,10 +88,12 @@ __device__ __forceinline__ void tdgl_f32_body(const Fp32Bufs& b, const StepConst
         }
         // the next plane: the column two planes up and its cell word now, its
         // halo cell (an L2 hit, loaded after this plane's evaluation so its
-        // registers are free during it)
+        // registers are free during it).  The tile's last plane loads the
+        // ghost plane above the tile again (unused) rather than a plane
+        // beyond it.
         Col32 above_n;
         const int kn = min(k + 1, k1 - 1);
-        ld(cc, min(k + 2, nz - 1), above_n);
+        ld(cc, min(k + 2, min(k1, nz - 1)), above_n);
         const uint32_t cw_n = b.cw[cc + plane * kn];
         float T9[9];
 #pragma unroll
