@@ -79,14 +79,27 @@ __device__ __forceinline__ bool res_barrier(unsigned* sync, int* err, int q, int
         const unsigned target = (unsigned)(q + 1) * (unsigned)ngrp;
         unsigned spins = 0;
         int ok = 1;
-        while (__hip_atomic_load(sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        auto give_up = [&]() {
             if (++spins > RES_SPIN_LIMIT ||
                 ((spins & 1023u) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
                 __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 ok = 0;
-                break;
+                return true;
             }
             __builtin_amdgcn_s_sleep(1);
+            return false;
+        };
+        // Two polls in flight: each poll's load goes out before the previous
+        // one's value is tested (the loop alternates between two values, so
+        // no copy waits for the younger load), and the release is seen about
+        // half a load round trip sooner than by polls issued one after
+        // another.
+        unsigned va = __hip_atomic_load(sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        while (true) {
+            const unsigned vb = __hip_atomic_load(sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (va >= target || give_up()) break;
+            va = __hip_atomic_load(sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (vb >= target || give_up()) break;
         }
         sok = ok;
     }
