@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -480,6 +481,20 @@ def main():
     bpc = rl["bytes_per_cell"]
     achieved = rl["achieved"]
     traffic = measured_traffic(rl["kernel"], rl["iterations_per_launch"])
+    # The same launch priced at the bytes of an unfused three-pass PCG
+    # iteration (SpMV: read p and eps, write q; update: read x, p, r, q, D^-1,
+    # write x, r; direction: read r, D^-1, p, write p: 14 FP32 words per
+    # cell), and the timed steps at that PCG's bytes per cell-step (100 +
+    # 56 N_it + 44 per FP64 residual refinement every 20 iterations).
+    # Fractions above 1 mean the fused and resident kernels move fewer bytes
+    # than the unfused iteration would, not that HBM ran past its peak.
+    unf_bpc = 14.0 * 4.0
+    unf_achieved = achieved * unf_bpc / bpc
+    step_bytes = sum(100.0 + 56.0 * n + 44.0 * math.ceil(n / 20.0) for n in head["its"])
+    unfused = {"bytes_per_cell_iteration": unf_bpc, "achieved": unf_achieved, "frac": unf_achieved / HBM_PEAK_GBS,
+               "step_bytes_per_cell": step_bytes / len(head["its"]),
+               "step_achieved": step_bytes * N / head["dt"] / 1e9,
+               "step_frac": step_bytes * N / head["dt"] / 1e9 / HBM_PEAK_GBS}
 
     adj = None if args.no_adjoint else run_adjoint(args)
     loop = None if args.no_loop else run_loop_errors(inp)
@@ -524,7 +539,8 @@ def main():
                      "ms_per_iteration": rl["ms_per_iteration"],
                      "launches_timed": rl["launches_timed"], "iterations_timed": rl["iterations_timed"],
                      "per": "launch; algorithmic bytes = bytes_per_cell x voxels x CG iterations of the "
-                            "launch (replacement and re-formation passes amortized)"},
+                            "launch (replacement and re-formation passes amortized)",
+                     "unfused_pcg": unfused},
         "cg_form": head["form"],
         "kernel_ms": {k: v[1] for k, v in kt.items()},
         "cpu_baseline": cpu,
