@@ -281,8 +281,7 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
     float af = 0.0f, bf = 0.0f;
     bool fresh = false;
     bool fold_now = false;  // an iteration that a periodic residual replacement follows
-    auto decide = [&](const double (&t)[3], bool cap) {
-        PcgState st = load_state(&sst[dpar]);
+    auto decide = [&](const double (&t)[3], bool cap, PcgState st) {
         double a = 0.0, bb = 0.0;
         int m = res_advance(st, cg_decide(st, t, a, bb));
         if (m != 1 && cap) {
@@ -314,7 +313,7 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
         }
         wg_total(a, sredB, tot);
     }
-    decide(tot, false);
+    decide(tot, false, load_state(&sst[dpar]));
     const float ih2 = (float)c.inv_h2;
     const int qcap = 4 * sst[0].max_iter + 16;
     int q = 0;             // grid barriers passed
@@ -577,6 +576,12 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
                 bstore(RP, ((slot * (unsigned)G + (unsigned)blk) * 4u + (unsigned)qq) * 8u, 0u, v);
             }
         }
+        // the control block for the decision after the barrier, read before
+        // the barrier is waited for (a pass with partials has passed a
+        // workgroup barrier since the last decision wrote it; the first pass
+        // of a replacement has not, and makes no decision)
+        PcgState st_next;
+        if (partials) st_next = load_state(&sst[dpar]);
         if (!res_barrier(b.sync, b.err, q, G, blk0, sok)) break;
         ++q;
         if (!partials) {  // RESID continues with its second pass
@@ -599,7 +604,7 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
         ring_r = mode == 2;
         const bool folded = fold_now;  // the pass just ended did a replacement's first pass
         if (mode == 2) tot[2] = 0.0;
-        decide(tot, q >= qcap);
+        decide(tot, q >= qcap, st_next);
         resid_x = mode == 2 && folded;
     }
     // the solution's correction goes back; x_hi is already in memory
