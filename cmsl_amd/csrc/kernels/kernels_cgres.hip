@@ -145,7 +145,7 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
     __shared__ float rR[NR], sR[NR], dR[NR];
     __shared__ HPair<float> sHP[64];
     __shared__ double sH64[16];
-    __shared__ double xP[2][2][RTH][RTH];  // RESID: [half][plane parity]
+    __shared__ double xP[2][4][RTH][RTH];  // RESID: [half][plane % 4]
     __shared__ double sredA[3][RNW], sredB[3][RNW];
     __shared__ int sok;
 
@@ -499,9 +499,9 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
             // edge columns) is staged in LDS as doubles over uT, which this
             // pass does not use; the column's x on planes k0 - 1 .. k0 + NH
             // (clamped into the film) is loaded up front, b two planes ahead.
-            // Both halves walk their planes together, each plane's x on tile
-            // and ring staged in xP[half][plane parity] (one barrier per
-            // plane).
+            // Both halves walk their planes together, two planes per
+            // barrier, each plane's x on tile and ring staged in
+            // xP[half][plane % 4].
             double* xR = reinterpret_cast<double*>(&uT[0][0][0]);
             const unsigned o8 = colp * 8u;
             double rx[RPT];
@@ -524,45 +524,57 @@ __device__ __forceinline__ void cgres_body(const ResBufs& b, const StepConsts& c
             for (int m = 0; m < RPT; ++m) xR[tidp + RNT * m] = rx[m];
             __syncthreads();
             const int rl = (int)(tidp & (RT * RT - 1));  // ring loaders: 64 threads of each half
+            static_assert(NH % 2 == 0, "the replacement stages its planes in pairs");
 #pragma unroll
-            for (int kk = 0; kk < NH; ++kk) {
-                const int k = k0 + kk;
-                double (*S)[RTH] = xP[tz][kk & 1];
-                const double xm = xq[kk], x0 = xq[kk + 1], xp = xq[kk + 2];
-                S[typ + 1][txp + 1] = x0;
-                if (rl < 4 * RT) {  // the plane's 64 ring cells
-                    const int side = rl / RT, pos = rl % RT;
-                    const unsigned e = (unsigned)((side * NZ + k) * RT + pos);
-                    int kq, ly, lx;
-                    ring_lds(e, kq, ly, lx);
-                    S[ly][lx] = xR[e];
+            for (int kk = 0; kk < NH; kk += 2) {
+                // planes kk and kk + 1 of both halves staged (four plane
+                // buffers: a pair's buffers are written again two pairs
+                // later, after the next pair's barrier), b two pairs ahead
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const int kj = kk + j, k = k0 + kj;
+                    double (*S)[RTH] = xP[tz][kj & 3];
+                    S[typ + 1][txp + 1] = xq[kj + 1];
+                    if (rl < 4 * RT) {  // the plane's 64 ring cells
+                        const int side = rl / RT, pos = rl % RT;
+                        const unsigned e = (unsigned)((side * NZ + k) * RT + pos);
+                        int kq, ly, lx;
+                        ring_lds(e, kq, ly, lx);
+                        S[ly][lx] = xR[e];
+                    }
+                    if (kj + 2 < NH) bq[kj + 2] = bload<double>(Rb, o8, (unsigned)(k + 2) * pl * 8u);
                 }
-                if (kk + 2 < NH) bq[kk + 2] = bload<double>(Rb, o8, (unsigned)(k + 2) * pl * 8u);
-                const double bv = bq[kk];
                 __syncthreads();
-                double xu = xp, xd = xm;
-                if (c.zbc != 0) {
-                    if (k == 0) xd = xu;
-                    else if (k == NZ - 1) xu = xd;
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const int kj = kk + j, k = k0 + kj;
+                    const double(*S)[RTH] = xP[tz][kj & 3];
+                    const double xm = xq[kj], x0 = xq[kj + 1], xp = xq[kj + 2];
+                    const double bv = bq[kj];
+                    double xu = xp, xd = xm;
+                    if (c.zbc != 0) {
+                        if (k == 0) xd = xu;
+                        else if (k == NZ - 1) xu = xd;
+                    }
+                    const unsigned code = code_of(kj);
+                    // the row as a stencil (an identity row's code reads table
+                    // entry 0, and its value is replaced below)
+                    const double* Hr = sH64 + (code & 3u) * 4u;
+                    const double num = Hr[(code >> 2) & 3u] * (S[typ + 1][txp + 2] - x0) -
+                                       Hr[(code >> 4) & 3u] * (x0 - S[typ + 1][txp]) +
+                                       Hr[(code >> 6) & 3u] * (S[typ + 2][txp + 1] - x0) -
+                                       Hr[(code >> 8) & 3u] * (x0 - S[typ][txp + 1]) +
+                                       Hr[(code >> 10) & 3u] * (xu - x0) - Hr[(code >> 12) & 3u] * (x0 - xd);
+                    double Ax = -(num * c.inv_h2);
+                    if (c.zbc != 0 || k == 0 || k == NZ - 1) {  // planes that can hold identity rows
+                        if (code & SC_IDENTITY) Ax = op_identity(x0, k, c);
+                    }
+                    const float rv = (float)(bv - Ax);
+                    r[kj] = rv;
+                    cg_dot_add<MIR>(acc[0], rv, rv * d[kj], c, k);
+                    acc[1] += (double)rv * (double)rv;
+                    eT[k][typ][txp] = rv;
                 }
-                const unsigned code = code_of(kk);
-                // the row as a stencil (an identity row's code reads table
-                // entry 0, and its value is replaced below)
-                const double* Hr = sH64 + (code & 3u) * 4u;
-                const double num = Hr[(code >> 2) & 3u] * (S[typ + 1][txp + 2] - x0) -
-                                   Hr[(code >> 4) & 3u] * (x0 - S[typ + 1][txp]) +
-                                   Hr[(code >> 6) & 3u] * (S[typ + 2][txp + 1] - x0) -
-                                   Hr[(code >> 8) & 3u] * (x0 - S[typ][txp + 1]) + Hr[(code >> 10) & 3u] * (xu - x0) -
-                                   Hr[(code >> 12) & 3u] * (x0 - xd);
-                double Ax = -(num * c.inv_h2);
-                if (c.zbc != 0 || k == 0 || k == NZ - 1) {  // planes that can hold identity rows
-                    if (code & SC_IDENTITY) Ax = op_identity(x0, k, c);
-                }
-                const float rv = (float)(bv - Ax);
-                r[kk] = rv;
-                cg_dot_add<MIR>(acc[0], rv, rv * d[kk], c, k);
-                acc[1] += (double)rv * (double)rv;
-                eT[k][typ][txp] = rv;
             }
             wave_slots(acc, sredA);
             publish_staged(slot);
