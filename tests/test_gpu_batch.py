@@ -212,9 +212,10 @@ def test_resident_groups_match_their_own_runs():
 def test_lanes_leave_every_film_in_the_same_state():
     # resident groups (depth 16 and 24), one streaming group of two films on
     # the batch kernels and one single-film streaming group, run with 1, 2
-    # and 4 lanes: the streaming groups then run side by side on their own
-    # streams and host threads, and every film's iterations, <Pmz> samples,
-    # P and potential are bit-identical to the one-lane run
+    # and 4 lanes (and 2 lanes with plain launches): the streaming groups
+    # then run side by side on their own streams and host threads, and every
+    # film's iterations, <Pmz> samples, P and potential are bit-identical to
+    # the one-lane run
     shapes = [(64, 48, 16, 12), (32, 48, 24, 6), (40, 72, 12, 9), (96, 32, 10, 15), (32, 32, 8, 6)]
     kz = [0, 0, 2, 2, 2]
     inps = [film_inputs(nx, ny, nz, ng, fracs=(0.3, 0.5, 0.2), seed=61 + f, jitter=0.2)
@@ -223,10 +224,10 @@ def test_lanes_leave_every_film_in_the_same_state():
     amps = [3.0, 2.0, 2.5, 1.5, 2.2]
     Ez = np.stack([inp.field_nd(H.triangle_wave(S, amps[f], S)) for f, inp in enumerate(inps)], axis=1)
     out = []
-    for lanes in (1, 2, 4):
-        _say(f"lanes={lanes}: 2 resident films, 3 streaming films in 2 groups, {S} steps")
+    for lanes, graphs in ((1, True), (2, True), (4, True), (2, False)):
+        _say(f"lanes={lanes} graphs={graphs}: 2 resident films, 3 streaming films in 2 groups, {S} steps")
         films = [_film(inp, k) for inp, k in zip(inps, kz)]
-        batch = H.Batch(films, tile_kz=2, lanes=lanes)
+        batch = H.Batch(films, tile_kz=2, lanes=lanes, use_graphs=graphs)
         batch.set_shape(2, 1.5)
         kinds = [batch.film_group(f)[1:] for f in range(len(films))]
         assert kinds == [(1, 2), (1, 2), (2, 0), (1, 1), (2, 0)], kinds
